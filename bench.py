@@ -1,0 +1,196 @@
+"""Benchmark of the MI355X UMAP/densMAP fitting path (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints ONE JSON line.  Workloads:
+  epochs   the "adam_gpu" epoch loop (500 Adam epochs, k=15-style graph of
+           1M nodes, 2-D output): one step = one full 500-epoch optimisation
+           with inputs resident in HBM.
+
+The CPU baseline is the C restatement of the reference's parallel Adam
+(optimise_embedding_adam_parallel) timed on this host's cores on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "manifolds-rs_amd"))
+
+METRIC = "UMAP fit sec @1M×64 k=15 (recall@15=1.0 vs FP64); edge-updates/s; served p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def synthetic_knn_graph(n: int, k: int, n_epochs: int, seed: int = 42):
+    """A UMAP-shaped graph for the epoch workload: exact kNN (k) of n points drawn
+    from 50 2-D Gaussian blobs, membership weights exp(-(d - rho)/sigma) with
+    sigma the row's mean excess distance, fuzzy union W + W^T - W*W^T, weak
+    edges (< max/n_epochs) dropped.  Returns a row-sorted adjacency CSR."""
+    from scipy.sparse import csr_matrix
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(0, 20, size=(50, 2))
+    pts = centres[rng.integers(0, 50, size=n)] + rng.normal(size=(n, 2))
+    d, idx = cKDTree(pts).query(pts, k=k + 1, workers=-1)
+    d, idx = d[:, 1:] ** 2, idx[:, 1:]
+    rho = d[:, :1]
+    sig = np.maximum((d - rho).mean(axis=1, keepdims=True), 1e-6)
+    w = np.exp(-(d - rho) / sig).astype(np.float32)
+    W = csr_matrix((w.ravel(), idx.ravel(), np.arange(0, n * k + 1, k)), shape=(n, n))
+    S = (W + W.T - W.multiply(W.T)).tocsr()
+    S.data[S.data < S.data.max() / n_epochs] = 0
+    S.eliminate_zeros()
+    S.sort_indices()
+    return S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(np.float32)
+
+
+def bench_epochs(args):
+    import torch
+
+    import mrs_amd as M
+
+    n, k, n_epochs = args.n, 15, 500
+    t0 = time.time()
+    adj = synthetic_knn_graph(n, k, n_epochs)
+    log(f"[bench] graph n={n} nnz={adj[0][-1]} built in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(42)
+    embd0 = rng.uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = n_epochs
+
+    ctx = M.Context(0)
+    st = M.UmapGpuState(ctx, embd0, *adj, p)
+    E = st.n_edges
+    per = st.read(M.BUF_PER)
+    counts = M.schedule_active_counts(per, n_epochs)
+    A_total = int(counts.sum())
+    d_embd0 = torch.from_numpy(embd0).cuda()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def step():
+        st.reset(d_embd0)
+        st.run_epochs(0, n_epochs, 42, M.EPOCH_EDGE_ATOMIC)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    times = []
+    st.set_profiling(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        ev0.record(stream)
+        step()
+        ev1.record(stream)
+        ev1.synchronize()
+        times.append(ev0.elapsed_time(ev1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    st.set_profiling(False)
+    kt = st.kernel_times()
+    ms_per_step = float(np.mean(times))
+
+    # algorithmic bytes of the edge kernel: 16 B/edge streamed (i, j, next, per)
+    # + per active edge 4 B schedule write, 2x8 B endpoint gathers, 2r x8 B
+    # negative gathers, 2x8 B gradient scatter
+    r = p.neg_sample_rate
+    edge_ms, edge_launches = kt["epoch_edges"]
+    bytes_edge_total = 16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)
+    edge_avg_ms = edge_ms / edge_launches
+    bytes_per_launch = bytes_edge_total / n_epochs
+    achieved = bytes_per_launch / (edge_avg_ms * 1e-3) / 1e9
+    adam_ms, adam_launches = kt["adam_clear"]
+
+    res = dict(
+        metric=METRIC,
+        value=ms_per_step / 1e3,
+        unit="s",
+        n_gpus=1,
+        steps=args.steps,
+        warmup=args.warmup,
+        ms_per_step=ms_per_step,
+        higher_is_better=False,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f32",
+        data="synthetic",
+        config=dict(workload=f"adam_gpu epoch loop only: {n_epochs} epochs, n={n}, "
+                    f"2-D output, k={k} kNN graph of 50 2-D blobs, E={E}",
+                    n=n, k=k, n_epochs=n_epochs, n_edges=E, edge_updates=A_total),
+        edge_updates_per_s=A_total / (ms_per_step / 1e3),
+        kernels={kk: dict(total_ms=v[0], launches=v[1], avg_us=1e3 * v[0] / v[1])
+                 for kk, v in kt.items()},
+        roofline=dict(bound="hbm", kernel="k_epoch_edges", achieved=achieved,
+                      peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                      traffic=None, avg_launch_us=edge_avg_ms * 1e3,
+                      algorithmic_bytes_per_launch=bytes_per_launch),
+        wall_s=wall,
+    )
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_epochs(embd0, adj, p, args)
+    return res
+
+
+def cpu_baseline_epochs(embd0, adj, p, args):
+    """Reference CPU path (parallel Adam restated in C, LUT pow, OpenMP) on a
+    bounded sample: the first S epochs of the same 1M-node fit, scaled to 500."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as orc
+
+    import mrs_amd as M
+
+    cores = os.cpu_count() or 1
+    S = args.cpu_epochs
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = p.n_epochs
+    # run S epochs of the 500-epoch schedule: n_epochs stays 500 for the lr decay,
+    # the oracle is stopped after S epochs by giving it a truncated copy
+    q2 = M.UmapOptimParams.default_2d()
+    q2.n_epochs = S
+    t0 = time.perf_counter()
+    orc.optimise_adam_parallel(embd0, *adj, q2, 42, use_lut=True, n_threads=cores)
+    dt = time.perf_counter() - t0
+    return dict(value=dt * p.n_epochs / S, unit="s", cores=cores, kind="port",
+                sample=f"{S} of {p.n_epochs} epochs of the same graph, scaled x{p.n_epochs / S:g}",
+                measured_s=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="epochs")
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-epochs", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps(dict(error="this build measures one GPU only; multi-GPU runs are "
+                                    "out of scope (the path would shard by query rows / node "
+                                    "ranges, see DESIGN.md)", n_gpus=args.gpus)))
+        sys.exit(2)
+    if args.workload == "epochs":
+        res = bench_epochs(args)
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * mrs_amd.h -- C ABI of the MI355X-native UMAP/densMAP fitting path.
+ *
+ * Drop-in boundary for the `umap_gpu` / `densmap_gpu` fit of
+ * GregorLueg/manifolds-rs v0.3.15 with the "adam_gpu" optimiser and the
+ * "exhaustive_gpu" kNN.  Each entry point names the reference interface it
+ * replaces (path:line into the reference crate).  Plain pointers and sizes
+ * only; float32 data; row-major buffers.  Any pointer argument may be host or
+ * device memory: device pointers (e.g. from hipMalloc) are used in place,
+ * host pointers are staged through the context's stream.
+ *
+ * Status codes (int32) map 1:1 onto ManifoldsError (src/errors.rs):
+ *   MRS_OK, NoData, NoGraphEdges, DegenerateLocalRadii, CubeclUtilsError
+ *   (device/allocation failures), and invalid arguments.
+ * mrs_last_error() returns the message of the last failure on this thread.
+ *
+ * Threading: calls are synchronous on the caller's thread (as the reference's
+ * are: every readback is a sync).  Use one mrs_ctx per host thread.
+ */
+#ifndef MRS_AMD_H
+#define MRS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRS_OK 0
+#define MRS_NO_DATA 1                /* ManifoldsError::NoData, src/errors.rs:78 */
+#define MRS_NO_GRAPH_EDGES 2         /* ManifoldsError::NoGraphEdges, src/errors.rs:82 */
+#define MRS_DEGENERATE_LOCAL_RADII 3 /* ManifoldsError::DegenerateLocalRadii, src/errors.rs:101 */
+#define MRS_DEVICE_ERROR 4           /* ManifoldsError::CubeclUtilsError, src/errors.rs:54 */
+#define MRS_INVALID_ARGUMENT 5
+
+/* epoch-kernel strategies for the "adam_gpu" optimiser */
+#define MRS_EPOCH_EDGE_ATOMIC 0   /* edge-parallel, FP32 atomic scatter-add (default, fastest) */
+#define MRS_EPOCH_NODE_EXACT 1    /* node-parallel CSR walk, deterministic, reference order */
+
+/* exhaustive kNN precisions */
+#define MRS_KNN_FP32 0            /* FP32 MFMA candidates + exact FP32 re-rank */
+#define MRS_KNN_BF16_RERANK 1     /* BF16 MFMA candidates + exact FP32 re-rank */
+
+typedef struct mrs_ctx mrs_ctx;
+typedef struct mrs_umap_state mrs_umap_state;
+typedef struct mrs_graph mrs_graph;
+
+/* UmapOptimParams<f32>, src/training/umap_optimisers.rs:22-44 */
+typedef struct {
+    float a, b, lr, gamma;
+    int32_t n_epochs, neg_sample_rate;
+    float min_dist, beta1, beta2, eps;
+} mrs_umap_optim_params;
+
+/* DensParams<f32>, src/utils/density.rs:69-79 */
+typedef struct {
+    float lambda, frac, var_shift;
+} mrs_dens_params;
+
+/* DensState<f32>, src/utils/density.rs:140-160 (r and mu_sum are [n]) */
+typedef struct {
+    mrs_dens_params params;
+    const float *r;
+    const float *mu_sum;
+    float mu_tot;
+} mrs_dens_state;
+
+/* ------------------------------------------------------------------ misc */
+
+/* message of the last failure on the calling thread ("" if none) */
+const char *mrs_last_error(void);
+
+/* Context: device ordinal, HBM budget in bytes (0 = no limit). */
+int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx **out);
+int mrs_ctx_destroy(mrs_ctx *ctx);
+/* block until all work queued on the context's stream has finished */
+int mrs_ctx_synchronize(mrs_ctx *ctx);
+/* the context's HIP stream (hipStream_t) so callers can order their own work */
+void *mrs_ctx_stream(mrs_ctx *ctx);
+
+/* parse_umap_optimiser_gpu, src/training/umap_optimiser_gpu.rs:44-52.
+ * 0 adam_gpu, 1 adam_parallel, 2 adam, 3 sgd; -1 if unknown. */
+int32_t mrs_parse_umap_optimiser_gpu(const char *s);
+
+/* parse_ann_search_gpu, src/data/nearest_neighbours_gpu.rs:203-210.
+ * 0 ivf_gpu, 1 nndescent_gpu, 2 exhaustive_gpu; -1 if unknown. */
+int32_t mrs_parse_ann_search_gpu(const char *s);
+
+/* Host evaluation of the negative-sample hash the epoch kernels use (same
+ * code path, including the invariant-divisor remainder): gpu_hash_neg /
+ * cpu_hash_neg, src/training/umap_optimiser_gpu.rs:611-624, :1379-1391.  n >= 1. */
+uint32_t mrs_hash_neg(uint32_t seed, uint32_t node, uint32_t epoch, uint32_t edge_local,
+                      uint32_t neg, uint32_t n);
+
+/* ------------------------------------------------- optimiser (adam_gpu) */
+
+/* UmapGpuState::upload (src/training/umap_optimiser_gpu.rs:374-452) preceded by
+ * UmapCsrGraph::from_graph (:132-209).  The graph is an adjacency list in CSR
+ * form: graph[i] = (adj_cols, adj_vals)[adj_off[i] .. adj_off[i+1]).
+ * embd is [n * n_dim] row-major.  dens may be NULL (plain UMAP). */
+int mrs_umap_state_create(mrs_ctx *ctx, const float *embd, int64_t n, int32_t n_dim,
+                          const int64_t *adj_off, const int32_t *adj_cols,
+                          const float *adj_vals, const mrs_umap_optim_params *params,
+                          const mrs_dens_state *dens, mrs_umap_state **out);
+int mrs_umap_state_destroy(mrs_umap_state *st);
+
+/* sizes of a state: n, n_dim, n_edges */
+int mrs_umap_state_info(const mrs_umap_state *st, int64_t *n, int32_t *n_dim,
+                        int64_t *n_edges);
+
+/* Read/write a state buffer to/from caller memory.  which:
+ *  0 embd [n*dim] f32     1 m [n*dim]          2 v [n*dim]
+ *  3 node_grad [n*dim]    4 has_update [n] u32
+ *  5 node_edge_offsets [n+1] u32  6 csr_edge_idx [2E] u32  7 csr_other_node [2E] u32
+ *  8 epochs_per_sample [E] f32    9 epoch_of_next_sample [E] f32   10 mu [E] f32
+ * 11 dens_re [n]  12 dens_phi_sum [n]  13 dens_weight [n]  14 dens_r [n]
+ * 15 edge endpoints [2E] u32 (i, j interleaved, i < j, edge order) */
+int mrs_umap_state_read(mrs_umap_state *st, int32_t which, void *out);
+int mrs_umap_state_write(mrs_umap_state *st, int32_t which, const void *in);
+
+/* launch_grad_accum, umap_optimiser_gpu.rs:1016-1068 (node kernel :682-860).
+ * dens_coeff == NULL selects the plain-UMAP arm. */
+int mrs_launch_grad_accum(mrs_umap_state *st, int32_t epoch, uint32_t seed,
+                          const float *dens_coeff);
+/* launch_dens_update, umap_optimiser_gpu.rs:1089-1143: radii, FP64 statistics
+ * and weights, all on the device; returns dens_coeff. */
+int mrs_launch_dens_update(mrs_umap_state *st, float *dens_coeff_out);
+/* launch_adam_update, umap_optimiser_gpu.rs:1155-1196 */
+int mrs_launch_adam_update(mrs_umap_state *st, int32_t epoch);
+/* launch_edge_schedule_update, umap_optimiser_gpu.rs:1205-1227 */
+int mrs_launch_edge_schedule_update(mrs_umap_state *st, int32_t epoch);
+/* launch_grad_norm, umap_optimiser_gpu.rs:1245-1272 */
+int mrs_launch_grad_norm(mrs_umap_state *st, float *norm_out);
+
+/* Edge-parallel counterpart of launch_grad_accum + launch_edge_schedule_update:
+ * one thread per undirected edge, FP32 atomic scatter-add of the gradient into
+ * node_grad and has_update, schedule cursors advanced in place.  node_grad and
+ * has_update are cleared first if a node-parallel launch left them dirty.
+ * dens_coeff == NULL selects the plain-UMAP arm; otherwise the weights and radii
+ * of the last mrs_launch_dens_update are used with this coefficient. */
+int mrs_launch_epoch_edges(mrs_umap_state *st, int32_t epoch, uint32_t seed,
+                           const float *dens_coeff);
+/* Adam step (launch_adam_update semantics) for flagged nodes that also clears
+ * node_grad and has_update, closing an edge-parallel epoch. */
+int mrs_launch_adam_clear(mrs_umap_state *st, int32_t epoch);
+
+/* The epoch loop of optimise_embedding_adam_gpu (umap_optimiser_gpu.rs:1336-1360)
+ * for epochs [first, first+count), with the chosen strategy
+ * (MRS_EPOCH_EDGE_ATOMIC or MRS_EPOCH_NODE_EXACT).  Asynchronous on the
+ * context stream; a density epoch synchronises nothing (all statistics stay on
+ * the device). */
+int mrs_run_epochs(mrs_umap_state *st, int32_t first, int32_t count, uint64_t seed,
+                   int32_t strategy);
+
+/* Per-kernel timing of mrs_run_epochs with HIP events on the context stream
+ * (no counterpart in the reference; used by the benchmark).  enable=1 starts
+ * recording, enable=0 stops; mrs_umap_state_kernel_times waits for the
+ * recorded work and returns accumulated milliseconds and launch counts per
+ * kind: 0 edge epoch, 1 adam+clear, 2 node gradient, 3 adam, 4 schedule,
+ * 5 densMAP update (arrays of 8). */
+int mrs_umap_state_set_profiling(mrs_umap_state *st, int32_t enable);
+int mrs_umap_state_kernel_times(mrs_umap_state *st, double *ms_out, int64_t *count_out);
+
+/* Reset a state to the start of a fit (UmapGpuState::upload's initial values,
+ * umap_optimiser_gpu.rs:411-441): embd from the caller ([n*dim], host or device),
+ * m = v = node_grad = 0, has_update = 0, epoch_of_next_sample = epochs_per_sample. */
+int mrs_umap_state_reset(mrs_umap_state *st, const float *embd);
+
+/* Number of edges that tick in each epoch of the schedule
+ * (epoch_of_next_sample <= epoch, umap_optimiser_gpu.rs:940-955), replayed on the
+ * host in f32 from epochs_per_sample [n_edges]; counts_out is [n_epochs].  The
+ * sum is the fit's total edge-updates. */
+int mrs_schedule_active_counts(const float *epochs_per_sample, int64_t n_edges,
+                               int32_t n_epochs, int64_t *counts_out);
+
+/* optimise_embedding_adam_gpu, src/training/umap_optimiser_gpu.rs:1302-1369.
+ * embd [n*n_dim] is updated in place. */
+int mrs_optimise_embedding_adam_gpu(mrs_ctx *ctx, float *embd, int64_t n, int32_t n_dim,
+                                    const int64_t *adj_off, const int32_t *adj_cols,
+                                    const float *adj_vals,
+                                    const mrs_umap_optim_params *params,
+                                    const mrs_dens_state *dens, uint64_t seed,
+                                    int32_t verbose, int32_t strategy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,202 @@
+// Context, error plumbing, pointer classification and a device exclusive scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <string>
+
+#include "epoch_common.hpp"
+#include "mrs_internal.hpp"
+
+namespace mrs {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+bool is_device_ptr(const void* ptr) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // unregistered host memory: clear the sticky error
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+namespace {
+
+constexpr int kScanWg = 256;
+constexpr int kScanItems = 8;  // per thread
+constexpr int kScanTile = kScanWg * kScanItems;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of kScanWg values; returns the block total.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t& excl) {
+    __shared__ uint32_t wsum[kScanWg / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (int q = 0; q < kScanWg / 64; q++) {
+        if (q < w) base += wsum[q];
+        total += wsum[q];
+    }
+    excl = base + inc - v;
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(kScanWg) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out, uint64_t n,
+                                                        uint32_t* __restrict__ tile_sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        vals[q] = (base + q < n) ? in[base + q] : 0u;
+        s += vals[q];
+    }
+    uint32_t excl;
+    uint32_t total = block_excl_scan(s, excl);
+    uint32_t run = excl;
+#pragma unroll
+    for (int q = 0; q < kScanItems; q++) {
+        if (base + q < n) out[base + q] = run;
+        run += vals[q];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanWg) void k_scan_add(uint32_t* __restrict__ out, uint64_t n,
+                                                      const uint32_t* __restrict__ tile_offs) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    const uint32_t add = tile_offs[blockIdx.x];
+    for (int q = threadIdx.x; q < kScanTile; q += kScanWg)
+        if (base + q < n) out[base + q] += add;
+}
+
+__global__ void k_copy_total(const uint32_t* __restrict__ scanned,
+                             const uint32_t* __restrict__ in, uint64_t n,
+                             uint32_t* __restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + in[n - 1];
+}
+
+}  // namespace
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev,
+                        hipStream_t s) {
+    if (n == 0) {
+        if (total_dev) MRS_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+        return;
+    }
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    DevBuf<uint32_t> sums(tiles), offs(tiles);
+    // the last input element is needed for the total when scanning in place
+    DevBuf<uint32_t> last(1);
+    MRS_HIP(hipMemcpyAsync(last.p, in + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanWg), 0, s, in, out, n,
+                       sums.p);
+    MRS_CHECK_LAUNCH();
+    if (tiles > 1) {
+        uint32_t* dummy_total = nullptr;
+        exclusive_scan_u32(sums.p, offs.p, tiles, dummy_total, s);
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanWg), 0, s, out, n,
+                           offs.p);
+        MRS_CHECK_LAUNCH();
+    }
+    if (total_dev) {
+        hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, out, last.p, n, total_dev);
+        MRS_CHECK_LAUNCH();
+    }
+    // temporaries are freed on return: make sure the queued work no longer needs them
+    MRS_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace mrs
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* mrs_last_error(void) { return mrs::g_last_error.c_str(); }
+
+int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx** out) {
+    return mrs::guarded([&] {
+        if (!out) throw mrs::Error(MRS_INVALID_ARGUMENT, "out is NULL");
+        int count = 0;
+        MRS_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw mrs::Error(MRS_DEVICE_ERROR, "no such HIP device: " + std::to_string(device));
+        MRS_HIP(hipSetDevice(device));
+        auto* c = new mrs_ctx();
+        c->device = device;
+        c->hbm_budget = hbm_budget;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw mrs::Error(MRS_DEVICE_ERROR, "hipStreamCreate failed");
+        }
+        *out = c;
+    });
+}
+
+int mrs_ctx_destroy(mrs_ctx* ctx) {
+    return mrs::guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int mrs_ctx_synchronize(mrs_ctx* ctx) {
+    return mrs::guarded([&] {
+        if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
+        MRS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void* mrs_ctx_stream(mrs_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+static std::string lower(const char* s) {
+    std::string r = s ? s : "";
+    std::transform(r.begin(), r.end(), r.begin(), [](unsigned char c) { return std::tolower(c); });
+    return r;
+}
+
+int32_t mrs_parse_umap_optimiser_gpu(const char* s) {
+    std::string t = lower(s);
+    if (t == "adam_gpu") return 0;
+    if (t == "adam_parallel") return 1;
+    if (t == "adam") return 2;
+    if (t == "sgd") return 3;
+    return -1;
+}
+
+uint32_t mrs_hash_neg(uint32_t seed, uint32_t node, uint32_t epoch, uint32_t edge_local,
+                      uint32_t neg, uint32_t n) {
+    if (n == 0) return 0;
+    return mrs::hash_neg(seed, node, epoch, edge_local, neg, mrs::make_fastmod(n));
+}
+
+int32_t mrs_parse_ann_search_gpu(const char* s) {
+    std::string t = lower(s);
+    if (t == "ivf_gpu" || t == "ivf") return 0;
+    if (t == "nndescent_gpu" || t == "nndescent") return 1;
+    if (t == "exhaustive_gpu" || t == "exhaustive") return 2;
+    return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,851 @@
+// The "adam_gpu" layout epoch on CDNA4 (gfx950).
+//
+// Two strategies share one device state:
+//
+//  * MRS_EPOCH_EDGE_ATOMIC (default): one thread per undirected edge over a
+//    coalesced (i, j, next, per) edge stream.  An active edge computes the
+//    attractive (and densMAP) term once, applies it with opposite signs to both
+//    endpoints, draws `neg_rate` hashed negatives for each endpoint exactly as
+//    the reference's per-CSR-entry walk does (same (node, edge_local) keys), and
+//    scatters the per-endpoint sums with FP32 atomics.  The smaller endpoint's
+//    contributions are first summed across the wavefront (edges are sorted by
+//    i, so runs of equal i are contiguous), cutting its atomics by the run
+//    length.  The schedule advance is fused (the edge thread owns next[e]).
+//    A node kernel then applies Adam to flagged nodes and clears grad/flags.
+//
+//  * MRS_EPOCH_NODE_EXACT: the reference's node-parallel CSR walk
+//    (umap_grad_accum, src/training/umap_optimiser_gpu.rs:682-860), summing in
+//    the reference's order; deterministic bit-for-bit run to run.  Followed by
+//    the element-wise Adam kernel (:887-923) and the schedule kernel (:940-955).
+//
+// densMAP (both strategies): umap_dens_radii (:487-544) as a node kernel, then
+// the correlation statistics of src/utils/density.rs:464-508 reduced in FP64 on
+// the device with a fixed grid (deterministic), then umap_dens_weights
+// (:568-586) fused with the final reduction.  No host readback per epoch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "umap_state.hpp"
+
+namespace mrs {
+
+namespace {
+
+constexpr int kWg = 256;
+
+// ---------------------------------------------------------------- helpers
+
+template <int DIM>
+struct Dim {
+    static constexpr int value = DIM;
+};
+
+__device__ inline float clampf(float g, float c) {
+    if (g > c) g = c;
+    if (g < -c) g = -c;
+    return g;
+}
+
+// ---------------------------------------------------- node-exact gradient
+
+// umap_grad_accum, umap_optimiser_gpu.rs:682-860.  DIM == 0: run-time n_dim <= 16.
+template <int DIM, bool DENS>
+__global__ __launch_bounds__(kWg) void k_grad_accum_node(
+    const float* __restrict__ embd, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ eidx, const uint32_t* __restrict__ other_node,
+    const float* __restrict__ next, float* __restrict__ grad, uint32_t* __restrict__ has_update,
+    uint32_t n, int dim_rt, uint32_t epoch, float epoch_f, uint32_t seed, FastMod fm, float a,
+    float b, float two_a_b, float two_gamma_b, int neg_rate, const float* __restrict__ mu,
+    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
+    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    constexpr int MAXD = DIM > 0 ? DIM : 16;
+    const int nd = DIM > 0 ? DIM : dim_rt;
+    uint32_t node = blockIdx.x * kWg + threadIdx.x;
+    if (node >= n) return;
+    const float two = 2.0f, one = 1.0f;
+    float ys[MAXD], g[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; d++) {
+        ys[d] = d < nd ? embd[(size_t)node * nd + d] : 0.0f;
+        g[d] = 0.0f;
+    }
+    float coeff = 0.0f, re_self = 0.0f, phi_self = 0.0f, w_self = 0.0f;
+    if (DENS) {
+        coeff = dens_scalars[3];
+        re_self = dens_re[node];
+        phi_self = dens_phi[node];
+        w_self = dens_w[node];
+    }
+    uint32_t active = 0, edge_local = 0;
+    const uint32_t start = off[node], end = off[node + 1];
+    for (uint32_t pos = start; pos < end; pos++, edge_local++) {
+        const uint32_t e = eidx[pos];
+        if (next[e] > epoch_f) continue;
+        active = 1;
+        const uint32_t other = other_node[pos];
+        float yo[MAXD];
+        float dist_sq = 0.0f;
+#pragma unroll
+        for (int d = 0; d < MAXD; d++) {
+            yo[d] = d < nd ? embd[(size_t)other * nd + d] : 0.0f;
+            if (d < nd) {
+                float diff = ys[d] - yo[d];
+                dist_sq += diff * diff;
+            }
+        }
+        if (dist_sq >= kGradDistSqThreshold) {
+            float dsb = powf(dist_sq, b);
+            float denom = one + a * dsb;
+            float gc = two_a_b * dsb / (dist_sq * denom);
+#pragma unroll
+            for (int d = 0; d < MAXD; d++)
+                if (d < nd) g[d] += two * gc * (yo[d] - ys[d]);
+            if (DENS) {
+                float phi = one / denom;
+                float dphi_term = a * b * dsb / dist_sq * phi;
+                float common = one - b * (one - phi);
+                float dr_self = (phi / phi_self) * (common / expf(re_self) + dphi_term);
+                float dr_other =
+                    (phi / dens_phi[other]) * (common / expf(dens_re[other]) + dphi_term);
+                float cor = coeff * (w_self * dr_self + dens_w[other] * dr_other) / mu[e];
+                float two_cor = two * cor;
+#pragma unroll
+                for (int d = 0; d < MAXD; d++)
+                    if (d < nd) g[d] += two * clampf(two_cor * (ys[d] - yo[d]), kGradClipVal);
+            }
+        }
+        for (int s = 0; s < neg_rate; s++) {
+            uint32_t k = hash_neg(seed, node, epoch, edge_local, (uint32_t)s, fm);
+            if (k == node) continue;
+            float yk[MAXD];
+            float dsk = 0.0f;
+#pragma unroll
+            for (int d = 0; d < MAXD; d++) {
+                yk[d] = d < nd ? embd[(size_t)k * nd + d] : 0.0f;
+                if (d < nd) {
+                    float diff = ys[d] - yk[d];
+                    dsk += diff * diff;
+                }
+            }
+            float safe = dsk + kGradRepEps;
+            float dsb = powf(safe, b);
+            float denom = safe * (one + a * dsb);
+            float gc = clampf(two_gamma_b / denom, kGradClipVal);
+#pragma unroll
+            for (int d = 0; d < MAXD; d++)
+                if (d < nd) g[d] += gc * (ys[d] - yk[d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < MAXD; d++)
+        if (d < nd) grad[(size_t)node * nd + d] = g[d];
+    has_update[node] = active;
+}
+
+// umap_adam_update, umap_optimiser_gpu.rs:887-923 (one thread per element)
+__global__ __launch_bounds__(kWg) void k_adam_elem(const float* __restrict__ grad,
+                                                   const uint32_t* __restrict__ has_update,
+                                                   float* __restrict__ m, float* __restrict__ v,
+                                                   float* __restrict__ embd, uint64_t n_total,
+                                                   int dim, float fused_lr, float epsc,
+                                                   float omb1, float omb2) {
+    uint64_t i = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (i >= n_total) return;
+    if (has_update[i / dim] == 0) return;
+    float g = grad[i];
+    float mo = m[i];
+    float mn = mo + omb1 * (g - mo);
+    m[i] = mn;
+    float vo = v[i];
+    float vn = vo + omb2 * (g * g - vo);
+    v[i] = vn;
+    embd[i] += fused_lr * mn / (sqrtf(vn) + epsc);
+}
+
+// umap_edge_schedule_update, umap_optimiser_gpu.rs:940-955
+__global__ __launch_bounds__(kWg) void k_schedule(const float* __restrict__ per,
+                                                  float* __restrict__ next, uint64_t n_edges,
+                                                  float epoch_f) {
+    uint64_t e = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (e >= n_edges) return;
+    float nx = next[e];
+    if (nx <= epoch_f) next[e] = nx + per[e];
+}
+
+// umap_grad_norm_sq, umap_optimiser_gpu.rs:976-997
+__global__ __launch_bounds__(kWg) void k_grad_norm_sq(const float* __restrict__ grad,
+                                                      float* __restrict__ partial, uint32_t n,
+                                                      int dim) {
+    uint32_t node = blockIdx.x * kWg + threadIdx.x;
+    if (node >= n) return;
+    float acc = 0.0f;
+    for (int d = 0; d < dim; d++) {
+        float g = grad[(size_t)node * dim + d];
+        acc += g * g;
+    }
+    partial[node] = acc;
+}
+
+// ----------------------------------------------------------------- densMAP
+
+// umap_dens_radii, umap_optimiser_gpu.rs:487-544
+template <int DIM>
+__global__ __launch_bounds__(kWg) void k_dens_radii(const float* __restrict__ embd,
+                                                    const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ other_node,
+                                                    float* __restrict__ re,
+                                                    float* __restrict__ phi_sum, uint32_t n,
+                                                    int dim_rt, float a, float b,
+                                                    float dens_eps, float tiny) {
+    constexpr int MAXD = DIM > 0 ? DIM : 16;
+    const int nd = DIM > 0 ? DIM : dim_rt;
+    uint32_t node = blockIdx.x * kWg + threadIdx.x;
+    if (node >= n) return;
+    float ys[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; d++) ys[d] = d < nd ? embd[(size_t)node * nd + d] : 0.0f;
+    float sum_sq = 0.0f, sum_phi = 0.0f;
+    for (uint32_t pos = off[node]; pos < off[node + 1]; pos++) {
+        uint32_t o = other_node[pos];
+        float dist_sq = 0.0f;
+#pragma unroll
+        for (int d = 0; d < MAXD; d++)
+            if (d < nd) {
+                float diff = ys[d] - embd[(size_t)o * nd + d];
+                dist_sq += diff * diff;
+            }
+        float dsb = powf(dist_sq, b);
+        float phi = 1.0f / (1.0f + a * dsb);
+        sum_sq += phi * dist_sq;
+        sum_phi += phi;
+    }
+    float denom = sum_phi < tiny ? tiny : sum_phi;
+    re[node] = logf(dens_eps + sum_sq / denom);
+    phi_sum[node] = sum_phi;
+}
+
+__device__ inline double block_sum_f64(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kWg / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Pass 1 of correlation_stats (src/utils/density.rs:471-481): per-block FP64
+// partial sums of re over a fixed grid (so the order, and the result, is
+// identical run to run).
+__global__ __launch_bounds__(kWg) void k_dens_sum(const float* __restrict__ re, uint32_t n,
+                                                  double* __restrict__ partials) {
+    __shared__ double sh[kWg];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += kDensBlocks * kWg)
+        acc += (double)re[i];
+    double s = block_sum_f64(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Pass 2 (:483-500): every block derives the same mean from the pass-1
+// partials, then its centred partials of sum (re-mean)^2 and sum (re-mean)*r.
+__global__ __launch_bounds__(kWg) void k_dens_centred(const float* __restrict__ re,
+                                                      const float* __restrict__ r, uint32_t n,
+                                                      double* __restrict__ partials) {
+    __shared__ double sh[kWg];
+    __shared__ double mean_sh;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kDensBlocks; q++) t += partials[q];
+        mean_sh = t / (double)(n > 1 ? n : 1);
+    }
+    __syncthreads();
+    const double mean = mean_sh;
+    double ss = 0.0, cross = 0.0;
+    for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += kDensBlocks * kWg) {
+        double d = (double)re[i] - mean;
+        ss += d * d;
+        cross += d * (double)r[i];
+    }
+    double s1 = block_sum_f64(ss, sh);
+    double s2 = block_sum_f64(cross, sh);
+    if (threadIdx.x == 0) {
+        partials[kDensBlocks + blockIdx.x] = s1;
+        partials[2 * kDensBlocks + blockIdx.x] = s2;
+    }
+}
+
+// Final reduction + umap_dens_weights (umap_optimiser_gpu.rs:568-586) + the
+// dens_coeff of launch_dens_update (:1141-1142).  The f64 statistics are cast
+// to f32 before use, as the reference's correlation_stats::<f32> returns them.
+__global__ __launch_bounds__(kWg) void k_dens_weights(
+    const float* __restrict__ re, const float* __restrict__ r, uint32_t n,
+    const double* __restrict__ partials, float* __restrict__ weight,
+    float* __restrict__ scalars, float var_shift, float lambda, float mu_tot) {
+    __shared__ float st_sh[4];
+    if (threadIdx.x == 0) {
+        double t = 0.0, ss = 0.0, cross = 0.0;
+        for (int q = 0; q < kDensBlocks; q++) t += partials[q];
+        for (int q = 0; q < kDensBlocks; q++) ss += partials[kDensBlocks + q];
+        for (int q = 0; q < kDensBlocks; q++) cross += partials[2 * kDensBlocks + q];
+        double mean = t / (double)(n > 1 ? n : 1);
+        double denom = (double)(n > 2 ? n - 1 : 1);
+        double var = ss / denom, cov = cross / denom;
+        double sd = sqrt(var + (double)var_shift);
+        float mean_f = (float)mean, sd_f = (float)sd, cov_f = (float)cov;
+        float denom_f = (float)(n > 2 ? n - 1 : 1);
+        st_sh[0] = mean_f;
+        st_sh[1] = sd_f;
+        st_sh[2] = cov_f;
+        st_sh[3] = lambda * mu_tot / (sd_f * denom_f);
+        if (blockIdx.x == 0) {
+            for (int q = 0; q < 4; q++) scalars[q] = st_sh[q];
+        }
+    }
+    __syncthreads();
+    const float mean_f = st_sh[0], sd_f = st_sh[1], cov_f = st_sh[2];
+    const float inv_var = 1.0f / (sd_f * sd_f);
+    for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += kDensBlocks * kWg)
+        weight[i] = r[i] - cov_f * (re[i] - mean_f) * inv_var;
+}
+
+// ------------------------------------------------------ edge-parallel epoch
+
+// Segmented sum over lanes holding equal keys (runs are contiguous): afterwards
+// the first lane of each run holds the run's total.
+template <int D>
+__device__ inline void seg_reduce(uint32_t key, float (&v)[D]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int offd = 1; offd < 64; offd <<= 1) {
+        uint32_t ko = __shfl_down(key, offd, 64);
+        bool take = (lane + offd < 64) && ko == key;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            float vo = __shfl_down(v[d], offd, 64);
+            if (take) v[d] += vo;
+        }
+    }
+}
+
+// One thread per undirected edge (i < j).  DIM is the compile-time embedding
+// dimension (1..8).  Reads next/per coalesced; active edges gather y_i, y_j
+// and 2*neg_rate negatives; the edge's schedule cursor is advanced in place.
+template <int DIM, bool DENS>
+__global__ __launch_bounds__(kWg) void k_epoch_edges(
+    const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
+    const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
+    float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
+    float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
+    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
+    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
+    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    const uint32_t e = blockIdx.x * kWg + threadIdx.x;
+    const float two = 2.0f, one = 1.0f;
+    bool active = false;
+    if (e < n_edges) {
+        float nx = next[e];
+        if (nx <= epoch_f) {
+            active = true;
+            next[e] = nx + per[e];
+        }
+    }
+    uint32_t key = 0xFFFFFFFFu;
+    float gi[DIM], gj[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) gi[d] = gj[d] = 0.0f;
+    uint32_t j = 0;
+    if (active) {
+        const uint2 ij = reinterpret_cast<const uint2*>(edge_ij)[e];
+        const uint32_t i = ij.x;
+        j = ij.y;
+        key = i;
+        const uint32_t lc = edge_loc[e];
+        const uint32_t li = lc & 0xFFFFu, lj = lc >> 16;
+        float yi[DIM], yj[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            yi[d] = embd[(size_t)i * DIM + d];
+            yj[d] = embd[(size_t)j * DIM + d];
+        }
+        float dist_sq = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            float diff = yi[d] - yj[d];
+            dist_sq += diff * diff;
+        }
+        if (dist_sq >= kGradDistSqThreshold) {
+            float dsb = powf(dist_sq, b);
+            float denom = one + a * dsb;
+            float gc = two_a_b * dsb / (dist_sq * denom);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                float t = two * gc * (yj[d] - yi[d]);
+                gi[d] += t;
+                gj[d] -= t;
+            }
+            if (DENS) {
+                float phi = one / denom;
+                float dphi_term = a * b * dsb / dist_sq * phi;
+                float common = one - b * (one - phi);
+                float dr_i = (phi / dens_phi[i]) * (common / expf(dens_re[i]) + dphi_term);
+                float dr_j = (phi / dens_phi[j]) * (common / expf(dens_re[j]) + dphi_term);
+                float cor = dens_scalars[3] * (dens_w[i] * dr_i + dens_w[j] * dr_j) / mu[e];
+                float two_cor = two * cor;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    float t = two * clampf(two_cor * (yi[d] - yj[d]), kGradClipVal);
+                    gi[d] += t;
+                    gj[d] -= t;
+                }
+            }
+        }
+        // negatives of both endpoints: gather all rows first, then consume
+        for (int s = 0; s < neg_rate; s++) {
+            uint32_t ki = hash_neg(seed, i, epoch, li, (uint32_t)s, fm);
+            uint32_t kj = hash_neg(seed, j, epoch, lj, (uint32_t)s, fm);
+            float yki[DIM], ykj[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                yki[d] = embd[(size_t)ki * DIM + d];
+                ykj[d] = embd[(size_t)kj * DIM + d];
+            }
+            if (ki != i) {
+                float dsk = 0.0f;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    float diff = yi[d] - yki[d];
+                    dsk += diff * diff;
+                }
+                float safe = dsk + kGradRepEps;
+                float gc = clampf(two_gamma_b / (safe * (one + a * powf(safe, b))), kGradClipVal);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gi[d] += gc * (yi[d] - yki[d]);
+            }
+            if (kj != j) {
+                float dsk = 0.0f;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    float diff = yj[d] - ykj[d];
+                    dsk += diff * diff;
+                }
+                float safe = dsk + kGradRepEps;
+                float gc = clampf(two_gamma_b / (safe * (one + a * powf(safe, b))), kGradClipVal);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gj[d] += gc * (yj[d] - ykj[d]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)j * DIM + d], gj[d]);
+        flag[j] = 1u;
+    }
+    // smaller endpoint: combine runs of equal i within the wavefront first
+    seg_reduce<DIM>(key, gi);
+    const uint32_t kprev = __shfl_up(key, 1, 64);
+    const bool head = ((threadIdx.x & 63) == 0) || kprev != key;
+    if (head && key != 0xFFFFFFFFu) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)key * DIM + d], gi[d]);
+        flag[key] = 1u;
+    }
+}
+
+// Adam step for flagged nodes, then clear grad and flag for the next epoch.
+template <int DIM>
+__global__ __launch_bounds__(kWg) void k_adam_clear(float* __restrict__ grad,
+                                                    uint32_t* __restrict__ flag,
+                                                    float* __restrict__ m, float* __restrict__ v,
+                                                    float* __restrict__ embd, uint32_t n,
+                                                    float fused_lr, float epsc, float omb1,
+                                                    float omb2, float* __restrict__ norm_partial) {
+    const uint32_t node = blockIdx.x * kWg + threadIdx.x;
+    if (node >= n) return;
+    if (flag[node] == 0u) {
+        if (norm_partial) norm_partial[node] = 0.0f;
+        return;
+    }
+    float nrm = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const size_t q = (size_t)node * DIM + d;
+        float g = grad[q];
+        nrm += g * g;
+        float mo = m[q];
+        float mn = mo + omb1 * (g - mo);
+        m[q] = mn;
+        float vo = v[q];
+        float vn = vo + omb2 * (g * g - vo);
+        v[q] = vn;
+        embd[q] += fused_lr * mn / (sqrtf(vn) + epsc);
+        grad[q] = 0.0f;
+    }
+    flag[node] = 0u;
+    if (norm_partial) norm_partial[node] = nrm;
+}
+
+template <class F>
+void dispatch_dim(int dim, F&& f) {
+    switch (dim) {
+        case 1: f(Dim<1>{}); break;
+        case 2: f(Dim<2>{}); break;
+        case 3: f(Dim<3>{}); break;
+        case 4: f(Dim<4>{}); break;
+        default: f(Dim<0>{}); break;
+    }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- host side
+
+AdamScalars adam_scalars(const mrs_umap_optim_params& p, int32_t epoch) {
+    const float one = 1.0f;
+    float lr_alpha = p.lr * (one - (float)epoch / (float)p.n_epochs);
+    float t = (float)(epoch + 1);
+    float beta1t = powf(p.beta1, t);
+    float beta2t = powf(p.beta2, t);
+    float sqrt_b2t1 = sqrtf(one - beta2t);
+    float ad_scale = sqrt_b2t1 / (one - beta1t);
+    AdamScalars s;
+    s.epsc = sqrt_b2t1 * p.eps;
+    s.fused_lr = lr_alpha * ad_scale;
+    return s;
+}
+
+bool dens_is_active(const mrs_dens_params& dp, int32_t epoch, int32_t n_epochs) {
+    if (dp.lambda <= 0.0f || n_epochs == 0) return false;
+    float progress = (float)(epoch + 1) / (float)n_epochs;
+    return progress > 1.0f - dp.frac;
+}
+
+void launch_grad_accum_node(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
+    const mrs_umap_optim_params& p = st->params;
+    const float two = 2.0f;
+    const uint32_t n = (uint32_t)st->n;
+    FastMod fm = make_fastmod(n);
+    hipStream_t s = st->ctx->stream;
+    dispatch_dim(st->dim, [&](auto dd) {
+        constexpr int D = decltype(dd)::value;
+        auto kern = dens ? k_grad_accum_node<D, true> : k_grad_accum_node<D, false>;
+        hipLaunchKernelGGL(kern, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p, st->off.p,
+                           st->csr_edge_idx.p, st->csr_other.p, st->next.p, st->grad.p,
+                           st->has_update.p, n, st->dim, (uint32_t)epoch, (float)epoch, seed, fm,
+                           p.a, p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,
+                           st->mu.p, st->dens_re.p, st->dens_phi.p, st->dens_w.p,
+                           st->dens_scalars.p);
+    });
+    MRS_CHECK_LAUNCH();
+    st->grad_clean = false;
+}
+
+void launch_adam_elem(mrs_umap_state* st, int32_t epoch) {
+    AdamScalars sc = adam_scalars(st->params, epoch);
+    uint64_t total = (uint64_t)st->n * st->dim;
+    hipLaunchKernelGGL(k_adam_elem, dim3(div_up(total, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->grad.p, st->has_update.p, st->m.p, st->v.p, st->embd.p, total,
+                       st->dim, sc.fused_lr, sc.epsc, 1.0f - st->params.beta1,
+                       1.0f - st->params.beta2);
+    MRS_CHECK_LAUNCH();
+}
+
+void launch_schedule(mrs_umap_state* st, int32_t epoch) {
+    if (st->n_edges == 0) return;
+    hipLaunchKernelGGL(k_schedule, dim3(div_up(st->n_edges, kWg)), dim3(kWg), 0,
+                       st->ctx->stream, st->per.p, st->next.p, (uint64_t)st->n_edges,
+                       (float)epoch);
+    MRS_CHECK_LAUNCH();
+}
+
+void launch_dens_update_dev(mrs_umap_state* st) {
+    if (!st->dens_enabled) throw Error(MRS_INVALID_ARGUMENT, "state has no densMAP buffers");
+    const uint32_t n = (uint32_t)st->n;
+    hipStream_t s = st->ctx->stream;
+    dispatch_dim(st->dim, [&](auto dd) {
+        constexpr int D = decltype(dd)::value;
+        hipLaunchKernelGGL(k_dens_radii<D>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
+                           st->off.p, st->csr_other.p, st->dens_re.p, st->dens_phi.p, n, st->dim,
+                           st->params.a, st->params.b, (float)kDensEps, FLT_EPSILON);
+    });
+    MRS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dens_sum, dim3(kDensBlocks), dim3(kWg), 0, s, st->dens_re.p, n,
+                       st->dens_partials.p);
+    MRS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dens_centred, dim3(kDensBlocks), dim3(kWg), 0, s, st->dens_re.p,
+                       st->dens_r.p, n, st->dens_partials.p);
+    MRS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dens_weights, dim3(kDensBlocks), dim3(kWg), 0, s, st->dens_re.p,
+                       st->dens_r.p, n, st->dens_partials.p, st->dens_w.p, st->dens_scalars.p,
+                       st->dens_params.var_shift, st->dens_params.lambda, st->mu_tot);
+    MRS_CHECK_LAUNCH();
+}
+
+void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
+    if (!st->loc_packed)
+        throw Error(MRS_INVALID_ARGUMENT,
+                    "a node degree exceeds 65535; use the node-exact epoch strategy");
+    if (!st->grad_clean) {
+        MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), st->ctx->stream));
+        MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), st->ctx->stream));
+        st->grad_clean = true;
+    }
+    const mrs_umap_optim_params& p = st->params;
+    const float two = 2.0f;
+    FastMod fm = make_fastmod((uint32_t)st->n);
+    const uint32_t ne = (uint32_t)st->n_edges;
+    hipStream_t s = st->ctx->stream;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(div_up(ne, kWg)), dim3(kWg), 0, s, st->edge_ij.p,
+                           st->edge_loc.p, st->per.p, st->next.p, st->embd.p, st->grad.p,
+                           st->has_update.p, ne, (uint32_t)epoch, (float)epoch, seed, fm, p.a,
+                           p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,
+                           st->mu.p, st->dens_re.p, st->dens_phi.p, st->dens_w.p,
+                           st->dens_scalars.p);
+    };
+    switch (st->dim) {
+        case 1: dens ? go(k_epoch_edges<1, true>) : go(k_epoch_edges<1, false>); break;
+        case 2: dens ? go(k_epoch_edges<2, true>) : go(k_epoch_edges<2, false>); break;
+        case 3: dens ? go(k_epoch_edges<3, true>) : go(k_epoch_edges<3, false>); break;
+        case 4: dens ? go(k_epoch_edges<4, true>) : go(k_epoch_edges<4, false>); break;
+        case 8: dens ? go(k_epoch_edges<8, true>) : go(k_epoch_edges<8, false>); break;
+        default:
+            throw Error(MRS_INVALID_ARGUMENT,
+                        "edge-parallel epoch supports n_dim in {1,2,3,4,8}; use node-exact");
+    }
+    MRS_CHECK_LAUNCH();
+}
+
+void launch_adam_clear(mrs_umap_state* st, int32_t epoch, float* norm_partial) {
+    AdamScalars sc = adam_scalars(st->params, epoch);
+    const uint32_t n = (uint32_t)st->n;
+    const float omb1 = 1.0f - st->params.beta1, omb2 = 1.0f - st->params.beta2;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(div_up(n, kWg)), dim3(kWg), 0, st->ctx->stream,
+                           st->grad.p, st->has_update.p, st->m.p, st->v.p, st->embd.p, n,
+                           sc.fused_lr, sc.epsc, omb1, omb2, norm_partial);
+    };
+    switch (st->dim) {
+        case 1: go(k_adam_clear<1>); break;
+        case 2: go(k_adam_clear<2>); break;
+        case 3: go(k_adam_clear<3>); break;
+        case 4: go(k_adam_clear<4>); break;
+        case 8: go(k_adam_clear<8>); break;
+        default: throw Error(MRS_INVALID_ARGUMENT, "unsupported n_dim for adam_clear");
+    }
+    MRS_CHECK_LAUNCH();
+}
+
+namespace {
+
+// RAII bracket that records a start/stop event pair around one launch when the
+// state is profiling.
+struct ProfScope {
+    mrs_umap_state* st;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mrs_umap_state* s, int k) : st(s), kind(k) {
+        if (!st->profiling) return;
+        MRS_HIP(hipEventCreate(&a));
+        MRS_HIP(hipEventCreate(&b));
+        MRS_HIP(hipEventRecord(a, st->ctx->stream));
+    }
+    ~ProfScope() {
+        if (!st->profiling) return;
+        (void)hipEventRecord(b, st->ctx->stream);
+        st->prof_events.push_back({kind, {a, b}});
+    }
+};
+
+}  // namespace
+
+void prof_collect(mrs_umap_state* st) {
+    for (auto& ev : st->prof_events) {
+        float ms = 0.0f;
+        MRS_HIP(hipEventSynchronize(ev.second.second));
+        MRS_HIP(hipEventElapsedTime(&ms, ev.second.first, ev.second.second));
+        st->prof_ms[ev.first] += ms;
+        st->prof_count[ev.first] += 1;
+        (void)hipEventDestroy(ev.second.first);
+        (void)hipEventDestroy(ev.second.second);
+    }
+    st->prof_events.clear();
+}
+
+void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed, int strategy) {
+    const int32_t N = st->params.n_epochs;
+    if (strategy != MRS_EPOCH_EDGE_ATOMIC && strategy != MRS_EPOCH_NODE_EXACT)
+        throw Error(MRS_INVALID_ARGUMENT, "unknown epoch strategy");
+    if (strategy == MRS_EPOCH_EDGE_ATOMIC && !st->loc_packed) strategy = MRS_EPOCH_NODE_EXACT;
+    if (strategy == MRS_EPOCH_EDGE_ATOMIC) {
+        int d = st->dim;
+        if (!(d == 1 || d == 2 || d == 3 || d == 4 || d == 8)) strategy = MRS_EPOCH_NODE_EXACT;
+    }
+    for (int32_t e = first; e < first + count; e++) {
+        bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
+        if (dens) {
+            ProfScope ps(st, 5);
+            launch_dens_update_dev(st);
+        }
+        if (strategy == MRS_EPOCH_NODE_EXACT) {
+            { ProfScope ps(st, 2); launch_grad_accum_node(st, e, seed, dens); }
+            { ProfScope ps(st, 3); launch_adam_elem(st, e); }
+            { ProfScope ps(st, 4); launch_schedule(st, e); }
+        } else {
+            { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens); }
+            { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
+        }
+    }
+}
+
+// ------------------------------------------------------------- state build
+
+HostCsr host_csr_from_graph(int64_t n, const int64_t* adj_off, const int32_t* adj_cols,
+                            const float* adj_vals) {
+    if (n == 0) throw Error(MRS_NO_DATA, "UMAP: empty data (NoData)");
+    HostCsr c;
+    c.n = n;
+    std::vector<uint32_t> degree((size_t)n, 0);
+    int64_t ne = 0;
+    float max_w = 0.0f;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = adj_off[i]; p < adj_off[i + 1]; p++) {
+            int64_t j = adj_cols[p];
+            if (j < 0 || j >= n) throw Error(MRS_INVALID_ARGUMENT, "graph column out of range");
+            if (i < j) {
+                degree[i]++;
+                degree[j]++;
+                if (adj_vals[p] > max_w) max_w = adj_vals[p];
+                ne++;
+            }
+        }
+    if (ne == 0) throw Error(MRS_NO_GRAPH_EDGES, "UMAP: no edges to optimise (NoGraphEdges)");
+    if (2 * ne > (int64_t)UINT32_MAX) throw Error(MRS_INVALID_ARGUMENT, "graph too large");
+    c.n_edges = ne;
+    c.off.assign((size_t)n + 1, 0);
+    for (int64_t i = 0; i < n; i++) c.off[i + 1] = c.off[i] + degree[i];
+    c.csr_edge_idx.resize((size_t)(2 * ne));
+    c.csr_other.resize((size_t)(2 * ne));
+    c.edge_ij.resize((size_t)(2 * ne));
+    c.edge_loc.resize((size_t)ne);
+    c.per.resize((size_t)ne);
+    c.mu.resize((size_t)ne);
+    std::vector<uint32_t> cursor(c.off.begin(), c.off.end() - 1);
+    int64_t e = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = adj_off[i]; p < adj_off[i + 1]; p++) {
+            int64_t j = adj_cols[p];
+            if (!(i < j)) continue;
+            uint32_t pi = cursor[i]++, pj = cursor[j]++;
+            c.csr_edge_idx[pi] = (uint32_t)e;
+            c.csr_other[pi] = (uint32_t)j;
+            c.csr_edge_idx[pj] = (uint32_t)e;
+            c.csr_other[pj] = (uint32_t)i;
+            uint32_t li = pi - c.off[i], lj = pj - c.off[j];
+            if (li > 0xFFFFu || lj > 0xFFFFu) c.loc_packed = false;
+            c.edge_loc[e] = (li & 0xFFFFu) | ((lj & 0xFFFFu) << 16);
+            c.edge_ij[2 * e] = (uint32_t)i;
+            c.edge_ij[2 * e + 1] = (uint32_t)j;
+            float w = adj_vals[p];
+            float norm = w / max_w;
+            c.per[e] = norm > 0.0f ? 1.0f / norm : 1e8f;
+            c.mu[e] = w;
+            e++;
+        }
+    return c;
+}
+
+void state_alloc_nodes(mrs_umap_state* st, const float* embd) {
+    hipStream_t s = st->ctx->stream;
+    const size_t nd = (size_t)st->n * st->dim;
+    st->embd.alloc(nd);
+    st->m.alloc(nd);
+    st->v.alloc(nd);
+    st->grad.alloc(nd);
+    st->has_update.alloc((size_t)st->n);
+    st->norm_partial.alloc((size_t)st->n);
+    MRS_HIP(hipMemcpyAsync(st->embd.p, embd, nd * sizeof(float), hipMemcpyDefault, s));
+    MRS_HIP(hipMemsetAsync(st->m.p, 0, st->m.bytes(), s));
+    MRS_HIP(hipMemsetAsync(st->v.p, 0, st->v.bytes(), s));
+    MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), s));
+    MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), s));
+    st->grad_clean = true;
+    st->dens_scalars.alloc(4);
+    MRS_HIP(hipMemsetAsync(st->dens_scalars.p, 0, st->dens_scalars.bytes(), s));
+    if (!st->dens_enabled) {
+        // length-1 dummies keep every kernel argument valid on the plain path
+        st->dens_r.alloc(1);
+        st->dens_re.alloc(1);
+        st->dens_phi.alloc(1);
+        st->dens_w.alloc(1);
+    }
+}
+
+void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float* r,
+                      float mu_tot) {
+    hipStream_t s = st->ctx->stream;
+    const size_t n = (size_t)st->n;
+    st->dens_enabled = true;
+    st->dens_params = dp;
+    st->mu_tot = mu_tot;
+    st->dens_r.alloc(n);
+    st->dens_re.alloc(n);
+    st->dens_phi.alloc(n);
+    st->dens_w.alloc(n);
+    st->dens_partials.alloc(3 * kDensBlocks);
+    MRS_HIP(hipMemcpyAsync(st->dens_r.p, r, n * sizeof(float), hipMemcpyDefault, s));
+    MRS_HIP(hipMemsetAsync(st->dens_re.p, 0, st->dens_re.bytes(), s));
+    MRS_HIP(hipMemsetAsync(st->dens_phi.p, 0, st->dens_phi.bytes(), s));
+    MRS_HIP(hipMemsetAsync(st->dens_w.p, 0, st->dens_w.bytes(), s));
+}
+
+}  // namespace mrs
+
+mrs_umap_state::~mrs_umap_state() {
+    for (auto& ev : prof_events) {
+        (void)hipEventDestroy(ev.second.first);
+        (void)hipEventDestroy(ev.second.second);
+    }
+}
+
+namespace mrs {
+
+mrs_umap_state* make_state(mrs_ctx* ctx, const float* embd, int64_t n, int32_t dim,
+                           const HostCsr& c, const mrs_umap_optim_params& p,
+                           const mrs_dens_state* dens) {
+    auto* st = new mrs_umap_state();
+    try {
+        st->ctx = ctx;
+        st->n = n;
+        st->dim = dim;
+        st->n_edges = c.n_edges;
+        st->params = p;
+        st->loc_packed = c.loc_packed;
+        hipStream_t s = ctx->stream;
+        auto up = [&](auto& buf, const auto& vec) {
+            buf.alloc(vec.size());
+            MRS_HIP(hipMemcpyAsync(buf.p, vec.data(), buf.bytes(), hipMemcpyHostToDevice, s));
+        };
+        up(st->off, c.off);
+        up(st->csr_edge_idx, c.csr_edge_idx);
+        up(st->csr_other, c.csr_other);
+        up(st->edge_ij, c.edge_ij);
+        up(st->edge_loc, c.edge_loc);
+        up(st->per, c.per);
+        up(st->next, c.per);  // cursors start one period from epoch 0
+        up(st->mu, c.mu);
+        if (dens) state_alloc_dens(st, dens->params, dens->r, dens->mu_tot);
+        state_alloc_nodes(st, embd);
+        MRS_HIP(hipStreamSynchronize(s));  // host vectors die on return
+    } catch (...) {
+        delete st;
+        throw;
+    }
+    return st;
+}
+
+}  // namespace mrs

@@ -1,0 +1,137 @@
+// Internal helpers shared by the HIP translation units of libmrs_amd.
+//
+// Status codes are 1:1 with the reference's ManifoldsError variants that this
+// path can raise (src/errors.rs:7-118): NoData, NoGraphEdges,
+// DegenerateLocalRadii; device and allocation failures (the reference's
+// CubeclUtilsError) map to MRS_DEVICE_ERROR; bad arguments to
+// MRS_INVALID_ARGUMENT.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mrs_amd.h"
+
+namespace mrs {
+
+// Error carrying an mrs_status; thrown inside the library, converted to a
+// status code at the C-ABI boundary.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define MRS_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess)                                                            \
+            throw ::mrs::Error(MRS_DEVICE_ERROR, std::string(#call " failed: ") +        \
+                                                     hipGetErrorString(_e) + " at " +    \
+                                                     __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+#define MRS_CHECK_LAUNCH() MRS_HIP(hipGetLastError())
+
+// Run `body` and translate exceptions into a status code.
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return MRS_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MRS_DEVICE_ERROR;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MRS_INVALID_ARGUMENT;
+    }
+}
+
+// Owning device allocation (no copy, movable).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) MRS_HIP(hipMalloc((void**)&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return sizeof(T) * n; }
+};
+
+// True if `ptr` is device (or managed) memory visible to the current device.
+bool is_device_ptr(const void* ptr);
+
+// A borrowed-or-staged view of caller memory: if the caller passed a device
+// pointer it is used in place, otherwise the data is staged through a device
+// buffer (uploaded on construction when `upload` is set).
+template <class T>
+struct DevView {
+    T* p = nullptr;
+    const void* host = nullptr;
+    size_t n = 0;
+    DevBuf<T> staged;
+    DevView(const T* caller, size_t count, hipStream_t s, bool upload = true) : n(count) {
+        if (!caller || count == 0) return;
+        if (is_device_ptr(caller)) {
+            p = const_cast<T*>(caller);
+        } else {
+            staged.alloc(count);
+            p = staged.p;
+            host = caller;
+            if (upload)
+                MRS_HIP(hipMemcpyAsync(p, caller, sizeof(T) * count, hipMemcpyHostToDevice, s));
+        }
+    }
+    bool is_staged() const { return host != nullptr; }
+    // copy device contents back to the caller's host buffer (no-op if in place)
+    void download(hipStream_t s) {
+        if (host && n)
+            MRS_HIP(hipMemcpyAsync(const_cast<void*>(host), p, sizeof(T) * n,
+                                   hipMemcpyDeviceToHost, s));
+    }
+};
+
+inline unsigned div_up(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Exclusive prefix sum of u32 flags/counts on the device (in place allowed).
+// Returns the total through `total_dev` (device scalar) if non-null.
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev,
+                        hipStream_t s);
+
+}  // namespace mrs
+
+// Context: one stream and device per handle; one handle per host thread.
+struct mrs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t hbm_budget = 0;
+    int verbose = 0;
+};

@@ -1,0 +1,93 @@
+// Device-resident optimiser state of the "adam_gpu" path: the MI355X
+// counterpart of UmapGpuState (src/training/umap_optimiser_gpu.rs:278-348).
+#pragma once
+
+#include "epoch_common.hpp"
+#include "mrs_internal.hpp"
+
+struct mrs_umap_state {
+    mrs_ctx* ctx = nullptr;
+    int64_t n = 0;
+    int32_t dim = 0;
+    int64_t n_edges = 0;
+    mrs_umap_optim_params params{};
+
+    // [n*dim] row-major
+    mrs::DevBuf<float> embd, m, v, grad;
+    // [n] per-node "any edge fired this epoch"
+    mrs::DevBuf<uint32_t> has_update;
+    // CSR: every undirected edge under both endpoints, in edge order
+    mrs::DevBuf<uint32_t> off;           // [n+1]
+    mrs::DevBuf<uint32_t> csr_edge_idx;  // [2E]
+    mrs::DevBuf<uint32_t> csr_other;     // [2E]
+    // edge stream for the edge-parallel kernel, edges sorted by (i, j), i < j
+    mrs::DevBuf<uint32_t> edge_ij;   // [2E] interleaved (i, j)
+    mrs::DevBuf<uint32_t> edge_loc;  // [E] local_i | local_j << 16 (CSR slot of the edge)
+    bool loc_packed = true;          // false if a degree >= 65536 (edge kernel unavailable)
+    mrs::DevBuf<float> per, next, mu;  // [E]
+
+    // grad/has_update are all-zero (ready for the accumulate-by-atomics kernel)
+    bool grad_clean = true;
+
+    // densMAP
+    bool dens_enabled = false;
+    mrs_dens_params dens_params{};
+    float mu_tot = 0.0f;
+    mrs::DevBuf<float> dens_r, dens_re, dens_phi, dens_w;  // [n] (or [1] when off)
+    mrs::DevBuf<double> dens_partials;                     // [3 * kDensBlocks]
+    mrs::DevBuf<float> dens_scalars;                       // mean, std, cov, coeff
+
+    mrs::DevBuf<float> norm_partial;  // [n]
+
+    // Optional per-kernel timing with HIP events on the context stream.
+    // kind: 0 edge epoch, 1 adam+clear, 2 node grad, 3 adam elem, 4 schedule, 5 densMAP update
+    bool profiling = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
+    double prof_ms[8] = {0};
+    int64_t prof_count[8] = {0};
+    ~mrs_umap_state();
+};
+
+namespace mrs {
+
+constexpr int kDensBlocks = 256;
+
+// Host CSR of an adjacency list (UmapCsrGraph::from_graph,
+// src/training/umap_optimiser_gpu.rs:132-209), plus the per-edge endpoint
+// arrays and local CSR slots used by the edge-parallel kernel.
+struct HostCsr {
+    int64_t n = 0, n_edges = 0;
+    std::vector<uint32_t> off, csr_edge_idx, csr_other, edge_ij, edge_loc;
+    std::vector<float> per, mu;
+    bool loc_packed = true;
+};
+// Throws Error(MRS_NO_DATA / MRS_NO_GRAPH_EDGES) like the reference.
+HostCsr host_csr_from_graph(int64_t n, const int64_t* adj_off, const int32_t* adj_cols,
+                            const float* adj_vals);
+
+// Allocate and upload a state from a host CSR.
+mrs_umap_state* make_state(mrs_ctx* ctx, const float* embd, int64_t n, int32_t dim,
+                           const HostCsr& csr, const mrs_umap_optim_params& p,
+                           const mrs_dens_state* dens);
+
+// Allocate the per-node buffers of a state whose graph arrays are already on
+// the device (filled by the caller).  embd is device or host memory [n*dim].
+void state_alloc_nodes(mrs_umap_state* st, const float* embd);
+void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float* r_dev_or_host,
+                      float mu_tot);
+
+AdamScalars adam_scalars(const mrs_umap_optim_params& p, int32_t epoch);
+bool dens_is_active(const mrs_dens_params& dp, int32_t epoch, int32_t n_epochs);
+
+// Per-epoch launchers (asynchronous on st->ctx->stream).
+void launch_grad_accum_node(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
+void launch_adam_elem(mrs_umap_state* st, int32_t epoch);
+void launch_schedule(mrs_umap_state* st, int32_t epoch);
+void launch_dens_update_dev(mrs_umap_state* st);  // radii + FP64 stats + weights, coeff on device
+void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
+void launch_adam_clear(mrs_umap_state* st, int32_t epoch, float* norm_partial);
+void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed, int strategy);
+// Wait for and accumulate the events recorded while profiling.
+void prof_collect(mrs_umap_state* st);
+
+}  // namespace mrs

@@ -1,0 +1,350 @@
+"""Python mirror of the manifolds-rs `adam_gpu` fitting path over the C ABI.
+
+Loads ``libmrs_amd.so`` (built in-tree next to this file) with ctypes.  The
+library is the product: there is no fallback, and every call fails loudly if
+the library or a HIP device is missing.  Names follow the reference crate
+(src/lib.rs, src/training/umap_optimiser_gpu.rs); buffers are numpy arrays
+(host) or torch tensors (device: used in place).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libmrs_amd.so"
+HEADER_PATH = _HERE.parent / "include" / "mrs_amd.h"
+
+MRS_OK = 0
+MRS_NO_DATA = 1
+MRS_NO_GRAPH_EDGES = 2
+MRS_DEGENERATE_LOCAL_RADII = 3
+MRS_DEVICE_ERROR = 4
+MRS_INVALID_ARGUMENT = 5
+
+EPOCH_EDGE_ATOMIC = 0
+EPOCH_NODE_EXACT = 1
+
+KNN_FP32 = 0
+KNN_BF16_RERANK = 1
+
+_STATUS_NAMES = {
+    1: "NoData",
+    2: "NoGraphEdges",
+    3: "DegenerateLocalRadii",
+    4: "DeviceError",
+    5: "InvalidArgument",
+}
+
+
+class ManifoldsError(RuntimeError):
+    """A non-zero status from the C ABI (ManifoldsError in the reference)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class UmapOptimParams(C.Structure):
+    """UmapOptimParams<f32> (src/training/umap_optimisers.rs:22-44)."""
+
+    _fields_ = [
+        ("a", C.c_float), ("b", C.c_float), ("lr", C.c_float), ("gamma", C.c_float),
+        ("n_epochs", C.c_int32), ("neg_sample_rate", C.c_int32),
+        ("min_dist", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+    ]
+
+    @classmethod
+    def default_2d(cls) -> "UmapOptimParams":
+        # UmapOptimParams::default_2d, src/training/umap_optimisers.rs:56-69
+        return cls(a=1.5, b=0.9, lr=1.0, gamma=1.0, n_epochs=500, neg_sample_rate=5,
+                   min_dist=0.1, beta1=0.5, beta2=0.9, eps=1e-7)
+
+
+class DensParams(C.Structure):
+    """DensParams<f32> (src/utils/density.rs:69-79); defaults :38-55."""
+
+    _fields_ = [("lambda_", C.c_float), ("frac", C.c_float), ("var_shift", C.c_float)]
+
+    @classmethod
+    def densmap_default(cls) -> "DensParams":
+        return cls(lambda_=2.0, frac=0.3, var_shift=0.1)
+
+
+class DensState(C.Structure):
+    """DensState<f32> (src/utils/density.rs:140-160)."""
+
+    _fields_ = [("params", DensParams), ("r", C.c_void_p), ("mu_sum", C.c_void_p),
+                ("mu_tot", C.c_float)]
+
+
+class UmapGraphParams(C.Structure):
+    """UmapGraphParams<f32> (src/data/graph.rs:30-61)."""
+
+    _fields_ = [("bandwidth", C.c_float), ("local_connectivity", C.c_float),
+                ("mix_weight", C.c_float)]
+
+    @classmethod
+    def default(cls) -> "UmapGraphParams":
+        return cls(bandwidth=1e-5, local_connectivity=1.0, mix_weight=1.0)
+
+
+_lib = None
+
+
+def _declared_functions() -> list[str]:
+    """Names of every function declared in include/mrs_amd.h."""
+    import re
+
+    text = HEADER_PATH.read_text()
+    return re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(mrs_\w+)\s*\(", text, re.M)
+
+
+def lib() -> C.CDLL:
+    """Load libmrs_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(str(LIB_PATH))
+    i32, i64, u32, u64, f32p = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_void_p
+    vp = C.c_void_p
+    sig = {
+        "mrs_last_error": (C.c_char_p, []),
+        "mrs_ctx_create": (C.c_int, [i32, C.c_size_t, C.POINTER(vp)]),
+        "mrs_ctx_destroy": (C.c_int, [vp]),
+        "mrs_ctx_synchronize": (C.c_int, [vp]),
+        "mrs_ctx_stream": (vp, [vp]),
+        "mrs_parse_umap_optimiser_gpu": (i32, [C.c_char_p]),
+        "mrs_parse_ann_search_gpu": (i32, [C.c_char_p]),
+        "mrs_hash_neg": (u32, [u32, u32, u32, u32, u32, u32]),
+        "mrs_umap_state_create": (C.c_int, [vp, vp, i64, i32, vp, vp, vp,
+                                            C.POINTER(UmapOptimParams), C.POINTER(DensState),
+                                            C.POINTER(vp)]),
+        "mrs_umap_state_destroy": (C.c_int, [vp]),
+        "mrs_umap_state_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i32), C.POINTER(i64)]),
+        "mrs_umap_state_read": (C.c_int, [vp, i32, vp]),
+        "mrs_umap_state_write": (C.c_int, [vp, i32, vp]),
+        "mrs_launch_grad_accum": (C.c_int, [vp, i32, u32, C.POINTER(C.c_float)]),
+        "mrs_launch_dens_update": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "mrs_launch_adam_update": (C.c_int, [vp, i32]),
+        "mrs_launch_edge_schedule_update": (C.c_int, [vp, i32]),
+        "mrs_launch_grad_norm": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "mrs_run_epochs": (C.c_int, [vp, i32, i32, u64, i32]),
+        "mrs_launch_epoch_edges": (C.c_int, [vp, i32, u32, C.POINTER(C.c_float)]),
+        "mrs_launch_adam_clear": (C.c_int, [vp, i32]),
+        "mrs_umap_state_set_profiling": (C.c_int, [vp, i32]),
+        "mrs_umap_state_reset": (C.c_int, [vp, vp]),
+        "mrs_schedule_active_counts": (C.c_int, [vp, i64, i32, vp]),
+        "mrs_umap_state_kernel_times": (C.c_int, [vp, C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_int64)]),
+        "mrs_optimise_embedding_adam_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, vp,
+                                                      C.POINTER(UmapOptimParams),
+                                                      C.POINTER(DensState), u64, i32, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != MRS_OK:
+        msg = lib().mrs_last_error().decode(errors="replace")
+        raise ManifoldsError(rc, msg)
+
+
+def _ptr(x) -> int | None:
+    """Address of a numpy array (host) or torch tensor (host or device)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return x.data_ptr()
+    raise TypeError(f"unsupported buffer type {type(x)}")
+
+
+class Context:
+    """mrs_ctx: one HIP stream on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        _check(lib().mrs_ctx_create(device, hbm_budget, C.byref(h)))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _check(lib().mrs_ctx_synchronize(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return lib().mrs_ctx_stream(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().mrs_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def adjacency_to_csr(graph: list[list[tuple[int, float]]]):
+    """Vec<Vec<(usize, T)>> adjacency list -> (offsets int64, cols int32, vals f32)."""
+    off = np.zeros(len(graph) + 1, dtype=np.int64)
+    for i, row in enumerate(graph):
+        off[i + 1] = off[i] + len(row)
+    cols = np.array([j for row in graph for j, _ in row], dtype=np.int32)
+    vals = np.array([w for row in graph for _, w in row], dtype=np.float32)
+    return off, cols, vals
+
+
+# state buffer ids of mrs_umap_state_read/write
+BUF_EMBD, BUF_M, BUF_V, BUF_GRAD, BUF_HAS_UPDATE = 0, 1, 2, 3, 4
+BUF_OFFSETS, BUF_EDGE_IDX, BUF_OTHER, BUF_PER, BUF_NEXT, BUF_MU = 5, 6, 7, 8, 9, 10
+BUF_DENS_RE, BUF_DENS_PHI, BUF_DENS_W, BUF_DENS_R, BUF_EDGE_IJ = 11, 12, 13, 14, 15
+
+
+class UmapGpuState:
+    """UmapGpuState (src/training/umap_optimiser_gpu.rs:278-452) on the device."""
+
+    def __init__(self, ctx: Context, embd: np.ndarray, adj_off, adj_cols, adj_vals,
+                 params: UmapOptimParams, dens: DensState | None = None):
+        embd = np.ascontiguousarray(embd, dtype=np.float32) if isinstance(embd, np.ndarray) else embd
+        n, dim = embd.shape
+        self.ctx = ctx
+        self._keep = (embd, adj_off, adj_cols, adj_vals, dens)
+        h = C.c_void_p()
+        _check(lib().mrs_umap_state_create(
+            ctx.handle, _ptr(embd), n, dim, _ptr(adj_off), _ptr(adj_cols), _ptr(adj_vals),
+            C.byref(params), C.byref(dens) if dens is not None else None, C.byref(h)))
+        self.handle = h
+        n_, d_, e_ = C.c_int64(), C.c_int32(), C.c_int64()
+        _check(lib().mrs_umap_state_info(h, C.byref(n_), C.byref(d_), C.byref(e_)))
+        self.n, self.n_dim, self.n_edges = n_.value, d_.value, e_.value
+
+    def _shape(self, which: int):
+        n, d, e = self.n, self.n_dim, self.n_edges
+        table = {0: (n * d, np.float32), 1: (n * d, np.float32), 2: (n * d, np.float32),
+                 3: (n * d, np.float32), 4: (n, np.uint32), 5: (n + 1, np.uint32),
+                 6: (2 * e, np.uint32), 7: (2 * e, np.uint32), 8: (e, np.float32),
+                 9: (e, np.float32), 10: (e, np.float32), 15: (2 * e, np.uint32)}
+        if which in (11, 12, 13, 14):
+            return (n, np.float32)
+        return table[which]
+
+    def read(self, which: int) -> np.ndarray:
+        size, dt = self._shape(which)
+        out = np.empty(size, dtype=dt)
+        _check(lib().mrs_umap_state_read(self.handle, which, _ptr(out)))
+        return out
+
+    def write(self, which: int, data: np.ndarray) -> None:
+        size, dt = self._shape(which)
+        arr = np.ascontiguousarray(data, dtype=dt).reshape(-1)
+        if arr.size != size:
+            raise ValueError(f"buffer {which} expects {size} elements, got {arr.size}")
+        _check(lib().mrs_umap_state_write(self.handle, which, _ptr(arr)))
+
+    def launch_grad_accum(self, epoch: int, seed: int, dens_coeff: float | None = None) -> None:
+        c = C.c_float(dens_coeff) if dens_coeff is not None else None
+        _check(lib().mrs_launch_grad_accum(self.handle, epoch, seed,
+                                           C.byref(c) if c is not None else None))
+
+    def launch_dens_update(self) -> float:
+        c = C.c_float()
+        _check(lib().mrs_launch_dens_update(self.handle, C.byref(c)))
+        return c.value
+
+    def launch_adam_update(self, epoch: int) -> None:
+        _check(lib().mrs_launch_adam_update(self.handle, epoch))
+
+    def launch_edge_schedule_update(self, epoch: int) -> None:
+        _check(lib().mrs_launch_edge_schedule_update(self.handle, epoch))
+
+    def launch_epoch_edges(self, epoch: int, seed: int, dens_coeff: float | None = None) -> None:
+        c = C.c_float(dens_coeff) if dens_coeff is not None else None
+        _check(lib().mrs_launch_epoch_edges(self.handle, epoch, seed,
+                                            C.byref(c) if c is not None else None))
+
+    def launch_adam_clear(self, epoch: int) -> None:
+        _check(lib().mrs_launch_adam_clear(self.handle, epoch))
+
+    def launch_grad_norm(self) -> float:
+        c = C.c_float()
+        _check(lib().mrs_launch_grad_norm(self.handle, C.byref(c)))
+        return c.value
+
+    def run_epochs(self, first: int, count: int, seed: int, strategy: int = EPOCH_EDGE_ATOMIC):
+        _check(lib().mrs_run_epochs(self.handle, first, count, seed, strategy))
+
+    def reset(self, embd) -> None:
+        """Back to the start of a fit with the given initial embedding."""
+        if isinstance(embd, np.ndarray):
+            embd = np.ascontiguousarray(embd, dtype=np.float32)
+        _check(lib().mrs_umap_state_reset(self.handle, _ptr(embd)))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().mrs_umap_state_set_profiling(self.handle, 1 if on else 0))
+
+    KERNEL_KINDS = ("epoch_edges", "adam_clear", "grad_accum_node", "adam_elem", "schedule",
+                    "dens_update")
+
+    def kernel_times(self) -> dict:
+        """{kind: (total_ms, launches)} of the launches recorded while profiling."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_int64 * 8)()
+        _check(lib().mrs_umap_state_kernel_times(self.handle, ms, cnt))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_KINDS) if cnt[i]}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().mrs_umap_state_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def optimise_embedding_adam_gpu(ctx: Context, embd: np.ndarray, adj_off, adj_cols, adj_vals,
+                                params: UmapOptimParams, dens: DensState | None = None,
+                                seed: int = 42, verbose: int = 0,
+                                strategy: int = EPOCH_EDGE_ATOMIC) -> np.ndarray:
+    """optimise_embedding_adam_gpu (umap_optimiser_gpu.rs:1302-1369); returns a new array."""
+    out = np.ascontiguousarray(embd, dtype=np.float32).copy()
+    n, dim = out.shape
+    _check(lib().mrs_optimise_embedding_adam_gpu(
+        ctx.handle, _ptr(out), n, dim, _ptr(adj_off), _ptr(adj_cols), _ptr(adj_vals),
+        C.byref(params), C.byref(dens) if dens is not None else None, seed, verbose, strategy))
+    return out
+
+
+def schedule_active_counts(epochs_per_sample, n_epochs: int) -> np.ndarray:
+    """Edges ticking per epoch (host replay of the f32 schedule); sum = edge-updates."""
+    per = np.ascontiguousarray(epochs_per_sample, dtype=np.float32)
+    out = np.zeros(n_epochs, np.int64)
+    _check(lib().mrs_schedule_active_counts(_ptr(per), per.size, n_epochs, _ptr(out)))
+    return out

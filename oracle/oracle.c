@@ -1,0 +1,974 @@
+/*
+ * oracle.c -- CPU restatement of the manifolds-rs UMAP/densMAP fitting path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline
+ * ("port").  Loaded by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg; never linked into or called by the product library.
+ *
+ * Built with -O2 -ffp-contract=off (no fused multiply-add unless written as
+ * fmaf), so float arithmetic follows the reference's f32 operation order.
+ *
+ * Pinning (what fixes each function independently of its own formula):
+ *   orc_hash_neg            pure u32 arithmetic; golden values in tests/golden
+ *   orc_csr_from_graph      reference golden triangle layout / periods / errors
+ *                           (umap_optimiser_gpu.rs:1716-1785)
+ *   orc_grad_accum          hand-computed (1.2,0,-1.2,0) case (:2060-2092), empty
+ *                           epoch = 0 (:1995-2003), finite differences of the UMAP
+ *                           loss for the attractive term, brute-force restatement
+ *                           in float64 for the repulsive term
+ *   orc_adam_update         hand-computed m=0.1, v=0.001, y=1 (:2211-2242),
+ *                           inactive node preserved (:2192-2209), torch.optim-free
+ *                           closed form of bias-corrected Adam
+ *   orc_schedule_update     golden schedules (:2342-2379)
+ *   orc_dens_radii          closed form on a 2-node graph
+ *   orc_correlation_stats   reference known values (density.rs:651-676)
+ *   orc_dens_state          reference known value (density.rs:592-649)
+ *   orc_dens_is_active      reference cases (density.rs:691-718)
+ *   orc_smooth_knn_dist     invariant |sum exp - ln k| < tol (or 64 iterations),
+ *                           rho = d[0] (graph.rs:1441-1471, tests/umap_tests.rs)
+ *   orc_knn_to_coo          closed form, self excluded (graph.rs:1473-1510)
+ *   orc_symmetrise          goldens 0.86/0.76/0.92 (graph.rs:1512-1563) and a
+ *                           dense-matrix restatement on random small graphs
+ *   orc_filter_weak_edges   closed-form threshold on random inputs
+ *   orc_knn_exact_*         scipy cdist + lexsort brute force on small inputs
+ *   orc_fit_params          bound/shape invariants (umap_optimisers.rs:1559-1593)
+ *   orc_optimise_adam_gpu   composition of the pinned pieces above
+ *   orc_optimise_adam_parallel, orc_optimise_sgd
+ *                           parity unpinned: the reference draws negatives from
+ *                           rand 0.9.5 SmallRng streams that cannot be run here;
+ *                           checked only through quality invariants
+ *                           (cluster separation, trustworthiness).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ hash */
+
+/* umap_optimiser_gpu.rs:611-624 */
+uint32_t orc_hash_neg(uint32_t seed, uint32_t node, uint32_t epoch, uint32_t edge_local,
+                      uint32_t neg, uint32_t n) {
+    uint32_t h = seed ^ (node * 0x9E3779B1u) ^ (epoch * 0x85EBCA77u) ^
+                 (edge_local * 0xC2B2AE3Du) ^ (neg * 0x27D4EB2Fu);
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h % n;
+}
+
+/* ------------------------------------------------------------------- CSR */
+
+/* UmapCsrGraph::from_graph, umap_optimiser_gpu.rs:132-209 */
+int orc_csr_from_graph(int64_t n, const int64_t *adj_off, const int32_t *adj_cols,
+                       const float *adj_vals, int64_t *n_edges_out,
+                       uint32_t *node_edge_offsets, uint32_t *csr_edge_idx,
+                       uint32_t *csr_other_node, float *epochs_per_sample, float *mu,
+                       int32_t *edge_i, int32_t *edge_j) {
+    if (n == 0) return 1;
+    int64_t ne = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = adj_off[i]; p < adj_off[i + 1]; p++)
+            if (i < adj_cols[p]) ne++;
+    *n_edges_out = ne;
+    if (ne == 0) return 2;
+    if (!node_edge_offsets) return 0;
+
+    uint32_t *degree = (uint32_t *)calloc((size_t)n, sizeof(uint32_t));
+    float max_w = 0.0f;
+    int64_t e = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = adj_off[i]; p < adj_off[i + 1]; p++) {
+            int64_t j = adj_cols[p];
+            if (i < j) {
+                degree[i]++;
+                degree[j]++;
+                float w = adj_vals[p];
+                if (w > max_w) max_w = w;
+                if (edge_i) edge_i[e] = (int32_t)i;
+                if (edge_j) edge_j[e] = (int32_t)j;
+                e++;
+            }
+        }
+    node_edge_offsets[0] = 0;
+    for (int64_t i = 0; i < n; i++) node_edge_offsets[i + 1] = node_edge_offsets[i] + degree[i];
+    uint32_t *cursor = degree; /* reuse */
+    for (int64_t i = 0; i < n; i++) cursor[i] = node_edge_offsets[i];
+    e = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = adj_off[i]; p < adj_off[i + 1]; p++) {
+            int64_t j = adj_cols[p];
+            if (i < j) {
+                uint32_t pi = cursor[i]++;
+                csr_edge_idx[pi] = (uint32_t)e;
+                csr_other_node[pi] = (uint32_t)j;
+                uint32_t pj = cursor[j]++;
+                csr_edge_idx[pj] = (uint32_t)e;
+                csr_other_node[pj] = (uint32_t)i;
+                float w = adj_vals[p];
+                float norm = w / max_w;
+                epochs_per_sample[e] = norm > 0.0f ? 1.0f / norm : 1e8f;
+                mu[e] = w;
+                e++;
+            }
+        }
+    free(degree);
+    return 0;
+}
+
+/* ------------------------------------------------------- gradient mirror */
+
+/* cpu_grad_accum, umap_optimiser_gpu.rs:1461-1579 */
+void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                    const uint32_t *csr_edge_idx, const uint32_t *csr_other,
+                    const float *next_sample, int32_t epoch, uint32_t seed, int32_t neg_rate,
+                    const orc_optim_params *p, const float *mu, const float *dens_re,
+                    const float *dens_phi_sum, const float *dens_weight, float dens_coeff,
+                    float *grad, uint32_t *has_update) {
+    const float two = 2.0f, one = 1.0f;
+    const float clip = ORC_GRAD_CLIP_VAL, rep_eps = ORC_GRAD_REP_EPS;
+    const float two_a_b = two * p->a * p->b;
+    const float two_gamma_b = two * p->gamma * p->b;
+    const float epoch_t = (float)epoch;
+    const int dens = dens_re != NULL;
+    memset(grad, 0, sizeof(float) * (size_t)n * (size_t)n_dim);
+    for (int64_t node = 0; node < n; node++) {
+        const float *ys = embd + node * n_dim;
+        float *g = grad + node * n_dim;
+        uint32_t active = 0, edge_local = 0;
+        for (uint32_t pos = off[node]; pos < off[node + 1]; pos++, edge_local++) {
+            uint32_t e = csr_edge_idx[pos];
+            if (next_sample[e] > epoch_t) continue;
+            active = 1;
+            uint32_t other = csr_other[pos];
+            const float *yo = embd + (int64_t)other * n_dim;
+            float dist_sq = 0.0f;
+            for (int d = 0; d < n_dim; d++) {
+                float diff = ys[d] - yo[d];
+                dist_sq += diff * diff;
+            }
+            if (dist_sq >= ORC_GRAD_DIST_SQ_THRESHOLD) {
+                float dist_sq_b = powf(dist_sq, p->b);
+                float denom = one + p->a * dist_sq_b;
+                float grad_coeff = two_a_b * dist_sq_b / (dist_sq * denom);
+                for (int d = 0; d < n_dim; d++) {
+                    float delta = yo[d] - ys[d];
+                    g[d] += two * grad_coeff * delta;
+                }
+                if (dens) {
+                    float phi = one / denom;
+                    float dphi_term = p->a * p->b * dist_sq_b / dist_sq * phi;
+                    float common = one - p->b * (one - phi);
+                    float dr_self =
+                        (phi / dens_phi_sum[node]) * (common / expf(dens_re[node]) + dphi_term);
+                    float dr_other =
+                        (phi / dens_phi_sum[other]) * (common / expf(dens_re[other]) + dphi_term);
+                    float cor = dens_coeff *
+                                (dens_weight[node] * dr_self + dens_weight[other] * dr_other) /
+                                mu[e];
+                    float two_cor = two * cor;
+                    for (int d = 0; d < n_dim; d++) {
+                        float delta = ys[d] - yo[d];
+                        float gg = two_cor * delta;
+                        if (gg > clip) gg = clip;
+                        if (gg < -clip) gg = -clip;
+                        g[d] += two * gg;
+                    }
+                }
+            }
+            for (int32_t s = 0; s < neg_rate; s++) {
+                uint32_t k = orc_hash_neg(seed, (uint32_t)node, (uint32_t)epoch, edge_local,
+                                          (uint32_t)s, (uint32_t)n);
+                if ((int64_t)k == node) continue;
+                const float *yk = embd + (int64_t)k * n_dim;
+                float dsk = 0.0f;
+                for (int d = 0; d < n_dim; d++) {
+                    float diff = ys[d] - yk[d];
+                    dsk += diff * diff;
+                }
+                float safe = dsk + rep_eps;
+                float dsb = powf(safe, p->b);
+                float denom = safe * (one + p->a * dsb);
+                float gc = two_gamma_b / denom;
+                if (gc > clip) gc = clip;
+                if (gc < -clip) gc = -clip;
+                for (int d = 0; d < n_dim; d++) {
+                    float delta = ys[d] - yk[d];
+                    g[d] += gc * delta;
+                }
+            }
+        }
+        if (has_update) has_update[node] = active;
+    }
+}
+
+/* ------------------------------------------------------------------- Adam */
+
+/* launch_adam_update scalars, umap_optimiser_gpu.rs:1168-1175 */
+void orc_adam_scalars(const orc_optim_params *p, int32_t epoch, int32_t n_epochs,
+                      float *fused_lr, float *epsc) {
+    const float one = 1.0f;
+    float lr_alpha = p->lr * (one - (float)epoch / (float)n_epochs);
+    float t = (float)(epoch + 1);
+    float beta1t = powf(p->beta1, t);
+    float beta2t = powf(p->beta2, t);
+    float sqrt_b2t1 = sqrtf(one - beta2t);
+    float ad_scale = sqrt_b2t1 / (one - beta1t);
+    *epsc = sqrt_b2t1 * p->eps;
+    *fused_lr = lr_alpha * ad_scale;
+}
+
+/* cpu_adam_update, umap_optimiser_gpu.rs:1584-1627 */
+void orc_adam_update(const float *grad, const uint32_t *has_update, float *m, float *v,
+                     float *embd, int64_t n, int32_t n_dim, const orc_optim_params *p,
+                     int32_t epoch, int32_t n_epochs) {
+    float fused, epsc;
+    orc_adam_scalars(p, epoch, n_epochs, &fused, &epsc);
+    const float omb1 = 1.0f - p->beta1, omb2 = 1.0f - p->beta2;
+    for (int64_t node = 0; node < n; node++) {
+        if (!has_update[node]) continue;
+        for (int d = 0; d < n_dim; d++) {
+            int64_t i = node * n_dim + d;
+            float g = grad[i];
+            float m_new = m[i] + omb1 * (g - m[i]);
+            m[i] = m_new;
+            float v_new = v[i] + omb2 * (g * g - v[i]);
+            v[i] = v_new;
+            embd[i] += fused * m_new / (sqrtf(v_new) + epsc);
+        }
+    }
+}
+
+/* umap_edge_schedule_update, umap_optimiser_gpu.rs:940-955 */
+void orc_schedule_update(const float *per, float *next, int64_t n_edges, int32_t epoch) {
+    const float et = (float)epoch;
+    for (int64_t e = 0; e < n_edges; e++)
+        if (next[e] <= et) next[e] += per[e];
+}
+
+/* ---------------------------------------------------------------- densMAP */
+
+/* cpu_dens_radii, umap_optimiser_gpu.rs:1409-1453 */
+void orc_dens_radii(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                    const uint32_t *csr_other, float a, float b, float *re, float *phi_sum) {
+    const float dens_eps = (float)ORC_DENS_EPS;
+    for (int64_t node = 0; node < n; node++) {
+        const float *ys = embd + node * n_dim;
+        float sum_sq = 0.0f, sum_phi = 0.0f;
+        for (uint32_t pos = off[node]; pos < off[node + 1]; pos++) {
+            const float *yo = embd + (int64_t)csr_other[pos] * n_dim;
+            float dist_sq = 0.0f;
+            for (int d = 0; d < n_dim; d++) {
+                float diff = ys[d] - yo[d];
+                dist_sq += diff * diff;
+            }
+            float dsb = powf(dist_sq, b);
+            float phi = 1.0f / (1.0f + a * dsb);
+            sum_sq += phi * dist_sq;
+            sum_phi += phi;
+        }
+        float den = sum_phi < FLT_EPSILON ? FLT_EPSILON : sum_phi;
+        re[node] = logf(dens_eps + sum_sq / den);
+        phi_sum[node] = sum_phi;
+    }
+}
+
+/* correlation_stats, src/utils/density.rs:464-508 */
+void orc_correlation_stats(const float *re, const float *r, int64_t n, float var_shift,
+                           double out[3]) {
+    const int64_t C = ORC_DENS_REDUCE_CHUNK;
+    double denom = (double)((n - 1) > 1 ? (n - 1) : 1);
+    double total = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += C) {
+        double s = 0.0;
+        int64_t c1 = c0 + C < n ? c0 + C : n;
+        for (int64_t i = c0; i < c1; i++) s += (double)re[i];
+        total += s;
+    }
+    double mean = total / (double)(n > 1 ? n : 1);
+    double ss = 0.0, cross = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += C) {
+        double s1 = 0.0, s2 = 0.0;
+        int64_t c1 = c0 + C < n ? c0 + C : n;
+        for (int64_t i = c0; i < c1; i++) {
+            double d = (double)re[i] - mean;
+            s1 += d * d;
+            s2 += d * (double)r[i];
+        }
+        ss += s1;
+        cross += s2;
+    }
+    double var = ss / denom, cov = cross / denom;
+    out[0] = mean;
+    out[1] = sqrt(var + (double)var_shift);
+    out[2] = cov;
+}
+
+/* launch_dens_update host part, umap_optimiser_gpu.rs:1122-1142, and
+ * update_density_weights, umap_optimisers.rs:519-537 */
+float orc_dens_weights(const float *r, const float *re, int64_t n, float var_shift,
+                       float lambda, float mu_tot, float *weight) {
+    double st[3];
+    orc_correlation_stats(re, r, n, var_shift, st);
+    float mean = (float)st[0], sd = (float)st[1], cov = (float)st[2];
+    float inv_var = 1.0f / (sd * sd);
+    for (int64_t i = 0; i < n; i++) weight[i] = r[i] - cov * (re[i] - mean) * inv_var;
+    float denom = (float)((n - 1) > 1 ? (n - 1) : 1);
+    return lambda * mu_tot / (sd * denom);
+}
+
+/* DensState::is_active, src/utils/density.rs:244-250 */
+int orc_dens_is_active(float lambda, float frac, int32_t epoch, int32_t n_epochs) {
+    if (lambda <= 0.0f || n_epochs == 0) return 0;
+    float progress = (float)(epoch + 1) / (float)n_epochs;
+    return progress > 1.0f - frac;
+}
+
+/* ------------------------------------------------------ GPU driver replay */
+
+/* optimise_embedding_adam_gpu, umap_optimiser_gpu.rs:1302-1369 */
+int orc_optimise_adam_gpu(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,
+                          const int32_t *adj_cols, const float *adj_vals,
+                          const orc_optim_params *p, const float *dens_r, float dens_lambda,
+                          float dens_frac, float dens_var_shift, float dens_mu_tot,
+                          uint64_t seed) {
+    if (n == 0) return 1;
+    int64_t ne = 0;
+    int st = orc_csr_from_graph(n, adj_off, adj_cols, adj_vals, &ne, NULL, NULL, NULL, NULL,
+                                NULL, NULL, NULL);
+    if (st) return st;
+    uint32_t *off = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    uint32_t *eidx = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(2 * ne));
+    uint32_t *oth = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(2 * ne));
+    float *per = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *next = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *mu = (float *)malloc(sizeof(float) * (size_t)ne);
+    orc_csr_from_graph(n, adj_off, adj_cols, adj_vals, &ne, off, eidx, oth, per, mu, NULL, NULL);
+    memcpy(next, per, sizeof(float) * (size_t)ne);
+    size_t nd = (size_t)n * (size_t)n_dim;
+    float *m = (float *)calloc(nd, sizeof(float));
+    float *v = (float *)calloc(nd, sizeof(float));
+    float *g = (float *)calloc(nd, sizeof(float));
+    uint32_t *hu = (uint32_t *)calloc((size_t)n, sizeof(uint32_t));
+    float *re = NULL, *phi = NULL, *w = NULL;
+    if (dens_r) {
+        re = (float *)calloc((size_t)n, sizeof(float));
+        phi = (float *)calloc((size_t)n, sizeof(float));
+        w = (float *)calloc((size_t)n, sizeof(float));
+    }
+    uint32_t seed32 = (uint32_t)seed;
+    for (int32_t e = 0; e < p->n_epochs; e++) {
+        int dens_on = dens_r && orc_dens_is_active(dens_lambda, dens_frac, e, p->n_epochs);
+        float coeff = 0.0f;
+        if (dens_on) {
+            orc_dens_radii(embd, n, n_dim, off, oth, p->a, p->b, re, phi);
+            coeff = orc_dens_weights(dens_r, re, n, dens_var_shift, dens_lambda, dens_mu_tot, w);
+        }
+        orc_grad_accum(embd, n, n_dim, off, eidx, oth, next, e, seed32, p->neg_sample_rate, p,
+                       mu, dens_on ? re : NULL, phi, w, coeff, g, hu);
+        orc_adam_update(g, hu, m, v, embd, n, n_dim, p, e, p->n_epochs);
+        orc_schedule_update(per, next, ne, e);
+    }
+    free(off); free(eidx); free(oth); free(per); free(next); free(mu);
+    free(m); free(v); free(g); free(hu); free(re); free(phi); free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------ graph build */
+
+/* local_connectivity_distance, src/data/graph.rs:107-125 */
+static float local_connectivity_distance(const float *d, int32_t len, float lc) {
+    if (lc <= 0.0f || len == 0) return 0.0f;
+    float offset = lc - 1.0f;
+    if (offset < 0.0f) offset = 0.0f;
+    float whole = floorf(offset);
+    float fraction = offset - whole;
+    int64_t idx = (int64_t)whole;
+    if (idx > len - 1) idx = len - 1;
+    if (fraction > 0.0f && idx + 1 < len) return d[idx] * (1.0f - fraction) + d[idx + 1] * fraction;
+    return d[idx];
+}
+
+/* smooth_knn_dist, src/data/graph.rs:499-565 */
+void orc_smooth_knn_dist(const float *dist, int64_t n, int32_t k, float local_connectivity,
+                         float bandwidth, int32_t n_iter, float *sigma, float *rho) {
+    const double target = log((double)k);
+    const double tol = (double)bandwidth;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const float *d = dist + i * k;
+        float adj[1024];
+        float rh = local_connectivity_distance(d, k, local_connectivity);
+        for (int j = 0; j < k; j++) {
+            float a = d[j] - rh;
+            adj[j] = a > 0.0f ? a : 0.0f;
+        }
+        float lo = 0.0f, hi = FLT_MAX, mid = 1.0f;
+        for (int it = 0; it < n_iter; it++) {
+            float inv_mid = 1.0f / mid;
+            float s = 0.0f;
+            for (int j = 0; j < k; j++) s += expf(-adj[j] * inv_mid);
+            double val = (double)s;
+            if (fabs(val - target) < tol) break;
+            if (val > target) {
+                hi = mid;
+                mid = (lo + hi) / 2.0f;
+            } else {
+                lo = mid;
+                if (hi == FLT_MAX) mid *= 2.0f;
+                else mid = (lo + hi) / 2.0f;
+            }
+        }
+        sigma[i] = mid;
+        rho[i] = rh;
+    }
+}
+
+/* knn_to_coo, src/data/graph.rs:585-662 */
+int64_t orc_knn_to_coo(const int32_t *idx, const float *dist, int64_t n, int32_t k,
+                       const float *sigma, const float *rho, int32_t *rows, int32_t *cols,
+                       float *vals) {
+    int64_t w = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float s = sigma[i], r = rho[i];
+        int smooth = s > 0.0f;
+        float inv_sigma = smooth ? 1.0f / s : 0.0f;
+        for (int j = 0; j < k; j++) {
+            int32_t c = idx[i * k + j];
+            if (c == i) continue;
+            float a = dist[i * k + j] - r;
+            if (a < 0.0f) a = 0.0f;
+            float wt = smooth ? expf(-a * inv_sigma) : (a > 0.0f ? 0.0f : 1.0f);
+            rows[w] = (int32_t)i;
+            cols[w] = c;
+            vals[w] = wt;
+            w++;
+        }
+    }
+    return w;
+}
+
+typedef struct {
+    int32_t c;
+    float w;
+    int64_t ord;
+} orc_entry;
+
+static int cmp_entry(const void *a, const void *b) {
+    const orc_entry *x = (const orc_entry *)a, *y = (const orc_entry *)b;
+    if (x->c != y->c) return x->c < y->c ? -1 : 1;
+    return x->ord < y->ord ? -1 : (x->ord > y->ord);
+}
+
+/* symmetrise_csr + symmetrise_graph, src/data/graph.rs:185-468, 684-694 */
+int64_t orc_symmetrise(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
+                       int64_t nnz, float mix_weight, int32_t *out_rows, int32_t *out_cols,
+                       float *out_vals) {
+    if (n == 0 || nnz == 0) return 0;
+    const float directed = 1.0f - mix_weight;
+    /* coo_to_sorted_csr: counting sort by row, then sort each row by column */
+    int64_t *ip = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < nnz; e++) ip[rows[e] + 1]++;
+    for (int64_t i = 0; i < n; i++) ip[i + 1] += ip[i];
+    orc_entry *ent = (orc_entry *)malloc(sizeof(orc_entry) * (size_t)nnz);
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    memcpy(cur, ip, sizeof(int64_t) * (size_t)n);
+    for (int64_t e = 0; e < nnz; e++) {
+        int64_t p = cur[rows[e]]++;
+        ent[p].c = cols[e];
+        ent[p].w = vals[e];
+        ent[p].ord = e;
+    }
+    for (int64_t i = 0; i < n; i++)
+        qsort(ent + ip[i], (size_t)(ip[i + 1] - ip[i]), sizeof(orc_entry), cmp_entry);
+    /* transpose_csr: counting sort visiting rows in order */
+    int64_t *tp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < nnz; e++) tp[ent[e].c + 1]++;
+    for (int64_t i = 0; i < n; i++) tp[i + 1] += tp[i];
+    orc_entry *tent = (orc_entry *)malloc(sizeof(orc_entry) * (size_t)nnz);
+    memcpy(cur, tp, sizeof(int64_t) * (size_t)n);
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = ip[r]; p < ip[r + 1]; p++) {
+            int64_t q = cur[ent[p].c]++;
+            tent[q].c = (int32_t)r;
+            tent[q].w = ent[p].w;
+            tent[q].ord = 0;
+        }
+    /* merge_row for every row, in row order */
+    int64_t written = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t ia = ip[i], ea = ip[i + 1], ib = tp[i], eb = tp[i + 1];
+        while (ia < ea || ib < eb) {
+            int32_t col;
+            if (ia < ea && ib < eb) col = ent[ia].c < tent[ib].c ? ent[ia].c : tent[ib].c;
+            else if (ia < ea) col = ent[ia].c;
+            else col = tent[ib].c;
+            float wij = 0.0f, wji = 0.0f;
+            while (ia < ea && ent[ia].c == col) wij = ent[ia++].w;
+            while (ib < eb && tent[ib].c == col) wji = tent[ib++].w;
+            float uni = wij + wji - wij * wji;
+            float wsym = mix_weight * uni + directed * wij;
+            if (wsym > 0.0f) {
+                out_rows[written] = (int32_t)i;
+                out_cols[written] = col;
+                out_vals[written] = wsym;
+                written++;
+            }
+        }
+    }
+    free(ip); free(ent); free(cur); free(tp); free(tent);
+    return written;
+}
+
+/* filter_weak_edges, src/data/graph.rs:776-853 */
+int64_t orc_filter_weak_edges(int32_t *rows, int32_t *cols, float *vals, int64_t nnz,
+                              int32_t n_epochs) {
+    float max_w = 0.0f;
+    for (int64_t e = 0; e < nnz; e++)
+        if (vals[e] > max_w) max_w = vals[e];
+    float thr = max_w / (float)n_epochs;
+    int64_t w = 0;
+    for (int64_t e = 0; e < nnz; e++)
+        if (vals[e] >= thr) {
+            rows[w] = rows[e];
+            cols[w] = cols[e];
+            vals[w] = vals[e];
+            w++;
+        }
+    return w;
+}
+
+/* ------------------------------------------------------------------- kNN */
+
+#define KNN_MAXK 256
+
+void orc_knn_exact_f32(const float *x, int64_t n, int32_t d, int32_t k, int64_t q0, int64_t q1,
+                       int32_t *idx, float *dist) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t q = q0; q < q1; q++) {
+        float bd[KNN_MAXK];
+        int32_t bi[KNN_MAXK];
+        int cnt = 0;
+        const float *xq = x + q * d;
+        for (int64_t j = 0; j < n; j++) {
+            if (j == q) continue;
+            const float *xj = x + j * d;
+            float s = 0.0f;
+            for (int t = 0; t < d; t++) {
+                float df = xq[t] - xj[t];
+                s = fmaf(df, df, s);
+            }
+            if (cnt == k && !(s < bd[k - 1])) continue; /* ties: keep smaller index (seen first) */
+            int p = cnt < k ? cnt : k - 1;
+            while (p > 0 && (s < bd[p - 1])) {
+                bd[p] = bd[p - 1];
+                bi[p] = bi[p - 1];
+                p--;
+            }
+            bd[p] = s;
+            bi[p] = (int32_t)j;
+            if (cnt < k) cnt++;
+        }
+        for (int t = 0; t < k; t++) {
+            idx[(q - q0) * k + t] = t < cnt ? bi[t] : -1;
+            dist[(q - q0) * k + t] = t < cnt ? bd[t] : INFINITY;
+        }
+    }
+}
+
+void orc_knn_exact_f64(const float *x, int64_t n, int32_t d, int32_t k, int64_t q0, int64_t q1,
+                       int32_t *idx, double *dist) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t q = q0; q < q1; q++) {
+        double bd[KNN_MAXK];
+        int32_t bi[KNN_MAXK];
+        int cnt = 0;
+        const float *xq = x + q * d;
+        for (int64_t j = 0; j < n; j++) {
+            if (j == q) continue;
+            const float *xj = x + j * d;
+            double s = 0.0;
+            for (int t = 0; t < d; t++) {
+                double df = (double)xq[t] - (double)xj[t];
+                s += df * df;
+            }
+            if (cnt == k && !(s < bd[k - 1])) continue;
+            int p = cnt < k ? cnt : k - 1;
+            while (p > 0 && (s < bd[p - 1])) {
+                bd[p] = bd[p - 1];
+                bi[p] = bi[p - 1];
+                p--;
+            }
+            bd[p] = s;
+            bi[p] = (int32_t)j;
+            if (cnt < k) cnt++;
+        }
+        for (int t = 0; t < k; t++) {
+            idx[(q - q0) * k + t] = t < cnt ? bi[t] : -1;
+            dist[(q - q0) * k + t] = t < cnt ? bd[t] : INFINITY;
+        }
+    }
+}
+
+/* --------------------------------------------------------------- DensState */
+
+/* DensState::new, src/utils/density.rs:181-201, with edge_distances_from_knn
+ * :309-334, original_log_radii :358-398 and standardise :410-438 */
+int orc_dens_state(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
+                   int64_t nnz, const int32_t *knn_idx, const float *knn_dist, int32_t k,
+                   int32_t distances_squared, float *r_out, float *mu_sum_out,
+                   float *mu_tot_out) {
+    if (n == 0) return 1;
+    double *ro = (double *)calloc((size_t)n, sizeof(double));
+    double *ms = (double *)calloc((size_t)n, sizeof(double));
+    for (int64_t e = 0; e < nnz; e++) {
+        int32_t i = rows[e], j = cols[e];
+        float raw = 0.0f;
+        int found = 0;
+        for (int t = 0; t < k; t++)
+            if (knn_idx[(int64_t)i * k + t] == j) {
+                raw = knn_dist[(int64_t)i * k + t];
+                found = 1;
+                break;
+            }
+        if (!found)
+            for (int t = 0; t < k; t++)
+                if (knn_idx[(int64_t)j * k + t] == i) {
+                    raw = knn_dist[(int64_t)j * k + t];
+                    break;
+                }
+        float dsq = distances_squared ? raw : raw * raw;
+        double mu = (double)vals[e];
+        ro[i] += mu * (double)dsq;
+        ms[i] += mu;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        double den = ms[i] > DBL_EPSILON ? ms[i] : DBL_EPSILON;
+        ro[i] = log(ORC_DENS_EPS + ro[i] / den);
+    }
+    int status = 0;
+    if (n < 2) {
+        status = 3;
+    } else {
+        double mean = 0.0;
+        for (int64_t i = 0; i < n; i++) mean += ro[i];
+        mean /= (double)n;
+        double var = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            double dd = ro[i] - mean;
+            var += dd * dd;
+        }
+        var /= (double)(n - 1);
+        double sd = sqrt(var);
+        if (isnan(sd) || sd <= DBL_EPSILON) status = 3;
+        else
+            for (int64_t i = 0; i < n; i++) ro[i] = (ro[i] - mean) / sd;
+    }
+    if (status == 0) {
+        double tot = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            r_out[i] = (float)ro[i];
+            mu_sum_out[i] = (float)ms[i];
+            tot += (double)mu_sum_out[i];
+        }
+        *mu_tot_out = (float)tot;
+    }
+    free(ro);
+    free(ms);
+    return status;
+}
+
+/* -------------------------------------------------------------- fit_params */
+
+/* UmapOptimParams::fit_params, src/training/umap_optimisers.rs:139-209 */
+void orc_fit_params(float min_dist, float spread, int32_t n_iter, float *a_out, float *b_out) {
+    if (n_iter <= 0) n_iter = 300;
+    const int np = 300;
+    float xv[300], yv[300];
+    float max_x = spread * 3.0f;
+    float step = max_x / (float)(np - 1);
+    for (int i = 0; i < np; i++) {
+        float x = step * (float)i;
+        xv[i] = x;
+        yv[i] = x < min_dist ? 1.0f : expf(-(x - min_dist) / spread);
+    }
+    float a = 1.0f, b = 1.0f;
+    const float two = 2.0f;
+    for (int it = 0; it < n_iter; it++) {
+        float ga = 0.0f, gb = 0.0f;
+        for (int i = 0; i < np; i++) {
+            float x = xv[i];
+            if (x <= 0.0f) continue;
+            float x2b = powf(x, two * b);
+            float denom = 1.0f + a * x2b;
+            float pred = 1.0f / denom;
+            float err = pred - yv[i];
+            ga += err * (-x2b / (denom * denom));
+            float lx = logf(x);
+            gb += err * (-two * a * x2b * lx / (denom * denom));
+        }
+        ga /= (float)np;
+        gb /= (float)np;
+        a -= 1.0f * ga;
+        b -= 1.0f * gb;
+        a = a < 0.001f ? 0.001f : a;
+        a = a > 10.0f ? 10.0f : a;
+        b = b < 0.1f ? 0.1f : b;
+        b = b > 2.0f ? 2.0f : b;
+    }
+    *a_out = a;
+    *b_out = b;
+}
+
+/* ------------------------------------------------- CPU optimisers (baseline) */
+
+typedef struct {
+    float b, max_val, inv_step;
+    float *table;
+    int size;
+} orc_lut;
+
+/* FastPowLut::new / get, src/training/umap_optimisers.rs:325-397 */
+static void lut_init(orc_lut *L, float b, float max_val, int size) {
+    L->b = b;
+    L->max_val = max_val;
+    L->size = size;
+    float step = max_val / (float)(size - 1);
+    L->inv_step = 1.0f / step;
+    L->table = (float *)malloc(sizeof(float) * (size_t)size);
+    for (int i = 0; i < size; i++) L->table[i] = powf(step * (float)i, b);
+}
+
+static inline float lut_get(const orc_lut *L, float x) {
+    if (x >= L->max_val) return powf(x, L->b);
+    float idx_f = x * L->inv_step;
+    int64_t idx = idx_f > 0.0f ? (int64_t)idx_f : 0;
+    if (idx >= L->size - 1) return L->table[L->size - 1];
+    float rem = idx_f - (float)idx;
+    float y0 = L->table[idx], y1 = L->table[idx + 1];
+    return y0 + rem * (y1 - y0);
+}
+
+/* optimise_embedding_adam_parallel, src/training/umap_optimisers.rs:1170-1492
+ * (plain-UMAP arm; the densMAP pass 0 at :1303-1317 is covered by the GPU
+ * replay above). */
+int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,
+                               const int32_t *adj_cols, const float *adj_vals,
+                               const orc_optim_params *p, uint64_t seed, int32_t use_lut,
+                               int32_t n_threads) {
+    if (n == 0) return 1;
+    int64_t ne = 0;
+    int st = orc_csr_from_graph(n, adj_off, adj_cols, adj_vals, &ne, NULL, NULL, NULL, NULL,
+                                NULL, NULL, NULL);
+    if (st) return st;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    uint32_t *off = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    uint32_t *eidx = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(2 * ne));
+    uint32_t *oth = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(2 * ne));
+    float *per = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *mu = (float *)malloc(sizeof(float) * (size_t)ne);
+    int32_t *ei = (int32_t *)malloc(sizeof(int32_t) * (size_t)ne);
+    orc_csr_from_graph(n, adj_off, adj_cols, adj_vals, &ne, off, eidx, oth, per, mu, ei, NULL);
+    float *next = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *per_neg = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *next_neg = (float *)malloc(sizeof(float) * (size_t)ne);
+    const float nsr = (float)p->neg_sample_rate;
+    for (int64_t e = 0; e < ne; e++) {
+        next[e] = per[e];
+        per_neg[e] = per[e] / nsr;
+        next_neg[e] = per_neg[e];
+    }
+    size_t nd = (size_t)n * (size_t)n_dim;
+    float *m = (float *)calloc(nd, sizeof(float));
+    float *v = (float *)calloc(nd, sizeof(float));
+    float *g = (float *)calloc(nd, sizeof(float));
+    unsigned char *hu = (unsigned char *)calloc((size_t)n, 1);
+    const float two = 2.0f;
+    const float two_a_b = two * p->a * p->b, two_gamma_b = two * p->gamma * p->b;
+    const float clip = 4.0f, ceps = 0.001f;
+    const int b_is_one = fabsf(p->b - 1.0f) < 1e-10f;
+    orc_lut L;
+    lut_init(&L, p->b, 25.0f, 65536);
+    const uint32_t seed32 = (uint32_t)seed;
+    const float nef = (float)p->n_epochs;
+    for (int32_t epoch = 0; epoch < p->n_epochs; epoch++) {
+        const float lr = p->lr * (1.0f - (float)epoch / nef);
+        const float et = (float)epoch;
+        const float t = (float)(epoch + 1);
+        const float sqrt_b2t1 = sqrtf(1.0f - powf(p->beta2, t));
+        const float ad_scale = sqrt_b2t1 / (1.0f - powf(p->beta1, t));
+        const float epsc = sqrt_b2t1 * p->eps;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; i++) {
+            float *gi = g + i * n_dim;
+            for (int d = 0; d < n_dim; d++) gi[d] = 0.0f;
+            const float *yi = embd + i * n_dim;
+            int any = 0;
+            uint32_t slot = 0;
+            for (uint32_t pos = off[i]; pos < off[i + 1]; pos++, slot++) {
+                uint32_t e = eidx[pos];
+                if (next[e] > et) continue;
+                any = 1;
+                uint32_t o = oth[pos];
+                const float *yo = embd + (int64_t)o * n_dim;
+                float dsq = 0.0f;
+                for (int d = 0; d < n_dim; d++) {
+                    float df = yi[d] - yo[d];
+                    dsq += df * df;
+                }
+                if (dsq >= 1e-8f) {
+                    float dsb = b_is_one ? dsq : (use_lut ? lut_get(&L, dsq) : powf(dsq, p->b));
+                    float denom = 1.0f + p->a * dsb;
+                    float gc = two_a_b * dsb / (dsq * denom);
+                    for (int d = 0; d < n_dim; d++) gi[d] += two * gc * (yo[d] - yi[d]);
+                }
+                if (ei[e] == (int32_t)i) { /* is_smaller */
+                    float nf = floorf((et - next_neg[e]) / per_neg[e]);
+                    int64_t nneg = nf > 0.0f ? (int64_t)nf : 0;
+                    for (int64_t s = 0; s < nneg; s++) {
+                        uint32_t kk = orc_hash_neg(seed32, (uint32_t)i, (uint32_t)epoch, slot,
+                                                   (uint32_t)s, (uint32_t)n);
+                        if ((int64_t)kk == i) continue;
+                        const float *yk = embd + (int64_t)kk * n_dim;
+                        float dk = 0.0f;
+                        for (int d = 0; d < n_dim; d++) {
+                            float df = yi[d] - yk[d];
+                            dk += df * df;
+                        }
+                        float safe = dk + ceps;
+                        float dsb = b_is_one ? safe : (use_lut ? lut_get(&L, safe) : powf(safe, p->b));
+                        float denom = safe * (1.0f + p->a * dsb);
+                        float gc = two_gamma_b / denom;
+                        if (gc > clip) gc = clip;
+                        if (gc < -clip) gc = -clip;
+                        for (int d = 0; d < n_dim; d++) gi[d] += gc * (yi[d] - yk[d]);
+                    }
+                }
+            }
+            hu[i] = (unsigned char)any;
+        }
+        const float omb1 = 1.0f - p->beta1, omb2 = 1.0f - p->beta2;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            if (!hu[i]) continue;
+            for (int d = 0; d < n_dim; d++) {
+                int64_t q = i * n_dim + d;
+                float gg = g[q];
+                m[q] += omb1 * (gg - m[q]);
+                v[q] += omb2 * (gg * gg - v[q]);
+                embd[q] += lr * ad_scale * m[q] / (sqrtf(v[q]) + epsc);
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < ne; e++) {
+            if (next[e] <= et) {
+                next[e] += per[e];
+                float nf = floorf((et - next_neg[e]) / per_neg[e]);
+                int64_t nneg = nf > 0.0f ? (int64_t)nf : 0;
+                next_neg[e] += (float)nneg * per_neg[e];
+            }
+        }
+    }
+    free(L.table);
+    free(off); free(eidx); free(oth); free(per); free(mu); free(ei); free(next);
+    free(per_neg); free(next_neg); free(m); free(v); free(g); free(hu);
+    return 0;
+}
+
+/* optimise_embedding_sgd, src/training/umap_optimisers.rs:653-865 (plain UMAP) */
+int orc_optimise_sgd(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,
+                     const int32_t *adj_cols, const float *adj_vals, const orc_optim_params *p,
+                     uint64_t seed) {
+    if (n == 0) return 1;
+    int64_t ne = adj_off[n];
+    if (ne == 0) return 2;
+    float max_w = 0.0f;
+    for (int64_t e = 0; e < ne; e++)
+        if (adj_vals[e] > max_w) max_w = adj_vals[e];
+    float *per = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *next = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *per_neg = (float *)malloc(sizeof(float) * (size_t)ne);
+    float *next_neg = (float *)malloc(sizeof(float) * (size_t)ne);
+    int32_t *src = (int32_t *)malloc(sizeof(int32_t) * (size_t)ne);
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t q = adj_off[i]; q < adj_off[i + 1]; q++) src[q] = (int32_t)i;
+    const float nsr = (float)p->neg_sample_rate;
+    for (int64_t e = 0; e < ne; e++) {
+        float norm = adj_vals[e] / max_w;
+        per[e] = norm > 0.0f ? 1.0f / norm : 1e8f;
+        next[e] = per[e];
+        per_neg[e] = per[e] / nsr;
+        next_neg[e] = per_neg[e];
+    }
+    const float two = 2.0f;
+    const float two_a_b = two * p->a * p->b, two_gamma_b = two * p->gamma * p->b;
+    const float clip = 4.0f;
+    const int b_is_one = fabsf(p->b - 1.0f) < 1e-10f;
+    const int b_is_half = fabsf(p->b - 0.5f) < 1e-10f;
+#define FPOW(x) (b_is_one ? (x) : (b_is_half ? sqrtf(x) : powf((x), p->b)))
+    const uint32_t seed32 = (uint32_t)seed;
+    const float nef = (float)p->n_epochs;
+    for (int32_t epoch = 0; epoch < p->n_epochs; epoch++) {
+        const float lr = p->lr * (1.0f - (float)epoch / nef);
+        const float et = (float)epoch;
+        for (int64_t e = 0; e < ne; e++) {
+            if (next[e] > et) continue;
+            int64_t i = src[e], j = adj_cols[e];
+            float *yi = embd + i * n_dim, *yj = embd + j * n_dim;
+            float dsq = 0.0f;
+            for (int d = 0; d < n_dim; d++) {
+                float df = yi[d] - yj[d];
+                dsq += df * df;
+            }
+            if (dsq >= 1e-8f) {
+                float dsb = FPOW(dsq);
+                float denom = 1.0f + p->a * dsb;
+                float gc = two_a_b * dsb / (dsq * denom);
+                for (int d = 0; d < n_dim; d++) {
+                    float delta = yj[d] - yi[d];
+                    float gd = gc * delta;
+                    if (gd > clip) gd = clip;
+                    if (gd < -clip) gd = -clip;
+                    yi[d] += gd * lr;
+                    yj[d] -= gd * lr;
+                }
+            }
+            next[e] += per[e];
+            float nf = floorf((et - next_neg[e]) / per_neg[e]);
+            int64_t nneg = nf > 0.0f ? (int64_t)nf : 0;
+            for (int64_t s = 0; s < nneg; s++) {
+                uint32_t kk = orc_hash_neg(seed32, (uint32_t)i, (uint32_t)epoch,
+                                           (uint32_t)(e - adj_off[i]), (uint32_t)s, (uint32_t)n);
+                if ((int64_t)kk == i) continue;
+                float *yk = embd + (int64_t)kk * n_dim;
+                float dk = 0.0f;
+                for (int d = 0; d < n_dim; d++) {
+                    float df = yi[d] - yk[d];
+                    dk += df * df;
+                }
+                float safe = dk + 0.001f;
+                float dsb = FPOW(safe);
+                float denom = safe * (1.0f + p->a * dsb);
+                float gc = two_gamma_b / denom;
+                if (gc > clip) gc = clip;
+                if (gc < -clip) gc = -clip;
+                for (int d = 0; d < n_dim; d++) yi[d] += gc * (yi[d] - yk[d]) * lr;
+            }
+            next_neg[e] += (float)nneg * per_neg[e];
+        }
+    }
+#undef FPOW
+    free(per); free(next); free(per_neg); free(next_neg); free(src);
+    return 0;
+}

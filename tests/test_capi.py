@@ -1,0 +1,56 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol,
+and its host-side logic (string keys, the negative hash) matches the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mrs_amd as M
+import pyoracle as orc
+
+
+def test_library_exports_every_declared_symbol():
+    L = M.lib()
+    names = M._declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_parse_string_keys():
+    # parse_umap_optimiser_gpu (umap_optimiser_gpu.rs:44-52): case-insensitive
+    L = M.lib()
+    assert L.mrs_parse_umap_optimiser_gpu(b"adam_gpu") == 0
+    assert L.mrs_parse_umap_optimiser_gpu(b"ADAM_GPU") == 0
+    assert L.mrs_parse_umap_optimiser_gpu(b"adam_parallel") == 1
+    assert L.mrs_parse_umap_optimiser_gpu(b"adam") == 2
+    assert L.mrs_parse_umap_optimiser_gpu(b"sgd") == 3
+    assert L.mrs_parse_umap_optimiser_gpu(b"nope") == -1
+    # parse_ann_search_gpu (nearest_neighbours_gpu.rs:203-210)
+    assert L.mrs_parse_ann_search_gpu(b"exhaustive_gpu") == 2
+    assert L.mrs_parse_ann_search_gpu(b"Exhaustive") == 2
+    assert L.mrs_parse_ann_search_gpu(b"ivf") == 0
+    assert L.mrs_parse_ann_search_gpu(b"nndescent_gpu") == 1
+    assert L.mrs_parse_ann_search_gpu(b"bogus") == -1
+
+
+def test_hash_matches_oracle_including_fast_remainder():
+    L = M.lib()
+    rng = np.random.default_rng(0)
+    ns = [1, 2, 3, 5, 7, 10, 641, 65537, 99_991, 100_000, 1_000_000, 2**31 - 1, 2**31,
+          2**32 - 5, 2**32 - 1]
+    for n in ns:
+        for _ in range(400):
+            a = [int(v) for v in rng.integers(0, 2**32, size=5, dtype=np.uint64)]
+            assert L.mrs_hash_neg(*a, n) == orc.hash_neg(*a, n), (a, n)
+
+
+def test_ctx_create_without_device_reports_device_error():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    h = C.c_void_p()
+    rc = M.lib().mrs_ctx_create(0, 0, C.byref(h))
+    assert rc == M.MRS_DEVICE_ERROR
+    assert M.lib().mrs_last_error()

@@ -1,0 +1,340 @@
+"""GPU parity of the "adam_gpu" epoch kernels against the CPU oracle.
+
+Ports the reference's kernel tests (src/training/umap_optimiser_gpu.rs:1786-2505)
+and adds seeded larger graphs.  Tolerances: the reference's own (bit-exact for
+attractive-only, 1e-4 with negatives, 1e-5/1e-6 for Adam); 1e-4 relative
+(floor 1e-3) for per-epoch gradients on larger graphs.
+"""
+import numpy as np
+import pytest
+
+import mrs_amd as M
+import pyoracle as orc
+from helpers import assert_close, blobs, rel_err, triangle_setup, umap_graph_from_data
+
+pytestmark = pytest.mark.gpu
+
+
+def make_state(ctx, embd, graph, params, dens=None):
+    adj = M.adjacency_to_csr(graph) if isinstance(graph, list) else graph
+    return M.UmapGpuState(ctx, embd, *adj, params, dens), adj
+
+
+def oracle_csr(adj):
+    st, c = orc.csr_from_graph(*adj)
+    assert st == 0
+    return c
+
+
+# ------------------------------------------------------- upload / state
+
+
+def test_state_roundtrips(ctx):
+    embd, graph, params = triangle_setup()
+    st, adj = make_state(ctx, embd, graph, params)
+    c = oracle_csr(adj)
+    assert np.array_equal(st.read(M.BUF_EMBD), embd.ravel())
+    for b in (M.BUF_M, M.BUF_V, M.BUF_GRAD):
+        assert np.all(st.read(b) == 0)
+    assert np.array_equal(st.read(M.BUF_OFFSETS), c["off"])
+    assert np.array_equal(st.read(M.BUF_EDGE_IDX), c["edge_idx"])
+    assert np.array_equal(st.read(M.BUF_OTHER), c["other"])
+    assert np.array_equal(st.read(M.BUF_PER), c["per"])
+    assert np.array_equal(st.read(M.BUF_NEXT), c["per"])
+
+
+def test_upload_3d_embedding(ctx):
+    _, graph, params = triangle_setup()
+    embd = np.arange(9, dtype=np.float32).reshape(3, 3)
+    st, _ = make_state(ctx, embd, graph, params)
+    assert st.n_dim == 3
+    assert np.array_equal(st.read(M.BUF_EMBD), embd.ravel())
+
+
+def test_state_errors(ctx):
+    _, _, params = triangle_setup()
+    with pytest.raises(M.ManifoldsError) as e:
+        make_state(ctx, np.zeros((3, 2), np.float32), [[], [], []], params)
+    assert e.value.status == M.MRS_NO_GRAPH_EDGES
+
+
+# ------------------------------------------------------------ gradient
+
+
+def test_grad_empty_epoch(ctx):
+    embd, graph, params = triangle_setup()
+    st, _ = make_state(ctx, embd, graph, params)
+    st.launch_grad_accum(0, 42)
+    assert np.all(st.read(M.BUF_GRAD) == 0)
+
+
+def test_grad_attractive_only_bit_exact(ctx):
+    # :2005-2032
+    embd, graph, params = triangle_setup()
+    params.neg_sample_rate = 0
+    st, adj = make_state(ctx, embd, graph, params)
+    st.launch_grad_accum(5, 42)
+    c = oracle_csr(adj)
+    want, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 5, 42, 0, params)
+    assert_close(st.read(M.BUF_GRAD), want, 0.0, "attractive-only")
+
+
+def test_grad_with_negatives_matches_cpu(ctx):
+    # :2034-2058
+    embd, graph, params = triangle_setup()
+    st, adj = make_state(ctx, embd, graph, params)
+    st.launch_grad_accum(5, 42)
+    c = oracle_csr(adj)
+    want, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 5, 42, 5, params)
+    assert_close(st.read(M.BUF_GRAD), want, 1e-4, "with-negatives")
+
+
+def test_grad_single_edge_hand_computed(ctx):
+    # :2060-2092
+    embd = np.array([[0.0, 0.0], [3.0, 0.0]], np.float32)
+    p = M.UmapOptimParams(a=1.0, b=1.0, lr=1.0, gamma=1.0, n_epochs=10, neg_sample_rate=0,
+                          min_dist=0.1, beta1=0.9, beta2=0.999, eps=1e-7)
+    st, _ = make_state(ctx, embd, [[(1, 1.0)], [(0, 1.0)]], p)
+    st.launch_grad_accum(5, 42)
+    assert_close(st.read(M.BUF_GRAD), [1.2, 0.0, -1.2, 0.0], 1e-5, "hand-computed")
+    st.launch_epoch_edges(5, 42)
+    assert_close(st.read(M.BUF_GRAD), [1.2, 0.0, -1.2, 0.0], 1e-5, "hand-computed edges")
+
+
+def test_grad_deterministic(ctx):
+    embd, graph, params = triangle_setup()
+    st, _ = make_state(ctx, embd, graph, params)
+    st.launch_grad_accum(5, 42)
+    g1 = st.read(M.BUF_GRAD)
+    st.launch_grad_accum(5, 42)
+    assert np.array_equal(g1, st.read(M.BUF_GRAD))
+
+
+def _random_graph(n=3000, d=8, k=15, seed=21):
+    x, _ = blobs(n, d, 6, seed=seed)
+    _, _, _, adj = umap_graph_from_data(x, k)
+    rng = np.random.default_rng(seed)
+    embd = rng.uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    return embd, adj
+
+
+@pytest.mark.parametrize("epoch", [1, 3, 17])
+def test_grad_random_graph_node_and_edge_kernels(ctx, epoch):
+    embd, adj = _random_graph()
+    params = M.UmapOptimParams.default_2d()
+    st, _ = make_state(ctx, embd, adj, params)
+    c = oracle_csr(adj)
+    nxt = st.read(M.BUF_NEXT)
+    want, hu = orc.grad_accum(embd, c, nxt, epoch, 42, 5, params)
+    st.launch_grad_accum(epoch, 42)
+    g_node = st.read(M.BUF_GRAD).reshape(-1, 2)
+    assert np.array_equal(st.read(M.BUF_HAS_UPDATE), hu)
+    assert rel_err(g_node, want, 1e-3) < 1e-4
+    # edge-parallel: same gradient by atomics, same flags, schedule advanced
+    st.launch_epoch_edges(epoch, 42)
+    g_edge = st.read(M.BUF_GRAD).reshape(-1, 2)
+    assert np.array_equal(st.read(M.BUF_HAS_UPDATE), hu)
+    assert rel_err(g_edge, want, 1e-3) < 1e-4
+    assert np.array_equal(st.read(M.BUF_NEXT), orc.schedule_update(c["per"], nxt, epoch))
+
+
+# ------------------------------------------------------------------ Adam
+
+
+def _run_adam(ctx, embd, m, v, grad, hu, n, dim, params, epoch):
+    graph = [[((i + 1) % max(n, 2), 1.0)] for i in range(max(n, 2))]
+    e = np.zeros((max(n, 2), dim), np.float32)
+    e[:n] = np.asarray(embd, np.float32).reshape(n, dim)
+    st, _ = make_state(ctx, e, graph, params)
+
+    def pad(a, dt=np.float32):
+        out = np.zeros(max(n, 2) * (dim if dt == np.float32 else 1), dt)
+        a = np.asarray(a, dt).ravel()
+        out[:a.size] = a
+        return out
+
+    st.write(M.BUF_M, pad(m))
+    st.write(M.BUF_V, pad(v))
+    st.write(M.BUF_GRAD, pad(grad))
+    st.write(M.BUF_HAS_UPDATE, pad(hu, np.uint32))
+    st.launch_adam_update(epoch)
+    k = n * dim
+    return st.read(M.BUF_EMBD)[:k], st.read(M.BUF_M)[:k], st.read(M.BUF_V)[:k]
+
+
+def test_adam_inactive_node_preserved(ctx):
+    p = M.UmapOptimParams.default_2d()
+    p.beta1, p.beta2, p.n_epochs = 0.9, 0.999, 10
+    embd = np.array([1.5, -2.5, 7.0, 8.0], np.float32)
+    m = np.array([0.1, 0.2, 0.3, 0.4], np.float32)
+    v = np.array([0.01, 0.02, 0.03, 0.04], np.float32)
+    e, mm, vv = _run_adam(ctx, embd, m, v, np.full(4, 5.0), [0, 0], 2, 2, p, 3)
+    assert np.array_equal(e, embd) and np.array_equal(mm, m) and np.array_equal(vv, v)
+
+
+def test_adam_single_step_hand_computed(ctx):
+    p = M.UmapOptimParams(a=1.5, b=0.9, lr=1.0, gamma=1.0, n_epochs=10, neg_sample_rate=5,
+                          min_dist=0.1, beta1=0.9, beta2=0.999, eps=1e-7)
+    e, mm, vv = _run_adam(ctx, [0.0], [0.0], [0.0], [1.0], [1], 1, 1, p, 0)
+    assert_close(mm, [0.1], 1e-6, "m")
+    assert_close(vv, [0.001], 1e-6, "v")
+    assert_close(e, [1.0], 1e-4, "embd")
+
+
+def test_adam_matches_cpu_reference(ctx):
+    # :2244-2295
+    p = M.UmapOptimParams(a=1.5, b=0.9, lr=1.0, gamma=1.0, n_epochs=10, neg_sample_rate=5,
+                          min_dist=0.1, beta1=0.9, beta2=0.999, eps=1e-7)
+    n, dim = 5, 2
+    embd = np.arange(n * dim, dtype=np.float32) * 0.5 - 1.0
+    m = np.arange(n * dim, dtype=np.float32) * 0.01
+    v = np.arange(n * dim, dtype=np.float32) * 0.001 + 1e-6
+    grad = (((np.arange(n * dim) * 7 + 3) % 5) * 0.1).astype(np.float32)
+    hu = np.array([1, 0, 1, 1, 0], np.uint32)
+    e, mm, vv = _run_adam(ctx, embd, m, v, grad, hu, n, dim, p, 4)
+    ew, mw, vw = orc.adam_update(grad, hu, m, v, embd, p, 4, 10)
+    assert_close(e, ew, 1e-5, "embd")
+    assert_close(mm, mw, 1e-6, "m")
+    assert_close(vv, vw, 1e-6, "v")
+
+
+# -------------------------------------------------------------- schedule
+
+
+@pytest.mark.parametrize("epoch,want", [(5, [2.0, 4.0, 2.0]), (0, [1.0, 2.0, 1.0]),
+                                        (1, [2.0, 2.0, 2.0])])
+def test_schedule_goldens(ctx, epoch, want):
+    # :2342-2379
+    embd, graph, params = triangle_setup()
+    st, _ = make_state(ctx, embd, graph, params)
+    st.launch_edge_schedule_update(epoch)
+    assert list(st.read(M.BUF_NEXT)) == want
+
+
+# --------------------------------------------------------------- densMAP
+
+
+def _dens_setup(n):
+    # dens_setup, :1873-1885
+    r = np.arange(n, dtype=np.float32) - (n - 1) / 2.0
+    norm = np.sqrt((r * r).sum() / (n - 1))
+    r = (r / norm).astype(np.float32)
+    mu_sum = np.full(n, 2.0, np.float32)
+    ds = M.DensState(params=M.DensParams(lambda_=2.0, frac=1.0, var_shift=0.1),
+                     r=r.ctypes.data, mu_sum=mu_sum.ctypes.data, mu_tot=2.0 * n)
+    return ds, (r, mu_sum)
+
+
+def test_dens_radii_matches_cpu(ctx):
+    # :1900-1924
+    embd, graph, params = triangle_setup()
+    ds, keep = _dens_setup(3)
+    st, adj = make_state(ctx, embd, graph, params, ds)
+    st.launch_dens_update()
+    re_w, phi_w = orc.dens_radii(embd, oracle_csr(adj), params.a, params.b)
+    assert_close(st.read(M.BUF_DENS_RE), re_w, 1e-5, "dens-re")
+    assert_close(st.read(M.BUF_DENS_PHI), phi_w, 1e-5, "dens-phi")
+
+
+def test_grad_with_density_matches_cpu(ctx):
+    # :1926-1993
+    embd, graph, params = triangle_setup()
+    params.neg_sample_rate = 0
+    ds, (r, _) = _dens_setup(3)
+    st, adj = make_state(ctx, embd, graph, params, ds)
+    coeff = st.launch_dens_update()
+    st.launch_grad_accum(5, 42, coeff)
+    got = st.read(M.BUF_GRAD)
+    c = oracle_csr(adj)
+    re, phi = orc.dens_radii(embd, c, params.a, params.b)
+    w, coeff_w = orc.dens_weights(r, re, 0.1, 2.0, 6.0)
+    assert abs(coeff - coeff_w) <= 1e-6 * abs(coeff_w)
+    want, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 5, 42, 0, params,
+                             dict(re=re, phi_sum=phi, weight=w, coeff=coeff_w))
+    assert_close(got, want, 1e-4, "with-density")
+    st.launch_epoch_edges(5, 42, coeff)
+    assert_close(st.read(M.BUF_GRAD), want, 1e-4, "with-density edges")
+    plain, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 5, 42, 0, params)
+    assert np.max(np.abs(got - plain.ravel())) > 1e-6
+
+
+def test_dens_random_graph_stats_and_grad(ctx):
+    embd, adj = _random_graph(n=2500, seed=31)
+    rng = np.random.default_rng(31)
+    r = rng.normal(size=embd.shape[0]).astype(np.float32)
+    r = ((r - r.mean()) / r.std(ddof=1)).astype(np.float32)
+    mu_sum = np.ones_like(r)
+    ds = M.DensState(params=M.DensParams(lambda_=2.0, frac=0.3, var_shift=0.1),
+                     r=r.ctypes.data, mu_sum=mu_sum.ctypes.data, mu_tot=1234.5)
+    params = M.UmapOptimParams.default_2d()
+    st, _ = make_state(ctx, embd, adj, params, ds)
+    c = oracle_csr(adj)
+    coeff = st.launch_dens_update()
+    re, phi = orc.dens_radii(embd, c, params.a, params.b)
+    assert rel_err(st.read(M.BUF_DENS_RE), re, 1e-3) < 1e-5
+    w, coeff_w = orc.dens_weights(r, re, 0.1, 2.0, 1234.5)
+    assert abs(coeff - coeff_w) <= 1e-5 * abs(coeff_w)
+    assert rel_err(st.read(M.BUF_DENS_W), w, 1e-3) < 1e-4
+    want, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 7, 42, 5, params,
+                             dict(re=re, phi_sum=phi, weight=w, coeff=coeff_w))
+    st.launch_grad_accum(7, 42, coeff)
+    assert rel_err(st.read(M.BUF_GRAD).reshape(-1, 2), want, 1e-3) < 1e-4
+    st.launch_epoch_edges(7, 42, coeff)
+    assert rel_err(st.read(M.BUF_GRAD).reshape(-1, 2), want, 1e-3) < 1e-4
+
+
+# ------------------------------------------------------------- driver
+
+
+def test_driver_preserves_graph_structure(ctx):
+    # :2386-2448
+    graph = [[(1, 1.0), (2, 1.0)], [(0, 1.0), (2, 1.0)], [(0, 1.0), (1, 1.0), (3, 0.1)],
+             [(2, 0.1), (4, 1.0), (5, 1.0)], [(3, 1.0), (5, 1.0)], [(3, 1.0), (4, 1.0)]]
+    embd = np.array([[0, 0], [10, 0], [0, 10], [10, 10], [-5, -5], [15, 15]], np.float32)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 200
+    adj = M.adjacency_to_csr(graph)
+    for strat in (M.EPOCH_EDGE_ATOMIC, M.EPOCH_NODE_EXACT):
+        out = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=strat)
+        dist = lambda i, j: float(np.linalg.norm(out[i] - out[j]))
+        intra = (dist(0, 1) + dist(0, 2) + dist(1, 2) + dist(3, 4) + dist(3, 5) + dist(4, 5)) / 6
+        inter = np.mean([dist(i, j) for i in (0, 1) for j in (3, 4, 5)])
+        assert inter > 1.5 * intra
+        assert np.all(np.isfinite(out))
+
+
+def test_driver_node_exact_bit_identical_rerun_and_oracle(ctx):
+    # :2474-2505 and the CPU replay of the whole driver
+    embd, adj = _random_graph(n=1500, seed=41)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 30
+    a = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_NODE_EXACT)
+    b = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_NODE_EXACT)
+    assert np.array_equal(a, b)
+    want = orc.optimise_adam_gpu(embd, *adj, p, 42)
+    assert np.max(np.abs(a - want)) < 1e-3
+
+
+def test_driver_edge_atomic_matches_oracle(ctx):
+    embd, adj = _random_graph(n=1500, seed=43)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 30
+    got = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_EDGE_ATOMIC)
+    want = orc.optimise_adam_gpu(embd, *adj, p, 42)
+    assert np.max(np.abs(got - want)) < 1e-2
+
+
+def test_driver_densmap_matches_oracle(ctx):
+    embd, adj = _random_graph(n=1200, seed=45)
+    rng = np.random.default_rng(45)
+    r = rng.normal(size=embd.shape[0]).astype(np.float32)
+    mu_sum = np.ones_like(r)
+    ds = M.DensState(params=M.DensParams(lambda_=2.0, frac=0.3, var_shift=0.1),
+                     r=r.ctypes.data, mu_sum=mu_sum.ctypes.data, mu_tot=float(adj[2].sum()))
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 20
+    got = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, dens=ds, seed=42,
+                                        strategy=M.EPOCH_NODE_EXACT)
+    want = orc.optimise_adam_gpu(embd, *adj, p, 42, dens_r=r, dens_params=(2.0, 0.3, 0.1),
+                                 mu_tot=float(adj[2].sum()))
+    assert np.max(np.abs(got - want)) < 1e-3
