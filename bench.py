@@ -3,9 +3,13 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Prints ONE JSON line.  Workloads:
-  epochs   the "adam_gpu" epoch loop (500 Adam epochs, k=15-style graph of
-           1M nodes, 2-D output): one step = one full 500-epoch optimisation
-           with inputs resident in HBM.
+  fit      (default) the full umap_gpu fit of the precision form: n=1,000,000,
+           d=64 f32 (50 Gaussians on a random 8-D subspace + N(0, 0.05) noise),
+           exhaustive kNN k=15 in BF16 on MFMA + exact FP32 re-rank of a
+           64-candidate shortlist, fuzzy graph, spectral layout, 500 Adam epochs,
+           2-D output.  One step = one fit with the data resident in HBM.
+  epochs   the "adam_gpu" epoch loop alone (500 Adam epochs on a 1M-node
+           k=15-style graph): one step = one full 500-epoch optimisation.
 
 The CPU baseline is the C restatement of the reference's parallel Adam
 (optimise_embedding_adam_parallel) timed on this host's cores on a bounded
@@ -170,12 +174,144 @@ def cpu_baseline_epochs(embd0, adj, p, args):
                 measured_s=dt)
 
 
+def bench_fit(args):
+    import torch
+
+    import mrs_amd as M
+    import synthetic
+
+    n, d, k, n_epochs = args.n, 64, 15, 500
+    t0 = time.time()
+    x, _ = synthetic.subspace(n=n, d=d)
+    log(f"[bench] data {x.shape} generated in {time.time() - t0:.1f}s")
+    ctx = M.Context(0)
+    xd = torch.from_numpy(x).cuda()
+    torch.cuda.synchronize()
+    p = M.UmapParamsGpu.default()
+    p.k = k
+    p.knn_precision = M.KNN_BF16_RERANK
+    p.knn_shortlist = 4 * k
+    p.optim_params.n_epochs = n_epochs
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def step():
+        return M.umap_gpu(ctx, xd, p, seed=42)
+
+    for _ in range(args.warmup):
+        step()
+        log(f"[bench] warmup fit: {M.last_fit_timings()}")
+    torch.cuda.synchronize()
+    times, stages = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t_s = time.perf_counter()
+        out = step()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t_s) * 1e3)
+        stages.append(M.last_fit_timings())
+        log(f"[bench] fit {times[-1]:.1f} ms: {stages[-1]}")
+    ms_per_step = float(np.mean(times))
+    st = {kk: float(np.mean([s_[kk] for s_ in stages])) for kk in stages[0]}
+
+    # kNN roofline: the shortlist kernel is the dominant one; algorithmic work
+    # 2 n^2 d FLOPs per launch (one launch per fit), BF16 dense MFMA peak 2.5 PFLOP/s
+    _, _, kt = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 4 * k, timed=True)
+    flops = 2.0 * n * n * d
+    achieved = flops / (kt["shortlist_ms"] * 1e-3) / 1e12
+
+    # exact edge-updates: schedule replay of this fit's graph
+    idx, dist = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 4 * k)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
+    e0 = torch.zeros((n, 2), device="cuda")
+    stt = M.UmapGpuState(ctx, e0, params=p.optim_params, graph=g)
+    per = stt.read(M.BUF_PER)
+    A_total = int(M.schedule_active_counts(per, n_epochs).sum())
+    E = stt.n_edges
+    # epoch kernel: one profiled 500-epoch run
+    stt.reset(e0)
+    stt.set_profiling(True)
+    stt.run_epochs(0, n_epochs, 42, M.EPOCH_EDGE_ATOMIC)
+    stt.set_profiling(False)
+    ktimes = stt.kernel_times()
+    edge_ms, edge_launches = ktimes["epoch_edges"]
+    r = p.optim_params.neg_sample_rate
+    bytes_edge = (16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)) / n_epochs
+    epoch_gbs = bytes_edge / (edge_ms / edge_launches * 1e-3) / 1e9
+
+    res = dict(
+        metric=METRIC, value=ms_per_step / 1e3, unit="s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=False, scaling="weak",
+        vs_baseline=None, dtype="bf16-mfma/f32", data="synthetic",
+        config=dict(workload="umap_gpu fit, n=1,000,000 x d=64 f32 (50 Gaussians on a random "
+                    "8-D subspace + N(0,0.05)), exhaustive kNN k=15 BF16 MFMA + FP32 re-rank of "
+                    "64 candidates, spectral init, 500 Adam epochs, 2-D output",
+                    n=n, d=d, k=k, n_epochs=n_epochs, n_edges=E, edge_updates=A_total),
+        stages_ms=st,
+        edge_updates_per_s=A_total / (st["epochs_ms"] * 1e-3),
+        roofline=dict(bound="mfma", kernel="k_shortlist<bf16>", achieved=achieved, peak=2500.0,
+                      unit="TFLOP/s", frac=achieved / 2500.0, traffic=None,
+                      avg_launch_ms=kt["shortlist_ms"], algorithmic_flops_per_launch=flops),
+        epoch_kernel=dict(bound="hbm", kernel="k_epoch_edges", achieved=epoch_gbs,
+                          peak=HBM_PEAK_GBS, unit="GB/s", frac=epoch_gbs / HBM_PEAK_GBS,
+                          avg_launch_us=1e3 * edge_ms / edge_launches,
+                          algorithmic_bytes_per_launch=bytes_edge),
+    )
+    if not args.no_cpu:
+        knn_host = (idx.cpu().numpy(), dist.cpu().numpy())
+        res["cpu_baseline"] = cpu_baseline_fit(x, g, knn_host, p, args)
+    return res
+
+
+def cpu_baseline_fit(x, g, knn_host, p, args):
+    """Reference CPU path on bounded samples of the same fit: exhaustive kNN
+    (FP32 brute force, OpenMP) for Q query rows against all n, scaled by n/Q;
+    host graph (smooth_knn_dist, knn_to_coo, symmetrise, filter) on the full
+    kNN; parallel Adam (LUT pow, OpenMP) for S of 500 epochs, scaled."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as orc
+
+    import mrs_amd as M
+
+    cores = os.cpu_count() or 1
+    n = x.shape[0]
+    k = p.k
+    Q = args.cpu_queries
+    t0 = time.perf_counter()
+    orc.knn_exact(x, k, 0, Q)
+    t_knn = (time.perf_counter() - t0) * n / Q
+    # host graph stages on the full kNN (the GPU's exact result, copied to the host)
+    idx_h, dist_h = knn_host
+    t0 = time.perf_counter()
+    sg, rh = orc.smooth_knn_dist(dist_h, k)
+    rr, cc, vv = orc.knn_to_coo(idx_h, dist_h, sg, rh)
+    rr, cc, vv = orc.symmetrise(n, rr, cc, vv, 1.0)
+    orc.filter_weak_edges(rr, cc, vv, p.optim_params.n_epochs)
+    t_graph = time.perf_counter() - t0
+    r, c, v = g.coo()
+    sys.path.insert(0, str(ROOT / "tests"))
+    from helpers import coo_to_adjacency
+
+    adj = coo_to_adjacency(n, r, c, v)
+    q2 = M.UmapOptimParams.default_2d()
+    q2.n_epochs = args.cpu_epochs
+    init = np.random.default_rng(0).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    t0 = time.perf_counter()
+    orc.optimise_adam_parallel(init, *adj, q2, 42, use_lut=True, n_threads=cores)
+    t_adam = (time.perf_counter() - t0) * p.optim_params.n_epochs / args.cpu_epochs
+    return dict(value=t_knn + t_graph + t_adam, unit="s", cores=cores, kind="port",
+                sample=f"kNN: {Q} of {n} query rows x all {n} (scaled x{n / Q:g}); graph: "
+                       f"full; parallel Adam: {args.cpu_epochs} of {p.optim_params.n_epochs} "
+                       f"epochs on the same graph (scaled); spectral init not included",
+                knn_s=t_knn, graph_s=t_graph, adam_s=t_adam)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="epochs")
+    ap.add_argument("--workload", default="fit")
+    ap.add_argument("--cpu-queries", type=int, default=512)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-epochs", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -185,7 +321,9 @@ def main():
                                     "out of scope (the path would shard by query rows / node "
                                     "ranges, see DESIGN.md)", n_gpus=args.gpus)))
         sys.exit(2)
-    if args.workload == "epochs":
+    if args.workload == "fit":
+        res = bench_fit(args)
+    elif args.workload == "epochs":
         res = bench_epochs(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")

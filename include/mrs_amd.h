@@ -53,6 +53,11 @@ typedef struct {
     float min_dist, beta1, beta2, eps;
 } mrs_umap_optim_params;
 
+/* UmapGraphParams<f32>, src/data/graph.rs:30-61 (defaults 1e-5, 1.0, 1.0) */
+typedef struct {
+    float bandwidth, local_connectivity, mix_weight;
+} mrs_umap_graph_params;
+
 /* DensParams<f32>, src/utils/density.rs:69-79 */
 typedef struct {
     float lambda, frac, var_shift;
@@ -65,6 +70,25 @@ typedef struct {
     const float *mu_sum;
     float mu_tot;
 } mrs_dens_state;
+
+/* UmapParamsGpu<f32> (src/lib.rs:3218-3241) with NearestNeighbourParamsGpu's
+ * dist_metric (src/data/nearest_neighbours_gpu.rs:34-62) and this build's kNN
+ * and epoch knobs.  Fill with mrs_umap_params_gpu_default(). */
+typedef struct {
+    int32_t n_dim;
+    int32_t k;
+    const char *optimiser;       /* "adam_gpu" (default; unknown strings fall back to it) */
+    const char *ann_type;        /* "exhaustive_gpu" (approximate types run the exact search) */
+    const char *initialisation;  /* "spectral" (default) | "random" */
+    float init_range;            /* NaN = None (the reference's Option<T>) */
+    const char *dist_metric;     /* "euclidean" */
+    int32_t knn_precision;       /* MRS_KNN_FP32 | MRS_KNN_BF16_RERANK */
+    int32_t knn_shortlist;       /* re-rank shortlist (0 = 64, or 128 when k > 64) */
+    mrs_umap_graph_params umap_graph_params;
+    mrs_umap_optim_params optim_params;
+    int32_t randomised;          /* accepted for parity with UmapParamsGpu; unused */
+    int32_t epoch_strategy;      /* MRS_EPOCH_EDGE_ATOMIC (default) | MRS_EPOCH_NODE_EXACT */
+} mrs_umap_params_gpu;
 
 /* ------------------------------------------------------------------ misc */
 
@@ -93,6 +117,78 @@ int32_t mrs_parse_ann_search_gpu(const char *s);
 uint32_t mrs_hash_neg(uint32_t seed, uint32_t node, uint32_t epoch, uint32_t edge_local,
                       uint32_t neg, uint32_t n);
 
+/* ------------------------------------------------------------- fits */
+
+/* UmapParamsGpu::default (src/lib.rs:3244-3260) with ann_type "exhaustive_gpu". */
+void mrs_umap_params_gpu_default(mrs_umap_params_gpu *p);
+
+/* umap_gpu (src/lib.rs:3465-3487 -> umap_gpu_inner :3510-3708): data [n*d] f32
+ * row-major; optional precomputed kNN (knn_idx/knn_dist [n*knn_k], self
+ * excluded, squared distances; pass NULL to search); out [n_dim*n] is the
+ * transposed embedding (out[dim*n + i]), as the reference returns [n_dim][n]. */
+int mrs_umap_gpu(mrs_ctx *ctx, const float *data, int64_t n, int32_t d, const int32_t *knn_idx,
+                 const float *knn_dist, int32_t knn_k, const mrs_umap_params_gpu *params,
+                 uint64_t seed, int32_t verbose, float *out);
+/* densmap_gpu (src/lib.rs:3829-3851): the same with the density term. */
+int mrs_densmap_gpu(mrs_ctx *ctx, const float *data, int64_t n, int32_t d,
+                    const int32_t *knn_idx, const float *knn_dist, int32_t knn_k,
+                    const mrs_umap_params_gpu *params, const mrs_dens_params *dens,
+                    uint64_t seed, int32_t verbose, float *out);
+/* Stage times of the calling thread's last fit (no counterpart in the reference;
+ * its timings are verbose prints): out[0..7] = kNN ms, graph ms, densMAP state
+ * ms, init ms, epochs ms, total ms, undirected edges, kNN shortlist-kernel ms. */
+int mrs_last_fit_timings(double *out);
+
+/* ------------------------------------------------------------- kNN */
+
+/* Exhaustive kNN of run_ann_search_gpu with "exhaustive_gpu"
+ * (src/data/nearest_neighbours_gpu.rs:257-365; the k+1 self query of
+ * ann-search-rs 0.5.2 followed by the self filter and take(k) of :350-362).
+ * x [n*d] f32 row-major; idx [n*k] int32, dist [n*k] f32 squared Euclidean,
+ * ascending, self excluded, ties broken by the smaller index.  precision:
+ * MRS_KNN_FP32 or MRS_KNN_BF16_RERANK (MFMA candidates, exact FP32 re-rank of a
+ * shortlist of max(shortlist, k) rounded up to 64 or 128 candidates). */
+int mrs_knn_exhaustive(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
+                       int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
+/* Same, returning stage times in ms_out[0..4]: pack, shortlist, merge+re-rank,
+ * database splits used, shortlist size (no counterpart in the reference; used
+ * by the benchmark). */
+int mrs_knn_exhaustive_timed(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
+                             int32_t precision, int32_t shortlist, int32_t *idx, float *dist,
+                             double *ms_out);
+
+/* ------------------------------------------------------------ graph */
+
+/* The graph half of construct_umap_graph_gpu (src/lib.rs:3376-3441) on the
+ * device: smooth_knn_dist (src/data/graph.rs:499-565, bandwidth search in FP64),
+ * knn_to_coo (:585-662), symmetrise_graph (:684-694) and filter_weak_edges
+ * (:776-853), plus the epoch layout of UmapCsrGraph::from_graph
+ * (src/training/umap_optimiser_gpu.rs:132-209).  knn_idx/knn_dist are [n*k]
+ * (self excluded, ascending squared distances), host or device memory. */
+int mrs_fuzzy_graph(mrs_ctx *ctx, const int32_t *knn_idx, const float *knn_dist, int64_t n,
+                    int32_t k, const mrs_umap_graph_params *params, int32_t n_epochs,
+                    mrs_graph **out);
+int mrs_graph_destroy(mrs_graph *g);
+/* n, number of COO entries after filtering (both directions), undirected edges */
+int mrs_graph_info(const mrs_graph *g, int64_t *n, int64_t *nnz, int64_t *n_edges);
+/* the filtered CoordinateList (src/data/structures.rs:22-31): row-major, columns
+ * ascending; buffers of length nnz (host or device) */
+int mrs_graph_export_coo(const mrs_graph *g, int32_t *rows, int32_t *cols, float *vals);
+/* sigma and rho of smooth_knn_dist, [n] each */
+int mrs_graph_sigma_rho(const mrs_graph *g, float *sigma, float *rho);
+/* DensState::new (src/utils/density.rs:181-201) from the device graph and the
+ * kNN it was built from: r [n], mu_sum [n], mu_tot. */
+int mrs_dens_state_from_graph(mrs_ctx *ctx, const mrs_graph *g, const int32_t *knn_idx,
+                              const float *knn_dist, int32_t distances_squared, float *r_out,
+                              float *mu_sum_out, float *mu_tot_out);
+
+/* spectral_layout (src/data/init.rs:986-1004) of a device graph: out [n*n_dim]
+ * row-major (host or device).  range NaN = SPECTRAL_RANGE (10).  data [n*d]
+ * (optional, NULL allowed) places component centroids on the disconnected path
+ * as the reference's feature matrix does (:539-595). */
+int mrs_spectral_layout(mrs_ctx *ctx, const mrs_graph *g, int32_t n_dim, uint64_t seed,
+                        float range, const float *data, int32_t d, float *out);
+
 /* ------------------------------------------------- optimiser (adam_gpu) */
 
 /* UmapGpuState::upload (src/training/umap_optimiser_gpu.rs:374-452) preceded by
@@ -103,6 +199,12 @@ int mrs_umap_state_create(mrs_ctx *ctx, const float *embd, int64_t n, int32_t n_
                           const int64_t *adj_off, const int32_t *adj_cols,
                           const float *adj_vals, const mrs_umap_optim_params *params,
                           const mrs_dens_state *dens, mrs_umap_state **out);
+/* The same state built from a device graph (no host adjacency list): the
+ * device-side equivalent of coo_to_adjacency_list + UmapCsrGraph::from_graph +
+ * upload (src/lib.rs:3612, umap_optimiser_gpu.rs:132-209, :374-452). */
+int mrs_umap_state_from_graph(mrs_ctx *ctx, const mrs_graph *g, const float *embd,
+                              int32_t n_dim, const mrs_umap_optim_params *params,
+                              const mrs_dens_state *dens, mrs_umap_state **out);
 int mrs_umap_state_destroy(mrs_umap_state *st);
 
 /* sizes of a state: n, n_dim, n_edges */

@@ -1,0 +1,18 @@
+// Exhaustive kNN on the device (MFMA shortlist + exact FP32 re-rank).
+#pragma once
+
+#include "mrs_internal.hpp"
+
+namespace mrs {
+
+struct KnnTimings {
+    float pack_ms = 0, shortlist_ms = 0, rerank_ms = 0;
+    int splits = 0, shortlist = 0;
+};
+
+// x: device [n][d] f32; out_idx/out_dist: device [n][k].
+void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32_t k,
+                        int32_t precision, int32_t shortlist, int32_t* out_idx, float* out_dist,
+                        KnnTimings* timings);
+
+}  // namespace mrs

@@ -1,0 +1,754 @@
+// Spectral initial layout on the device: spectral_layout (src/data/init.rs:986-1004).
+//
+//   find_connected_components (:349-391)  min-label hooking + pointer jumping;
+//       a component's id is its smallest vertex, which orders components the
+//       way the reference's BFS from ascending start vertices does.
+//   graph_to_negative_normalised_adjacency (:210-256)  M = -D^-1/2 W D^-1/2
+//       (degrees accumulated per row in COO order, f64).
+//   compute_smallest_eigenpairs_lanczos (src/utils/math.rs:401-627)  Lanczos
+//       with full modified Gram-Schmidt re-orthogonalisation in f64, run for
+//       ALL components at once: vectors are global, the SpMV is block
+//       diagonal by construction, and every inner product is a segmented
+//       reduction over the component's vertices (fixed chunking, so the result
+//       is identical run to run).  The small tridiagonal problems are solved
+//       on the host (cyclic Jacobi).
+//   finalise_spectral_embedding (:905-960) / multi_component_init (:413-537)
+//       with the simplex / data-centroid spectral / scatter meta layouts
+//       (:539-766).
+//
+// The reference's StdRng streams (Lanczos start vector, noise, random fills)
+// cannot be reproduced here; counter-based hashes draw from the same
+// distributions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "spectral.hpp"
+
+namespace mrs {
+
+namespace {
+
+constexpr int kWg = 256;
+constexpr int kChunk = 4096;
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double u01(uint64_t seed, uint64_t stream, uint64_t i) {
+    uint64_t z = mix64(seed * 0x9E3779B97F4A7C15ull + stream * 0xD1B54A32D192ED03ull + i);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ __launch_bounds__(kWg) void k_row_offsets(const uint32_t* __restrict__ coo_row,
+                                                     uint64_t nnz, uint32_t n,
+                                                     uint32_t* __restrict__ off) {
+    uint32_t v = blockIdx.x * kWg + threadIdx.x;
+    if (v > n) return;
+    uint64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (coo_row[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    off[v] = (uint32_t)lo;
+}
+
+__global__ __launch_bounds__(kWg) void k_degrees(const uint32_t* __restrict__ off,
+                                                 const float* __restrict__ val, uint32_t n,
+                                                 double* __restrict__ dinv) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    double dg = 0.0;
+    for (uint32_t p = off[i]; p < off[i + 1]; p++) dg += (double)val[p];
+    dinv[i] = dg > 1e-8 ? 1.0 / sqrt(dg) : 0.0;
+}
+
+__global__ __launch_bounds__(kWg) void k_cc_init(uint32_t* __restrict__ lab, uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i < n) lab[i] = i;
+}
+
+__global__ __launch_bounds__(kWg) void k_cc_hook(const uint32_t* __restrict__ row,
+                                                 const uint32_t* __restrict__ col, uint64_t nnz,
+                                                 uint32_t* __restrict__ lab,
+                                                 uint32_t* __restrict__ changed) {
+    uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (p >= nnz) return;
+    uint32_t a = lab[row[p]], b = lab[col[p]];
+    if (a == b) return;
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    uint32_t old = atomicMin(&lab[hi], lo);
+    if (old > lo) *changed = 1u;
+}
+
+__global__ __launch_bounds__(kWg) void k_cc_jump(uint32_t* __restrict__ lab, uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    uint32_t r = lab[i];
+    while (lab[r] != r) r = lab[r];
+    lab[i] = r;
+}
+
+// y = M x with M = -D^-1/2 W D^-1/2 (self loops excluded)
+__global__ __launch_bounds__(kWg) void k_spmv(const uint32_t* __restrict__ off,
+                                              const uint32_t* __restrict__ col,
+                                              const float* __restrict__ val,
+                                              const double* __restrict__ dinv,
+                                              const double* __restrict__ x, uint32_t n,
+                                              double* __restrict__ y) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    const double di = dinv[i];
+    for (uint32_t p = off[i]; p < off[i + 1]; p++) {
+        const uint32_t j = col[p];
+        if (j == i) continue;
+        acc += (-di * (double)val[p] * dinv[j]) * x[j];
+    }
+    y[i] = acc;
+}
+
+struct Chunk {
+    uint32_t comp, begin, end;
+};
+
+__device__ inline double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kWg / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// partial[chunk] = sum over the chunk's vertices of a[i]*b[i]
+__global__ __launch_bounds__(kWg) void k_seg_dot(const Chunk* __restrict__ chunks,
+                                                 const uint32_t* __restrict__ perm,
+                                                 const double* __restrict__ a,
+                                                 const double* __restrict__ b,
+                                                 double* __restrict__ partial) {
+    __shared__ double sh[kWg];
+    const Chunk c = chunks[blockIdx.x];
+    double acc = 0.0;
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
+        const uint32_t i = perm[q];
+        acc += a[i] * b[i];
+    }
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[comp] = (sqrt?) sum of the comp's chunk partials, in chunk order
+__global__ __launch_bounds__(kWg) void k_seg_finish(const double* __restrict__ partial,
+                                                    const uint32_t* __restrict__ comp_chunk0,
+                                                    uint32_t n_comp, int do_sqrt,
+                                                    double* __restrict__ out) {
+    uint32_t c = blockIdx.x * kWg + threadIdx.x;
+    if (c >= n_comp) return;
+    double t = 0.0;
+    for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) t += partial[q];
+    out[c] = do_sqrt ? sqrt(t) : t;
+}
+
+// w[i] -= coef[comp(i)] * v[i] for live components
+__global__ __launch_bounds__(kWg) void k_axpy_seg(double* __restrict__ w,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ coef,
+                                                  const uint32_t* __restrict__ comp_of,
+                                                  const uint8_t* __restrict__ live, uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu || !live[c]) return;
+    w[i] -= coef[c] * v[i];
+}
+
+// w -= alpha v + beta_prev v_old  (the three-term recurrence, math.rs:478-483)
+__global__ __launch_bounds__(kWg) void k_recur(double* __restrict__ w, const double* __restrict__ v,
+                                               const double* __restrict__ v_old,
+                                               const double* __restrict__ alpha,
+                                               const double* __restrict__ beta_prev, int has_prev,
+                                               const uint32_t* __restrict__ comp_of,
+                                               const uint8_t* __restrict__ live, uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu || !live[c]) return;
+    double x = w[i] - alpha[c] * v[i];
+    if (has_prev) x -= beta_prev[c] * v_old[i];
+    w[i] = x;
+}
+
+// v_old = v; v = w / beta (live comps); basis copy done separately
+__global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double* __restrict__ v_old,
+                                                 const double* __restrict__ w,
+                                                 const double* __restrict__ beta,
+                                                 const uint32_t* __restrict__ comp_of,
+                                                 const uint8_t* __restrict__ live, uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu || !live[c]) return;
+    v_old[i] = v[i];
+    v[i] = w[i] / beta[c];
+}
+
+__global__ __launch_bounds__(kWg) void k_scale_seg(double* __restrict__ v,
+                                                   const double* __restrict__ s,
+                                                   const uint32_t* __restrict__ comp_of,
+                                                   uint32_t n) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu) return;
+    v[i] = s[c] > 0.0 ? v[i] / s[c] : v[i];
+}
+
+__global__ __launch_bounds__(kWg) void k_start_vec(double* __restrict__ v, uint32_t n,
+                                                   uint64_t seed) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i < n) v[i] = u01(seed, 1, i) - 0.5;
+}
+
+// evec[i] = sum_j V_j[i] * C_comp(i)[j][sel]  for sel in [0, n_sel)
+__global__ __launch_bounds__(kWg) void k_ritz(const double* __restrict__ V, uint32_t n,
+                                              int max_iter, const double* __restrict__ coef,
+                                              const uint32_t* __restrict__ coef_off,
+                                              const int* __restrict__ n_done,
+                                              const int* __restrict__ n_sel,
+                                              const uint32_t* __restrict__ comp_of, int sel,
+                                              double* __restrict__ out) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu || sel >= n_sel[c]) return;
+    const int m = n_done[c];
+    const double* C = coef + coef_off[c];  // [m][n_sel]
+    double acc = 0.0;
+    for (int j = 0; j < m; j++) acc += V[(size_t)j * n + i] * C[j * n_sel[c] + sel];
+    out[i] = acc;
+}
+
+// ---------------------------------------------------------- host helpers
+
+// eigen-decomposition of a symmetric tridiagonal matrix (cyclic Jacobi);
+// returns eigenvalues ascending and eigenvectors as columns of vecs [m][m].
+void tridiag_eig(const std::vector<double>& alpha, const std::vector<double>& beta, int m,
+                 std::vector<double>& evals, std::vector<double>& vecs) {
+    std::vector<double> A((size_t)m * m, 0.0), V((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) {
+        A[i * m + i] = alpha[i];
+        V[i * m + i] = 1.0;
+        if (i + 1 < m) A[i * m + i + 1] = A[(i + 1) * m + i] = beta[i];
+    }
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = 0; q < m; q++) {
+                tot += A[p * m + q] * A[p * m + q];
+                if (p != q) off += A[p * m + q] * A[p * m + q];
+            }
+        if (off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                const double apq = A[p * m + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; k++) {
+                    const double akp = A[k * m + p], akq = A[k * m + q];
+                    A[k * m + p] = c * akp - s * akq;
+                    A[k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double apk = A[p * m + k], aqk = A[q * m + k];
+                    A[p * m + k] = c * apk - s * aqk;
+                    A[q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double vkp = V[k * m + p], vkq = V[k * m + q];
+                    V[k * m + p] = c * vkp - s * vkq;
+                    V[k * m + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return A[a * m + a] < A[b * m + b]; });
+    evals.resize(m);
+    vecs.assign((size_t)m * m, 0.0);
+    for (int r = 0; r < m; r++) {
+        evals[r] = A[order[r] * m + order[r]];
+        for (int k = 0; k < m; k++) vecs[k * m + r] = V[k * m + order[r]];
+    }
+}
+
+// Host Lanczos for a small dense symmetric graph (the component meta graph):
+// smallest n_eig eigenvectors of -D^-1/2 W D^-1/2, as [n][n_sel] (f32 values).
+std::vector<std::vector<float>> host_small_spectral(const std::vector<double>& W, int n,
+                                                    int n_comp, uint64_t seed) {
+    std::vector<std::vector<float>> emb(n, std::vector<float>(n_comp, 0.0f));
+    if (n <= n_comp + 1) {
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < n_comp; d++) emb[i][d] = (float)(2.0 * u01(seed, 7, i * 16 + d) - 1.0);
+        return emb;
+    }
+    std::vector<double> dinv(n);
+    for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int j = 0; j < n; j++) s += W[(size_t)i * n + j];
+        dinv[i] = s > 1e-8 ? 1.0 / std::sqrt(s) : 0.0;
+    }
+    auto mv = [&](const std::vector<double>& x, std::vector<double>& y) {
+        for (int i = 0; i < n; i++) {
+            double a = 0;
+            for (int j = 0; j < n; j++)
+                if (j != i) a += (-dinv[i] * W[(size_t)i * n + j] * dinv[j]) * x[j];
+            y[i] = a;
+        }
+    };
+    const int n_eig = std::min(n_comp + 1, n);
+    const int n_iter = std::min(n_eig * 4 + 30, n);
+    std::vector<double> v(n), v_old(n, 0.0), w(n), basis;
+    for (int i = 0; i < n; i++) v[i] = u01(seed, 1, i) - 0.5;
+    double nv = 0;
+    for (double x : v) nv += x * x;
+    nv = std::sqrt(nv);
+    for (double& x : v) x /= nv;
+    std::vector<double> alpha, beta;
+    int done = 0;
+    for (int j = 0; j < n_iter; j++) {
+        basis.insert(basis.end(), v.begin(), v.end());
+        done = j + 1;
+        mv(v, w);
+        double a = 0;
+        for (int i = 0; i < n; i++) a += w[i] * v[i];
+        alpha.push_back(a);
+        for (int i = 0; i < n; i++) {
+            w[i] -= a * v[i];
+            if (j > 0) w[i] -= beta[j - 1] * v_old[i];
+        }
+        for (int k = 0; k <= j; k++) {
+            double c = 0;
+            for (int i = 0; i < n; i++) c += w[i] * basis[(size_t)k * n + i];
+            for (int i = 0; i < n; i++) w[i] -= c * basis[(size_t)k * n + i];
+        }
+        double b = 0;
+        for (double x : w) b += x * x;
+        b = std::sqrt(b);
+        beta.push_back(b);
+        if (b < 1e-12) break;
+        v_old = v;
+        for (int i = 0; i < n; i++) v[i] = w[i] / b;
+    }
+    std::vector<double> ev, vecs;
+    tridiag_eig(alpha, beta, done, ev, vecs);
+    const int n_sel = std::min(n_eig, done);
+    for (int d = 0; d < n_comp; d++) {
+        const int s = d + 1;
+        if (s < n_sel) {
+            std::vector<double> e(n, 0.0);
+            for (int j = 0; j < done; j++)
+                for (int i = 0; i < n; i++) e[i] += basis[(size_t)j * n + i] * vecs[j * done + s];
+            double nn = 0;
+            for (double x : e) nn += x * x;
+            nn = std::sqrt(nn);
+            for (int i = 0; i < n; i++) emb[i][d] = (float)(nn > 0 ? e[i] / nn : e[i]);
+        } else {
+            for (int i = 0; i < n; i++) emb[i][d] = (float)(2.0 * u01(seed + d, 3, i) - 1.0);
+        }
+    }
+    return emb;
+}
+
+__global__ __launch_bounds__(kWg) void k_centroid_partials(const Chunk* __restrict__ chunks,
+                                                           const uint32_t* __restrict__ perm,
+                                                           const float* __restrict__ x, int d,
+                                                           double* __restrict__ partial) {
+    const Chunk c = chunks[blockIdx.x];
+    for (int f = threadIdx.x; f < d; f += kWg) {
+        double acc = 0.0;
+        for (uint32_t q = c.begin; q < c.end; q++) acc += (double)x[(size_t)perm[q] * d + f];
+        partial[(size_t)blockIdx.x * d + f] = acc;
+    }
+}
+
+}  // namespace
+
+void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t seed,
+                         float range, float* out, const float* data, int32_t d) {
+    const uint32_t n = (uint32_t)g->n;
+    const uint64_t nnz = (uint64_t)g->nnz;
+    hipStream_t s = ctx->stream;
+    const int n_comp = dim;
+
+    // ---- CSR view of the filtered graph and D^-1/2
+    DevBuf<uint32_t> off((size_t)n + 1);
+    hipLaunchKernelGGL(k_row_offsets, dim3(div_up((uint64_t)n + 1, kWg)), dim3(kWg), 0, s,
+                       g->coo_row.p, nnz, n, off.p);
+    DevBuf<double> dinv(n);
+    hipLaunchKernelGGL(k_degrees, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, g->coo_val.p, n,
+                       dinv.p);
+    MRS_CHECK_LAUNCH();
+
+    // ---- connected components
+    DevBuf<uint32_t> lab(n), changed(1);
+    hipLaunchKernelGGL(k_cc_init, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, n);
+    for (int it = 0; it < 10000; it++) {
+        MRS_HIP(hipMemsetAsync(changed.p, 0, sizeof(uint32_t), s));
+        if (nnz)
+            hipLaunchKernelGGL(k_cc_hook, dim3(div_up(nnz, kWg)), dim3(kWg), 0, s, g->coo_row.p,
+                               g->coo_col.p, nnz, lab.p, changed.p);
+        hipLaunchKernelGGL(k_cc_jump, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, n);
+        MRS_CHECK_LAUNCH();
+        uint32_t ch = 0;
+        MRS_HIP(hipMemcpyAsync(&ch, changed.p, sizeof(ch), hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipStreamSynchronize(s));
+        if (!ch) break;
+    }
+    std::vector<uint32_t> hl(n);
+    MRS_HIP(hipMemcpyAsync(hl.data(), lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    MRS_HIP(hipStreamSynchronize(s));
+    // components in ascending order of their smallest vertex
+    std::vector<uint32_t> root_id(n, 0xFFFFFFFFu);
+    std::vector<uint32_t> roots;
+    for (uint32_t i = 0; i < n; i++)
+        if (hl[i] == i) {
+            root_id[i] = (uint32_t)roots.size();
+            roots.push_back(i);
+        }
+    const uint32_t ncomp = (uint32_t)roots.size();
+    std::vector<uint32_t> comp(n), size(ncomp, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        comp[i] = root_id[hl[i]];
+        size[comp[i]]++;
+    }
+    std::vector<uint32_t> start(ncomp + 1, 0);
+    for (uint32_t c = 0; c < ncomp; c++) start[c + 1] = start[c] + size[c];
+    std::vector<uint32_t> perm(n), cur(start.begin(), start.end() - 1);
+    for (uint32_t i = 0; i < n; i++) perm[cur[comp[i]]++] = i;
+
+    // which components take the spectral branch
+    const bool single = ncomp == 1;
+    std::vector<uint8_t> spectral(ncomp, 0);
+    for (uint32_t c = 0; c < ncomp; c++)
+        spectral[c] = single ? (size[c] > (uint32_t)n_comp + 1)
+                             : (size[c] >= 2u * n_comp && size[c] > (uint32_t)n_comp + 1);
+    // batched Lanczos over the spectral components
+    std::vector<uint32_t> lc_of(n, 0xFFFFFFFFu);  // local (Lanczos) component index
+    std::vector<uint32_t> lc_comp;                 // lanczos comp -> component
+    for (uint32_t c = 0; c < ncomp; c++)
+        if (spectral[c]) lc_comp.push_back(c);
+    const uint32_t nl = (uint32_t)lc_comp.size();
+    std::vector<uint32_t> comp_to_lc(ncomp, 0xFFFFFFFFu);
+    for (uint32_t q = 0; q < nl; q++) comp_to_lc[lc_comp[q]] = q;
+    for (uint32_t i = 0; i < n; i++) lc_of[i] = comp_to_lc[comp[i]];
+
+    const int n_eig = n_comp + 1;
+    std::vector<int> n_iter_c(nl), n_done(nl, 0), n_sel(nl, 0);
+    int max_iter = 0;
+    for (uint32_t q = 0; q < nl; q++) {
+        n_iter_c[q] = (int)std::min<uint32_t>(n_eig * 4 + 30, size[lc_comp[q]]);
+        max_iter = std::max(max_iter, n_iter_c[q]);
+    }
+    // per-vertex raw spectral coordinates [n][n_comp] (f32), filled per component
+    std::vector<float> raw((size_t)n * n_comp, 0.0f);
+    if (nl) {
+        std::vector<Chunk> chunks;
+        std::vector<uint32_t> chunk0(nl + 1, 0);
+        for (uint32_t q = 0; q < nl; q++) {
+            const uint32_t c = lc_comp[q];
+            chunk0[q] = (uint32_t)chunks.size();
+            for (uint32_t b = start[c]; b < start[c + 1]; b += kChunk)
+                chunks.push_back({q, b, std::min(b + kChunk, start[c + 1])});
+        }
+        chunk0[nl] = (uint32_t)chunks.size();
+        const uint32_t nch = (uint32_t)chunks.size();
+        DevBuf<Chunk> dchunks(nch);
+        DevBuf<uint32_t> dchunk0(nl + 1), dperm(n), dlc(n);
+        MRS_HIP(hipMemcpyAsync(dchunks.p, chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dchunk0.p, chunk0.data(), sizeof(uint32_t) * (nl + 1), hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dlc.p, lc_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        DevBuf<double> v(n), v_old(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
+            alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl);
+        DevBuf<uint8_t> live(nl);
+        std::vector<uint8_t> hlive(nl, 1);
+        MRS_HIP(hipMemcpyAsync(live.p, hlive.data(), nl, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemsetAsync(v_old.p, 0, v_old.bytes(), s));
+        const dim3 gn(div_up(n, kWg));
+        auto seg = [&](const double* a, const double* b, double* outp, int do_sqrt) {
+            hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, dperm.p, a, b,
+                               partial.p);
+            hipLaunchKernelGGL(k_seg_finish, dim3(div_up(nl, kWg)), dim3(kWg), 0, s, partial.p,
+                               dchunk0.p, nl, do_sqrt, outp);
+        };
+        hipLaunchKernelGGL(k_start_vec, gn, dim3(kWg), 0, s, v.p, n, seed);
+        seg(v.p, v.p, sc.p, 1);
+        hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, v.p, sc.p, dlc.p, n);
+        MRS_CHECK_LAUNCH();
+        std::vector<double> hbeta(nl);
+        for (int j = 0; j < max_iter; j++) {
+            // components past their iteration budget stop here
+            bool any = false;
+            for (uint32_t q = 0; q < nl; q++) {
+                if (hlive[q] && j >= n_iter_c[q]) hlive[q] = 0;
+                any = any || hlive[q];
+            }
+            if (!any) break;
+            MRS_HIP(hipMemcpyAsync(live.p, hlive.data(), nl, hipMemcpyHostToDevice, s));
+            MRS_HIP(hipMemcpyAsync(V.p + (size_t)j * n, v.p, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+            for (uint32_t q = 0; q < nl; q++)
+                if (hlive[q]) n_done[q] = j + 1;
+            hipLaunchKernelGGL(k_spmv, gn, dim3(kWg), 0, s, off.p, g->coo_col.p, g->coo_val.p,
+                               dinv.p, v.p, n, w.p);
+            seg(w.p, v.p, alpha_d.p + (size_t)j * nl, 0);
+            hipLaunchKernelGGL(k_recur, gn, dim3(kWg), 0, s, w.p, v.p, v_old.p,
+                               alpha_d.p + (size_t)j * nl,
+                               j > 0 ? beta_d.p + (size_t)(j - 1) * nl : beta_d.p, j > 0 ? 1 : 0,
+                               dlc.p, live.p, n);
+            for (int k = 0; k <= j; k++) {
+                seg(w.p, V.p + (size_t)k * n, sc.p, 0);
+                hipLaunchKernelGGL(k_axpy_seg, gn, dim3(kWg), 0, s, w.p, V.p + (size_t)k * n,
+                                   sc.p, dlc.p, live.p, n);
+            }
+            seg(w.p, w.p, beta_d.p + (size_t)j * nl, 1);
+            MRS_CHECK_LAUNCH();
+            MRS_HIP(hipMemcpyAsync(hbeta.data(), beta_d.p + (size_t)j * nl, sizeof(double) * nl,
+                                   hipMemcpyDeviceToHost, s));
+            MRS_HIP(hipStreamSynchronize(s));
+            hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p, v_old.p, w.p,
+                               beta_d.p + (size_t)j * nl, dlc.p, live.p, n);
+            for (uint32_t q = 0; q < nl; q++)
+                if (hlive[q] && hbeta[q] < 1e-12) hlive[q] = 0;  // breakdown
+        }
+        std::vector<double> ha((size_t)max_iter * nl), hb((size_t)max_iter * nl);
+        MRS_HIP(hipMemcpyAsync(ha.data(), alpha_d.p, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipMemcpyAsync(hb.data(), beta_d.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipStreamSynchronize(s));
+        // tridiagonal problems on the host; Ritz coefficients for the smallest pairs
+        std::vector<double> coef;
+        std::vector<uint32_t> coef_off(nl, 0);
+        for (uint32_t q = 0; q < nl; q++) {
+            const int m = n_done[q];
+            std::vector<double> al(m), be(std::max(m - 1, 0));
+            for (int j = 0; j < m; j++) al[j] = ha[(size_t)j * nl + q];
+            for (int j = 0; j + 1 < m; j++) be[j] = hb[(size_t)j * nl + q];
+            std::vector<double> ev, vecs;
+            tridiag_eig(al, be, m, ev, vecs);
+            n_sel[q] = std::min(n_eig, m);
+            coef_off[q] = (uint32_t)coef.size();
+            for (int j = 0; j < m; j++)
+                for (int t = 0; t < n_sel[q]; t++) coef.push_back(vecs[j * m + t]);
+        }
+        DevBuf<double> dcoef(coef.size() ? coef.size() : 1), evec(n);
+        DevBuf<uint32_t> dcoef_off(nl);
+        DevBuf<int> dn_done(nl), dn_sel(nl);
+        MRS_HIP(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dcoef_off.p, coef_off.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dn_done.p, n_done.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dn_sel.p, n_sel.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
+        std::vector<double> he(n);
+        for (int dcomp = 0; dcomp < n_comp; dcomp++) {
+            const int sel = dcomp + 1;  // skip the trivial smallest eigenvector
+            MRS_HIP(hipMemsetAsync(evec.p, 0, evec.bytes(), s));
+            hipLaunchKernelGGL(k_ritz, gn, dim3(kWg), 0, s, V.p, n, max_iter, dcoef.p,
+                               dcoef_off.p, dn_done.p, dn_sel.p, dlc.p, sel, evec.p);
+            seg(evec.p, evec.p, sc.p, 1);
+            hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, evec.p, sc.p, dlc.p, n);
+            MRS_CHECK_LAUNCH();
+            MRS_HIP(hipMemcpyAsync(he.data(), evec.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            MRS_HIP(hipStreamSynchronize(s));
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t q = lc_of[i];
+                if (q == 0xFFFFFFFFu) continue;
+                raw[(size_t)i * n_comp + dcomp] =
+                    sel < n_sel[q] ? (float)he[i]
+                                   : (float)(2.0 * u01(seed + dcomp, 3, i) - 1.0);
+            }
+        }
+    }
+
+    std::vector<float> emb((size_t)n * n_comp, 0.0f);
+    if (single) {
+        if (!spectral[0]) {
+            for (uint32_t i = 0; i < n; i++)
+                for (int dd = 0; dd < n_comp; dd++)
+                    emb[(size_t)i * n_comp + dd] = (float)(2.0 * u01(seed, 5, (uint64_t)i * n_comp + dd) - 1.0);
+        } else {
+            emb = raw;
+        }
+        // finalise_spectral_embedding: centre, max-abs -> range, N(0, 1e-4) noise
+        for (int dd = 0; dd < n_comp; dd++) {
+            double m = 0;
+            for (uint32_t i = 0; i < n; i++) m += emb[(size_t)i * n_comp + dd];
+            const float mean = (float)(m / n);
+            for (uint32_t i = 0; i < n; i++) emb[(size_t)i * n_comp + dd] -= mean;
+        }
+        float mx = 0.0f;
+        for (float x : emb) mx = std::max(mx, std::fabs(x));
+        if (mx > 1e-8f) {
+            const float sc = range / mx;
+            for (float& x : emb) x *= sc;
+        }
+        for (size_t q = 0; q < emb.size(); q++) {
+            const double u1 = std::max(u01(seed + 9999, 11, q), 1e-300), u2 = u01(seed + 9999, 13, q);
+            const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            emb[q] += (float)(z * 1e-4);
+        }
+    } else {
+        // component_meta_layout (:539-595)
+        std::vector<std::vector<float>> meta(ncomp, std::vector<float>(n_comp, 0.0f));
+        auto simplex = [&]() {
+            const uint32_t kk = (ncomp + 1) / 2;
+            for (auto& r : meta) std::fill(r.begin(), r.end(), 0.0f);
+            for (uint32_t l = 0; l < ncomp; l++) {
+                const uint32_t idx = l < kk ? l : l - kk;
+                if (idx < (uint32_t)n_comp) meta[l][idx] = l < kk ? 1.0f : -1.0f;
+            }
+        };
+        if (ncomp <= 2u * n_comp || !data) {
+            simplex();
+        } else if (ncomp > 2000) {
+            for (uint32_t l = 0; l < ncomp; l++)
+                for (int dd = 0; dd < n_comp; dd++)
+                    meta[l][dd] = (float)(2.0 * u01(seed, 17, (uint64_t)l * n_comp + dd) - 1.0);
+        } else {
+            // centroids in feature space, dense exp(-d^2 / max d^2) affinity, spectral
+            std::vector<Chunk> chunks;
+            std::vector<uint32_t> c0(ncomp + 1, 0);
+            for (uint32_t c = 0; c < ncomp; c++) {
+                c0[c] = (uint32_t)chunks.size();
+                for (uint32_t b = start[c]; b < start[c + 1]; b += kChunk)
+                    chunks.push_back({c, b, std::min(b + kChunk, start[c + 1])});
+            }
+            c0[ncomp] = (uint32_t)chunks.size();
+            DevBuf<Chunk> dch(chunks.size());
+            DevBuf<uint32_t> dperm(n);
+            DevBuf<double> part(chunks.size() * (size_t)d);
+            MRS_HIP(hipMemcpyAsync(dch.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice, s));
+            MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_centroid_partials, dim3((unsigned)chunks.size()), dim3(kWg), 0, s,
+                               dch.p, dperm.p, data, d, part.p);
+            MRS_CHECK_LAUNCH();
+            std::vector<double> hp(chunks.size() * (size_t)d);
+            MRS_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+            MRS_HIP(hipStreamSynchronize(s));
+            std::vector<double> cen((size_t)ncomp * d, 0.0);
+            for (uint32_t c = 0; c < ncomp; c++) {
+                for (uint32_t q = c0[c]; q < c0[c + 1]; q++)
+                    for (int f = 0; f < d; f++) cen[(size_t)c * d + f] += hp[(size_t)q * d + f];
+                for (int f = 0; f < d; f++) cen[(size_t)c * d + f] /= (double)size[c];
+            }
+            std::vector<double> W((size_t)ncomp * ncomp, 0.0);
+            double maxd2 = 0;
+            for (uint32_t a = 0; a < ncomp; a++)
+                for (uint32_t b = a + 1; b < ncomp; b++) {
+                    double d2 = 0;
+                    for (int f = 0; f < d; f++) {
+                        double df = cen[(size_t)a * d + f] - cen[(size_t)b * d + f];
+                        d2 += df * df;
+                    }
+                    W[(size_t)a * ncomp + b] = W[(size_t)b * ncomp + a] = d2;
+                    maxd2 = std::max(maxd2, d2);
+                }
+            const double scl = maxd2 > 1e-12 ? 1.0 / maxd2 : 1.0;
+            for (uint32_t a = 0; a < ncomp; a++)
+                for (uint32_t b = 0; b < ncomp; b++)
+                    W[(size_t)a * ncomp + b] = a == b ? 0.0 : std::exp(-W[(size_t)a * ncomp + b] * scl);
+            auto e = host_small_spectral(W, (int)ncomp, n_comp, seed);
+            for (uint32_t l = 0; l < ncomp; l++) meta[l] = e[l];
+        }
+        float mx = 0.0f;
+        for (auto& r : meta)
+            for (float x : r) mx = std::max(mx, std::fabs(x));
+        if (mx <= 1e-8f) {
+            simplex();
+            mx = 0.0f;
+            for (auto& r : meta)
+                for (float x : r) mx = std::max(mx, std::fabs(x));
+        }
+        if (mx > 1e-8f)
+            for (auto& r : meta)
+                for (float& x : r) x /= mx;
+        // per component placement around its centroid (multi_component_init :439-484)
+        for (uint32_t c = 0; c < ncomp; c++) {
+            float dr = 0.0f;  // meta_data_range: half the largest centroid distance
+            for (uint32_t o = 0; o < ncomp; o++) {
+                float d2 = 0;
+                for (int dd = 0; dd < n_comp; dd++) {
+                    float df = meta[c][dd] - meta[o][dd];
+                    d2 += df * df;
+                }
+                dr = std::max(dr, std::sqrt(d2));
+            }
+            dr /= 2.0f;
+            if (!spectral[c]) {
+                for (uint32_t q = start[c]; q < start[c + 1]; q++) {
+                    const uint32_t i = perm[q];
+                    for (int dd = 0; dd < n_comp; dd++) {
+                        const float u = (float)(2.0 * u01(seed, 19, (uint64_t)i * n_comp + dd) - 1.0);
+                        emb[(size_t)i * n_comp + dd] = meta[c][dd] + u * dr;
+                    }
+                }
+                continue;
+            }
+            float ma = 0.0f;
+            for (uint32_t q = start[c]; q < start[c + 1]; q++)
+                for (int dd = 0; dd < n_comp; dd++)
+                    ma = std::max(ma, std::fabs(raw[(size_t)perm[q] * n_comp + dd]));
+            const float expansion = ma > 1e-8f ? dr / ma : 1.0f;
+            for (uint32_t q = start[c]; q < start[c + 1]; q++) {
+                const uint32_t i = perm[q];
+                for (int dd = 0; dd < n_comp; dd++)
+                    emb[(size_t)i * n_comp + dd] =
+                        meta[c][dd] + raw[(size_t)i * n_comp + dd] * expansion;
+            }
+        }
+        for (int dd = 0; dd < n_comp; dd++) {
+            double m = 0;
+            for (uint32_t i = 0; i < n; i++) m += emb[(size_t)i * n_comp + dd];
+            const float mean = (float)(m / n);
+            for (uint32_t i = 0; i < n; i++) emb[(size_t)i * n_comp + dd] -= mean;
+        }
+        float m2 = 0.0f;
+        for (float x : emb) m2 = std::max(m2, std::fabs(x));
+        if (m2 > 1e-8f) {
+            const float sc = range / m2;
+            for (float& x : emb) x *= sc;
+        }
+    }
+    MRS_HIP(hipMemcpyAsync(out, emb.data(), sizeof(float) * emb.size(), hipMemcpyHostToDevice, s));
+    MRS_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace mrs
+
+extern "C" int mrs_spectral_layout(mrs_ctx* ctx, const mrs_graph* g, int32_t n_dim, uint64_t seed,
+                                   float range, const float* data, int32_t d, float* out) {
+    return mrs::guarded([&] {
+        if (!ctx || !g || !out) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (n_dim <= 0 || n_dim > 16) throw mrs::Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
+        hipStream_t s = ctx->stream;
+        mrs::DevView<float> xv(data, data ? (size_t)g->n * d : 0, s);
+        mrs::DevView<float> ov(out, (size_t)g->n * n_dim, s, false);
+        mrs::spectral_layout_dev(ctx, g, n_dim, seed, std::isnan(range) ? 10.0f : range, ov.p,
+                                 xv.p, data ? d : 0);
+        ov.download(s);
+        MRS_HIP(hipStreamSynchronize(s));
+    });
+}

@@ -93,6 +93,24 @@ class UmapGraphParams(C.Structure):
         return cls(bandwidth=1e-5, local_connectivity=1.0, mix_weight=1.0)
 
 
+class UmapParamsGpu(C.Structure):
+    """UmapParamsGpu<f32> (src/lib.rs:3218-3241) + kNN/epoch knobs of this build."""
+
+    _fields_ = [
+        ("n_dim", C.c_int32), ("k", C.c_int32), ("optimiser", C.c_char_p),
+        ("ann_type", C.c_char_p), ("initialisation", C.c_char_p), ("init_range", C.c_float),
+        ("dist_metric", C.c_char_p), ("knn_precision", C.c_int32), ("knn_shortlist", C.c_int32),
+        ("umap_graph_params", UmapGraphParams), ("optim_params", UmapOptimParams),
+        ("randomised", C.c_int32), ("epoch_strategy", C.c_int32),
+    ]
+
+    @classmethod
+    def default(cls) -> "UmapParamsGpu":
+        p = cls()
+        lib().mrs_umap_params_gpu_default(C.byref(p))
+        return p
+
+
 _lib = None
 
 
@@ -143,6 +161,26 @@ def lib() -> C.CDLL:
         "mrs_schedule_active_counts": (C.c_int, [vp, i64, i32, vp]),
         "mrs_umap_state_kernel_times": (C.c_int, [vp, C.POINTER(C.c_double),
                                                   C.POINTER(C.c_int64)]),
+        "mrs_knn_exhaustive": (C.c_int, [vp, vp, i64, i32, i32, i32, i32, vp, vp]),
+        "mrs_knn_exhaustive_timed": (C.c_int, [vp, vp, i64, i32, i32, i32, i32, vp, vp,
+                                               C.POINTER(C.c_double)]),
+        "mrs_fuzzy_graph": (C.c_int, [vp, vp, vp, i64, i32, C.POINTER(UmapGraphParams), i32,
+                                      C.POINTER(vp)]),
+        "mrs_graph_destroy": (C.c_int, [vp]),
+        "mrs_graph_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
+        "mrs_graph_export_coo": (C.c_int, [vp, vp, vp, vp]),
+        "mrs_graph_sigma_rho": (C.c_int, [vp, vp, vp]),
+        "mrs_dens_state_from_graph": (C.c_int, [vp, vp, vp, vp, i32, vp, vp,
+                                                C.POINTER(C.c_float)]),
+        "mrs_umap_state_from_graph": (C.c_int, [vp, vp, vp, i32, C.POINTER(UmapOptimParams),
+                                                C.POINTER(DensState), C.POINTER(vp)]),
+        "mrs_umap_params_gpu_default": (None, [C.POINTER(UmapParamsGpu)]),
+        "mrs_spectral_layout": (C.c_int, [vp, vp, i32, u64, C.c_float, vp, i32, vp]),
+        "mrs_umap_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, i32, C.POINTER(UmapParamsGpu), u64,
+                                   i32, vp]),
+        "mrs_densmap_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, i32, C.POINTER(UmapParamsGpu),
+                                      C.POINTER(DensParams), u64, i32, vp]),
+        "mrs_last_fit_timings": (C.c_int, [vp]),
         "mrs_optimise_embedding_adam_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, vp,
                                                       C.POINTER(UmapOptimParams),
                                                       C.POINTER(DensState), u64, i32, i32]),
@@ -225,19 +263,124 @@ BUF_OFFSETS, BUF_EDGE_IDX, BUF_OTHER, BUF_PER, BUF_NEXT, BUF_MU = 5, 6, 7, 8, 9,
 BUF_DENS_RE, BUF_DENS_PHI, BUF_DENS_W, BUF_DENS_R, BUF_EDGE_IJ = 11, 12, 13, 14, 15
 
 
+def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist: int = 0,
+                   timed: bool = False):
+    """run_ann_search_gpu("exhaustive_gpu"): (idx [n,k] int32, dist [n,k] f32 squared).
+
+    x may be a numpy array (results returned as numpy) or a device tensor
+    (results returned as device tensors)."""
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, d = x.shape
+        idx = np.empty((n, k), np.int32)
+        dist = np.empty((n, k), np.float32)
+    else:
+        import torch
+
+        n, d = x.shape
+        idx = torch.empty((n, k), dtype=torch.int32, device=x.device)
+        dist = torch.empty((n, k), dtype=torch.float32, device=x.device)
+    if timed:
+        ms = (C.c_double * 5)()
+        _check(lib().mrs_knn_exhaustive_timed(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
+                                              _ptr(idx), _ptr(dist), ms))
+        return idx, dist, dict(pack_ms=ms[0], shortlist_ms=ms[1], rerank_ms=ms[2],
+                               splits=int(ms[3]), shortlist=int(ms[4]))
+    _check(lib().mrs_knn_exhaustive(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
+                                    _ptr(idx), _ptr(dist)))
+    return idx, dist
+
+
+class FuzzyGraph:
+    """The device graph of construct_umap_graph_gpu (src/lib.rs:3376-3441)."""
+
+    def __init__(self, ctx: Context, knn_idx, knn_dist, params: UmapGraphParams | None = None,
+                 n_epochs: int = 500):
+        params = params or UmapGraphParams.default()
+        if isinstance(knn_idx, np.ndarray):
+            knn_idx = np.ascontiguousarray(knn_idx, dtype=np.int32)
+            knn_dist = np.ascontiguousarray(knn_dist, dtype=np.float32)
+        n, k = knn_idx.shape
+        self.ctx, self.k = ctx, k
+        h = C.c_void_p()
+        _check(lib().mrs_fuzzy_graph(ctx.handle, _ptr(knn_idx), _ptr(knn_dist), n, k,
+                                     C.byref(params), n_epochs, C.byref(h)))
+        self.handle = h
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().mrs_graph_info(h, C.byref(a), C.byref(b), C.byref(c)))
+        self.n, self.nnz, self.n_edges = a.value, b.value, c.value
+
+    def coo(self):
+        """(rows, cols, vals) of the filtered CoordinateList."""
+        r = np.empty(self.nnz, np.int32)
+        c = np.empty(self.nnz, np.int32)
+        v = np.empty(self.nnz, np.float32)
+        _check(lib().mrs_graph_export_coo(self.handle, _ptr(r), _ptr(c), _ptr(v)))
+        return r, c, v
+
+    def sigma_rho(self):
+        s = np.empty(self.n, np.float32)
+        r = np.empty(self.n, np.float32)
+        _check(lib().mrs_graph_sigma_rho(self.handle, _ptr(s), _ptr(r)))
+        return s, r
+
+    def dens_state(self, knn_idx, knn_dist, distances_squared: bool = True):
+        """DensState::new on the device: (r, mu_sum, mu_tot)."""
+        if isinstance(knn_idx, np.ndarray):
+            knn_idx = np.ascontiguousarray(knn_idx, dtype=np.int32)
+            knn_dist = np.ascontiguousarray(knn_dist, dtype=np.float32)
+        r = np.empty(self.n, np.float32)
+        ms = np.empty(self.n, np.float32)
+        tot = C.c_float()
+        _check(lib().mrs_dens_state_from_graph(self.ctx.handle, self.handle, _ptr(knn_idx),
+                                               _ptr(knn_dist), 1 if distances_squared else 0,
+                                               _ptr(r), _ptr(ms), C.byref(tot)))
+        return r, ms, tot.value
+
+    def spectral_layout(self, n_dim: int = 2, seed: int = 42, range_: float = float("nan"),
+                        data=None) -> np.ndarray:
+        """spectral_layout (src/data/init.rs:986-1004) of this graph: [n, n_dim]."""
+        out = np.empty((self.n, n_dim), np.float32)
+        d = 0
+        if data is not None:
+            if isinstance(data, np.ndarray):
+                data = np.ascontiguousarray(data, dtype=np.float32)
+            d = data.shape[1]
+        _check(lib().mrs_spectral_layout(self.ctx.handle, self.handle, n_dim, seed, range_,
+                                         _ptr(data), d, _ptr(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().mrs_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class UmapGpuState:
     """UmapGpuState (src/training/umap_optimiser_gpu.rs:278-452) on the device."""
 
-    def __init__(self, ctx: Context, embd: np.ndarray, adj_off, adj_cols, adj_vals,
-                 params: UmapOptimParams, dens: DensState | None = None):
+    def __init__(self, ctx: Context, embd: np.ndarray, adj_off=None, adj_cols=None,
+                 adj_vals=None, params: UmapOptimParams | None = None,
+                 dens: DensState | None = None, graph: FuzzyGraph | None = None):
         embd = np.ascontiguousarray(embd, dtype=np.float32) if isinstance(embd, np.ndarray) else embd
         n, dim = embd.shape
         self.ctx = ctx
-        self._keep = (embd, adj_off, adj_cols, adj_vals, dens)
+        self._keep = (embd, adj_off, adj_cols, adj_vals, dens, graph)
         h = C.c_void_p()
-        _check(lib().mrs_umap_state_create(
-            ctx.handle, _ptr(embd), n, dim, _ptr(adj_off), _ptr(adj_cols), _ptr(adj_vals),
-            C.byref(params), C.byref(dens) if dens is not None else None, C.byref(h)))
+        if graph is not None:
+            _check(lib().mrs_umap_state_from_graph(
+                ctx.handle, graph.handle, _ptr(embd), dim, C.byref(params),
+                C.byref(dens) if dens is not None else None, C.byref(h)))
+        else:
+            _check(lib().mrs_umap_state_create(
+                ctx.handle, _ptr(embd), n, dim, _ptr(adj_off), _ptr(adj_cols), _ptr(adj_vals),
+                C.byref(params), C.byref(dens) if dens is not None else None, C.byref(h)))
         self.handle = h
         n_, d_, e_ = C.c_int64(), C.c_int32(), C.c_int64()
         _check(lib().mrs_umap_state_info(h, C.byref(n_), C.byref(d_), C.byref(e_)))
@@ -348,3 +491,46 @@ def schedule_active_counts(epochs_per_sample, n_epochs: int) -> np.ndarray:
     out = np.zeros(n_epochs, np.int64)
     _check(lib().mrs_schedule_active_counts(_ptr(per), per.size, n_epochs, _ptr(out)))
     return out
+
+
+def umap_gpu(ctx: Context, data, params: UmapParamsGpu | None = None, precomputed_knn=None,
+             seed: int = 42, verbose: int = 0, dens: DensParams | None = None):
+    """umap_gpu / densmap_gpu (src/lib.rs:3465, :3829): returns [n_dim, n] (numpy for host
+    input, device tensor for device input)."""
+    params = params or UmapParamsGpu.default()
+    on_host = data is None or isinstance(data, np.ndarray)
+    if data is not None and on_host:
+        data = np.ascontiguousarray(data, dtype=np.float32)
+    if data is not None:
+        n, d = data.shape
+    else:
+        n, d = precomputed_knn[0].shape[0], 0
+    kidx = kdist = None
+    kk = 0
+    if precomputed_knn is not None:
+        kidx, kdist = precomputed_knn
+        if isinstance(kidx, np.ndarray):
+            kidx = np.ascontiguousarray(kidx, dtype=np.int32)
+            kdist = np.ascontiguousarray(kdist, dtype=np.float32)
+        kk = kidx.shape[1]
+    if on_host:
+        out = np.empty((params.n_dim, n), np.float32)
+    else:
+        import torch
+
+        out = torch.empty((params.n_dim, n), dtype=torch.float32, device=data.device)
+    if dens is None:
+        _check(lib().mrs_umap_gpu(ctx.handle, _ptr(data), n, d, _ptr(kidx), _ptr(kdist), kk,
+                                  C.byref(params), seed, verbose, _ptr(out)))
+    else:
+        _check(lib().mrs_densmap_gpu(ctx.handle, _ptr(data), n, d, _ptr(kidx), _ptr(kdist), kk,
+                                     C.byref(params), C.byref(dens), seed, verbose, _ptr(out)))
+    return out
+
+
+def last_fit_timings() -> dict:
+    out = (C.c_double * 8)()
+    _check(lib().mrs_last_fit_timings(out))
+    keys = ("knn_ms", "graph_ms", "dens_ms", "init_ms", "epochs_ms", "total_ms", "n_edges",
+            "knn_shortlist_ms")
+    return dict(zip(keys, list(out)))

@@ -1,0 +1,161 @@
+"""End-to-end umap_gpu / densmap_gpu on the device, against the reference's
+integration tests (tests/umap_gpu_tests.rs, tests/densmap_gpu_tests.rs) and
+against the CPU oracle on the same graph and initial layout.
+
+Bar: trustworthiness@15 of the GPU fit within 0.005 of the CPU optimiser
+started from the same layout on the oracle's graph.
+"""
+import numpy as np
+import pytest
+
+import mrs_amd as M
+import pyoracle as orc
+from helpers import coo_to_adjacency, umap_graph_from_data
+from metrics import spearman, trustworthiness
+
+pytestmark = pytest.mark.gpu
+
+
+def diagnostic_data(n_per_cluster, n_dim, seed):
+    """create_diagnostic_data (tests/commons/mod.rs:15-52): 5 centres, uniform +-1.5."""
+    rng = np.random.default_rng(seed)
+    centres = np.zeros((5, n_dim))
+    centres[1, 0] = 5.0
+    centres[2, 1] = 5.0
+    centres[3, 2] = 5.0
+    centres[4, :] = 10.0
+    x = np.concatenate([c + rng.uniform(-1.5, 1.5, size=(n_per_cluster, n_dim)) for c in centres])
+    labels = np.repeat(np.arange(5), n_per_cluster)
+    return x.astype(np.float32), labels
+
+
+def variable_density(n_per_cluster, n_dim, sds, seed):
+    rng = np.random.default_rng(seed)
+    xs, ls = [], []
+    for c, sd in enumerate(sds):
+        centre = np.zeros(n_dim)
+        centre[c % n_dim] = 20.0 * (c + 1)
+        xs.append(centre + rng.normal(scale=sd, size=(n_per_cluster, n_dim)))
+        ls.append(np.full(n_per_cluster, c))
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ls)
+
+
+def params(**kw):
+    p = M.UmapParamsGpu.default()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_fit_finite_and_separated(ctx):
+    # umap_gpu_integration_04, tests/umap_gpu_tests.rs:161-231
+    x, lab = diagnostic_data(50, 10, 123)
+    e = M.umap_gpu(ctx, x, params(), seed=42)
+    assert e.shape == (2, 250) and np.all(np.isfinite(e))
+    cen = np.stack([e[:, lab == c].mean(1) for c in range(5)])
+    dmin = min(np.linalg.norm(cen[a] - cen[b]) for a in range(5) for b in range(a + 1, 5))
+    intra = np.mean([np.linalg.norm(e[:, lab == c].T - cen[c], axis=1).mean() for c in range(5)])
+    assert dmin > 0.5 and dmin / intra > 0.3
+
+
+def test_fit_reproducible_and_precomputed_knn(ctx):
+    # umap_gpu_integration_05/06 (:234-307), with the deterministic epoch strategy
+    x, _ = diagnostic_data(50, 10, 42)
+    p = params(epoch_strategy=M.EPOCH_NODE_EXACT)
+    a = M.umap_gpu(ctx, x, p, seed=42)
+    b = M.umap_gpu(ctx, x, p, seed=42)
+    assert np.array_equal(a, b)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    c = M.umap_gpu(ctx, x, p, precomputed_knn=(idx, dist), seed=42)
+    assert np.max(np.abs(a - c)) < 1e-6
+    # the fast strategy stays close to the deterministic one
+    f = M.umap_gpu(ctx, x, params(), seed=42)
+    assert np.all(np.isfinite(f))
+
+
+@pytest.mark.parametrize("precision", [M.KNN_FP32, M.KNN_BF16_RERANK])
+def test_fit_trustworthiness_matches_cpu_optimiser(ctx, precision):
+    rng = np.random.default_rng(7)
+    n, d = 2500, 20
+    centres = rng.normal(0, 6, size=(8, d))
+    lab = rng.integers(0, 8, size=n)
+    x = (centres[lab] + rng.normal(size=(n, d))).astype(np.float32)
+    p = params(knn_precision=precision, epoch_strategy=M.EPOCH_EDGE_ATOMIC)
+    p.optim_params.n_epochs = 300
+    e_gpu = M.umap_gpu(ctx, x, p, seed=42).T
+    # CPU: oracle kNN + graph, the device's spectral layout of that same graph
+    idx, dist, (r, c, v), adj = umap_graph_from_data(x, 15, n_epochs=300)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=300)
+    init = g.spectral_layout(2, seed=42, data=x)
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = 300
+    e_cpu = orc.optimise_adam_parallel(init, *adj, q, 42)
+    t_gpu = trustworthiness(x, e_gpu, 15)
+    t_cpu = trustworthiness(x, e_cpu, 15)
+    assert t_gpu >= t_cpu - 0.005, (t_gpu, t_cpu)
+
+
+def test_fit_gpu_replay_matches_oracle_driver(ctx):
+    # same graph + same layout: the device fit equals the oracle's replay of
+    # optimise_embedding_adam_gpu (deterministic strategy)
+    x, _ = diagnostic_data(60, 8, 5)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    p = params(epoch_strategy=M.EPOCH_NODE_EXACT)
+    p.optim_params.n_epochs = 60
+    e = M.umap_gpu(ctx, x, p, precomputed_knn=(idx, dist), seed=42).T
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=60)
+    init = g.spectral_layout(2, seed=42, data=x)
+    r, c, v = g.coo()
+    adj = coo_to_adjacency(g.n, r, c, v)
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = 60
+    want = orc.optimise_adam_gpu(init, *adj, q, 42)
+    assert np.max(np.abs(e - want)) < 1e-3
+
+
+def test_spectral_layout_properties(ctx):
+    x, _ = diagnostic_data(60, 10, 3)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    g = M.FuzzyGraph(ctx, idx, dist)
+    emb = g.spectral_layout(2, seed=1, data=x)
+    assert emb.shape == (300, 2) and np.all(np.isfinite(emb))
+    assert np.max(np.abs(emb)) <= 10.0 + 1e-3  # SPECTRAL_RANGE (+ the 1e-4 noise)
+    assert np.array_equal(emb, g.spectral_layout(2, seed=1, data=x))
+
+
+def test_densmap_lambda_zero_matches_umap(ctx):
+    # densmap_gpu_integration_02 (tests/densmap_gpu_tests.rs:173-219)
+    x, _ = diagnostic_data(40, 10, 42)
+    p = params(epoch_strategy=M.EPOCH_NODE_EXACT)
+    plain = M.umap_gpu(ctx, x, p, seed=42)
+    dens = M.umap_gpu(ctx, x, p, seed=42, dens=M.DensParams(lambda_=0.0, frac=0.3, var_shift=0.1))
+    assert np.max(np.abs(plain - dens)) < 1e-4
+
+
+def test_densmap_finite_and_preserves_density(ctx):
+    # densmap_gpu_integration_01/03
+    x, lab = variable_density(60, 10, [0.2, 1.0, 4.0], 42)
+    e20 = M.umap_gpu(ctx, x, params(), seed=42, dens=M.DensParams(lambda_=20.0, frac=0.3, var_shift=0.1))
+    assert np.all(np.isfinite(e20))
+    plain = M.umap_gpu(ctx, x, params(), seed=42).T
+    dens = M.umap_gpu(ctx, x, params(), seed=42, dens=M.DensParams.densmap_default()).T
+
+    def radii(emb):
+        return np.array([np.linalg.norm(emb[lab == c] - emb[lab == c].mean(0), axis=1).mean()
+                         for c in range(3)])
+
+    orig = radii(x)
+    corr_p = np.corrcoef(np.log(orig), np.log(radii(plain)))[0, 1]
+    corr_d = np.corrcoef(np.log(orig), np.log(radii(dens)))[0, 1]
+    assert corr_d > corr_p
+
+
+def test_fit_errors(ctx):
+    x = np.zeros((10, 3), np.float32)
+    p = params(optimiser=b"sgd")
+    with pytest.raises(M.ManifoldsError) as e:
+        M.umap_gpu(ctx, x, p)
+    assert e.value.status == M.MRS_INVALID_ARGUMENT
+    with pytest.raises(M.ManifoldsError) as e:
+        M.umap_gpu(ctx, np.zeros((0, 3), np.float32), params())
+    assert e.value.status == M.MRS_NO_DATA

@@ -1,0 +1,127 @@
+"""GPU parity of the device graph construction (smooth_knn_dist, knn_to_coo,
+fuzzy union, filter, CSR/edge layout, DensState) against the CPU oracle.
+
+Structure (rows, columns, CSR, edge order) is bit-exact; weights are within
+1e-5 relative (device expf vs libm expf); sigma within 1e-4 relative because
+the device carries the bandwidth search in FP64 (the reference in f32).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mrs_amd as M
+import pyoracle as orc
+from helpers import blobs, coo_to_adjacency
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_graph(idx, dist, sigma, rho, mix, n_epochs):
+    n = idx.shape[0]
+    r, c, v = orc.knn_to_coo(idx, dist, sigma, rho)
+    r, c, v = orc.symmetrise(n, r, c, v, mix)
+    return orc.filter_weak_edges(r, c, v, n_epochs)
+
+
+def _knn(n=2000, d=8, k=15, seed=3):
+    x, _ = blobs(n, d, 5, seed=seed)
+    return orc.knn_exact(x, k)
+
+
+@pytest.mark.parametrize("mix", [1.0, 0.5, 0.0])
+def test_graph_matches_oracle(ctx, mix):
+    idx, dist = _knn()
+    gp = M.UmapGraphParams(bandwidth=1e-5, local_connectivity=1.0, mix_weight=mix)
+    g = M.FuzzyGraph(ctx, idx, dist, gp, n_epochs=500)
+    sg, rh = g.sigma_rho()
+    so, ro = orc.smooth_knn_dist(dist, idx.shape[1])
+    assert np.array_equal(rh, ro)
+    assert np.max(np.abs(sg - so) / so) < 1e-4
+    # the defining invariant holds for the device sigma
+    a = np.maximum(dist.astype(np.float64) - rh[:, None], 0.0)
+    val = np.exp(-a / sg[:, None].astype(np.float64)).sum(1)
+    assert np.mean(np.abs(val - math.log(idx.shape[1])) < 2e-5) > 0.99
+    # same sigma/rho through the oracle: identical structure, weights to expf ulps
+    r, c, v = g.coo()
+    orw, ocl, ovl = _oracle_graph(idx, dist, sg, rh, mix, 500)
+    assert np.array_equal(r, orw) and np.array_equal(c, ocl)
+    np.testing.assert_allclose(v, ovl, rtol=1e-5, atol=0)
+    # and against the oracle's own sigma the graph agrees to the search tolerance
+    orw2, ocl2, ovl2 = _oracle_graph(idx, dist, so, ro, mix, 500)
+    if np.array_equal(r, orw2) and np.array_equal(c, ocl2):
+        np.testing.assert_allclose(v, ovl2, rtol=1e-3)
+
+
+def test_graph_csr_layout_matches_host_from_graph(ctx):
+    idx, dist = _knn(seed=5)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r, c, v = g.coo()
+    adj = coo_to_adjacency(g.n, r, c, v)
+    p = M.UmapOptimParams.default_2d()
+    embd = np.zeros((g.n, 2), np.float32)
+    s_dev = M.UmapGpuState(ctx, embd, params=p, graph=g)
+    s_host = M.UmapGpuState(ctx, embd, *adj, p)
+    assert s_dev.n_edges == s_host.n_edges == g.n_edges
+    for b in (M.BUF_OFFSETS, M.BUF_EDGE_IDX, M.BUF_OTHER, M.BUF_PER, M.BUF_MU, M.BUF_EDGE_IJ,
+              M.BUF_NEXT):
+        assert np.array_equal(s_dev.read(b), s_host.read(b)), b
+
+
+def test_graph_hub_rows_take_the_long_sort_paths(ctx):
+    # node 0 sits in every row (in-degree n-1 > 2048) and nodes 1..40 in many:
+    # exercises the LDS and global-memory row sorts
+    rng = np.random.default_rng(9)
+    n, k = 3000, 12
+    idx = np.zeros((n, k), np.int32)
+    for i in range(n):
+        cand = [] if i == 0 else [0]
+        h = int(rng.integers(1, 41))
+        if h != i:
+            cand.append(h)
+        pool = np.setdiff1d(np.arange(n), [i] + cand)
+        cand += list(rng.choice(pool, k - len(cand), replace=False))
+        idx[i] = cand
+    dist = np.sort(rng.uniform(0.1, 5.0, size=(n, k)).astype(np.float32), axis=1)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=200)
+    sg, rh = g.sigma_rho()
+    r, c, v = g.coo()
+    orw, ocl, ovl = _oracle_graph(idx, dist, sg, rh, 1.0, 200)
+    assert np.array_equal(r, orw) and np.array_equal(c, ocl)
+    np.testing.assert_allclose(v, ovl, rtol=1e-5)
+    assert np.sum(r == 0) > 2048
+
+
+def test_graph_duplicates_and_zero_distances(ctx):
+    x, _ = blobs(800, 4, 3, seed=7)
+    x[100:140] = x[100]  # 40 identical points: zero distances, rho = 0
+    idx, dist = orc.knn_exact(x, 10)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    sg, rh = g.sigma_rho()
+    r, c, v = g.coo()
+    orw, ocl, ovl = _oracle_graph(idx, dist, sg, rh, 1.0, 500)
+    assert np.array_equal(r, orw) and np.array_equal(c, ocl)
+    np.testing.assert_allclose(v, ovl, rtol=1e-5)
+
+
+def test_dens_state_matches_oracle(ctx):
+    idx, dist = _knn(seed=11)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r_d, ms_d, tot_d = g.dens_state(idx, dist, True)
+    rr, cc, vv = g.coo()
+    st, r_o, ms_o, tot_o = orc.dens_state(g.n, rr, cc, vv, idx, dist, True)
+    assert st == 0
+    np.testing.assert_allclose(ms_d, ms_o, rtol=1e-6)
+    np.testing.assert_allclose(r_d, r_o, rtol=1e-5, atol=1e-6)
+    assert abs(tot_d - tot_o) <= 1e-6 * tot_o
+
+
+def test_dens_state_degenerate(ctx):
+    # a ring where every point sees its two neighbours at the same distance
+    n = 50
+    idx = np.stack([(np.arange(n) + 1) % n, (np.arange(n) - 1) % n], 1).astype(np.int32)
+    dist = np.ones((n, 2), np.float32)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    with pytest.raises(M.ManifoldsError) as e:
+        g.dens_state(idx, dist, True)
+    assert e.value.status == M.MRS_DEGENERATE_LOCAL_RADII
