@@ -305,7 +305,10 @@ __global__ __launch_bounds__(kWg) void k_flags(
     const uint32_t kp = (w > 0.0f && w >= thr) ? 1u : 0u;
     keep[p] = kp;
     upper[p] = (kp && c > r) ? 1u : 0u;
-    mirror[p] = find_in_row(u_col, u_off[c], u_off[c + 1], r);
+    // the union pattern is symmetric, so (c, r) exists; fall back to p itself
+    // rather than index past row c if that invariant were ever broken
+    const uint32_t q = find_in_row(u_col, u_off[c], u_off[c + 1], r);
+    mirror[p] = (q < u_off[c + 1] && u_col[q] == r) ? q : (uint32_t)p;
 }
 
 __global__ __launch_bounds__(kWg) void k_present(const uint32_t* __restrict__ u_row,
