@@ -279,6 +279,15 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
     t0 = time.perf_counter()
     orc.knn_exact(x, k, 0, Q)
     t_knn = (time.perf_counter() - t0) * n / Q
+    # recall@k and distance error of the GPU (BF16 + re-rank) kNN against an
+    # FP64 brute force on a seeded row sample (the CPU oracle is the checker)
+    rows = np.sort(np.random.default_rng(7).choice(n, size=args.recall_rows, replace=False))
+    idx64, d64 = orc.knn_exact_rows_f64(x, k, rows)
+    idx_h, dist_h = knn_host
+    hits = sum(len(np.intersect1d(idx_h[q], idx64[a])) for a, q in enumerate(rows))
+    recall = dict(rows=int(rows.size), recall_at_k=hits / (rows.size * k),
+                  exact_rows=int(sum(np.array_equal(idx_h[q], idx64[a]) for a, q in enumerate(rows))),
+                  max_rel_dist_err=float(np.max(np.abs(dist_h[rows] - d64) / np.maximum(d64, 1e-30))))
     # host graph stages on the full kNN (the GPU's exact result, copied to the host)
     idx_h, dist_h = knn_host
     t0 = time.perf_counter()
@@ -299,6 +308,7 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
     orc.optimise_adam_parallel(init, *adj, q2, 42, use_lut=True, n_threads=cores)
     t_adam = (time.perf_counter() - t0) * p.optim_params.n_epochs / args.cpu_epochs
     return dict(value=t_knn + t_graph + t_adam, unit="s", cores=cores, kind="port",
+                knn_recall_vs_fp64=recall,
                 sample=f"kNN: {Q} of {n} query rows x all {n} (scaled x{n / Q:g}); graph: "
                        f"full; parallel Adam: {args.cpu_epochs} of {p.optim_params.n_epochs} "
                        f"epochs on the same graph (scaled); spectral init not included",
@@ -312,6 +322,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fit")
     ap.add_argument("--cpu-queries", type=int, default=512)
+    ap.add_argument("--recall-rows", type=int, default=1000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-epochs", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

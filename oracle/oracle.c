@@ -5,11 +5,13 @@
  * ("port").  Loaded by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg; never linked into or called by the product library.
  *
- * Built with -O2 -ffp-contract=off (no fused multiply-add unless written as
+ * Built with -O3 -ffp-contract=off (no fused multiply-add unless written as
  * fmaf), so float arithmetic follows the reference's f32 operation order.
  *
  * Pinning (what fixes each function independently of its own formula):
- *   orc_hash_neg            pure u32 arithmetic; golden values in tests/golden
+ *   orc_hash_neg            the finaliser is C. Wellons' lowbias32 (checked with
+ *                           its published inverse); the pre-mix multipliers are
+ *                           the xxHash32 primes; reduction = plain % n
  *   orc_csr_from_graph      reference golden triangle layout / periods / errors
  *                           (umap_optimiser_gpu.rs:1716-1785)
  *   orc_grad_accum          hand-computed (1.2,0,-1.2,0) case (:2060-2092), empty
@@ -615,6 +617,13 @@ void orc_knn_exact_f64(const float *x, int64_t n, int32_t d, int32_t k, int64_t 
             dist[(q - q0) * k + t] = t < cnt ? bd[t] : INFINITY;
         }
     }
+}
+
+void orc_knn_exact_f64_rows(const float *x, int64_t n, int32_t d, int32_t k,
+                            const int64_t *rows, int64_t n_rows, int32_t *idx, double *dist) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t a = 0; a < n_rows; a++)
+        orc_knn_exact_f64(x, n, d, k, rows[a], rows[a] + 1, idx + a * k, dist + a * k);
 }
 
 /* --------------------------------------------------------------- DensState */

@@ -123,6 +123,10 @@ void orc_knn_exact_f32(const float *x, int64_t n, int32_t d, int32_t k, int64_t 
 void orc_knn_exact_f64(const float *x, int64_t n, int32_t d, int32_t k, int64_t q0, int64_t q1,
                        int32_t *idx, double *dist);
 
+/* the f64 brute force for an arbitrary list of query rows (OpenMP over rows) */
+void orc_knn_exact_f64_rows(const float *x, int64_t n, int32_t d, int32_t k,
+                            const int64_t *rows, int64_t n_rows, int32_t *idx, double *dist);
+
 /* edge_distances_from_knn + original_log_radii + standardise + mu_tot,
  * src/utils/density.rs:181-201, 309-438.  rows must be grouped as the COO is.
  * Returns 0 ok, 1 NoData, 3 DegenerateLocalRadii. */

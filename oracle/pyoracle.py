@@ -229,6 +229,18 @@ def knn_exact(x, k, q0=0, q1=None, f64=False):
     return idx, dist
 
 
+def knn_exact_rows_f64(x, k, rows):
+    """FP64 brute-force kNN (self excluded, ties by index) for the given query rows."""
+    x = f32(x)
+    n, d = x.shape
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    idx = np.zeros((rows.size, k), np.int32)
+    dist = np.zeros((rows.size, k), np.float64)
+    lib().orc_knn_exact_f64_rows(P(x), C.c_int64(n), C.c_int32(d), C.c_int32(k), P(rows),
+                                 C.c_int64(rows.size), P(idx), P(dist))
+    return idx, dist
+
+
 def dens_state(n, rows, cols, vals, knn_idx, knn_dist, distances_squared=True):
     rows, cols, vals = i32(rows), i32(cols), f32(vals)
     knn_idx, knn_dist = i32(knn_idx), f32(knn_dist)
