@@ -685,21 +685,61 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
         int d = st->dim;
         if (!(d == 1 || d == 2 || d == 3 || d == 4 || d == 8)) strategy = MRS_EPOCH_NODE_EXACT;
     }
-    for (int32_t e = first; e < first + count; e++) {
-        bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
-        if (dens) {
-            ProfScope ps(st, 5);
-            launch_dens_update_dev(st);
+    auto launches = [&]() {
+        for (int32_t e = first; e < first + count; e++) {
+            bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
+            if (dens) {
+                ProfScope ps(st, 5);
+                launch_dens_update_dev(st);
+            }
+            if (strategy == MRS_EPOCH_NODE_EXACT) {
+                { ProfScope ps(st, 2); launch_grad_accum_node(st, e, seed, dens); }
+                { ProfScope ps(st, 3); launch_adam_elem(st, e); }
+                { ProfScope ps(st, 4); launch_schedule(st, e); }
+            } else {
+                { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens); }
+                { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
+            }
         }
-        if (strategy == MRS_EPOCH_NODE_EXACT) {
-            { ProfScope ps(st, 2); launch_grad_accum_node(st, e, seed, dens); }
-            { ProfScope ps(st, 3); launch_adam_elem(st, e); }
-            { ProfScope ps(st, 4); launch_schedule(st, e); }
-        } else {
-            { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens); }
-            { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
-        }
+    };
+    hipStream_t s = st->ctx->stream;
+    if (st->profiling || !st->use_graphs || count < 16) {
+        launches();
+        return;
     }
+    // Capture the whole loop once and replay it: one launch per fit instead of
+    // 2-3 per epoch.  The accumulate-by-atomics buffers are cleared outside the
+    // capture (no memset nodes inside the graph).
+    if (strategy == MRS_EPOCH_EDGE_ATOMIC && !st->grad_clean) {
+        MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), s));
+        MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), s));
+        st->grad_clean = true;
+    }
+    const int64_t key[4] = {first, count, (int64_t)seed, strategy};
+    const bool hit = st->graph_exec && std::equal(key, key + 4, st->graph_key);
+    if (!hit) {
+        if (st->graph_exec) {
+            (void)hipGraphExecDestroy(st->graph_exec);
+            st->graph_exec = nullptr;
+        }
+        MRS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            launches();
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(s, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        MRS_HIP(hipStreamEndCapture(s, &graph));
+        hipError_t ie = hipGraphInstantiate(&st->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MRS_HIP(ie);
+        std::copy(key, key + 4, st->graph_key);
+    }
+    MRS_HIP(hipGraphLaunch(st->graph_exec, s));
+    if (strategy == MRS_EPOCH_NODE_EXACT) st->grad_clean = false;
 }
 
 // ------------------------------------------------------------- state build
@@ -806,6 +846,7 @@ void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float
 }  // namespace mrs
 
 mrs_umap_state::~mrs_umap_state() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (auto& ev : prof_events) {
         (void)hipEventDestroy(ev.second.first);
         (void)hipEventDestroy(ev.second.second);

@@ -47,6 +47,7 @@ constexpr int kWaves = 8;          // waves per workgroup in the shortlist kerne
 constexpr int kQPerWave = 32;      // queries per wave (MFMA B columns)
 constexpr int kTileRows = 64;      // database rows per LDS tile (two MFMA A tiles)
 constexpr int kQCap = 64;          // per-query queue capacity
+constexpr int kPF = 4;             // database tiles in flight (register prefetch)
 constexpr uint64_t kInfKey = ~0ull;
 
 // ---------------------------------------------------------------- keys
@@ -187,7 +188,7 @@ struct ShortlistArgs {
     uint32_t db_per_split;  // multiple of kTileRows
 };
 
-template <bool BF16, int E>
+template <bool BF16, int E, int CPT>
 __global__ __launch_bounds__(kWaves * 64) void k_shortlist(ShortlistArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int S = 64 * E;
@@ -258,63 +259,106 @@ __global__ __launch_bounds__(kWaves * 64) void k_shortlist(ShortlistArgs a) {
         }
     };
 
-    for (uint32_t t0 = db0; t0 < db1; t0 += kTileRows) {
-        __syncthreads();  // previous tile fully consumed
-        // stage 64 database rows (16 B per thread-iteration) and their norms
-        const int chunks_per_row = KP * elem / 16;
-        for (int c = tid; c < kTileRows * chunks_per_row; c += kWaves * 64) {
-            const int r = c / chunks_per_row, o = c % chunks_per_row;
-            const uint4 v = reinterpret_cast<const uint4*>(
-                reinterpret_cast<const unsigned char*>(a.P) + (size_t)(t0 + r) * KP * elem)[o];
-            *reinterpret_cast<uint4*>(tile + r * row_bytes + o * 16) = v;
+    // epilogue of one 32-row MFMA tile: any element beating the query's threshold?
+    auto epilogue = [&](const f32x16& acc, uint32_t rowbase) {
+        float m = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; r++) m = fmaxf(m, acc[r]);
+        if (__ballot(m > thr) == 0ull) return;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint32_t db = rowbase + (r >> 2) * 8 + h * 4 + (r & 3);
+            if (acc[r] > thr && db < a.n && db != q) {
+                const uint32_t pos = atomicAdd(&mycnt[col], 1u);
+                myq[col * kQCap + pos] = score_key(acc[r], db);
+            }
         }
-        if (tid < kTileRows) tile_negh[tid] = a.negh[t0 + tid];
-        __syncthreads();
-#pragma unroll 1
-        for (int rt = 0; rt < kTileRows / 32; rt++) {
-            const int arow = rt * 32 + col;
-            f32x16 acc;
+        // flush queues that the next 32-row tile could overflow
+        const bool full = lane < kQPerWave && mycnt[lane] > (uint32_t)(kQCap - 32);
+        uint64_t need = __ballot(full);
+        while (need) {
+            const int b = __builtin_ctzll(need);
+            need &= need - 1;
+            merge_query(b);
+        }
+        thr = mythr[col];
+    };
+
+    // both 32-row MFMA tiles of the staged 64-row tile, as two independent chains
+    auto process_tile = [&](uint32_t t0) {
+        f32x16 acc0, acc1;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float4 nv = *reinterpret_cast<const float4*>(tile_negh + rt * 32 + i * 8 + h * 4);
-                acc[i * 4 + 0] = nv.x;
-                acc[i * 4 + 1] = nv.y;
-                acc[i * 4 + 2] = nv.z;
-                acc[i * 4 + 3] = nv.w;
-            }
-            if (BF16) {
-                const bf16x8* ap = reinterpret_cast<const bf16x8*>(tile + arow * row_bytes + h * (KP / 2) * 2);
+        for (int i = 0; i < 4; i++) {
+            const float4 n0 = *reinterpret_cast<const float4*>(tile_negh + i * 8 + h * 4);
+            const float4 n1 = *reinterpret_cast<const float4*>(tile_negh + 32 + i * 8 + h * 4);
+            acc0[i * 4 + 0] = n0.x; acc0[i * 4 + 1] = n0.y; acc0[i * 4 + 2] = n0.z; acc0[i * 4 + 3] = n0.w;
+            acc1[i * 4 + 0] = n1.x; acc1[i * 4 + 1] = n1.y; acc1[i * 4 + 2] = n1.z; acc1[i * 4 + 3] = n1.w;
+        }
+        if (BF16) {
+            const bf16x8* a0 = reinterpret_cast<const bf16x8*>(tile + col * row_bytes + h * KP);
+            const bf16x8* a1 = reinterpret_cast<const bf16x8*>(tile + (32 + col) * row_bytes + h * KP);
 #pragma unroll
-                for (int s = 0; s < MAXSTEPS; s++)
-                    if (s < steps) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[s], bq[s], acc, 0, 0, 0);
-            } else {
-                const float* ap = reinterpret_cast<const float*>(tile + arow * row_bytes) + h * (KP / 2);
-#pragma unroll
-                for (int s = 0; s < MAXSTEPS; s++)
-                    if (s < steps) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[s], bqf[s], acc, 0, 0, 0);
-            }
-            // epilogue: any element beating this query's threshold?
-            float m = acc[0];
-#pragma unroll
-            for (int r = 1; r < 16; r++) m = fmaxf(m, acc[r]);
-            if (__ballot(m > thr) != 0ull) {
-#pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const uint32_t db = t0 + rt * 32 + (r >> 2) * 8 + h * 4 + (r & 3);
-                    if (acc[r] > thr && db < a.n && db != q) {
-                        const uint32_t pos = atomicAdd(&mycnt[col], 1u);
-                        myq[col * kQCap + pos] = score_key(acc[r], db);
-                    }
+            for (int st = 0; st < MAXSTEPS; st++)
+                if (st < steps) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[st], bq[st], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[st], bq[st], acc1, 0, 0, 0);
                 }
-                // flush queues that the next 32-row tile could overflow
-                const bool full = lane < kQPerWave && mycnt[lane] > (uint32_t)(kQCap - 32);
-                uint64_t need = __ballot(full);
-                while (need) {
-                    const int b = __builtin_ctzll(need);
-                    need &= need - 1;
-                    merge_query(b);
+        } else {
+            const float* a0 = reinterpret_cast<const float*>(tile + col * row_bytes) + h * (KP / 2);
+            const float* a1 = reinterpret_cast<const float*>(tile + (32 + col) * row_bytes) + h * (KP / 2);
+#pragma unroll
+            for (int st = 0; st < MAXSTEPS; st++)
+                if (st < steps) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[st], bqf[st], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[st], bqf[st], acc1, 0, 0, 0);
                 }
-                thr = mythr[col];
+        }
+        epilogue(acc0, t0);
+        epilogue(acc1, t0 + 32);
+    };
+
+    // Database tiles stream through registers kPF tiles ahead of the LDS stage,
+    // so the global/L2 latency of tile t+kPF hides behind the MFMAs of tile t.
+    const int cpr = KP * elem / 16;           // 16-byte chunks per row
+    const int tile_chunks = kTileRows * cpr;  // contiguous in P
+    const uint32_t ntiles = (db1 - db0) / kTileRows;
+    uint4 pre[kPF][CPT];
+    float pre_n[kPF];
+    auto load_tile = [&](uint4 (&dst)[CPT], float& dn, uint32_t t) {
+        const uint32_t t0 = db0 + t * kTileRows;
+        const uint4* base = reinterpret_cast<const uint4*>(
+            reinterpret_cast<const unsigned char*>(a.P) + (size_t)t0 * KP * elem);
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * kWaves * 64;
+            if (c < tile_chunks) dst[j] = base[c];
+        }
+        if (tid < kTileRows) dn = a.negh[t0 + tid];
+    };
+    auto store_tile = [&](const uint4 (&src)[CPT], float dn) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int c = tid + j * kWaves * 64;
+            if (c < tile_chunks) {
+                const int r = c / cpr, o = c - r * cpr;
+                *reinterpret_cast<uint4*>(tile + r * row_bytes + o * 16) = src[j];
+            }
+        }
+        if (tid < kTileRows) tile_negh[tid] = dn;
+    };
+#pragma unroll
+    for (int p = 0; p < kPF; p++)
+        if ((uint32_t)p < ntiles) load_tile(pre[p], pre_n[p], p);
+    for (uint32_t tb = 0; tb < ntiles; tb += kPF) {
+#pragma unroll
+        for (int p = 0; p < kPF; p++) {
+            const uint32_t t = tb + p;
+            if (t < ntiles) {
+                __syncthreads();  // the previous tile's LDS reads are done
+                store_tile(pre[p], pre_n[p]);
+                if (t + kPF < ntiles) load_tile(pre[p], pre_n[p], t + kPF);
+                __syncthreads();
+                process_tile(db0 + t * kTileRows);
             }
         }
     }
@@ -461,12 +505,15 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         hipLaunchKernelGGL(kern, dim3(qblocks, splits), dim3(kWaves * 64), lds, s, sa);
     };
     MRS_HIP(hipEventRecord(ev[1], s));
+    // 16-byte chunks each thread stages per 64-row tile
+    const int cpt = (kTileRows * (KP * (bf16 ? 2 : 4) / 16) + kWaves * 64 - 1) / (kWaves * 64);
     if (bf16) {
-        if (E == 1) launch(k_shortlist<true, 1>);
-        else launch(k_shortlist<true, 2>);
+        if (cpt == 1) E == 1 ? launch(k_shortlist<true, 1, 1>) : launch(k_shortlist<true, 2, 1>);
+        else E == 1 ? launch(k_shortlist<true, 1, 2>) : launch(k_shortlist<true, 2, 2>);
     } else {
-        if (E == 1) launch(k_shortlist<false, 1>);
-        else launch(k_shortlist<false, 2>);
+        if (cpt == 1) E == 1 ? launch(k_shortlist<false, 1, 1>) : launch(k_shortlist<false, 2, 1>);
+        else if (cpt == 2) E == 1 ? launch(k_shortlist<false, 1, 2>) : launch(k_shortlist<false, 2, 2>);
+        else E == 1 ? launch(k_shortlist<false, 1, 4>) : launch(k_shortlist<false, 2, 4>);
     }
     MRS_CHECK_LAUNCH();
     MRS_HIP(hipEventRecord(ev[2], s));

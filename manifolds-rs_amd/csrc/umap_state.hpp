@@ -45,6 +45,11 @@ struct mrs_umap_state {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     double prof_ms[8] = {0};
     int64_t prof_count[8] = {0};
+    // the epoch loop captured as a HIP graph (one launch per fit); the key is
+    // the (first, count, seed, strategy) it was captured for
+    bool use_graphs = true;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_key[4] = {-1, -1, -1, -1};
     ~mrs_umap_state();
 };
 

@@ -338,3 +338,38 @@ def test_driver_densmap_matches_oracle(ctx):
     want = orc.optimise_adam_gpu(embd, *adj, p, 42, dens_r=r, dens_params=(2.0, 0.3, 0.1),
                                  mu_tot=float(adj[2].sum()))
     assert np.max(np.abs(got - want)) < 1e-3
+
+
+@pytest.mark.parametrize("dens_on", [False, True])
+def test_graph_replay_matches_eager(ctx, dens_on):
+    # run_epochs(count >= 16) replays a captured HIP graph; it must equal the
+    # eager epoch-by-epoch launches bit for bit (node-exact strategy), on every
+    # replay, with eager work and syncs in between
+    embd, adj = _random_graph(n=1200, seed=51)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 40
+    ds = None
+    if dens_on:
+        rng = np.random.default_rng(51)
+        r = rng.normal(size=embd.shape[0]).astype(np.float32)
+        ms = np.ones_like(r)
+        ds = M.DensState(params=M.DensParams(lambda_=2.0, frac=0.5, var_shift=0.1),
+                         r=r.ctypes.data, mu_sum=ms.ctypes.data, mu_tot=float(adj[2].sum()))
+    st, _ = make_state(ctx, embd, adj, p, ds)
+    for e in range(40):
+        st.run_epochs(e, 1, 42, M.EPOCH_NODE_EXACT)
+    eager = st.read(M.BUF_EMBD)
+    for rep in range(3):
+        st.reset(embd)
+        st.run_epochs(0, 40, 42, M.EPOCH_NODE_EXACT)
+        got = st.read(M.BUF_EMBD)
+        assert np.array_equal(got, eager), f"replay {rep}"
+        st.launch_grad_accum(3, 7)  # unrelated eager work between replays
+    # the atomic strategy through the graph stays close to its eager run
+    st.reset(embd)
+    for e in range(40):
+        st.run_epochs(e, 1, 42, M.EPOCH_EDGE_ATOMIC)
+    eager_a = st.read(M.BUF_EMBD)
+    st.reset(embd)
+    st.run_epochs(0, 40, 42, M.EPOCH_EDGE_ATOMIC)
+    assert np.max(np.abs(st.read(M.BUF_EMBD) - eager_a)) < 1e-2
