@@ -219,6 +219,13 @@ def bench_fit(args):
     flops = 2.0 * n * n * d
     achieved = flops / (kt["shortlist_ms"] * 1e-3) / 1e12
 
+    # plain FP32 GPU kNN beside it (same shortlist/re-rank machinery, FP32 MFMA)
+    _, _, kt32 = M.knn_exhaustive(ctx, xd, k, M.KNN_FP32, 0, timed=True)
+    knn_cmp = dict(bf16_rerank_ms=kt["pack_ms"] + kt["shortlist_ms"] + kt["rerank_ms"],
+                   fp32_ms=kt32["pack_ms"] + kt32["shortlist_ms"] + kt32["rerank_ms"])
+    knn_cmp["speedup_bf16_over_fp32"] = knn_cmp["fp32_ms"] / knn_cmp["bf16_rerank_ms"]
+    knn_cmp["fp32_shortlist_tflops"] = flops / (kt32["shortlist_ms"] * 1e-3) / 1e12
+
     # exact edge-updates: schedule replay of this fit's graph
     idx, dist = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 4 * k)
     g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
@@ -247,6 +254,7 @@ def bench_fit(args):
                     "64 candidates, spectral init, 500 Adam epochs, 2-D output",
                     n=n, d=d, k=k, n_epochs=n_epochs, n_edges=E, edge_updates=A_total),
         stages_ms=st,
+        knn_fp32_vs_bf16=knn_cmp,
         edge_updates_per_s=A_total / (st["epochs_ms"] * 1e-3),
         roofline=dict(bound="mfma", kernel="k_shortlist<bf16>", achieved=achieved, peak=2500.0,
                       unit="TFLOP/s", frac=achieved / 2500.0, traffic=None,

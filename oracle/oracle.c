@@ -765,13 +765,16 @@ static inline float lut_get(const orc_lut *L, float x) {
     return y0 + rem * (y1 - y0);
 }
 
-/* optimise_embedding_adam_parallel, src/training/umap_optimisers.rs:1170-1492
- * (plain-UMAP arm; the densMAP pass 0 at :1303-1317 is covered by the GPU
- * replay above). */
-int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,
-                               const int32_t *adj_cols, const float *adj_vals,
-                               const orc_optim_params *p, uint64_t seed, int32_t use_lut,
-                               int32_t n_threads) {
+/* optimise_embedding_adam_parallel, src/training/umap_optimisers.rs:1170-1492,
+ * with the densMAP pass 0 (:1303-1317: accumulate_umap_radii :468-511,
+ * update_density_weights :519-537) and density_edge_coeff (:564-592, with its
+ * max(eps) floors) when dens_r is given. */
+int orc_optimise_adam_parallel_dens(float *embd, int64_t n, int32_t n_dim,
+                                    const int64_t *adj_off, const int32_t *adj_cols,
+                                    const float *adj_vals, const orc_optim_params *p,
+                                    uint64_t seed, int32_t use_lut, int32_t n_threads,
+                                    const float *dens_r, float dens_lambda, float dens_frac,
+                                    float dens_var_shift, float dens_mu_tot) {
     if (n == 0) return 1;
     int64_t ne = 0;
     int st = orc_csr_from_graph(n, adj_off, adj_cols, adj_vals, &ne, NULL, NULL, NULL, NULL,
@@ -811,6 +814,13 @@ int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int6
     lut_init(&L, p->b, 25.0f, 65536);
     const uint32_t seed32 = (uint32_t)seed;
     const float nef = (float)p->n_epochs;
+    float *re = NULL, *phis = NULL, *dw = NULL;
+    if (dens_r) {
+        re = (float *)calloc((size_t)n, sizeof(float));
+        phis = (float *)calloc((size_t)n, sizeof(float));
+        dw = (float *)calloc((size_t)n, sizeof(float));
+    }
+    const float feps = FLT_EPSILON;
     for (int32_t epoch = 0; epoch < p->n_epochs; epoch++) {
         const float lr = p->lr * (1.0f - (float)epoch / nef);
         const float et = (float)epoch;
@@ -818,6 +828,30 @@ int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int6
         const float sqrt_b2t1 = sqrtf(1.0f - powf(p->beta2, t));
         const float ad_scale = sqrt_b2t1 / (1.0f - powf(p->beta1, t));
         const float epsc = sqrt_b2t1 * p->eps;
+        const int dens_on = dens_r && orc_dens_is_active(dens_lambda, dens_frac, epoch, p->n_epochs);
+        float dcoeff = 0.0f;
+        if (dens_on) {
+#pragma omp parallel for schedule(dynamic, 256)
+            for (int64_t i = 0; i < n; i++) {
+                const float *yi = embd + i * n_dim;
+                float ssq = 0.0f, sphi = 0.0f;
+                for (uint32_t pos = off[i]; pos < off[i + 1]; pos++) {
+                    const float *yo = embd + (int64_t)oth[pos] * n_dim;
+                    float dsq = 0.0f;
+                    for (int d = 0; d < n_dim; d++) {
+                        float df = yi[d] - yo[d];
+                        dsq += df * df;
+                    }
+                    float dsb = b_is_one ? dsq : lut_get(&L, dsq);
+                    float phi = 1.0f / (1.0f + p->a * dsb);
+                    ssq += phi * dsq;
+                    sphi += phi;
+                }
+                phis[i] = sphi;
+                re[i] = logf((float)ORC_DENS_EPS + ssq / (sphi > feps ? sphi : feps));
+            }
+            dcoeff = orc_dens_weights(dens_r, re, n, dens_var_shift, dens_lambda, dens_mu_tot, dw);
+        }
 #pragma omp parallel for schedule(dynamic, 256)
         for (int64_t i = 0; i < n; i++) {
             float *gi = g + i * n_dim;
@@ -841,6 +875,24 @@ int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int6
                     float denom = 1.0f + p->a * dsb;
                     float gc = two_a_b * dsb / (dsq * denom);
                     for (int d = 0; d < n_dim; d++) gi[d] += two * gc * (yo[d] - yi[d]);
+                    if (dens_on) {
+                        float phi = 1.0f / denom;
+                        float dphi_term = p->a * p->b * dsb / dsq * phi;
+                        float common = 1.0f - p->b * (1.0f - phi);
+                        float pi_ = phis[i] > feps ? phis[i] : feps;
+                        float po_ = phis[o] > feps ? phis[o] : feps;
+                        float dr_self = (phi / pi_) * (common / expf(re[i]) + dphi_term);
+                        float dr_other = (phi / po_) * (common / expf(re[o]) + dphi_term);
+                        float mue = mu[e] > feps ? mu[e] : feps;
+                        float cor = dcoeff * (dw[i] * dr_self + dw[o] * dr_other) / mue;
+                        float two_cor = two * cor;
+                        for (int d = 0; d < n_dim; d++) {
+                            float gg = two_cor * (yi[d] - yo[d]);
+                            if (gg > clip) gg = clip;
+                            if (gg < -clip) gg = -clip;
+                            gi[d] += two * gg;
+                        }
+                    }
                 }
                 if (ei[e] == (int32_t)i) { /* is_smaller */
                     float nf = floorf((et - next_neg[e]) / per_neg[e]);
@@ -892,7 +944,16 @@ int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int6
     free(L.table);
     free(off); free(eidx); free(oth); free(per); free(mu); free(ei); free(next);
     free(per_neg); free(next_neg); free(m); free(v); free(g); free(hu);
+    free(re); free(phis); free(dw);
     return 0;
+}
+
+int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,
+                               const int32_t *adj_cols, const float *adj_vals,
+                               const orc_optim_params *p, uint64_t seed, int32_t use_lut,
+                               int32_t n_threads) {
+    return orc_optimise_adam_parallel_dens(embd, n, n_dim, adj_off, adj_cols, adj_vals, p, seed,
+                                           use_lut, n_threads, NULL, 0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 /* optimise_embedding_sgd, src/training/umap_optimisers.rs:653-865 (plain UMAP) */

@@ -147,6 +147,14 @@ int orc_optimise_adam_parallel(float *embd, int64_t n, int32_t n_dim, const int6
                                const orc_optim_params *p, uint64_t seed, int32_t use_lut,
                                int32_t n_threads);
 
+/* the same with the densMAP term (dens_r NULL = plain UMAP) */
+int orc_optimise_adam_parallel_dens(float *embd, int64_t n, int32_t n_dim,
+                                    const int64_t *adj_off, const int32_t *adj_cols,
+                                    const float *adj_vals, const orc_optim_params *p,
+                                    uint64_t seed, int32_t use_lut, int32_t n_threads,
+                                    const float *dens_r, float dens_lambda, float dens_frac,
+                                    float dens_var_shift, float dens_mu_tot);
+
 /* optimise_embedding_sgd, src/training/umap_optimisers.rs:653-865 (plain UMAP,
  * negatives injected from the hash as above). */
 int orc_optimise_sgd(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_off,

@@ -262,15 +262,17 @@ def fit_params(min_dist, spread, n_iter=300):
 
 
 def optimise_adam_parallel(embd, adj_off, adj_cols, adj_vals, params, seed, use_lut=True,
-                           n_threads=0):
+                           n_threads=0, dens_r=None, dens_params=None, mu_tot=0.0):
     out = f32(embd).copy()
     n, dim = out.shape
     p = params_from(params)
     adj_off = np.ascontiguousarray(adj_off, dtype=np.int64)
-    st = lib().orc_optimise_adam_parallel(P(out), C.c_int64(n), C.c_int32(dim), P(adj_off),
-                                          P(i32(adj_cols)), P(f32(adj_vals)), C.byref(p),
-                                          C.c_uint64(seed), C.c_int32(1 if use_lut else 0),
-                                          C.c_int32(n_threads))
+    lam, frac, vs = (dens_params if dens_params is not None else (0.0, 0.0, 0.0))
+    st = lib().orc_optimise_adam_parallel_dens(
+        P(out), C.c_int64(n), C.c_int32(dim), P(adj_off), P(i32(adj_cols)), P(f32(adj_vals)),
+        C.byref(p), C.c_uint64(seed), C.c_int32(1 if use_lut else 0), C.c_int32(n_threads),
+        P(f32(dens_r)) if dens_r is not None else None, C.c_float(lam), C.c_float(frac),
+        C.c_float(vs), C.c_float(mu_tot))
     if st:
         raise RuntimeError(f"oracle status {st}")
     return out

@@ -423,3 +423,37 @@ def test_cpu_optimisers_separate_two_cliques():
         intra = (dist(0, 1) + dist(0, 2) + dist(1, 2) + dist(3, 4) + dist(3, 5) + dist(4, 5)) / 6
         inter = np.mean([dist(i, j) for i in (0, 1) for j in (3, 4, 5)])
         assert inter > 1.5 * intra and np.all(np.isfinite(out))
+
+
+def test_cpu_densmap_raises_radius_correlation():
+    # densMAP's purpose (Narayan et al. 2021): embedding local radii track the
+    # original ones better than plain UMAP's (tests/densmap_tests.rs analogue)
+    from helpers import umap_graph_from_data
+    from metrics import spearman
+
+    rng = np.random.default_rng(3)
+    sds = [0.3, 1.0, 3.0]
+    x = np.concatenate([rng.normal(scale=s, size=(150, 6)) + 15 * i for i, s in enumerate(sds)])
+    x = x.astype(np.float32)
+    idx, dist, (r, c, v), adj = umap_graph_from_data(x, 15, n_epochs=200)
+    st, R, ms, mu_tot = orc.dens_state(x.shape[0], r, c, v, idx, dist, True)
+    assert st == 0
+    init = rng.uniform(-10, 10, size=(x.shape[0], 2)).astype(np.float32)
+    p = mrs_amd.UmapOptimParams.default_2d()
+    p.n_epochs = 200
+    plain = orc.optimise_adam_parallel(init, *adj, p, 42)
+    dens = orc.optimise_adam_parallel(init, *adj, p, 42, dens_r=R, dens_params=(2.0, 0.3, 0.1),
+                                      mu_tot=mu_tot)
+
+    def emb_radii(e):
+        out = np.zeros(x.shape[0])
+        off = adj[0]
+        for i in range(x.shape[0]):
+            nb = adj[1][off[i]:off[i + 1]]
+            d2 = ((e[nb] - e[i]) ** 2).sum(1)
+            phi = 1 / (1 + p.a * d2 ** p.b)
+            out[i] = np.log(1e-8 + (phi * d2).sum() / phi.sum())
+        return out
+
+    assert np.all(np.isfinite(dens))
+    assert spearman(R, emb_radii(dens)) > spearman(R, emb_radii(plain)) + 0.05
