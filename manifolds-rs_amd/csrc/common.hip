@@ -87,10 +87,12 @@ __global__ __launch_bounds__(kScanWg) void k_scan_add(uint32_t* __restrict__ out
         if (base + q < n) out[base + q] += add;
 }
 
+// total = last exclusive prefix + the saved last input element (last_in holds
+// exactly one value: the input may have been overwritten by an in-place scan)
 __global__ void k_copy_total(const uint32_t* __restrict__ scanned,
-                             const uint32_t* __restrict__ in, uint64_t n,
+                             const uint32_t* __restrict__ last_in, uint64_t n,
                              uint32_t* __restrict__ total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + in[n - 1];
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + last_in[0];
 }
 
 }  // namespace

@@ -580,6 +580,9 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         uint32_t tot[3];
         MRS_HIP(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
         MRS_HIP(hipStreamSynchronize(s));
+        // the totals size every output array below: check them against the union
+        if (tot[0] > m || tot[1] > m || tot[2] > m)
+            throw Error(MRS_DEVICE_ERROR, "graph scan totals exceed the union size");
         g->nnz = tot[0];
         g->n_edges = tot[1];
         if ((uint64_t)tot[2] != 2ull * tot[1])

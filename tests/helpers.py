@@ -62,3 +62,30 @@ def rel_err(got, want, floor):
     got = np.asarray(got, np.float64)
     want = np.asarray(want, np.float64)
     return float(np.max(np.abs(got - want) / np.maximum(np.abs(want), floor)))
+
+
+def assert_scaled_close(got, want, rtol, ctx=""):
+    """max |got - want| <= rtol * max |want|.
+
+    Sums of many clipped terms cancel, so an element-wise relative bar is
+    meaningless near zero; the error of such a sum is bounded by the size of
+    its terms, for which the largest magnitude in the array stands in.
+    """
+    got = np.asarray(got, np.float64).ravel()
+    want = np.asarray(want, np.float64).ravel()
+    assert got.shape == want.shape, f"{ctx}: shape {got.shape} vs {want.shape}"
+    if want.size == 0:
+        return
+    d = np.abs(got - want)
+    scale = max(float(np.max(np.abs(want))), 1e-30)
+    i = int(np.argmax(d))
+    assert np.all(np.isfinite(got)), f"{ctx}: non-finite output"
+    assert d[i] <= rtol * scale, (
+        f"{ctx}: max diff {d[i]:.3e} at {i} (got {got[i]!r}, want {want[i]!r}); scale {scale:.3e}, "
+        f"scaled {d[i] / scale:.3e} > {rtol:.1e}; median diff {float(np.median(d)):.3e}")
+
+
+def diff_stats(got, want):
+    d = np.abs(np.asarray(got, np.float64).ravel() - np.asarray(want, np.float64).ravel())
+    return (f"max {d.max():.3e} p99 {np.percentile(d, 99):.3e} median {np.median(d):.3e} "
+            f"frac>1e-4 {np.mean(d > 1e-4):.2e}")
