@@ -337,8 +337,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU only; multi-GPU runs are "
-                                    "out of scope (the path would shard by query rows / node "
-                                    "ranges, see DESIGN.md)", n_gpus=args.gpus)))
+                                    "out of scope (the kNN would shard by query rows and the "
+                                    "epochs by node ranges)", n_gpus=args.gpus)))
         sys.exit(2)
     if args.workload == "fit":
         res = bench_fit(args)

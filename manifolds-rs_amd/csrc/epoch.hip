@@ -317,21 +317,31 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 
 // ------------------------------------------------------ edge-parallel epoch
 
-// Segmented sum over lanes holding equal keys (runs are contiguous): afterwards
-// the first lane of each run holds the run's total.
+// Segmented sum over the wavefront's contiguous runs of equal keys; returns
+// whether this lane heads its run, and the head then holds the run's total.
+// A run is named by its head lane, not by the key: the same key can recur
+// after a gap (inactive lanes in between), and those are separate runs whose
+// heads each add their own part -- reducing by key alone would let the
+// doubling steps reach across the gap and count the later run twice.
 template <int D>
-__device__ inline void seg_reduce(uint32_t key, float (&v)[D]) {
+__device__ inline bool seg_reduce(uint32_t key, float (&v)[D]) {
     const int lane = threadIdx.x & 63;
+    const uint32_t kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint32_t run = 63u - (uint32_t)__clzll(heads & upto);
 #pragma unroll
     for (int offd = 1; offd < 64; offd <<= 1) {
-        uint32_t ko = __shfl_down(key, offd, 64);
-        bool take = (lane + offd < 64) && ko == key;
+        const uint32_t ro = __shfl_down(run, offd, 64);
+        const bool take = (lane + offd < 64) && ro == run;
 #pragma unroll
         for (int d = 0; d < D; d++) {
             float vo = __shfl_down(v[d], offd, 64);
             if (take) v[d] += vo;
         }
     }
+    return head;
 }
 
 // One thread per undirected edge (i < j).  DIM is the compile-time embedding
@@ -446,9 +456,7 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
         flag[j] = 1u;
     }
     // smaller endpoint: combine runs of equal i within the wavefront first
-    seg_reduce<DIM>(key, gi);
-    const uint32_t kprev = __shfl_up(key, 1, 64);
-    const bool head = ((threadIdx.x & 63) == 0) || kprev != key;
+    const bool head = seg_reduce<DIM>(key, gi);
     if (head && key != 0xFFFFFFFFu) {
 #pragma unroll
         for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)key * DIM + d], gi[d]);

@@ -129,6 +129,14 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    # One HIP runtime per process.  PyTorch's ROCm wheels carry their own
+    # libamdhip64 / libhsa-runtime64 under the same sonames: with torch loaded
+    # first this library binds to those; loaded first itself, it would bring
+    # up /opt/rocm's runtime beside torch's, and torch would then find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     i32, i64, u32, u64, f32p = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_void_p
     vp = C.c_void_p
