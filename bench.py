@@ -323,6 +323,21 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
                 knn_s=t_knn, graph_s=t_graph, adam_s=t_adam)
 
 
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB by a plain elementwise kernel: it
+    reads and writes exactly mib MiB, so HBM byte counters read on it give the
+    factor between counter units and bytes."""
+    import torch
+
+    src = torch.ones(mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: {mib} MiB read + {mib} MiB written")
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,12 +349,17 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-epochs", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy a device buffer of this many MiB (a plain copy kernel of "
+                         "known size, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU only; multi-GPU runs are "
                                     "out of scope (the kNN would shard by query rows and the "
                                     "epochs by node ranges)", n_gpus=args.gpus)))
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     if args.workload == "fit":
         res = bench_fit(args)
     elif args.workload == "epochs":

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cmath>
 
@@ -42,12 +43,12 @@ namespace {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef unsigned short u16;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;  // 16-byte staging chunk
 
-constexpr int kWaves = 8;          // waves per workgroup in the shortlist kernel
+constexpr int kWavesMax = 8;       // most waves per workgroup in the shortlist kernel
 constexpr int kQPerWave = 32;      // queries per wave (MFMA B columns)
-constexpr int kTileRows = 64;      // database rows per LDS tile (two MFMA A tiles)
-constexpr int kQCap = 64;          // per-query queue capacity
-constexpr int kPF = 4;             // database tiles in flight (register prefetch)
+constexpr int kSplitAlign = 128;   // database splits are multiples of every stage size
+constexpr int kStageChunks = 2;    // 16-byte chunks per thread at kWavesMax waves (1024 per stage)
 constexpr uint64_t kInfKey = ~0ull;
 
 // ---------------------------------------------------------------- keys
@@ -68,10 +69,28 @@ __device__ inline float key_score(uint64_t key) {
     return key == kInfKey ? -INFINITY : unord_f32(~(uint32_t)(key >> 32));
 }
 
-__device__ inline uint64_t shfl_xor64(uint64_t v, int m) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    lo = __shfl_xor(lo, m, 64);
-    hi = __shfl_xor(hi, m, 64);
+// v from lane (lane ^ m).  The sorting networks below only ask for constant
+// masks (their loops are fully unrolled), and each mask takes the cheapest
+// cross-lane path: DPP quad permutes and row mirrors (VALU) for 1, 2, 3, 7, 15;
+// ds_swizzle's xor mode (32-lane groups, no LDS memory access) for 4, 8, 16, 31;
+// ds_bpermute only for the masks that cross the two 32-lane halves.
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
+    switch (m) {
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+        case 3: return __builtin_amdgcn_mov_dpp(v, 0x1B, 0xF, 0xF, true);   // quad_perm [3,2,1,0]
+        case 7: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+        case 15: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true); // row_mirror
+        case 4: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (4 << 10));
+        case 8: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (8 << 10));
+        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (16 << 10));
+        case 31: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (31 << 10));
+        default: return __shfl_xor(v, m, 64);
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = lane_xor((uint32_t)v, m), hi = lane_xor((uint32_t)(v >> 32), m);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -183,41 +202,141 @@ struct ShortlistArgs {
     const void* P;        // packed [npad][KP]
     const float* negh;    // [npad]
     uint64_t* lists;      // [splits][npad][S]
+    uint64_t* queue;      // [splits][npad][QC] per-query candidate queues
     uint32_t n, npad;
     int KP;
-    uint32_t db_per_split;  // multiple of kTileRows
+    uint32_t db_per_split;  // multiple of kSplitAlign
 };
 
-template <bool BF16, int E, int CPT>
-__global__ __launch_bounds__(kWaves * 64) void k_shortlist(ShortlistArgs a) {
+// Wave-wide merge of query b's queue (c entries) into its sorted global list;
+// returns the list's new threshold (the score of its last entry).
+template <int E, int QC>
+__device__ inline float merge_query(int b, uint32_t c, int lane, uint64_t* __restrict__ lists,
+                                    const uint64_t* __restrict__ myq) {
+    constexpr int S = 64 * E;
+    // the queue entries were stored by this wave's lanes: complete those
+    // stores before any lane reads another lane's entry
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    uint64_t one[1] = {(uint32_t)lane < c ? myq[b * QC + lane] : kInfKey};
+    wave_sort<1>(one);
+    uint64_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = lists[(size_t)b * S + e * 64 + lane];
+    wave_merge<E>(v, one[0]);
+#pragma unroll
+    for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[e];
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v[E - 1] >> 32), 63);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v[E - 1], 63);
+    return key_score(((uint64_t)hi << 32) | lo);
+}
+
+// Epilogue of one 32-row MFMA tile: the lane's 16 scores (one query, rows
+// rowbase + (r/4)*8 + h*4 + r%4) against the query's threshold.  A query's
+// queue count qn and threshold thr live in registers, identical in its two
+// lanes (h = 0, 1); the halves learn each other's insert count with one lane
+// swap, so queueing costs no LDS round trip.  Most tiles leave on the
+// max/ballot test; a queue that could overflow on the next tile is merged.
+template <int E, int QC>
+__device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h, int col,
+                                int lane, float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
+                                uint64_t* __restrict__ myq) {
+    if (q - rowbase < 32u) {  // the query's own row is in this tile: exclude it
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (rowbase + (r >> 2) * 8 + h * 4 + (r & 3) == q) acc[r] = -INFINITY;
+    }
+    float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+        gm[g] = fmaxf(fmaxf(acc[g * 4], acc[g * 4 + 1]), fmaxf(acc[g * 4 + 2], acc[g * 4 + 3]));
+    const float m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
+    if (__ballot(m > thr) == 0ull) return;
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) c += acc[r] > thr ? 1u : 0u;
+    const uint32_t co = __shfl_xor(c, 32, 64);
+    uint32_t pos = qn + (h ? co : 0u);
+    qn += c + co;
+    uint64_t* dst = myq + col * QC;
+    // a group no lane of the wave passes is skipped as a whole
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+        if (gm[g] > thr) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (acc[g * 4 + j] > thr)
+                    dst[pos++] = score_key(acc[g * 4 + j], rowbase + g * 8 + h * 4 + j);
+        }
+    uint64_t need = __ballot(qn > (uint32_t)(QC - 32)) & 0xFFFFFFFFull;  // lanes 0..31: one per query
+    while (need) {
+        const int b = __builtin_ctzll(need);
+        need &= need - 1;
+        const float t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
+        if (col == b) {
+            qn = 0;
+            thr = t;
+        }
+    }
+}
+
+// Global -> registers: this thread's 16-byte chunks of an RT-row stage (rows
+// are contiguous in P) and, for tid < RT, the row's -0.5|y|^2.
+template <int CPT, int RT, int W>
+__device__ inline void stage_load(u32x4 (&r)[CPT], float& rn, const unsigned char* __restrict__ P,
+                                  const float* __restrict__ negh, uint32_t row0, int row_elems_bytes,
+                                  int chunks, int tid) {
+    const u32x4* base = reinterpret_cast<const u32x4*>(P + (size_t)row0 * row_elems_bytes);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const int c = tid + j * W * 64;
+        if (c < chunks) r[j] = base[c];
+    }
+    if (tid < RT) rn = negh[row0 + tid];
+}
+
+// Registers -> LDS stage buffer (rows padded to row_bytes against bank conflicts).
+template <int CPT, int RT, int W>
+__device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned char* tile,
+                                   float* tile_negh, int row_bytes, int cpr, int chunks, int tid) {
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const int c = tid + j * W * 64;
+        if (c < chunks) {
+            const int row = c / cpr, o = c - row * cpr;
+            *reinterpret_cast<u32x4*>(tile + row * row_bytes + o * 16) = r[j];
+        }
+    }
+    if (tid < RT) tile_negh[tid] = rn;
+}
+
+// The distance GEMM + top-S filter.  One workgroup = W waves x 32 queries; the
+// database split [db0, db1) streams through two LDS stage buffers of RT rows
+// (double buffering: one barrier per stage), each stage prefetched into
+// registers one stage ahead.  QC = per-query queue capacity.
+template <bool BF16, int E, int RT, int QC, int W>
+__global__ __launch_bounds__(W * 64) void k_shortlist(ShortlistArgs a) {
+    constexpr int CPT = (kStageChunks * kWavesMax) / W;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int S = 64 * E;
+    constexpr int elem = BF16 ? 2 : 4;
     const int KP = a.KP;
-    const int elem = BF16 ? 2 : 4;
-    const int row_bytes = KP * elem + 16;  // padded LDS row
-    unsigned char* tile = smem;                                      // [kTileRows][row_bytes]
-    float* tile_negh = reinterpret_cast<float*>(smem + kTileRows * row_bytes);  // [64]
-    uint64_t* queue = reinterpret_cast<uint64_t*>(tile_negh + kTileRows);      // [waves][32][QCap]
-    uint32_t* qcnt = reinterpret_cast<uint32_t*>(queue + kWaves * kQPerWave * kQCap);
-    float* qthr = reinterpret_cast<float*>(qcnt + kWaves * kQPerWave);
+    const int row_bytes = KP * elem + 16;
+    const int buf_bytes = RT * row_bytes;
+    unsigned char* tile0 = smem;
+    unsigned char* tile1 = smem + buf_bytes;
+    float* negh0 = reinterpret_cast<float*>(smem + 2 * buf_bytes);
+    float* negh1 = negh0 + RT;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, h = lane >> 5, col = lane & 31;
-    const uint32_t qbase = (blockIdx.x * kWaves + wave) * kQPerWave;
+    const uint32_t qbase = (blockIdx.x * W + wave) * kQPerWave;
     const uint32_t split = blockIdx.y;
     const uint32_t db0 = split * a.db_per_split;
     const uint32_t db1 = min(db0 + a.db_per_split, a.npad);
     uint64_t* lists = a.lists + ((size_t)split * a.npad + qbase) * S;
-    uint64_t* myq = queue + (size_t)wave * kQPerWave * kQCap;
-    uint32_t* mycnt = qcnt + wave * kQPerWave;
-    float* mythr = qthr + wave * kQPerWave;
+    uint64_t* myq = a.queue + ((size_t)split * a.npad + qbase) * QC;
 
-    // init this wave's lists, counters and thresholds
     for (int i = lane; i < kQPerWave * S; i += 64) lists[i] = kInfKey;
-    if (lane < kQPerWave) {
-        mycnt[lane] = 0;
-        mythr[lane] = -INFINITY;
-    }
     // B fragments: this lane's query (column) over its half of K
     const uint32_t q = qbase + col;
     constexpr int MAXSTEPS = BF16 ? 8 : 64;  // KP <= 128
@@ -237,140 +356,114 @@ __global__ __launch_bounds__(kWaves * 64) void k_shortlist(ShortlistArgs a) {
             if (s < steps) bqf[s] = src[s];
     }
     float thr = -INFINITY;
+    uint32_t qn = 0;
+
+    const unsigned char* P = reinterpret_cast<const unsigned char*>(a.P);
+    const int row_elems_bytes = KP * elem;
+    const int cpr = row_elems_bytes / 16;  // 16-byte chunks per row
+    const int chunks = RT * cpr;
+    const uint32_t nst = (db1 - db0) / RT;
+    u32x4 R[CPT];
+    float Rn = 0.0f;
+    if (nst > 0) {
+        stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0, row_elems_bytes, chunks, tid);
+        stage_store<CPT, RT, W>(R, Rn, tile0, negh0, row_bytes, cpr, chunks, tid);
+    }
+    // everything loaded so far (query fragments, stage 0) is complete here, so
+    // inside the loop only the one-stage-ahead prefetch is ever in flight and
+    // the MFMAs do not wait on it
+    __builtin_amdgcn_s_waitcnt(0);
+    if (nst > 1) stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0 + RT, row_elems_bytes, chunks, tid);
     __syncthreads();
 
-    // wave-wide merge of query b's queue into its list
-    auto merge_query = [&](int b) {
-        const uint32_t c = mycnt[b];
-        uint64_t qv = (uint32_t)lane < c ? myq[b * kQCap + lane] : kInfKey;
-        uint64_t one[1] = {qv};
-        wave_sort<1>(one);
-        uint64_t v[E];
+    for (uint32_t st = 0; st < nst; st++) {
+        const bool odd = st & 1u;
+        const unsigned char* cur = odd ? tile1 : tile0;
+        const float* curn = odd ? negh1 : negh0;
+        // stage st+1 -> the other buffer (last read in stage st-1, before the
+        // barrier that closed it), then fetch stage st+2 into registers
+        if (st + 1 < nst)
+            stage_store<CPT, RT, W>(R, Rn, odd ? tile0 : tile1, odd ? negh0 : negh1, row_bytes, cpr,
+                                    chunks, tid);
+        if (st + 2 < nst)
+            stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0 + (st + 2) * RT, row_elems_bytes, chunks,
+                                   tid);
+        const uint32_t r0 = db0 + st * RT;
 #pragma unroll
-        for (int e = 0; e < E; e++) v[e] = lists[(size_t)b * S + e * 64 + lane];
-        wave_merge<E>(v, one[0]);
+        for (int sp = 0; sp < RT / 32; sp += 2) {
+            // Two independent accumulation chains: the stage's two 32-row MFMA
+            // tiles, or (32-row stages) the even and odd K steps of one tile.
+            constexpr bool kPair = RT >= 64;
+            f32x16 acc0, acc1;
 #pragma unroll
-        for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[e];
-        const uint64_t last = __shfl((uint32_t)(v[E - 1] >> 32), 63, 64);
-        const uint64_t lastlo = __shfl((uint32_t)v[E - 1], 63, 64);
-        if (lane == 0) {
-            mythr[b] = key_score((last << 32) | lastlo);
-            mycnt[b] = 0;
-        }
-    };
-
-    // epilogue of one 32-row MFMA tile: any element beating the query's threshold?
-    auto epilogue = [&](const f32x16& acc, uint32_t rowbase) {
-        float m = acc[0];
-#pragma unroll
-        for (int r = 1; r < 16; r++) m = fmaxf(m, acc[r]);
-        if (__ballot(m > thr) == 0ull) return;
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const uint32_t db = rowbase + (r >> 2) * 8 + h * 4 + (r & 3);
-            if (acc[r] > thr && db < a.n && db != q) {
-                const uint32_t pos = atomicAdd(&mycnt[col], 1u);
-                myq[col * kQCap + pos] = score_key(acc[r], db);
-            }
-        }
-        // flush queues that the next 32-row tile could overflow
-        const bool full = lane < kQPerWave && mycnt[lane] > (uint32_t)(kQCap - 32);
-        uint64_t need = __ballot(full);
-        while (need) {
-            const int b = __builtin_ctzll(need);
-            need &= need - 1;
-            merge_query(b);
-        }
-        thr = mythr[col];
-    };
-
-    // both 32-row MFMA tiles of the staged 64-row tile, as two independent chains
-    auto process_tile = [&](uint32_t t0) {
-        f32x16 acc0, acc1;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float4 n0 = *reinterpret_cast<const float4*>(tile_negh + i * 8 + h * 4);
-            const float4 n1 = *reinterpret_cast<const float4*>(tile_negh + 32 + i * 8 + h * 4);
-            acc0[i * 4 + 0] = n0.x; acc0[i * 4 + 1] = n0.y; acc0[i * 4 + 2] = n0.z; acc0[i * 4 + 3] = n0.w;
-            acc1[i * 4 + 0] = n1.x; acc1[i * 4 + 1] = n1.y; acc1[i * 4 + 2] = n1.z; acc1[i * 4 + 3] = n1.w;
-        }
-        if (BF16) {
-            const bf16x8* a0 = reinterpret_cast<const bf16x8*>(tile + col * row_bytes + h * KP);
-            const bf16x8* a1 = reinterpret_cast<const bf16x8*>(tile + (32 + col) * row_bytes + h * KP);
-#pragma unroll
-            for (int st = 0; st < MAXSTEPS; st++)
-                if (st < steps) {
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[st], bq[st], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[st], bq[st], acc1, 0, 0, 0);
+            for (int i = 0; i < 4; i++) {
+                const float4 n0 = *reinterpret_cast<const float4*>(curn + sp * 32 + i * 8 + h * 4);
+                acc0[i * 4 + 0] = n0.x; acc0[i * 4 + 1] = n0.y; acc0[i * 4 + 2] = n0.z; acc0[i * 4 + 3] = n0.w;
+                if (kPair) {
+                    const float4 n1 = *reinterpret_cast<const float4*>(curn + (sp + 1) * 32 + i * 8 + h * 4);
+                    acc1[i * 4 + 0] = n1.x; acc1[i * 4 + 1] = n1.y; acc1[i * 4 + 2] = n1.z; acc1[i * 4 + 3] = n1.w;
+                } else {
+                    acc1[i * 4 + 0] = 0.0f; acc1[i * 4 + 1] = 0.0f; acc1[i * 4 + 2] = 0.0f; acc1[i * 4 + 3] = 0.0f;
                 }
-        } else {
-            const float* a0 = reinterpret_cast<const float*>(tile + col * row_bytes) + h * (KP / 2);
-            const float* a1 = reinterpret_cast<const float*>(tile + (32 + col) * row_bytes) + h * (KP / 2);
-#pragma unroll
-            for (int st = 0; st < MAXSTEPS; st++)
-                if (st < steps) {
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[st], bqf[st], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[st], bqf[st], acc1, 0, 0, 0);
-                }
-        }
-        epilogue(acc0, t0);
-        epilogue(acc1, t0 + 32);
-    };
-
-    // Database tiles stream through registers kPF tiles ahead of the LDS stage,
-    // so the global/L2 latency of tile t+kPF hides behind the MFMAs of tile t.
-    const int cpr = KP * elem / 16;           // 16-byte chunks per row
-    const int tile_chunks = kTileRows * cpr;  // contiguous in P
-    const uint32_t ntiles = (db1 - db0) / kTileRows;
-    uint4 pre[kPF][CPT];
-    float pre_n[kPF];
-    auto load_tile = [&](uint4 (&dst)[CPT], float& dn, uint32_t t) {
-        const uint32_t t0 = db0 + t * kTileRows;
-        const uint4* base = reinterpret_cast<const uint4*>(
-            reinterpret_cast<const unsigned char*>(a.P) + (size_t)t0 * KP * elem);
-#pragma unroll
-        for (int j = 0; j < CPT; j++) {
-            const int c = tid + j * kWaves * 64;
-            if (c < tile_chunks) dst[j] = base[c];
-        }
-        if (tid < kTileRows) dn = a.negh[t0 + tid];
-    };
-    auto store_tile = [&](const uint4 (&src)[CPT], float dn) {
-#pragma unroll
-        for (int j = 0; j < CPT; j++) {
-            const int c = tid + j * kWaves * 64;
-            if (c < tile_chunks) {
-                const int r = c / cpr, o = c - r * cpr;
-                *reinterpret_cast<uint4*>(tile + r * row_bytes + o * 16) = src[j];
             }
-        }
-        if (tid < kTileRows) tile_negh[tid] = dn;
-    };
+            const unsigned char* rowA0 = cur + (sp * 32 + col) * row_bytes;
+            const unsigned char* rowA1 = kPair ? cur + ((sp + 1) * 32 + col) * row_bytes : rowA0;
+            if (BF16) {
+                const bf16x8* a0 = reinterpret_cast<const bf16x8*>(rowA0 + h * KP);
+                const bf16x8* a1 = reinterpret_cast<const bf16x8*>(rowA1 + h * KP);
 #pragma unroll
-    for (int p = 0; p < kPF; p++)
-        if ((uint32_t)p < ntiles) load_tile(pre[p], pre_n[p], p);
-    for (uint32_t tb = 0; tb < ntiles; tb += kPF) {
+                for (int s = 0; s < MAXSTEPS; s++)
+                    if (s < steps) {
+                        if (kPair) {
+                            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
+                            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[s], bq[s], acc1, 0, 0, 0);
+                        } else if (s & 1) {
+                            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc1, 0, 0, 0);
+                        } else {
+                            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
+                        }
+                    }
+            } else {
+                // K steps come in fours (KP is a multiple of 8): one 16-byte LDS
+                // read feeds four MFMAs
+                const float4* a0 = reinterpret_cast<const float4*>(rowA0) + h * (KP / 8);
+                const float4* a1 = reinterpret_cast<const float4*>(rowA1) + h * (KP / 8);
 #pragma unroll
-        for (int p = 0; p < kPF; p++) {
-            const uint32_t t = tb + p;
-            if (t < ntiles) {
-                __syncthreads();  // the previous tile's LDS reads are done
-                store_tile(pre[p], pre_n[p]);
-                if (t + kPF < ntiles) load_tile(pre[p], pre_n[p], t + kPF);
-                __syncthreads();
-                process_tile(db0 + t * kTileRows);
+                for (int s4 = 0; s4 < MAXSTEPS / 4; s4++)
+                    if (s4 * 4 < steps) {
+                        const float4 x0 = a0[s4];
+                        const float4 x1 = kPair ? a1[s4] : x0;
+                        const float v0[4] = {x0.x, x0.y, x0.z, x0.w};
+                        const float v1[4] = {x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int s = s4 * 4 + u;
+                            if (kPair) {
+                                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
+                                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1[u], bqf[s], acc1, 0, 0, 0);
+                            } else if (u & 1) {
+                                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc1, 0, 0, 0);
+                            } else {
+                                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
+                            }
+                        }
+                    }
             }
+            if (!kPair) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
+            }
+            epilogue<E, QC>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq);
+            if (kPair) epilogue<E, QC>(acc1, r0 + (sp + 1) * 32, q, h, col, lane, thr, qn, lists, myq);
         }
+        __syncthreads();
     }
     // final flush of every non-empty queue
-    {
-        const bool any = lane < kQPerWave && mycnt[lane] > 0;
-        uint64_t need = __ballot(any);
-        while (need) {
-            const int b = __builtin_ctzll(need);
-            need &= need - 1;
-            merge_query(b);
-        }
+    uint64_t need = __ballot(qn > 0) & 0xFFFFFFFFull;
+    while (need) {
+        const int b = __builtin_ctzll(need);
+        need &= need - 1;
+        merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
     }
 }
 
@@ -438,12 +531,31 @@ __global__ __launch_bounds__(256) void k_rerank(const float* __restrict__ x, uin
     }
 }
 
-size_t shortlist_lds_bytes(bool bf16, int KP) {
+size_t shortlist_lds_bytes(bool bf16, int KP, int RT) {
     const int elem = bf16 ? 2 : 4;
     const size_t row_bytes = (size_t)KP * elem + 16;
-    return kTileRows * row_bytes + kTileRows * sizeof(float) +
-           (size_t)kWaves * kQPerWave * kQCap * sizeof(uint64_t) +
-           (size_t)kWaves * kQPerWave * (sizeof(uint32_t) + sizeof(float));
+    return 2 * RT * row_bytes + 2 * RT * sizeof(float);
+}
+
+constexpr size_t kLdsCu = 160 * 1024;  // LDS per CU on gfx950
+
+struct Shape {
+    int W, RT, QC;  // waves per workgroup, rows per stage, queue capacity
+};
+
+// Candidate shapes, deepest stage first: the first whose two stage buffers
+// leave LDS for two workgroups per CU (else one) and whose stage fits the
+// staging registers is used.  Deeper stages mean fewer barriers per MFMA.
+constexpr Shape kShapes[] = {{4, 128, 64}, {4, 64, 64}, {4, 32, 64}};
+
+Shape shortlist_shape(bool bf16, int KP) {
+    for (int per_cu : {2, 1})
+        for (const Shape& sh : kShapes) {
+            const size_t lds = shortlist_lds_bytes(bf16, KP, sh.RT);
+            const int chunks = sh.RT * (KP * (bf16 ? 2 : 4) / 16);
+            if (lds * per_cu <= kLdsCu && chunks <= kStageChunks * kWavesMax * 64) return sh;
+        }
+    throw Error(MRS_INVALID_ARGUMENT, "d too large for the shortlist kernel");
 }
 
 }  // namespace
@@ -463,9 +575,10 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     const bool bf16 = precision == MRS_KNN_BF16_RERANK;
     const uint32_t n = (uint32_t)n64;
     // bf16: K steps of 16 (8 per lane half); f32: steps of 2, rows kept 16-byte multiples
-    const int KP = bf16 ? ((d + 15) / 16) * 16 : ((d + 3) / 4) * 4;
-    const uint32_t qpb = kWaves * kQPerWave;  // queries per block
-    const uint32_t npad = ((n + qpb - 1) / qpb) * qpb;  // also a multiple of kTileRows
+    const int KP = bf16 ? ((d + 15) / 16) * 16 : ((d + 7) / 8) * 8;
+    // queries are padded to whole 8-wave blocks (a multiple of every shape's
+    // block and of kSplitAlign)
+    const uint32_t npad = ((n + 255) / 256) * 256;
     hipStream_t s = ctx->stream;
 
     hipEvent_t ev[4];
@@ -482,39 +595,60 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                            KP, (void*)P.p, negh.p);
     MRS_CHECK_LAUNCH();
 
-    const uint32_t qblocks = npad / qpb;
+    const Shape sh = shortlist_shape(bf16, KP);
+    const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
     int splits = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
-    uint32_t db_per = ((npad + splits - 1) / splits + kTileRows - 1) / kTileRows * kTileRows;
+    uint32_t db_per = ((npad + splits - 1) / splits + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
     splits = (int)((npad + db_per - 1) / db_per);
     const int S = 64 * E;
     DevBuf<uint64_t> lists((size_t)splits * npad * S);
+    DevBuf<uint64_t> queue((size_t)splits * npad * sh.QC);
 
     ShortlistArgs sa;
     sa.P = P.p;
     sa.negh = negh.p;
     sa.lists = lists.p;
+    sa.queue = queue.p;
     sa.n = n;
     sa.npad = npad;
     sa.KP = KP;
     sa.db_per_split = db_per;
-    const size_t lds = shortlist_lds_bytes(bf16, KP);
-    if (lds > 160 * 1024) throw Error(MRS_INVALID_ARGUMENT, "d too large for the shortlist kernel");
-    auto launch = [&](auto kern) {
+    const size_t lds = shortlist_lds_bytes(bf16, KP, sh.RT);
+    auto launch = [&](auto kern, int W) {
         MRS_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-        hipLaunchKernelGGL(kern, dim3(qblocks, splits), dim3(kWaves * 64), lds, s, sa);
+        hipLaunchKernelGGL(kern, dim3(qblocks, splits), dim3(W * 64), lds, s, sa);
     };
     MRS_HIP(hipEventRecord(ev[1], s));
-    // 16-byte chunks each thread stages per 64-row tile
-    const int cpt = (kTileRows * (KP * (bf16 ? 2 : 4) / 16) + kWaves * 64 - 1) / (kWaves * 64);
-    if (bf16) {
-        if (cpt == 1) E == 1 ? launch(k_shortlist<true, 1, 1>) : launch(k_shortlist<true, 2, 1>);
-        else E == 1 ? launch(k_shortlist<true, 1, 2>) : launch(k_shortlist<true, 2, 2>);
-    } else {
-        if (cpt == 1) E == 1 ? launch(k_shortlist<false, 1, 1>) : launch(k_shortlist<false, 2, 1>);
-        else if (cpt == 2) E == 1 ? launch(k_shortlist<false, 1, 2>) : launch(k_shortlist<false, 2, 2>);
-        else E == 1 ? launch(k_shortlist<false, 1, 4>) : launch(k_shortlist<false, 2, 4>);
-    }
+    bool launched = false;
+    auto pick = [&](auto b16, auto e, auto rt, auto qc, auto w) {
+        constexpr bool B = decltype(b16)::value;
+        constexpr int EE = decltype(e)::value, RTT = decltype(rt)::value;
+        constexpr int QCC = decltype(qc)::value, WW = decltype(w)::value;
+        if (launched || B != bf16 || EE != E || RTT != sh.RT || QCC != sh.QC || WW != sh.W) return;
+        launch(k_shortlist<B, EE, RTT, QCC, WW>, WW);
+        launched = true;
+    };
+    using std::integral_constant;
+    using T = integral_constant<bool, true>;
+    using F = integral_constant<bool, false>;
+    using I1 = integral_constant<int, 1>;
+    using I2 = integral_constant<int, 2>;
+    using I4 = integral_constant<int, 4>;
+    using I32 = integral_constant<int, 32>;
+    using I64 = integral_constant<int, 64>;
+    using I128 = integral_constant<int, 128>;
+    auto all = [&](auto rt, auto qc, auto w) {
+        pick(T{}, I1{}, rt, qc, w);
+        pick(T{}, I2{}, rt, qc, w);
+        pick(F{}, I1{}, rt, qc, w);
+        pick(F{}, I2{}, rt, qc, w);
+    };
+    all(I128{}, I64{}, I4{});
+    all(I64{}, I64{}, I4{});
+    all(I32{}, I64{}, I4{});
+    if (!launched)
+        throw Error(MRS_INVALID_ARGUMENT, "no shortlist kernel for this (precision, d) shape");
     MRS_CHECK_LAUNCH();
     MRS_HIP(hipEventRecord(ev[2], s));
     if (splits > 1) {
