@@ -6,8 +6,11 @@
 //   graph_to_negative_normalised_adjacency (:210-256)  M = -D^-1/2 W D^-1/2
 //       (degrees accumulated per row in COO order, f64).
 //   compute_smallest_eigenpairs_lanczos (src/utils/math.rs:401-627)  Lanczos
-//       with full modified Gram-Schmidt re-orthogonalisation in f64, run for
-//       ALL components at once: vectors are global, the SpMV is block
+//       with full re-orthogonalisation in f64 (the reference: modified
+//       Gram-Schmidt; here classical Gram-Schmidt applied twice, equally
+//       orthogonal, a few blocked launches per step instead of two per basis
+//       vector), run for ALL components at once: vectors are global, the SpMV
+//       (16 lanes per row over precomputed f64 matrix values) is block
 //       diagonal by construction, and every inner product is a segmented
 //       reduction over the component's vertices (fixed chunking, so the result
 //       is identical run to run).  The small tridiagonal problems are solved
@@ -96,28 +99,107 @@ __global__ __launch_bounds__(kWg) void k_cc_jump(uint32_t* __restrict__ lab, uin
     lab[i] = r;
 }
 
-// y = M x with M = -D^-1/2 W D^-1/2 (self loops excluded)
-__global__ __launch_bounds__(kWg) void k_spmv(const uint32_t* __restrict__ off,
-                                              const uint32_t* __restrict__ col,
-                                              const float* __restrict__ val,
-                                              const double* __restrict__ dinv,
-                                              const double* __restrict__ x, uint32_t n,
-                                              double* __restrict__ y) {
+// M's values once per layout: mval[p] = -d_i^-1/2 w_p d_j^-1/2 (0 on the diagonal),
+// the same product order the matrix-vector product used per entry.
+__global__ __launch_bounds__(kWg) void k_norm_vals(const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ col,
+                                                   const float* __restrict__ val,
+                                                   const double* __restrict__ dinv, uint32_t n,
+                                                   double* __restrict__ mval) {
     uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= n) return;
-    double acc = 0.0;
     const double di = dinv[i];
     for (uint32_t p = off[i]; p < off[i + 1]; p++) {
         const uint32_t j = col[p];
-        if (j == i) continue;
-        acc += (-di * (double)val[p] * dinv[j]) * x[j];
+        mval[p] = j == i ? 0.0 : -di * (double)val[p] * dinv[j];
     }
-    y[i] = acc;
+}
+
+// y = M x, 16 lanes per row (strided partial sums + a fixed xor tree: the
+// result does not depend on scheduling).
+constexpr int kRowLanes = 16;
+__global__ __launch_bounds__(kWg) void k_spmv_rows(const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ col,
+                                                   const double* __restrict__ mval,
+                                                   const double* __restrict__ x, uint32_t n,
+                                                   double* __restrict__ y) {
+    const uint32_t row = (blockIdx.x * kWg + threadIdx.x) / kRowLanes;
+    const int sub = threadIdx.x & (kRowLanes - 1);
+    double acc = 0.0;
+    if (row < n)
+        for (uint32_t p = off[row] + sub; p < off[row + 1]; p += kRowLanes) acc += mval[p] * x[col[p]];
+#pragma unroll
+    for (int o = kRowLanes / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+    if (row < n && sub == 0) y[row] = acc;
 }
 
 struct Chunk {
     uint32_t comp, begin, end;
 };
+
+// Classical Gram-Schmidt, blocked: partial[chunk][k] = sum over the chunk's
+// vertices of V_k[i] * w[i] for k in [k0, k0 + kKB), one block per (chunk, k-block).
+constexpr int kKB = 8;
+__global__ __launch_bounds__(kWg) void k_multidot(const Chunk* __restrict__ chunks,
+                                                  const uint32_t* __restrict__ perm,
+                                                  const double* __restrict__ V, uint32_t n,
+                                                  const double* __restrict__ w, int m,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[kKB][kWg];
+    const Chunk c = chunks[blockIdx.x];
+    const int k0 = blockIdx.y * kKB;
+    double acc[kKB];
+#pragma unroll
+    for (int t = 0; t < kKB; t++) acc[t] = 0.0;
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
+        const uint32_t i = perm[q];
+        const double wi = w[i];
+#pragma unroll
+        for (int t = 0; t < kKB; t++)
+            if (k0 + t < m) acc[t] += V[(size_t)(k0 + t) * n + i] * wi;
+    }
+#pragma unroll
+    for (int t = 0; t < kKB; t++) sh[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    for (int st = kWg / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+#pragma unroll
+            for (int t = 0; t < kKB; t++) sh[t][threadIdx.x] += sh[t][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kKB && k0 + (int)threadIdx.x < m)
+        partial[(size_t)blockIdx.x * m + k0 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// h[comp][k] = sum of the component's chunk partials, in chunk order
+__global__ __launch_bounds__(kWg) void k_multidot_finish(const double* __restrict__ partial,
+                                                         const uint32_t* __restrict__ comp_chunk0,
+                                                         uint32_t n_comp, int m,
+                                                         double* __restrict__ h) {
+    const uint32_t t = blockIdx.x * kWg + threadIdx.x;
+    if (t >= n_comp * (uint32_t)m) return;
+    const uint32_t c = t / m, k = t % m;
+    double s = 0.0;
+    for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) s += partial[(size_t)q * m + k];
+    h[t] = s;
+}
+
+// w[i] -= sum_k V_k[i] h[comp(i)][k] for live components
+__global__ __launch_bounds__(kWg) void k_cgs_update(double* __restrict__ w,
+                                                    const double* __restrict__ V, uint32_t n,
+                                                    const double* __restrict__ h, int m,
+                                                    const uint32_t* __restrict__ comp_of,
+                                                    const uint8_t* __restrict__ live) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = comp_of[i];
+    if (c == 0xFFFFFFFFu || !live[c]) return;
+    const double* hc = h + (size_t)c * m;
+    double s = 0.0;
+    for (int k = 0; k < m; k++) s += V[(size_t)k * n + i] * hc[k];
+    w[i] -= s;
+}
 
 __device__ inline double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -158,19 +240,6 @@ __global__ __launch_bounds__(kWg) void k_seg_finish(const double* __restrict__ p
     double t = 0.0;
     for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) t += partial[q];
     out[c] = do_sqrt ? sqrt(t) : t;
-}
-
-// w[i] -= coef[comp(i)] * v[i] for live components
-__global__ __launch_bounds__(kWg) void k_axpy_seg(double* __restrict__ w,
-                                                  const double* __restrict__ v,
-                                                  const double* __restrict__ coef,
-                                                  const uint32_t* __restrict__ comp_of,
-                                                  const uint8_t* __restrict__ live, uint32_t n) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = comp_of[i];
-    if (c == 0xFFFFFFFFu || !live[c]) return;
-    w[i] -= coef[c] * v[i];
 }
 
 // w -= alpha v + beta_prev v_old  (the three-term recurrence, math.rs:478-483)
@@ -484,7 +553,10 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dlc.p, lc_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
         DevBuf<double> v(n), v_old(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
-            alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl);
+            alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl),
+            mpart((size_t)nch * max_iter), hcoef((size_t)nl * max_iter), mval(nnz ? nnz : 1);
+        hipLaunchKernelGGL(k_norm_vals, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, g->coo_col.p,
+                           g->coo_val.p, dinv.p, n, mval.p);
         DevBuf<uint8_t> live(nl);
         std::vector<uint8_t> hlive(nl, 1);
         MRS_HIP(hipMemcpyAsync(live.p, hlive.data(), nl, hipMemcpyHostToDevice, s));
@@ -514,17 +586,23 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
                                    hipMemcpyDeviceToDevice, s));
             for (uint32_t q = 0; q < nl; q++)
                 if (hlive[q]) n_done[q] = j + 1;
-            hipLaunchKernelGGL(k_spmv, gn, dim3(kWg), 0, s, off.p, g->coo_col.p, g->coo_val.p,
-                               dinv.p, v.p, n, w.p);
+            hipLaunchKernelGGL(k_spmv_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
+                               s, off.p, g->coo_col.p, mval.p, v.p, n, w.p);
             seg(w.p, v.p, alpha_d.p + (size_t)j * nl, 0);
             hipLaunchKernelGGL(k_recur, gn, dim3(kWg), 0, s, w.p, v.p, v_old.p,
                                alpha_d.p + (size_t)j * nl,
                                j > 0 ? beta_d.p + (size_t)(j - 1) * nl : beta_d.p, j > 0 ? 1 : 0,
                                dlc.p, live.p, n);
-            for (int k = 0; k <= j; k++) {
-                seg(w.p, V.p + (size_t)k * n, sc.p, 0);
-                hipLaunchKernelGGL(k_axpy_seg, gn, dim3(kWg), 0, s, w.p, V.p + (size_t)k * n,
-                                   sc.p, dlc.p, live.p, n);
+            // full re-orthogonalisation against V_0..V_j: classical Gram-Schmidt
+            // applied twice (as orthogonal as modified GS, 6 launches per step)
+            const int m = j + 1;
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
+                                   dchunks.p, dperm.p, V.p, n, w.p, m, mpart.p);
+                hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWg)),
+                                   dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p);
+                hipLaunchKernelGGL(k_cgs_update, gn, dim3(kWg), 0, s, w.p, V.p, n, hcoef.p, m,
+                                   dlc.p, live.p);
             }
             seg(w.p, w.p, beta_d.p + (size_t)j * nl, 1);
             MRS_CHECK_LAUNCH();
