@@ -37,6 +37,23 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(workload: str, kernel_prefix: str):
+    """HBM bytes per launch of a kernel, from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summaries under profiles/ (copy-kernel calibrated); None if absent."""
+    import re
+
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for block in re.findall(r"```json\n(.*?)```", f.read_text(), re.S):
+            try:
+                entries = json.loads(block)
+            except ValueError:
+                continue
+            for key, e in entries.items():
+                if key.startswith(f"{workload}:{kernel_prefix}"):
+                    return e["bytes_per_launch"], f"profiles/{f.name}"
+    return None, None
+
+
 def synthetic_knn_graph(n: int, k: int, n_epochs: int, seed: int = 42):
     """A UMAP-shaped graph for the epoch workload: exact kNN (k) of n points drawn
     from 50 2-D Gaussian blobs, membership weights exp(-(d - rho)/sigma) with
@@ -119,6 +136,7 @@ def bench_epochs(args):
     bytes_per_launch = bytes_edge_total / n_epochs
     achieved = bytes_per_launch / (edge_avg_ms * 1e-3) / 1e9
     adam_ms, adam_launches = kt["adam_clear"]
+    traffic, traffic_src = pmc_traffic("epochs", "k_epoch_edges<2")
 
     res = dict(
         metric=METRIC,
@@ -141,7 +159,8 @@ def bench_epochs(args):
                  for kk, v in kt.items()},
         roofline=dict(bound="hbm", kernel="k_epoch_edges", achieved=achieved,
                       peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, avg_launch_us=edge_avg_ms * 1e3,
+                      traffic=traffic, traffic_source=traffic_src,
+                      avg_launch_us=edge_avg_ms * 1e3,
                       algorithmic_bytes_per_launch=bytes_per_launch),
         wall_s=wall,
     )
@@ -244,6 +263,8 @@ def bench_fit(args):
     r = p.optim_params.neg_sample_rate
     bytes_edge = (16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)) / n_epochs
     epoch_gbs = bytes_edge / (edge_ms / edge_launches * 1e-3) / 1e9
+    sl_traffic, sl_src = pmc_traffic("fit", "k_shortlist<true")
+    ep_traffic, ep_src = pmc_traffic("fit", "k_epoch_edges<2")
 
     res = dict(
         metric=METRIC, value=ms_per_step / 1e3, unit="s", n_gpus=1, steps=args.steps,
@@ -257,10 +278,12 @@ def bench_fit(args):
         knn_fp32_vs_bf16=knn_cmp,
         edge_updates_per_s=A_total / (st["epochs_ms"] * 1e-3),
         roofline=dict(bound="mfma", kernel="k_shortlist<bf16>", achieved=achieved, peak=2500.0,
-                      unit="TFLOP/s", frac=achieved / 2500.0, traffic=None,
-                      avg_launch_ms=kt["shortlist_ms"], algorithmic_flops_per_launch=flops),
+                      unit="TFLOP/s", frac=achieved / 2500.0, traffic=sl_traffic,
+                      traffic_source=sl_src, avg_launch_ms=kt["shortlist_ms"],
+                      algorithmic_flops_per_launch=flops),
         epoch_kernel=dict(bound="hbm", kernel="k_epoch_edges", achieved=epoch_gbs,
                           peak=HBM_PEAK_GBS, unit="GB/s", frac=epoch_gbs / HBM_PEAK_GBS,
+                          traffic=ep_traffic, traffic_source=ep_src,
                           avg_launch_us=1e3 * edge_ms / edge_launches,
                           algorithmic_bytes_per_launch=bytes_edge),
     )

@@ -491,33 +491,53 @@ __global__ __launch_bounds__(256) void k_merge_splits(uint64_t* __restrict__ lis
     for (int e = 0; e < E; e++) lists[(size_t)q * S + e * 64 + lane] = v[e];
 }
 
-// exact FP32 re-rank of the shortlist; one wave per query
+// Exact FP32 re-rank of the shortlist; one wave per query, lane = candidate.
+// The wave first stages its 64 candidate rows into LDS with coalesced row
+// loads (one row per load instruction); each lane then accumulates its own
+// distance in dimension order with fmaf, the oracle's arithmetic.  Rows are
+// padded to d+1 floats so the lanes' column reads fall in distinct banks.
+constexpr int kRerankWaves = 4;
 template <int E>
-__global__ __launch_bounds__(256) void k_rerank(const float* __restrict__ x, uint32_t n, int d,
-                                                const uint64_t* __restrict__ lists, int k,
-                                                int32_t* __restrict__ out_idx,
-                                                float* __restrict__ out_dist) {
+__global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __restrict__ x,
+                                                              uint32_t n, int d,
+                                                              const uint64_t* __restrict__ lists,
+                                                              int k, int32_t* __restrict__ out_idx,
+                                                              float* __restrict__ out_dist) {
+    extern __shared__ float rsh[];  // per wave: 64 rows of d+1 floats, then the query row
     constexpr int S = 64 * E;
-    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (q >= n) return;
-    const float* xq = x + (size_t)q * d;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * kRerankWaves + wave;
+    if (q >= n) return;  // whole waves leave; nothing below is block-wide
+    const int ld = d + 1;
+    float* rows = rsh + (size_t)wave * (64 * ld + d);
+    float* xq = rows + 64 * ld;
+    for (int t = lane; t < d; t += 64) xq[t] = x[(size_t)q * d + t];
     uint64_t key[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const uint64_t lk = lists[(size_t)q * S + e * 64 + lane];
-        if (lk == kInfKey) {
-            key[e] = kInfKey;
-            continue;
-        }
         const uint32_t c = (uint32_t)lk;
-        const float* xc = x + (size_t)c * d;
-        float s = 0.0f;
-        for (int t = 0; t < d; t++) {
-            const float df = xq[t] - xc[t];
-            s = fmaf(df, df, s);
+        const bool ok = lk != kInfKey;
+        for (int r = 0; r < 64; r++) {
+            const uint32_t cr = __builtin_amdgcn_readlane(ok ? c : 0xFFFFFFFFu, r);
+            if (cr == 0xFFFFFFFFu) continue;  // wave-uniform: an empty slot
+            for (int t = lane; t < d; t += 64) rows[r * ld + t] = x[(size_t)cr * d + t];
         }
-        key[e] = ((uint64_t)__float_as_uint(s) << 32) | c;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (!ok) {
+            key[e] = kInfKey;
+        } else {
+            const float* xc = rows + lane * ld;
+            float s = 0.0f;
+            for (int t = 0; t < d; t++) {
+                const float df = xq[t] - xc[t];
+                s = fmaf(df, df, s);
+            }
+            key[e] = ((uint64_t)__float_as_uint(s) << 32) | c;
+        }
+        __builtin_amdgcn_wave_barrier();  // rows are reused by the next e
     }
     wave_sort<E>(key);
 #pragma unroll
@@ -529,6 +549,10 @@ __global__ __launch_bounds__(256) void k_rerank(const float* __restrict__ x, uin
             out_dist[(size_t)q * k + g] = ok ? __uint_as_float((uint32_t)(key[e] >> 32)) : INFINITY;
         }
     }
+}
+
+size_t rerank_lds_bytes(int d) {
+    return (size_t)kRerankWaves * (64 * (d + 1) + d) * sizeof(float);
 }
 
 size_t shortlist_lds_bytes(bool bf16, int KP, int RT) {
@@ -660,12 +684,15 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                                npad, splits);
         MRS_CHECK_LAUNCH();
     }
-    if (E == 1)
-        hipLaunchKernelGGL(k_rerank<1>, dim3(div_up(n, 4)), dim3(256), 0, s, x, n, d, lists.p, k,
-                           out_idx, out_dist);
-    else
-        hipLaunchKernelGGL(k_rerank<2>, dim3(div_up(n, 4)), dim3(256), 0, s, x, n, d, lists.p, k,
-                           out_idx, out_dist);
+    const size_t rlds = rerank_lds_bytes(d);
+    auto rerank = [&](auto kern) {
+        MRS_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)rlds));
+        hipLaunchKernelGGL(kern, dim3(div_up(n, kRerankWaves)), dim3(kRerankWaves * 64), rlds, s, x,
+                           n, d, lists.p, k, out_idx, out_dist);
+    };
+    if (E == 1) rerank(k_rerank<1>);
+    else rerank(k_rerank<2>);
     MRS_CHECK_LAUNCH();
     MRS_HIP(hipEventRecord(ev[3], s));
     MRS_HIP(hipEventSynchronize(ev[3]));
