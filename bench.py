@@ -10,6 +10,8 @@ Prints ONE JSON line.  Workloads:
            2-D output.  One step = one fit with the data resident in HBM.
   epochs   the "adam_gpu" epoch loop alone (500 Adam epochs on a 1M-node
            k=15-style graph): one step = one full 500-epoch optimisation.
+  serve    the served stream: Poisson arrivals of uneven fits into the fit
+           server, fits/s and p50/p99 latency against sequential fits.
 
 The CPU baseline is the C restatement of the reference's parallel Adam
 (optimise_embedding_adam_parallel) timed on this host's cores on a bounded
@@ -361,6 +363,103 @@ def copy_probe(mib: int) -> None:
     del src, dst
 
 
+def bench_serve(args):
+    """Served stream: requests with n log-uniform in [2000, 200000], d in {32, 50,
+    100}, 5-40 Gaussian clusters, k = 15, 200 Adam epochs, Poisson arrivals at a
+    fixed rate; fits/s and p50/p99 latency of the server against the same
+    requests fitted one after another with mrs_umap_gpu."""
+    import threading
+
+    if args.hw_queues:
+        # hardware queues per process: each prep worker and the batcher get one,
+        # so their small kernels overlap instead of queueing behind each other
+        # (set before the HIP runtime starts)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+    import mrs_amd as M
+
+    try:  # a switched-off server refuses here, before any fit runs
+        M.Server(0, 0, args.prep_workers, args.max_active).close()
+    except M.ManifoldsError as e:
+        raise SystemExit(f"serve workload unavailable: {e}")
+    rng = np.random.default_rng(42)
+    # a pool of distinct datasets covering the size range; requests draw from it
+    # with their own fit seeds (every request recomputes kNN, graph and layout)
+    pool = []
+    for i in range(args.pool):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(200_000))))
+        d = int(rng.choice([32, 50, 100]))
+        c = int(rng.integers(5, 41))
+        centres = rng.normal(0, 5, size=(c, d)).astype(np.float32)
+        lab = rng.integers(0, c, size=n)
+        pool.append((centres[lab] + rng.normal(size=(n, d)).astype(np.float32)).astype(np.float32))
+    reqs = [(int(rng.integers(0, len(pool))), int(rng.integers(0, 2**31))) for _ in range(args.requests)]
+    p = M.UmapParamsGpu.default()
+    p.k = 15
+    p.optim_params.n_epochs = 200
+    p.knn_precision = M.KNN_BF16_RERANK
+    log(f"[bench] serve: {len(pool)} datasets, {len(reqs)} requests, "
+        f"{sum(pool[i].shape[0] for i, _ in reqs) / len(reqs):.0f} points per request on average")
+    # sequential baseline: the same requests (a seeded sample), one mrs_umap_gpu at a time
+    ctx = M.Context(0)
+    M.umap_gpu(ctx, pool[0], p, seed=1)  # warm-up
+    sample = reqs[: args.seq_sample]
+    stage_sum = {}
+    t0 = time.perf_counter()
+    for i, sd in sample:
+        M.umap_gpu(ctx, pool[i], p, seed=sd)
+        for kk, v in M.last_fit_timings().items():
+            stage_sum[kk] = stage_sum.get(kk, 0.0) + v
+    seq_s = (time.perf_counter() - t0) / len(sample)
+    seq_fps = 1.0 / seq_s
+    seq_stages = {kk: v / len(sample) for kk, v in stage_sum.items()}
+    log(f"[bench] sequential: {seq_s * 1e3:.1f} ms per fit ({seq_fps:.2f} fits/s); "
+        f"mean stages {seq_stages}")
+    rate = args.rate if args.rate > 0 else 8.0 * seq_fps
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    with M.Server(0, 0, args.prep_workers, args.max_active) as sv:
+        sv.wait(sv.submit(pool[0], p, seed=1))  # warm-up
+        tickets = [None] * len(reqs)
+        lat = [None] * len(reqs)
+
+        def collector():
+            for q in range(len(reqs)):
+                while tickets[q] is None:
+                    time.sleep(1e-4)
+                out, times = sv.wait(tickets[q])
+                lat[q] = times["latency_ms"]
+
+        th = threading.Thread(target=collector)
+        start = time.perf_counter()
+        th.start()
+        for q, ((i, sd), at) in enumerate(zip(reqs, arrivals)):
+            delay = start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[q] = sv.submit(pool[i], p, seed=sd)
+        th.join()
+        span = time.perf_counter() - start
+        st = sv.stats()
+    lat = np.array(lat)
+    served_fps = len(reqs) / span
+    return dict(
+        metric="served fits/s and p99 latency (UMAP fits, k=15, 200 epochs)",
+        value=served_fps, unit="fits/s", n_gpus=1, steps=1, warmup=1,
+        ms_per_step=span * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="bf16-mfma/f32", data="synthetic",
+        config=dict(workload=f"served stream: {len(reqs)} requests, Poisson arrivals at "
+                    f"{rate:.2f}/s, n log-uniform [2000, 200000], d in {{32, 50, 100}}, "
+                    f"5-40 clusters, k=15, 200 epochs, BF16 kNN", requests=len(reqs),
+                    rate_per_s=rate, prep_workers=args.prep_workers, max_active=args.max_active,
+                    hw_queues=os.environ.get("GPU_MAX_HW_QUEUES")),
+        sequential_fits_per_s=seq_fps, sequential_sample=len(sample),
+        sequential_stage_ms=seq_stages,
+        speedup_vs_sequential=served_fps / seq_fps,
+        latency_ms=dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                        mean=float(lat.mean()), max=float(lat.max())),
+        server=st,
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,6 +471,15 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-epochs", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
+    ap.add_argument("--pool", type=int, default=48, help="serve: distinct datasets")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serve: arrivals per second (0 = 8x the sequential fit rate)")
+    ap.add_argument("--seq-sample", type=int, default=40, help="serve: sequential sample size")
+    ap.add_argument("--prep-workers", type=int, default=4)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="serve: GPU_MAX_HW_QUEUES for the run (0 = leave as is; at most 32)")
+    ap.add_argument("--max-active", type=int, default=256)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a device buffer of this many MiB (a plain copy kernel of "
                          "known size, to scale hardware byte counters)")
@@ -387,6 +495,8 @@ def main():
         res = bench_fit(args)
     elif args.workload == "epochs":
         res = bench_epochs(args)
+    elif args.workload == "serve":
+        res = bench_serve(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     print(json.dumps(res))

@@ -285,6 +285,57 @@ int mrs_optimise_embedding_adam_gpu(mrs_ctx *ctx, float *embd, int64_t n, int32_
                                     const mrs_dens_state *dens, uint64_t seed,
                                     int32_t verbose, int32_t strategy);
 
+/* Host solver of the Lanczos tridiagonal problems of the spectral layout
+ * (compute_smallest_eigenpairs_lanczos's eigen step, src/utils/math.rs:560-590):
+ * symmetric tridiagonal diag [m], offdiag [m-1] -> eigenvalues ascending [m]
+ * and eigenvectors as columns of evecs [m][m] (row-major).  Host memory only;
+ * needs no device. */
+int mrs_tridiag_eigen(const double *diag, const double *offdiag, int32_t m, double *evals,
+                      double *evecs);
+
+/* ---------------------------------------------------------------- serving
+ * A fit server: continuous batching of many uneven umap_gpu fits on one GPU.
+ * The reference has no serving layer; each request has exactly the semantics
+ * of mrs_umap_gpu (src/lib.rs:3465-3487) with the same parameters and seed.
+ * Requests are admitted in arrival order while their estimated device memory
+ * (mrs_fit_bytes_estimate) fits the HBM budget and fewer than max_active fits
+ * are resident; prep_workers threads prepare admitted fits concurrently (kNN,
+ * graph, spectral layout, each worker on its own stream); one batcher advances
+ * every resident fit by one epoch per tick (one segmented launch pair for all
+ * edge-parallel fits, per-fit launches for node-exact ones).  Node-exact
+ * requests return exactly what mrs_umap_gpu returns; edge-parallel ones differ
+ * from it only by the FP32 atomic summation order. */
+typedef struct mrs_server mrs_server;
+
+/* Upper estimate of one fit's peak device bytes (preparation and resident state). */
+size_t mrs_fit_bytes_estimate(int64_t n, int32_t d, int32_t k, int32_t n_dim);
+
+/* hbm_budget 0 = half the device's free memory at creation. */
+int mrs_server_create(int32_t device, size_t hbm_budget, int32_t prep_workers, int32_t max_active,
+                      mrs_server **out);
+
+/* Queue one fit of data [n][d] (host or device; it must stay valid until the
+ * request completes).  params as for mrs_umap_gpu (densMAP is not served).
+ * A request larger than the whole budget is InvalidArgument. */
+int mrs_server_submit(mrs_server *sv, const float *data, int64_t n, int32_t d,
+                      const mrs_umap_params_gpu *params, uint64_t seed, int64_t *ticket);
+
+/* state: 0 queued, 1 preparing, 2 in epochs, 3 done, 4 failed. */
+int mrs_server_poll(mrs_server *sv, int64_t ticket, int32_t *state);
+
+/* Block until the request completes; copy its [n_dim][n] embedding to out
+ * (host or device; may be NULL) and return the request's status.
+ * times_ms (may be NULL) = {queued, preparing, submit-to-result}.  The ticket
+ * is consumed. */
+int mrs_server_wait(mrs_server *sv, int64_t ticket, float *out, double *times_ms);
+
+/* out[7] = {submitted, completed, failed, ticks, peak resident fits,
+ *           peak admitted bytes, budget bytes}. */
+int mrs_server_stats(mrs_server *sv, int64_t *out);
+
+/* Finishes every queued request, then stops the threads and frees everything. */
+int mrs_server_destroy(mrs_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,7 +3,10 @@
 
 #include <algorithm>
 #include <cctype>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "epoch_common.hpp"
 #include "mrs_internal.hpp"
@@ -11,6 +14,21 @@
 namespace mrs {
 
 static thread_local std::string g_last_error;
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local bool t_stream_set = false;
+
+StreamScope::StreamScope(hipStream_t s) : prev(t_stream), prev_set(t_stream_set) {
+    t_stream = s;
+    t_stream_set = true;
+}
+StreamScope::~StreamScope() {
+    t_stream = prev;
+    t_stream_set = prev_set;
+}
+bool current_stream(hipStream_t* s) {
+    if (t_stream_set) *s = t_stream;
+    return t_stream_set;
+}
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
@@ -122,8 +140,21 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t*
         hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, out, last.p, n, total_dev);
         MRS_CHECK_LAUNCH();
     }
-    // temporaries are freed on return: make sure the queued work no longer needs them
-    MRS_HIP(hipStreamSynchronize(s));
+    // temporaries are freed on return: unless that free is stream-ordered it
+    // is immediate, so the queued work must be finished first
+    if (!sums.ordered) MRS_HIP(hipStreamSynchronize(s));
+}
+
+void ensure_dynamic_lds(const void* kern, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> limit;
+    int dev = 0;
+    MRS_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& cur = limit[{dev, kern}];
+    if (bytes <= cur) return;
+    MRS_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
 }
 
 }  // namespace mrs
@@ -142,6 +173,12 @@ int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx** out) {
         if (device < 0 || device >= count)
             throw mrs::Error(MRS_DEVICE_ERROR, "no such HIP device: " + std::to_string(device));
         MRS_HIP(hipSetDevice(device));
+        // keep freed pool memory for reuse instead of returning it at each sync
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
         auto* c = new mrs_ctx();
         c->device = device;
         c->hbm_budget = hbm_budget;

@@ -18,6 +18,7 @@
 #include "graph.hpp"
 #include "knn.hpp"
 #include "spectral.hpp"
+#include "fit.hpp"
 #include "umap_state.hpp"
 
 namespace mrs {
@@ -72,15 +73,15 @@ void random_layout_dev(mrs_ctx* ctx, float* out, int64_t n, int32_t dim, uint64_
     MRS_CHECK_LAUNCH();
 }
 
-void umap_fit(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const int32_t* pre_idx,
-              const float* pre_dist, int32_t pre_k, const mrs_umap_params_gpu* p,
-              const mrs_dens_params* dens, uint64_t seed, int32_t verbose, float* out) {
+std::unique_ptr<mrs_umap_state> fit_prepare(mrs_ctx* ctx, const float* data, int64_t n, int32_t d,
+                                            const int32_t* pre_idx, const float* pre_dist,
+                                            int32_t pre_k, const mrs_umap_params_gpu* p,
+                                            const mrs_dens_params* dens, uint64_t seed,
+                                            int32_t verbose, FitTimings& t) {
     if (!p) throw Error(MRS_INVALID_ARGUMENT, "params is NULL");
     if (n <= 0) throw Error(MRS_NO_DATA, "UMAP: empty data (NoData)");
     if (p->n_dim <= 0 || p->n_dim > 16) throw Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
     hipStream_t s = ctx->stream;
-    FitTimings t;
-    const double t_all = now_ms();
 
     // parse_umap_optimiser_gpu, src/lib.rs:3541-3547: unknown -> AdamGpu
     const std::string opt = lower(p->optimiser);
@@ -177,8 +178,7 @@ void umap_fit(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const int32
     MRS_HIP(hipStreamSynchronize(s));
     t.init_ms = now_ms() - t0;
 
-    // ---- optimise (optimise_embedding_adam_gpu, umap_optimiser_gpu.rs:1302-1369)
-    t0 = now_ms();
+    // ---- the optimiser state (optimise_embedding_adam_gpu's upload, umap_optimiser_gpu.rs:1302-1325)
     if (g->n_edges == 0) throw Error(MRS_NO_GRAPH_EDGES, "UMAP: no edges to optimise (NoGraphEdges)");
     mrs_umap_state* raw_state = nullptr;
     {
@@ -186,23 +186,43 @@ void umap_fit(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const int32
                                            dens ? &ds : nullptr, &raw_state);
         if (rc != MRS_OK) throw Error(rc, mrs_last_error());
     }
-    std::unique_ptr<mrs_umap_state> st(raw_state);
-    g.reset();  // graph arrays now live in the state
+    t.n_edges = raw_state->n_edges;
+    return std::unique_ptr<mrs_umap_state>(raw_state);
+}
+
+void fit_output_async(const mrs_umap_state* st, float* out_dev) {
+    hipLaunchKernelGGL(k_transpose_out, dim3(div_up(st->n, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->embd.p, (uint32_t)st->n, st->dim, out_dev);
+    MRS_CHECK_LAUNCH();
+}
+
+void fit_output(const mrs_umap_state* st, float* out) {
+    hipStream_t s = st->ctx->stream;
+    DevView<float> ov(out, (size_t)st->n * st->dim, s, false);
+    hipLaunchKernelGGL(k_transpose_out, dim3(div_up(st->n, kWg)), dim3(kWg), 0, s, st->embd.p,
+                       (uint32_t)st->n, st->dim, ov.p);
+    MRS_CHECK_LAUNCH();
+    ov.download(s);
+    MRS_HIP(hipStreamSynchronize(s));
+}
+
+void umap_fit(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const int32_t* pre_idx,
+              const float* pre_dist, int32_t pre_k, const mrs_umap_params_gpu* p,
+              const mrs_dens_params* dens, uint64_t seed, int32_t verbose, float* out) {
+    FitTimings t;
+    const double t_all = now_ms();
+    std::unique_ptr<mrs_umap_state> st =
+        fit_prepare(ctx, data, n, d, pre_idx, pre_dist, pre_k, p, dens, seed, verbose, t);
+    // ---- optimise (optimise_embedding_adam_gpu, umap_optimiser_gpu.rs:1302-1369)
+    const double t0 = now_ms();
     if (verbose > 0)
         std::printf("Optimising embedding via Adam (GPU-accelerated) (%d epochs) on %lld edges...\n",
                     p->optim_params.n_epochs, (long long)st->n_edges);
     run_epochs(st.get(), 0, p->optim_params.n_epochs, (uint32_t)seed, p->epoch_strategy);
-    MRS_HIP(hipStreamSynchronize(s));
+    MRS_HIP(hipStreamSynchronize(ctx->stream));
     t.epochs_ms = now_ms() - t0;
-    t.n_edges = st->n_edges;
-
     // ---- output [n_dim][n]
-    DevView<float> ov(out, (size_t)n * dim, s, false);
-    hipLaunchKernelGGL(k_transpose_out, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
-                       (uint32_t)n, dim, ov.p);
-    MRS_CHECK_LAUNCH();
-    ov.download(s);
-    MRS_HIP(hipStreamSynchronize(s));
+    fit_output(st.get(), out);
     t.total_ms = now_ms() - t_all;
     g_last_fit = t;
     if (verbose > 0) std::printf("UMAP (GPU) complete!\n");
@@ -248,6 +268,7 @@ int mrs_umap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const in
     return mrs::guarded([&] {
         if (!ctx || !out) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
+        mrs::StreamScope scope(ctx->stream);
         mrs::umap_fit(ctx, data, n, d, knn_idx, knn_dist, knn_k, params, nullptr, seed, verbose,
                       out);
     });
@@ -260,6 +281,7 @@ int mrs_densmap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d,
     return mrs::guarded([&] {
         if (!ctx || !out || !dens) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
+        mrs::StreamScope scope(ctx->stream);
         mrs::umap_fit(ctx, data, n, d, knn_idx, knn_dist, knn_k, params, dens, seed, verbose,
                       out);
     });

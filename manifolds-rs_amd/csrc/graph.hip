@@ -552,7 +552,6 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
                            raw_off.p, n, nullptr, g->u_off.p, g->u_row.p, g->u_col.p, g->u_w.p,
                            mix_weight, 1.0f - mix_weight);
         MRS_CHECK_LAUNCH();
-        raw.release();
         // filter flags
         DevBuf<uint32_t> max_bits(2);
         MRS_HIP(hipMemsetAsync(max_bits.p, 0, max_bits.bytes(), s));
@@ -580,6 +579,7 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         uint32_t tot[3];
         MRS_HIP(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
         MRS_HIP(hipStreamSynchronize(s));
+        raw.release();  // the union read it; the stream is idle here
         // the totals size every output array below: check them against the union
         if (tot[0] > m || tot[1] > m || tot[2] > m)
             throw Error(MRS_DEVICE_ERROR, "graph scan totals exceed the union size");
@@ -678,6 +678,7 @@ int mrs_fuzzy_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_dist,
                     mrs_graph** out) {
     return mrs::guarded([&] {
         if (!ctx || !out || !gp) throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        mrs::StreamScope scope(ctx->stream);
         if (n <= 0 || !knn_idx || !knn_dist) throw Error(MRS_NO_DATA, "UMAP: empty kNN (NoData)");
         MRS_HIP(hipSetDevice(ctx->device));
         const size_t nk = (size_t)n * (size_t)k;
@@ -692,6 +693,7 @@ int mrs_graph_destroy(mrs_graph* g) {
     return mrs::guarded([&] {
         if (!g) return;
         (void)hipStreamSynchronize(g->ctx->stream);
+        mrs::StreamScope scope(g->ctx->stream);
         delete g;
     });
 }
@@ -735,6 +737,7 @@ int mrs_dens_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const int32_t* k
     return mrs::guarded([&] {
         if (!ctx || !g || !r_out || !mu_sum_out)
             throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        mrs::StreamScope scope(ctx->stream);
         const size_t nk = (size_t)g->n * (size_t)g->k;
         mrs::DevView<int32_t> idx(knn_idx, nk, ctx->stream);
         mrs::DevView<float> dist(knn_dist, nk, ctx->stream);
@@ -755,6 +758,7 @@ int mrs_umap_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const float* emb
     return mrs::guarded([&] {
         if (!ctx || !g || !embd || !params || !out)
             throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        mrs::StreamScope scope(ctx->stream);
         if (n_dim <= 0 || n_dim > 16) throw Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
         if (params->n_epochs <= 0) throw Error(MRS_INVALID_ARGUMENT, "n_epochs must be positive");
         if (g->n_edges == 0) throw Error(MRS_NO_GRAPH_EDGES, "UMAP: no edges to optimise (NoGraphEdges)");

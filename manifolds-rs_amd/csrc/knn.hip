@@ -639,8 +639,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     sa.db_per_split = db_per;
     const size_t lds = shortlist_lds_bytes(bf16, KP, sh.RT);
     auto launch = [&](auto kern, int W) {
-        MRS_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
+        ensure_dynamic_lds((const void*)kern, lds);
         hipLaunchKernelGGL(kern, dim3(qblocks, splits), dim3(W * 64), lds, s, sa);
     };
     MRS_HIP(hipEventRecord(ev[1], s));
@@ -686,8 +685,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     }
     const size_t rlds = rerank_lds_bytes(d);
     auto rerank = [&](auto kern) {
-        MRS_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)rlds));
+        ensure_dynamic_lds((const void*)kern, rlds);
         hipLaunchKernelGGL(kern, dim3(div_up(n, kRerankWaves)), dim3(kRerankWaves * 64), rlds, s, x,
                            n, d, lists.p, k, out_idx, out_dist);
     };
@@ -719,6 +717,7 @@ int mrs_knn_exhaustive(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32
     return mrs::guarded([&] {
         if (!ctx || !x || !idx || !dist) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
+        mrs::StreamScope scope(ctx->stream);
         mrs::DevView<float> xv(x, (size_t)n * d, ctx->stream);
         mrs::DevView<int32_t> iv(idx, (size_t)n * k, ctx->stream, false);
         mrs::DevView<float> dv(dist, (size_t)n * k, ctx->stream, false);
@@ -735,6 +734,7 @@ int mrs_knn_exhaustive_timed(mrs_ctx* ctx, const float* x, int64_t n, int32_t d,
     return mrs::guarded([&] {
         if (!ctx || !x || !idx || !dist) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
+        mrs::StreamScope scope(ctx->stream);
         mrs::DevView<float> xv(x, (size_t)n * d, ctx->stream);
         mrs::DevView<int32_t> iv(idx, (size_t)n * k, ctx->stream, false);
         mrs::DevView<float> dv(dist, (size_t)n * k, ctx->stream, false);

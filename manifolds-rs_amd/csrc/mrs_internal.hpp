@@ -58,28 +58,77 @@ int guarded(F&& body) {
     }
 }
 
-// Owning device allocation (no copy, movable).
+// The stream the calling thread works on inside the library (set for the
+// duration of an ABI call by StreamScope).  Device buffers are allocated and
+// freed in that stream's order from the device's memory pool: hipFree would
+// wait for every stream of the device, which serialises concurrent fits.
+struct StreamScope {
+    hipStream_t prev;
+    bool prev_set;
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+};
+bool current_stream(hipStream_t* s);  // false outside any StreamScope
+
+// Stream-ordered allocation (hipMallocAsync / hipFreeAsync on the StreamScope
+// stream) is switched off: a device memory fault seen while several fits ran
+// on several streams has no identified cause yet, and the pool's cross-stream
+// reuse is one of the paths that run exercised.  Every DevBuf uses
+// hipMalloc/hipFree, and every free follows a synchronisation of the stream
+// that used the buffer.
+constexpr bool kStreamOrderedAlloc = false;
+
+// Owning device allocation (no copy, movable).  With kStreamOrderedAlloc and
+// inside a StreamScope it is stream-ordered (freed on the thread's stream at
+// release, after the work queued there so far); otherwise hipMalloc/hipFree.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool ordered = false;
+    hipStream_t home = nullptr;  // stream of the allocation
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), ordered(o.ordered), home(o.home) {
+        o.p = nullptr;
+        o.n = 0;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; ordered = o.ordered; home = o.home;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) MRS_HIP(hipMalloc((void**)&p, sizeof(T) * count));
+        if (!count) return;
+        hipStream_t s;
+        ordered = kStreamOrderedAlloc && current_stream(&s);
+        if (ordered) {
+            home = s;
+            MRS_HIP(hipMallocAsync((void**)&p, sizeof(T) * count, s));
+        } else {
+            MRS_HIP(hipMalloc((void**)&p, sizeof(T) * count));
+        }
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (ordered) {
+                hipStream_t s;
+                if (!current_stream(&s)) s = home;
+                (void)hipFreeAsync(p, s);
+            } else {
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         n = 0;
     }
@@ -125,6 +174,12 @@ inline unsigned div_up(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) /
 // Returns the total through `total_dev` (device scalar) if non-null.
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev,
                         hipStream_t s);
+
+// Raise kernel `kern`'s dynamic-LDS limit on the current device to at least
+// `bytes`.  The limit is per function and shared by every thread, so it only
+// ever grows: a caller never lowers it under a launch another thread is about
+// to make.
+void ensure_dynamic_lds(const void* kern, size_t bytes);
 
 }  // namespace mrs
 

@@ -37,7 +37,8 @@ namespace mrs {
 namespace {
 
 constexpr int kWg = 256;
-constexpr int kChunk = 4096;
+constexpr int kChunk = 1024;      // rows per reduction chunk (fixed: results are run-to-run identical)
+constexpr int kCenChunk = 256;    // rows per centroid chunk
 
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -172,33 +173,56 @@ __global__ __launch_bounds__(kWg) void k_multidot(const Chunk* __restrict__ chun
         partial[(size_t)blockIdx.x * m + k0 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// h[comp][k] = sum of the component's chunk partials, in chunk order
+// h[comp][k] = sum of the component's chunk partials, in chunk order.  The
+// projection on the newest basis vector (k = m-1) is the Lanczos diagonal
+// coefficient: written to alpha (pass 0) or added to it (pass 1).
 __global__ __launch_bounds__(kWg) void k_multidot_finish(const double* __restrict__ partial,
                                                          const uint32_t* __restrict__ comp_chunk0,
                                                          uint32_t n_comp, int m,
-                                                         double* __restrict__ h) {
+                                                         double* __restrict__ h,
+                                                         double* __restrict__ alpha, int pass) {
     const uint32_t t = blockIdx.x * kWg + threadIdx.x;
     if (t >= n_comp * (uint32_t)m) return;
     const uint32_t c = t / m, k = t % m;
     double s = 0.0;
     for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) s += partial[(size_t)q * m + k];
     h[t] = s;
+    if ((int)k == m - 1) alpha[c] = pass ? alpha[c] + s : s;
 }
 
-// w[i] -= sum_k V_k[i] h[comp(i)][k] for live components
-__global__ __launch_bounds__(kWg) void k_cgs_update(double* __restrict__ w,
+// w -= sum_k V_k h[comp][k] over one chunk of a live component (one block per
+// chunk); with norm_partial, also the chunk's sum of the updated w^2.
+__global__ __launch_bounds__(kWg) void k_cgs_update(const Chunk* __restrict__ chunks,
+                                                    const uint32_t* __restrict__ perm,
+                                                    double* __restrict__ w,
                                                     const double* __restrict__ V, uint32_t n,
                                                     const double* __restrict__ h, int m,
-                                                    const uint32_t* __restrict__ comp_of,
-                                                    const uint8_t* __restrict__ live) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = comp_of[i];
-    if (c == 0xFFFFFFFFu || !live[c]) return;
-    const double* hc = h + (size_t)c * m;
-    double s = 0.0;
-    for (int k = 0; k < m; k++) s += V[(size_t)k * n + i] * hc[k];
-    w[i] -= s;
+                                                    const uint8_t* __restrict__ live,
+                                                    double* __restrict__ norm_partial) {
+    __shared__ double sh[kWg];
+    const Chunk c = chunks[blockIdx.x];
+    const bool on = live[c.comp] != 0;
+    const double* hc = h + (size_t)c.comp * m;
+    double acc = 0.0;
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
+        const uint32_t i = perm[q];
+        double x = w[i];
+        if (on) {
+            double s = 0.0;
+            for (int k = 0; k < m; k++) s += V[(size_t)k * n + i] * hc[k];
+            x -= s;
+            w[i] = x;
+        }
+        acc += x * x;
+    }
+    if (!norm_partial) return;  // uniform per block
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kWg / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) norm_partial[blockIdx.x] = sh[0];
 }
 
 __device__ inline double block_sum(double v, double* sh) {
@@ -242,24 +266,8 @@ __global__ __launch_bounds__(kWg) void k_seg_finish(const double* __restrict__ p
     out[c] = do_sqrt ? sqrt(t) : t;
 }
 
-// w -= alpha v + beta_prev v_old  (the three-term recurrence, math.rs:478-483)
-__global__ __launch_bounds__(kWg) void k_recur(double* __restrict__ w, const double* __restrict__ v,
-                                               const double* __restrict__ v_old,
-                                               const double* __restrict__ alpha,
-                                               const double* __restrict__ beta_prev, int has_prev,
-                                               const uint32_t* __restrict__ comp_of,
-                                               const uint8_t* __restrict__ live, uint32_t n) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = comp_of[i];
-    if (c == 0xFFFFFFFFu || !live[c]) return;
-    double x = w[i] - alpha[c] * v[i];
-    if (has_prev) x -= beta_prev[c] * v_old[i];
-    w[i] = x;
-}
-
-// v_old = v; v = w / beta (live comps); basis copy done separately
-__global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double* __restrict__ v_old,
+// v = w / beta_j and the next basis column V_{j+1} = v (live components)
+__global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double* __restrict__ vnext,
                                                  const double* __restrict__ w,
                                                  const double* __restrict__ beta,
                                                  const uint32_t* __restrict__ comp_of,
@@ -268,8 +276,23 @@ __global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double*
     if (i >= n) return;
     const uint32_t c = comp_of[i];
     if (c == 0xFFFFFFFFu || !live[c]) return;
-    v_old[i] = v[i];
-    v[i] = w[i] / beta[c];
+    const double x = w[i] / beta[c];
+    v[i] = x;
+    if (vnext) vnext[i] = x;
+}
+
+// Start of Lanczos step j: a component past its iteration budget, or whose
+// previous residual vanished (breakdown, math.rs:490-495), stops; a live one
+// records that step j belongs to its basis.
+__global__ __launch_bounds__(kWg) void k_lanczos_gate(uint8_t* __restrict__ live,
+                                                      int* __restrict__ n_done,
+                                                      const int* __restrict__ n_iter,
+                                                      const double* __restrict__ beta_prev,
+                                                      uint32_t nl, int j) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= nl) return;
+    if (live[q] && (j >= n_iter[q] || (beta_prev && beta_prev[q] < 1e-12))) live[q] = 0;
+    if (live[q]) n_done[q] = j + 1;
 }
 
 __global__ __launch_bounds__(kWg) void k_scale_seg(double* __restrict__ v,
@@ -310,60 +333,78 @@ __global__ __launch_bounds__(kWg) void k_ritz(const double* __restrict__ V, uint
 
 // ---------------------------------------------------------- host helpers
 
-// eigen-decomposition of a symmetric tridiagonal matrix (cyclic Jacobi);
-// returns eigenvalues ascending and eigenvectors as columns of vecs [m][m].
+}  // namespace
+
+// Eigen-decomposition of the symmetric tridiagonal matrix with diagonal
+// alpha[0..m) and off-diagonal beta[0..m-1): implicit QL iterations with
+// Wilkinson shifts, the rotations accumulated into the eigenvector matrix
+// (Golub & Van Loan, Matrix Computations, 4th ed., section 8.3.3: each sweep
+// chases the bulge of a shifted QL step from the bottom of the unreduced block
+// up to its top with Givens rotations).  Returns eigenvalues ascending and the
+// eigenvectors as columns of vecs [m][m] (row-major, vecs[k*m + r] =
+// component k of eigenvector r).  O(m^2) per eigenvalue.
 void tridiag_eig(const std::vector<double>& alpha, const std::vector<double>& beta, int m,
                  std::vector<double>& evals, std::vector<double>& vecs) {
-    std::vector<double> A((size_t)m * m, 0.0), V((size_t)m * m, 0.0);
-    for (int i = 0; i < m; i++) {
-        A[i * m + i] = alpha[i];
-        V[i * m + i] = 1.0;
-        if (i + 1 < m) A[i * m + i + 1] = A[(i + 1) * m + i] = beta[i];
-    }
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int p = 0; p < m; p++)
-            for (int q = 0; q < m; q++) {
-                tot += A[p * m + q] * A[p * m + q];
-                if (p != q) off += A[p * m + q] * A[p * m + q];
+    std::vector<double> d(alpha.begin(), alpha.begin() + m), e((size_t)m, 0.0);
+    for (int i = 0; i + 1 < m; i++) e[i] = beta[i];
+    std::vector<double> Z((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) Z[(size_t)i * m + i] = 1.0;
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < m; l++) {
+        for (int iter = 0; iter < 64; iter++) {
+            // end of the unreduced block that starts at l
+            int hi = l;
+            while (hi + 1 < m &&
+                   std::fabs(e[hi]) > eps * (std::fabs(d[hi]) + std::fabs(d[hi + 1])))
+                hi++;
+            if (hi == l) break;  // d[l] has converged
+            // Wilkinson shift from the leading 2x2 block
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[hi] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double sn = 1.0, cs = 1.0, p = 0.0;
+            bool deflated = false;
+            for (int i = hi - 1; i >= l; i--) {
+                const double f = sn * e[i], bb = cs * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {  // an exact split: the block below i is done
+                    d[i + 1] -= p;
+                    e[hi] = 0.0;
+                    deflated = true;
+                    break;
+                }
+                sn = f / r;
+                cs = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2.0 * cs * bb;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = cs * r - bb;
+                for (int k = 0; k < m; k++) {
+                    const double zi = Z[(size_t)k * m + i], zi1 = Z[(size_t)k * m + i + 1];
+                    Z[(size_t)k * m + i + 1] = sn * zi + cs * zi1;
+                    Z[(size_t)k * m + i] = cs * zi - sn * zi1;
+                }
             }
-        if (off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
-        for (int p = 0; p < m; p++)
-            for (int q = p + 1; q < m; q++) {
-                const double apq = A[p * m + q];
-                if (std::fabs(apq) < 1e-300) continue;
-                const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) /
-                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (int k = 0; k < m; k++) {
-                    const double akp = A[k * m + p], akq = A[k * m + q];
-                    A[k * m + p] = c * akp - s * akq;
-                    A[k * m + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < m; k++) {
-                    const double apk = A[p * m + k], aqk = A[q * m + k];
-                    A[p * m + k] = c * apk - s * aqk;
-                    A[q * m + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < m; k++) {
-                    const double vkp = V[k * m + p], vkq = V[k * m + q];
-                    V[k * m + p] = c * vkp - s * vkq;
-                    V[k * m + q] = s * vkp + c * vkq;
-                }
-            }
+            if (deflated) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[hi] = 0.0;
+        }
     }
     std::vector<int> order(m);
     std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(),
-              [&](int a, int b) { return A[a * m + a] < A[b * m + b]; });
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
     evals.resize(m);
     vecs.assign((size_t)m * m, 0.0);
-    for (int r = 0; r < m; r++) {
-        evals[r] = A[order[r] * m + order[r]];
-        for (int k = 0; k < m; k++) vecs[k * m + r] = V[k * m + order[r]];
+    for (int r2 = 0; r2 < m; r2++) {
+        evals[r2] = d[order[r2]];
+        for (int k = 0; k < m; k++) vecs[(size_t)k * m + r2] = Z[(size_t)k * m + order[r2]];
     }
 }
+
+namespace {
 
 // Host Lanczos for a small dense symmetric graph (the component meta graph):
 // smallest n_eig eigenvectors of -D^-1/2 W D^-1/2, as [n][n_sel] (f32 values).
@@ -552,16 +593,19 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dchunk0.p, chunk0.data(), sizeof(uint32_t) * (nl + 1), hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dlc.p, lc_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-        DevBuf<double> v(n), v_old(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
+        DevBuf<double> v(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
             alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl),
             mpart((size_t)nch * max_iter), hcoef((size_t)nl * max_iter), mval(nnz ? nnz : 1);
         hipLaunchKernelGGL(k_norm_vals, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, g->coo_col.p,
                            g->coo_val.p, dinv.p, n, mval.p);
+        // per-component control stays on the device: the loop below issues no
+        // host synchronisation until the tridiagonal coefficients are read back
         DevBuf<uint8_t> live(nl);
-        std::vector<uint8_t> hlive(nl, 1);
-        MRS_HIP(hipMemcpyAsync(live.p, hlive.data(), nl, hipMemcpyHostToDevice, s));
-        MRS_HIP(hipMemsetAsync(v_old.p, 0, v_old.bytes(), s));
-        const dim3 gn(div_up(n, kWg));
+        DevBuf<int> dn_iter(nl), dn_done(nl);
+        MRS_HIP(hipMemsetAsync(live.p, 1, nl, s));
+        MRS_HIP(hipMemsetAsync(dn_done.p, 0, dn_done.bytes(), s));
+        MRS_HIP(hipMemcpyAsync(dn_iter.p, n_iter_c.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
+        const dim3 gn(div_up(n, kWg)), gq(div_up(nl, kWg));
         auto seg = [&](const double* a, const double* b, double* outp, int do_sqrt) {
             hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, dperm.p, a, b,
                                partial.p);
@@ -571,49 +615,38 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         hipLaunchKernelGGL(k_start_vec, gn, dim3(kWg), 0, s, v.p, n, seed);
         seg(v.p, v.p, sc.p, 1);
         hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, v.p, sc.p, dlc.p, n);
+        MRS_HIP(hipMemcpyAsync(V.p, v.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         MRS_CHECK_LAUNCH();
-        std::vector<double> hbeta(nl);
         for (int j = 0; j < max_iter; j++) {
-            // components past their iteration budget stop here
-            bool any = false;
-            for (uint32_t q = 0; q < nl; q++) {
-                if (hlive[q] && j >= n_iter_c[q]) hlive[q] = 0;
-                any = any || hlive[q];
-            }
-            if (!any) break;
-            MRS_HIP(hipMemcpyAsync(live.p, hlive.data(), nl, hipMemcpyHostToDevice, s));
-            MRS_HIP(hipMemcpyAsync(V.p + (size_t)j * n, v.p, sizeof(double) * n,
-                                   hipMemcpyDeviceToDevice, s));
-            for (uint32_t q = 0; q < nl; q++)
-                if (hlive[q]) n_done[q] = j + 1;
+            double* alpha_j = alpha_d.p + (size_t)j * nl;
+            double* beta_j = beta_d.p + (size_t)j * nl;
+            hipLaunchKernelGGL(k_lanczos_gate, gq, dim3(kWg), 0, s, live.p, dn_done.p, dn_iter.p,
+                               j > 0 ? beta_j - nl : nullptr, nl, j);
             hipLaunchKernelGGL(k_spmv_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
                                s, off.p, g->coo_col.p, mval.p, v.p, n, w.p);
-            seg(w.p, v.p, alpha_d.p + (size_t)j * nl, 0);
-            hipLaunchKernelGGL(k_recur, gn, dim3(kWg), 0, s, w.p, v.p, v_old.p,
-                               alpha_d.p + (size_t)j * nl,
-                               j > 0 ? beta_d.p + (size_t)(j - 1) * nl : beta_d.p, j > 0 ? 1 : 0,
-                               dlc.p, live.p, n);
-            // full re-orthogonalisation against V_0..V_j: classical Gram-Schmidt
-            // applied twice (as orthogonal as modified GS, 6 launches per step)
+            // Orthogonalise w = M v_j against V_0..V_j by classical Gram-Schmidt
+            // applied twice.  The first pass's projections on v_j and v_{j-1}
+            // are alpha_j and beta_{j-1}, so it also performs the three-term
+            // recurrence (math.rs:478-483); alpha_j collects both passes' v_j
+            // projections, and the second pass emits the norm partials of w.
             const int m = j + 1;
             for (int pass = 0; pass < 2; pass++) {
                 hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
                                    dchunks.p, dperm.p, V.p, n, w.p, m, mpart.p);
                 hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWg)),
-                                   dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p);
-                hipLaunchKernelGGL(k_cgs_update, gn, dim3(kWg), 0, s, w.p, V.p, n, hcoef.p, m,
-                                   dlc.p, live.p);
+                                   dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p, alpha_j,
+                                   pass);
+                hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, dperm.p,
+                                   w.p, V.p, n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
             }
-            seg(w.p, w.p, beta_d.p + (size_t)j * nl, 1);
+            hipLaunchKernelGGL(k_seg_finish, gq, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
+                               beta_j);
+            hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p,
+                               j + 1 < max_iter ? V.p + (size_t)(j + 1) * n : nullptr, w.p, beta_j,
+                               dlc.p, live.p, n);
             MRS_CHECK_LAUNCH();
-            MRS_HIP(hipMemcpyAsync(hbeta.data(), beta_d.p + (size_t)j * nl, sizeof(double) * nl,
-                                   hipMemcpyDeviceToHost, s));
-            MRS_HIP(hipStreamSynchronize(s));
-            hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p, v_old.p, w.p,
-                               beta_d.p + (size_t)j * nl, dlc.p, live.p, n);
-            for (uint32_t q = 0; q < nl; q++)
-                if (hlive[q] && hbeta[q] < 1e-12) hlive[q] = 0;  // breakdown
         }
+        MRS_HIP(hipMemcpyAsync(n_done.data(), dn_done.p, sizeof(int) * nl, hipMemcpyDeviceToHost, s));
         std::vector<double> ha((size_t)max_iter * nl), hb((size_t)max_iter * nl);
         MRS_HIP(hipMemcpyAsync(ha.data(), alpha_d.p, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, s));
         MRS_HIP(hipMemcpyAsync(hb.data(), beta_d.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, s));
@@ -635,10 +668,9 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         }
         DevBuf<double> dcoef(coef.size() ? coef.size() : 1), evec(n);
         DevBuf<uint32_t> dcoef_off(nl);
-        DevBuf<int> dn_done(nl), dn_sel(nl);
+        DevBuf<int> dn_sel(nl);
         MRS_HIP(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dcoef_off.p, coef_off.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
-        MRS_HIP(hipMemcpyAsync(dn_done.p, n_done.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dn_sel.p, n_sel.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
         std::vector<double> he(n);
         for (int dcomp = 0; dcomp < n_comp; dcomp++) {
@@ -711,8 +743,8 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             std::vector<uint32_t> c0(ncomp + 1, 0);
             for (uint32_t c = 0; c < ncomp; c++) {
                 c0[c] = (uint32_t)chunks.size();
-                for (uint32_t b = start[c]; b < start[c + 1]; b += kChunk)
-                    chunks.push_back({c, b, std::min(b + kChunk, start[c + 1])});
+                for (uint32_t b = start[c]; b < start[c + 1]; b += kCenChunk)
+                    chunks.push_back({c, b, std::min(b + kCenChunk, start[c + 1])});
             }
             c0[ncomp] = (uint32_t)chunks.size();
             DevBuf<Chunk> dch(chunks.size());
@@ -820,6 +852,7 @@ extern "C" int mrs_spectral_layout(mrs_ctx* ctx, const mrs_graph* g, int32_t n_d
                                    float range, const float* data, int32_t d, float* out) {
     return mrs::guarded([&] {
         if (!ctx || !g || !out) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        mrs::StreamScope scope(ctx->stream);
         if (n_dim <= 0 || n_dim > 16) throw mrs::Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
         hipStream_t s = ctx->stream;
         mrs::DevView<float> xv(data, data ? (size_t)g->n * d : 0, s);
@@ -828,5 +861,18 @@ extern "C" int mrs_spectral_layout(mrs_ctx* ctx, const mrs_graph* g, int32_t n_d
                                  xv.p, data ? d : 0);
         ov.download(s);
         MRS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+extern "C" int mrs_tridiag_eigen(const double* diag, const double* offdiag, int32_t m, double* evals,
+                                 double* evecs) {
+    return mrs::guarded([&] {
+        if (m <= 0 || !diag || !evals || !evecs || (m > 1 && !offdiag))
+            throw mrs::Error(MRS_INVALID_ARGUMENT, "bad tridiagonal problem");
+        std::vector<double> a(diag, diag + m), b(offdiag ? offdiag : diag, offdiag ? offdiag + m - 1 : diag);
+        std::vector<double> ev, vv;
+        mrs::tridiag_eig(a, b, m, ev, vv);
+        std::copy(ev.begin(), ev.end(), evals);
+        std::copy(vv.begin(), vv.end(), evecs);
     });
 }

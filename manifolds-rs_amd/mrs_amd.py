@@ -189,6 +189,15 @@ def lib() -> C.CDLL:
         "mrs_densmap_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, i32, C.POINTER(UmapParamsGpu),
                                       C.POINTER(DensParams), u64, i32, vp]),
         "mrs_last_fit_timings": (C.c_int, [vp]),
+        "mrs_fit_bytes_estimate": (C.c_size_t, [i64, i32, i32, i32]),
+        "mrs_tridiag_eigen": (C.c_int, [vp, vp, i32, vp, vp]),
+        "mrs_server_create": (C.c_int, [i32, C.c_size_t, i32, i32, C.POINTER(vp)]),
+        "mrs_server_submit": (C.c_int, [vp, vp, i64, i32, C.POINTER(UmapParamsGpu), u64,
+                                        C.POINTER(i64)]),
+        "mrs_server_poll": (C.c_int, [vp, i64, C.POINTER(i32)]),
+        "mrs_server_wait": (C.c_int, [vp, i64, vp, vp]),
+        "mrs_server_stats": (C.c_int, [vp, vp]),
+        "mrs_server_destroy": (C.c_int, [vp]),
         "mrs_optimise_embedding_adam_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, vp,
                                                       C.POINTER(UmapOptimParams),
                                                       C.POINTER(DensState), u64, i32, i32]),
@@ -534,6 +543,92 @@ def umap_gpu(ctx: Context, data, params: UmapParamsGpu | None = None, precompute
         _check(lib().mrs_densmap_gpu(ctx.handle, _ptr(data), n, d, _ptr(kidx), _ptr(kdist), kk,
                                      C.byref(params), C.byref(dens), seed, verbose, _ptr(out)))
     return out
+
+
+def tridiag_eigen(diag, offdiag):
+    """Eigenpairs of a symmetric tridiagonal matrix (the spectral layout's host
+    solver): (eigenvalues ascending, eigenvectors as columns)."""
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    e = np.ascontiguousarray(offdiag, dtype=np.float64)
+    m = d.size
+    ev = np.empty(m, np.float64)
+    vv = np.empty((m, m), np.float64)
+    _check(lib().mrs_tridiag_eigen(_ptr(d), _ptr(e) if m > 1 else None, m, _ptr(ev), _ptr(vv)))
+    return ev, vv
+
+
+SERVE_QUEUED, SERVE_PREPARING, SERVE_EPOCHS, SERVE_DONE, SERVE_FAILED = range(5)
+
+
+def fit_bytes_estimate(n: int, d: int, k: int = 15, n_dim: int = 2) -> int:
+    return int(lib().mrs_fit_bytes_estimate(n, d, k, n_dim))
+
+
+class Server:
+    """Continuous-batching fit server (mrs_server_*): submit() queues one
+    umap_gpu fit and returns a ticket; wait(ticket) returns its [n_dim, n]
+    embedding (numpy for host data, a device tensor for device data) and the
+    request's {queued, preparing, latency} milliseconds.  Submitted arrays are
+    kept alive here until their request is collected."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, prep_workers: int = 4,
+                 max_active: int = 256):
+        h = C.c_void_p()
+        _check(lib().mrs_server_create(device, hbm_budget, prep_workers, max_active, C.byref(h)))
+        self.handle = h
+        self._live = {}
+
+    def submit(self, data, params: UmapParamsGpu | None = None, seed: int = 42) -> int:
+        params = params or UmapParamsGpu.default()
+        if isinstance(data, np.ndarray):
+            data = np.ascontiguousarray(data, dtype=np.float32)
+        n, d = data.shape
+        t = C.c_int64()
+        _check(lib().mrs_server_submit(self.handle, _ptr(data), n, d, C.byref(params), seed,
+                                       C.byref(t)))
+        self._live[t.value] = (data, params.n_dim, n)
+        return t.value
+
+    def poll(self, ticket: int) -> int:
+        st = C.c_int32()
+        _check(lib().mrs_server_poll(self.handle, ticket, C.byref(st)))
+        return st.value
+
+    def wait(self, ticket: int):
+        data, n_dim, n = self._live.pop(ticket)
+        if isinstance(data, np.ndarray):
+            out = np.empty((n_dim, n), np.float32)
+        else:
+            import torch
+
+            out = torch.empty((n_dim, n), dtype=torch.float32, device=data.device)
+        times = (C.c_double * 3)()
+        _check(lib().mrs_server_wait(self.handle, ticket, _ptr(out), times))
+        return out, dict(queued_ms=times[0], prepare_ms=times[1], latency_ms=times[2])
+
+    def stats(self) -> dict:
+        v = (C.c_int64 * 7)()
+        _check(lib().mrs_server_stats(self.handle, v))
+        keys = ("submitted", "completed", "failed", "ticks", "peak_resident", "peak_bytes",
+                "budget_bytes")
+        return dict(zip(keys, list(v)))
+
+    def close(self) -> None:
+        if self.handle:
+            _check(lib().mrs_server_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def last_fit_timings() -> dict:

@@ -54,3 +54,39 @@ def test_ctx_create_without_device_reports_device_error():
     rc = M.lib().mrs_ctx_create(0, 0, C.byref(h))
     assert rc == M.MRS_DEVICE_ERROR
     assert M.lib().mrs_last_error()
+
+
+def test_fit_bytes_estimate_grows_with_every_size():
+    base = M.fit_bytes_estimate(10_000, 50, 15, 2)
+    assert base > 10_000 * 50 * 4  # at least the data itself
+    assert M.fit_bytes_estimate(20_000, 50, 15, 2) > base
+    assert M.fit_bytes_estimate(10_000, 100, 15, 2) > base
+    assert M.fit_bytes_estimate(10_000, 50, 30, 2) > base
+    assert M.fit_bytes_estimate(0, 50, 15, 2) == 0
+
+
+def test_server_create_refuses_while_switched_off():
+    # the fit server stays off until the cause of a device fault seen under
+    # concurrent serving is shown; creation must say so, not half-start
+    h = C.c_void_p()
+    rc = M.lib().mrs_server_create(0, 0, 2, 8, C.byref(h))
+    assert rc == M.MRS_INVALID_ARGUMENT
+    assert b"switched off" in M.lib().mrs_last_error()
+    assert not h.value
+
+
+@pytest.mark.parametrize("m,seed", [(1, 0), (2, 1), (7, 2), (42, 3), (42, 4), (120, 5)])
+def test_tridiag_eigen_matches_numpy(m, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=m)
+    e = rng.normal(size=m - 1)
+    if seed == 4:  # repeated eigenvalues and an exact split
+        d[:] = 0.5
+        e[::3] = 0.0
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    ev, V = M.tridiag_eigen(d, e)
+    want = np.linalg.eigvalsh(T)
+    assert np.all(np.diff(ev) >= 0)
+    np.testing.assert_allclose(ev, want, atol=1e-12 * max(1.0, np.abs(want).max()))
+    np.testing.assert_allclose(V.T @ V, np.eye(m), atol=1e-12)
+    np.testing.assert_allclose(T @ V, V * ev[None, :], atol=1e-11)
