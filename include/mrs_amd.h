@@ -95,7 +95,9 @@ typedef struct {
 /* message of the last failure on the calling thread ("" if none) */
 const char *mrs_last_error(void);
 
-/* Context: device ordinal, HBM budget in bytes (0 = no limit). */
+/* Context: device ordinal, HBM budget in bytes (0 = no limit).  Graphs and
+ * optimiser states keep using their context's stream: destroy them before the
+ * context. */
 int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx **out);
 int mrs_ctx_destroy(mrs_ctx *ctx);
 /* block until all work queued on the context's stream has finished */

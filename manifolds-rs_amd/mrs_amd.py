@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -232,12 +233,17 @@ def _ptr(x) -> int | None:
 
 
 class Context:
-    """mrs_ctx: one HIP stream on one device."""
+    """mrs_ctx: one HIP stream on one device.  Graphs and optimiser states made
+    on a context hold its stream: closing the context closes them first."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._deps = weakref.WeakSet()
         h = C.c_void_p()
         _check(lib().mrs_ctx_create(device, hbm_budget, C.byref(h)))
         self.handle = h
+
+    def _adopt(self, obj) -> None:
+        self._deps.add(obj)
 
     def synchronize(self) -> None:
         _check(lib().mrs_ctx_synchronize(self.handle))
@@ -248,6 +254,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for obj in list(self._deps):
+                obj.close()
             lib().mrs_ctx_destroy(self.handle)
             self.handle = None
 
@@ -323,6 +331,7 @@ class FuzzyGraph:
         _check(lib().mrs_fuzzy_graph(ctx.handle, _ptr(knn_idx), _ptr(knn_dist), n, k,
                                      C.byref(params), n_epochs, C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().mrs_graph_info(h, C.byref(a), C.byref(b), C.byref(c)))
         self.n, self.nnz, self.n_edges = a.value, b.value, c.value
@@ -399,6 +408,7 @@ class UmapGpuState:
                 ctx.handle, _ptr(embd), n, dim, _ptr(adj_off), _ptr(adj_cols), _ptr(adj_vals),
                 C.byref(params), C.byref(dens) if dens is not None else None, C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         n_, d_, e_ = C.c_int64(), C.c_int32(), C.c_int64()
         _check(lib().mrs_umap_state_info(h, C.byref(n_), C.byref(d_), C.byref(e_)))
         self.n, self.n_dim, self.n_edges = n_.value, d_.value, e_.value
