@@ -100,19 +100,38 @@ __global__ __launch_bounds__(kWg) void k_cc_jump(uint32_t* __restrict__ lab, uin
     lab[i] = r;
 }
 
-// M's values once per layout: mval[p] = -d_i^-1/2 w_p d_j^-1/2 (0 on the diagonal),
-// the same product order the matrix-vector product used per entry.
-__global__ __launch_bounds__(kWg) void k_norm_vals(const uint32_t* __restrict__ off,
+// Degree of each vertex in component order: deg[q] = row length of perm[q].
+__global__ __launch_bounds__(kWg) void k_perm_degree(const uint32_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ perm, uint32_t n,
+                                                     uint32_t* __restrict__ deg) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t i = perm[q];
+    deg[q] = off[i + 1] - off[i];
+}
+
+// M in component order: row q is row perm[q] with its columns renamed through
+// inv (= perm^-1) and M's values mval = -d_i^-1/2 w d_j^-1/2 (0 on the
+// diagonal), entries kept in their row order.  Every Lanczos vector then lives
+// in component order, so a component's vertices are one contiguous range.
+__global__ __launch_bounds__(kWg) void k_perm_rows(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ col,
                                                    const float* __restrict__ val,
-                                                   const double* __restrict__ dinv, uint32_t n,
-                                                   double* __restrict__ mval) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
+                                                   const double* __restrict__ dinv,
+                                                   const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ inv,
+                                                   const uint32_t* __restrict__ poff, uint32_t n,
+                                                   uint32_t* __restrict__ pcol,
+                                                   double* __restrict__ pval) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t i = perm[q];
     const double di = dinv[i];
-    for (uint32_t p = off[i]; p < off[i + 1]; p++) {
+    uint32_t o = poff[q];
+    for (uint32_t p = off[i]; p < off[i + 1]; p++, o++) {
         const uint32_t j = col[p];
-        mval[p] = j == i ? 0.0 : -di * (double)val[p] * dinv[j];
+        pcol[o] = inv[j];
+        pval[o] = j == i ? 0.0 : -di * (double)val[p] * dinv[j];
     }
 }
 
@@ -142,7 +161,6 @@ struct Chunk {
 // vertices of V_k[i] * w[i] for k in [k0, k0 + kKB), one block per (chunk, k-block).
 constexpr int kKB = 8;
 __global__ __launch_bounds__(kWg) void k_multidot(const Chunk* __restrict__ chunks,
-                                                  const uint32_t* __restrict__ perm,
                                                   const double* __restrict__ V, uint32_t n,
                                                   const double* __restrict__ w, int m,
                                                   double* __restrict__ partial) {
@@ -153,11 +171,10 @@ __global__ __launch_bounds__(kWg) void k_multidot(const Chunk* __restrict__ chun
 #pragma unroll
     for (int t = 0; t < kKB; t++) acc[t] = 0.0;
     for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
-        const uint32_t i = perm[q];
-        const double wi = w[i];
+        const double wq = w[q];
 #pragma unroll
         for (int t = 0; t < kKB; t++)
-            if (k0 + t < m) acc[t] += V[(size_t)(k0 + t) * n + i] * wi;
+            if (k0 + t < m) acc[t] += V[(size_t)(k0 + t) * n + q] * wq;
     }
 #pragma unroll
     for (int t = 0; t < kKB; t++) sh[t][threadIdx.x] = acc[t];
@@ -193,7 +210,6 @@ __global__ __launch_bounds__(kWg) void k_multidot_finish(const double* __restric
 // w -= sum_k V_k h[comp][k] over one chunk of a live component (one block per
 // chunk); with norm_partial, also the chunk's sum of the updated w^2.
 __global__ __launch_bounds__(kWg) void k_cgs_update(const Chunk* __restrict__ chunks,
-                                                    const uint32_t* __restrict__ perm,
                                                     double* __restrict__ w,
                                                     const double* __restrict__ V, uint32_t n,
                                                     const double* __restrict__ h, int m,
@@ -205,13 +221,12 @@ __global__ __launch_bounds__(kWg) void k_cgs_update(const Chunk* __restrict__ ch
     const double* hc = h + (size_t)c.comp * m;
     double acc = 0.0;
     for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
-        const uint32_t i = perm[q];
-        double x = w[i];
+        double x = w[q];
         if (on) {
             double s = 0.0;
-            for (int k = 0; k < m; k++) s += V[(size_t)k * n + i] * hc[k];
+            for (int k = 0; k < m; k++) s += V[(size_t)k * n + q] * hc[k];
             x -= s;
-            w[i] = x;
+            w[q] = x;
         }
         acc += x * x;
     }
@@ -239,17 +254,13 @@ __device__ inline double block_sum(double v, double* sh) {
 
 // partial[chunk] = sum over the chunk's vertices of a[i]*b[i]
 __global__ __launch_bounds__(kWg) void k_seg_dot(const Chunk* __restrict__ chunks,
-                                                 const uint32_t* __restrict__ perm,
                                                  const double* __restrict__ a,
                                                  const double* __restrict__ b,
                                                  double* __restrict__ partial) {
     __shared__ double sh[kWg];
     const Chunk c = chunks[blockIdx.x];
     double acc = 0.0;
-    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
-        const uint32_t i = perm[q];
-        acc += a[i] * b[i];
-    }
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) acc += a[q] * b[q];
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -306,10 +317,12 @@ __global__ __launch_bounds__(kWg) void k_scale_seg(double* __restrict__ v,
     v[i] = s[c] > 0.0 ? v[i] / s[c] : v[i];
 }
 
-__global__ __launch_bounds__(kWg) void k_start_vec(double* __restrict__ v, uint32_t n,
+// start vector in component order (vertex perm[q]'s draw at position q)
+__global__ __launch_bounds__(kWg) void k_start_vec(double* __restrict__ v,
+                                                   const uint32_t* __restrict__ perm, uint32_t n,
                                                    uint64_t seed) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i < n) v[i] = u01(seed, 1, i) - 0.5;
+    uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q < n) v[q] = u01(seed, 1, perm[q]) - 0.5;
 }
 
 // evec[i] = sum_j V_j[i] * C_comp(i)[j][sel]  for sel in [0, n_sel)
@@ -329,6 +342,179 @@ __global__ __launch_bounds__(kWg) void k_ritz(const double* __restrict__ V, uint
     double acc = 0.0;
     for (int j = 0; j < m; j++) acc += V[(size_t)j * n + i] * C[j * n_sel[c] + sel];
     out[i] = acc;
+}
+
+// Position q's component by binary search of the component starts (start[c] <=
+// q < start[c+1]; components are contiguous in component order).
+__device__ inline uint32_t comp_at(const uint32_t* __restrict__ start, uint32_t ncomp, uint32_t q) {
+    uint32_t lo = 0, hi = ncomp - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// lcq[q] = Lanczos index of position q's component (0xFFFFFFFF: not spectral)
+__global__ __launch_bounds__(kWg) void k_fill_lcq(const uint32_t* __restrict__ start,
+                                                  uint32_t ncomp,
+                                                  const uint32_t* __restrict__ comp_to_lc,
+                                                  uint32_t n, uint32_t* __restrict__ lcq) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q < n) lcq[q] = comp_to_lc[comp_at(start, ncomp, q)];
+}
+
+// raw[q][dcomp] = the normalised Ritz vector (f32, eigenvectors.push(T::from),
+// math.rs:583), or a uniform(-1, 1) fill where the component has fewer
+// converged pairs than requested.
+__global__ __launch_bounds__(kWg) void k_raw_store(const double* __restrict__ evec,
+                                                   const uint32_t* __restrict__ lcq,
+                                                   const uint32_t* __restrict__ perm,
+                                                   const int* __restrict__ n_sel, int sel,
+                                                   int dcomp, int n_comp, uint64_t seed,
+                                                   uint32_t n, float* __restrict__ raw) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t c = lcq[q];
+    if (c == 0xFFFFFFFFu) return;
+    raw[(size_t)q * n_comp + dcomp] =
+        sel < n_sel[c] ? (float)evec[q] : (float)(2.0 * u01(seed + dcomp, 3, perm[q]) - 1.0);
+}
+
+// partial[chunk] = max |raw| over the chunk's rows (all n_comp coordinates)
+__global__ __launch_bounds__(kWg) void k_chunk_absmax(const Chunk* __restrict__ chunks,
+                                                      const float* __restrict__ raw, int n_comp,
+                                                      float* __restrict__ partial) {
+    __shared__ float sh[kWg];
+    const Chunk c = chunks[blockIdx.x];
+    float m = 0.0f;
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg)
+        for (int dd = 0; dd < n_comp; dd++) m = fmaxf(m, fabsf(raw[(size_t)q * n_comp + dd]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = kWg / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// multi_component_init placement (init.rs:439-484): a spectral component's
+// rows are meta + raw * expansion, the others meta + uniform(-1, 1) * range.
+__global__ __launch_bounds__(kWg) void k_place(const float* __restrict__ raw,
+                                               const uint32_t* __restrict__ start, uint32_t ncomp,
+                                               const uint32_t* __restrict__ perm,
+                                               const float* __restrict__ meta,
+                                               const float* __restrict__ expansion,
+                                               const float* __restrict__ drange,
+                                               const uint8_t* __restrict__ spectral, uint32_t n,
+                                               int n_comp, uint64_t seed,
+                                               float* __restrict__ emb) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t c = comp_at(start, ncomp, q);
+    const uint32_t i = perm[q];
+    for (int dd = 0; dd < n_comp; dd++) {
+        const size_t o = (size_t)q * n_comp + dd;
+        const float mc = meta[(size_t)c * n_comp + dd];
+        if (spectral[c]) {
+            emb[o] = mc + raw[o] * expansion[c];
+        } else {
+            const float u = (float)(2.0 * u01(seed, 19, (uint64_t)i * n_comp + dd) - 1.0);
+            emb[o] = mc + u * drange[c];
+        }
+    }
+}
+
+// a single component too small for the spectral branch: uniform(-1, 1)
+__global__ __launch_bounds__(kWg) void k_random_rows(uint32_t n, int n_comp, uint64_t seed,
+                                                     float* __restrict__ emb) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    for (int dd = 0; dd < n_comp; dd++)
+        emb[(size_t)i * n_comp + dd] = (float)(2.0 * u01(seed, 5, (uint64_t)i * n_comp + dd) - 1.0);
+}
+
+constexpr int kFinBlocks = 256;  // fixed grid of the finishing reductions (run-to-run identical)
+
+// per-block FP64 column sums of emb [n][n_comp]
+__global__ __launch_bounds__(kWg) void k_col_sums(const float* __restrict__ emb, uint32_t n,
+                                                  int n_comp, double* __restrict__ partials) {
+    __shared__ double sh[kWg];
+    for (int dd = 0; dd < n_comp; dd++) {
+        double acc = 0.0;
+        for (uint32_t q = blockIdx.x * kWg + threadIdx.x; q < n; q += kFinBlocks * kWg)
+            acc += (double)emb[(size_t)q * n_comp + dd];
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * n_comp + dd] = t;
+    }
+}
+
+// centre every column on its mean (f32, the mean rounded from the FP64 sum)
+// and emit per-block max |x|
+__global__ __launch_bounds__(kWg) void k_center_absmax(float* __restrict__ emb, uint32_t n,
+                                                       int n_comp,
+                                                       const double* __restrict__ partials,
+                                                       float* __restrict__ absmax) {
+    __shared__ float mean[16];
+    __shared__ float sh[kWg];
+    if ((int)threadIdx.x < n_comp) {
+        double t = 0.0;
+        for (int b = 0; b < kFinBlocks; b++) t += partials[(size_t)b * n_comp + threadIdx.x];
+        mean[threadIdx.x] = (float)(t / n);
+    }
+    __syncthreads();
+    float m = 0.0f;
+    for (uint32_t q = blockIdx.x * kWg + threadIdx.x; q < n; q += kFinBlocks * kWg)
+        for (int dd = 0; dd < n_comp; dd++) {
+            const size_t o = (size_t)q * n_comp + dd;
+            const float x = emb[o] - mean[dd];
+            emb[o] = x;
+            m = fmaxf(m, fabsf(x));
+        }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = kWg / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) absmax[blockIdx.x] = sh[0];
+}
+
+// scale max |x| to `range`, optionally add the N(0, 1e-4) noise of
+// finalise_spectral_embedding (Box-Muller on counter hashes), and write row q
+// to the caller's row perm[q]
+__global__ __launch_bounds__(kWg) void k_finish_rows(const float* __restrict__ emb, uint32_t n,
+                                                     int n_comp,
+                                                     const float* __restrict__ absmax,
+                                                     float range, int noise, uint64_t seed,
+                                                     const uint32_t* __restrict__ perm,
+                                                     float* __restrict__ out) {
+    __shared__ float mx_sh;
+    if (threadIdx.x == 0) {
+        float mx = 0.0f;
+        for (int b = 0; b < kFinBlocks; b++) mx = fmaxf(mx, absmax[b]);
+        mx_sh = mx;
+    }
+    __syncthreads();
+    const float mx = mx_sh;
+    const bool scale = mx > 1e-8f;
+    const float sc = scale ? range / mx : 1.0f;
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t i = perm[q];
+    for (int dd = 0; dd < n_comp; dd++) {
+        float x = emb[(size_t)q * n_comp + dd];
+        if (scale) x *= sc;
+        if (noise) {
+            const uint64_t f = (uint64_t)i * n_comp + dd;
+            const double u1 = fmax(u01(seed + 9999, 11, f), 1e-300), u2 = u01(seed + 9999, 13, f);
+            const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+            x += (float)(z * 1e-4);
+        }
+        out[(size_t)i * n_comp + dd] = x;
+    }
 }
 
 // ---------------------------------------------------------- host helpers
@@ -532,7 +718,11 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
     std::vector<uint32_t> hl(n);
     MRS_HIP(hipMemcpyAsync(hl.data(), lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
     MRS_HIP(hipStreamSynchronize(s));
-    // components in ascending order of their smallest vertex
+    // Components in ascending order of their smallest vertex.  Everything below
+    // works in component order: position q runs over the components one after
+    // another, each in ascending vertex order; perm[q] is the vertex at q and
+    // inv its inverse, so a component is the contiguous range
+    // [start[c], start[c+1]) and every vector access is a contiguous stream.
     std::vector<uint32_t> root_id(n, 0xFFFFFFFFu);
     std::vector<uint32_t> roots;
     for (uint32_t i = 0; i < n; i++)
@@ -541,15 +731,26 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             roots.push_back(i);
         }
     const uint32_t ncomp = (uint32_t)roots.size();
+    if (ncomp == 0 || ncomp > n) throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
     std::vector<uint32_t> comp(n), size(ncomp, 0);
     for (uint32_t i = 0; i < n; i++) {
-        comp[i] = root_id[hl[i]];
-        size[comp[i]]++;
+        const uint32_t c = root_id[hl[i]];
+        if (c == 0xFFFFFFFFu) throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
+        comp[i] = c;
+        size[c]++;
     }
     std::vector<uint32_t> start(ncomp + 1, 0);
     for (uint32_t c = 0; c < ncomp; c++) start[c + 1] = start[c] + size[c];
-    std::vector<uint32_t> perm(n), cur(start.begin(), start.end() - 1);
-    for (uint32_t i = 0; i < n; i++) perm[cur[comp[i]]++] = i;
+    std::vector<uint32_t> perm(n), inv(n), cur(start.begin(), start.end() - 1);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t q = cur[comp[i]]++;
+        perm[q] = i;
+        inv[i] = q;
+    }
+    DevBuf<uint32_t> dperm(n), dstart((size_t)ncomp + 1);
+    MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    MRS_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(uint32_t) * (ncomp + 1),
+                           hipMemcpyHostToDevice, s));
 
     // which components take the spectral branch
     const bool single = ncomp == 1;
@@ -557,47 +758,58 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
     for (uint32_t c = 0; c < ncomp; c++)
         spectral[c] = single ? (size[c] > (uint32_t)n_comp + 1)
                              : (size[c] >= 2u * n_comp && size[c] > (uint32_t)n_comp + 1);
-    // batched Lanczos over the spectral components
-    std::vector<uint32_t> lc_of(n, 0xFFFFFFFFu);  // local (Lanczos) component index
-    std::vector<uint32_t> lc_comp;                 // lanczos comp -> component
+    std::vector<uint32_t> lc_comp;  // lanczos comp -> component
     for (uint32_t c = 0; c < ncomp; c++)
         if (spectral[c]) lc_comp.push_back(c);
     const uint32_t nl = (uint32_t)lc_comp.size();
     std::vector<uint32_t> comp_to_lc(ncomp, 0xFFFFFFFFu);
     for (uint32_t q = 0; q < nl; q++) comp_to_lc[lc_comp[q]] = q;
-    for (uint32_t i = 0; i < n; i++) lc_of[i] = comp_to_lc[comp[i]];
-
-    const int n_eig = n_comp + 1;
-    std::vector<int> n_iter_c(nl), n_done(nl, 0), n_sel(nl, 0);
-    int max_iter = 0;
+    // reduction chunks of the spectral components (fixed: results are run-to-run identical)
+    std::vector<Chunk> chunks;
+    std::vector<uint32_t> chunk0(nl + 1, 0);
     for (uint32_t q = 0; q < nl; q++) {
-        n_iter_c[q] = (int)std::min<uint32_t>(n_eig * 4 + 30, size[lc_comp[q]]);
-        max_iter = std::max(max_iter, n_iter_c[q]);
+        const uint32_t c = lc_comp[q];
+        chunk0[q] = (uint32_t)chunks.size();
+        for (uint32_t b = start[c]; b < start[c + 1]; b += kChunk)
+            chunks.push_back({q, b, std::min(b + kChunk, start[c + 1])});
     }
-    // per-vertex raw spectral coordinates [n][n_comp] (f32), filled per component
-    std::vector<float> raw((size_t)n * n_comp, 0.0f);
+    chunk0[nl] = (uint32_t)chunks.size();
+    const uint32_t nch = (uint32_t)chunks.size();
+
+    // raw spectral coordinates [n][n_comp] (f32), component order
+    DevBuf<float> raw((size_t)n * n_comp);
+    MRS_HIP(hipMemsetAsync(raw.p, 0, raw.bytes(), s));
+    DevBuf<Chunk> dchunks(nch ? nch : 1);
     if (nl) {
-        std::vector<Chunk> chunks;
-        std::vector<uint32_t> chunk0(nl + 1, 0);
+        const int n_eig = n_comp + 1;
+        std::vector<int> n_iter_c(nl), n_done(nl, 0), n_sel(nl, 0);
+        int max_iter = 0;
         for (uint32_t q = 0; q < nl; q++) {
-            const uint32_t c = lc_comp[q];
-            chunk0[q] = (uint32_t)chunks.size();
-            for (uint32_t b = start[c]; b < start[c + 1]; b += kChunk)
-                chunks.push_back({q, b, std::min(b + kChunk, start[c + 1])});
+            n_iter_c[q] = (int)std::min<uint32_t>(n_eig * 4 + 30, size[lc_comp[q]]);
+            max_iter = std::max(max_iter, n_iter_c[q]);
         }
-        chunk0[nl] = (uint32_t)chunks.size();
-        const uint32_t nch = (uint32_t)chunks.size();
-        DevBuf<Chunk> dchunks(nch);
-        DevBuf<uint32_t> dchunk0(nl + 1), dperm(n), dlc(n);
+        DevBuf<uint32_t> dchunk0(nl + 1), dc2l(ncomp), dlc(n), dinvp(n), poff((size_t)n + 1);
         MRS_HIP(hipMemcpyAsync(dchunks.p, chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dchunk0.p, chunk0.data(), sizeof(uint32_t) * (nl + 1), hipMemcpyHostToDevice, s));
-        MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-        MRS_HIP(hipMemcpyAsync(dlc.p, lc_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dc2l.p, comp_to_lc.data(), sizeof(uint32_t) * ncomp, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dinvp.p, inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_fill_lcq, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dstart.p, ncomp,
+                           dc2l.p, n, dlc.p);
+        // M = -D^-1/2 W D^-1/2 in component order (rows keep their entry order,
+        // so every row sum below adds the same products in the same order)
+        hipLaunchKernelGGL(k_perm_degree, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, dperm.p,
+                           n, poff.p);
+        MRS_CHECK_LAUNCH();
+        exclusive_scan_u32(poff.p, poff.p, n, poff.p + n, s);
+        DevBuf<uint32_t> pcol(nnz ? nnz : 1);
+        DevBuf<double> mval(nnz ? nnz : 1);
+        hipLaunchKernelGGL(k_perm_rows, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p,
+                           g->coo_col.p, g->coo_val.p, dinv.p, dperm.p, dinvp.p, poff.p, n,
+                           pcol.p, mval.p);
+        MRS_CHECK_LAUNCH();
         DevBuf<double> v(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
             alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl),
-            mpart((size_t)nch * max_iter), hcoef((size_t)nl * max_iter), mval(nnz ? nnz : 1);
-        hipLaunchKernelGGL(k_norm_vals, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, g->coo_col.p,
-                           g->coo_val.p, dinv.p, n, mval.p);
+            mpart((size_t)nch * max_iter), hcoef((size_t)nl * max_iter);
         // per-component control stays on the device: the loop below issues no
         // host synchronisation until the tridiagonal coefficients are read back
         DevBuf<uint8_t> live(nl);
@@ -607,12 +819,11 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dn_iter.p, n_iter_c.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
         const dim3 gn(div_up(n, kWg)), gq(div_up(nl, kWg));
         auto seg = [&](const double* a, const double* b, double* outp, int do_sqrt) {
-            hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, dperm.p, a, b,
-                               partial.p);
-            hipLaunchKernelGGL(k_seg_finish, dim3(div_up(nl, kWg)), dim3(kWg), 0, s, partial.p,
-                               dchunk0.p, nl, do_sqrt, outp);
+            hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, a, b, partial.p);
+            hipLaunchKernelGGL(k_seg_finish, gq, dim3(kWg), 0, s, partial.p, dchunk0.p, nl,
+                               do_sqrt, outp);
         };
-        hipLaunchKernelGGL(k_start_vec, gn, dim3(kWg), 0, s, v.p, n, seed);
+        hipLaunchKernelGGL(k_start_vec, gn, dim3(kWg), 0, s, v.p, dperm.p, n, seed);
         seg(v.p, v.p, sc.p, 1);
         hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, v.p, sc.p, dlc.p, n);
         MRS_HIP(hipMemcpyAsync(V.p, v.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -623,7 +834,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             hipLaunchKernelGGL(k_lanczos_gate, gq, dim3(kWg), 0, s, live.p, dn_done.p, dn_iter.p,
                                j > 0 ? beta_j - nl : nullptr, nl, j);
             hipLaunchKernelGGL(k_spmv_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
-                               s, off.p, g->coo_col.p, mval.p, v.p, n, w.p);
+                               s, poff.p, pcol.p, mval.p, v.p, n, w.p);
             // Orthogonalise w = M v_j against V_0..V_j by classical Gram-Schmidt
             // applied twice.  The first pass's projections on v_j and v_{j-1}
             // are alpha_j and beta_{j-1}, so it also performs the three-term
@@ -632,12 +843,12 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             const int m = j + 1;
             for (int pass = 0; pass < 2; pass++) {
                 hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
-                                   dchunks.p, dperm.p, V.p, n, w.p, m, mpart.p);
+                                   dchunks.p, V.p, n, w.p, m, mpart.p);
                 hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWg)),
                                    dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p, alpha_j,
                                    pass);
-                hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, dperm.p,
-                                   w.p, V.p, n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
+                hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, w.p, V.p,
+                                   n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
             }
             hipLaunchKernelGGL(k_seg_finish, gq, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
                                beta_j);
@@ -672,7 +883,6 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dcoef_off.p, coef_off.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dn_sel.p, n_sel.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
-        std::vector<double> he(n);
         for (int dcomp = 0; dcomp < n_comp; dcomp++) {
             const int sel = dcomp + 1;  // skip the trivial smallest eigenvector
             MRS_HIP(hipMemsetAsync(evec.p, 0, evec.bytes(), s));
@@ -680,46 +890,24 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
                                dcoef_off.p, dn_done.p, dn_sel.p, dlc.p, sel, evec.p);
             seg(evec.p, evec.p, sc.p, 1);
             hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, evec.p, sc.p, dlc.p, n);
+            hipLaunchKernelGGL(k_raw_store, gn, dim3(kWg), 0, s, evec.p, dlc.p, dperm.p, dn_sel.p,
+                               sel, dcomp, n_comp, seed, n, raw.p);
             MRS_CHECK_LAUNCH();
-            MRS_HIP(hipMemcpyAsync(he.data(), evec.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            MRS_HIP(hipStreamSynchronize(s));
-            for (uint32_t i = 0; i < n; i++) {
-                const uint32_t q = lc_of[i];
-                if (q == 0xFFFFFFFFu) continue;
-                raw[(size_t)i * n_comp + dcomp] =
-                    sel < n_sel[q] ? (float)he[i]
-                                   : (float)(2.0 * u01(seed + dcomp, 3, i) - 1.0);
-            }
         }
+        // the stream is synchronised before these buffers are freed
+        MRS_HIP(hipStreamSynchronize(s));
     }
 
-    std::vector<float> emb((size_t)n * n_comp, 0.0f);
+    DevBuf<float> emb((size_t)n * n_comp);
+    const dim3 gn(div_up(n, kWg));
+    int noise = 0;
     if (single) {
-        if (!spectral[0]) {
-            for (uint32_t i = 0; i < n; i++)
-                for (int dd = 0; dd < n_comp; dd++)
-                    emb[(size_t)i * n_comp + dd] = (float)(2.0 * u01(seed, 5, (uint64_t)i * n_comp + dd) - 1.0);
-        } else {
-            emb = raw;
-        }
-        // finalise_spectral_embedding: centre, max-abs -> range, N(0, 1e-4) noise
-        for (int dd = 0; dd < n_comp; dd++) {
-            double m = 0;
-            for (uint32_t i = 0; i < n; i++) m += emb[(size_t)i * n_comp + dd];
-            const float mean = (float)(m / n);
-            for (uint32_t i = 0; i < n; i++) emb[(size_t)i * n_comp + dd] -= mean;
-        }
-        float mx = 0.0f;
-        for (float x : emb) mx = std::max(mx, std::fabs(x));
-        if (mx > 1e-8f) {
-            const float sc = range / mx;
-            for (float& x : emb) x *= sc;
-        }
-        for (size_t q = 0; q < emb.size(); q++) {
-            const double u1 = std::max(u01(seed + 9999, 11, q), 1e-300), u2 = u01(seed + 9999, 13, q);
-            const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-            emb[q] += (float)(z * 1e-4);
-        }
+        // finalise_spectral_embedding (init.rs:905-960)
+        if (!spectral[0])
+            hipLaunchKernelGGL(k_random_rows, gn, dim3(kWg), 0, s, n, n_comp, seed, emb.p);
+        else
+            MRS_HIP(hipMemcpyAsync(emb.p, raw.p, raw.bytes(), hipMemcpyDeviceToDevice, s));
+        noise = 1;
     } else {
         // component_meta_layout (:539-595)
         std::vector<std::vector<float>> meta(ncomp, std::vector<float>(n_comp, 0.0f));
@@ -739,23 +927,21 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
                     meta[l][dd] = (float)(2.0 * u01(seed, 17, (uint64_t)l * n_comp + dd) - 1.0);
         } else {
             // centroids in feature space, dense exp(-d^2 / max d^2) affinity, spectral
-            std::vector<Chunk> chunks;
+            std::vector<Chunk> cch;
             std::vector<uint32_t> c0(ncomp + 1, 0);
             for (uint32_t c = 0; c < ncomp; c++) {
-                c0[c] = (uint32_t)chunks.size();
+                c0[c] = (uint32_t)cch.size();
                 for (uint32_t b = start[c]; b < start[c + 1]; b += kCenChunk)
-                    chunks.push_back({c, b, std::min(b + kCenChunk, start[c + 1])});
+                    cch.push_back({c, b, std::min(b + kCenChunk, start[c + 1])});
             }
-            c0[ncomp] = (uint32_t)chunks.size();
-            DevBuf<Chunk> dch(chunks.size());
-            DevBuf<uint32_t> dperm(n);
-            DevBuf<double> part(chunks.size() * (size_t)d);
-            MRS_HIP(hipMemcpyAsync(dch.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice, s));
-            MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_centroid_partials, dim3((unsigned)chunks.size()), dim3(kWg), 0, s,
+            c0[ncomp] = (uint32_t)cch.size();
+            DevBuf<Chunk> dch(cch.size());
+            DevBuf<double> part(cch.size() * (size_t)d);
+            MRS_HIP(hipMemcpyAsync(dch.p, cch.data(), sizeof(Chunk) * cch.size(), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_centroid_partials, dim3((unsigned)cch.size()), dim3(kWg), 0, s,
                                dch.p, dperm.p, data, d, part.p);
             MRS_CHECK_LAUNCH();
-            std::vector<double> hp(chunks.size() * (size_t)d);
+            std::vector<double> hp(cch.size() * (size_t)d);
             MRS_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
             MRS_HIP(hipStreamSynchronize(s));
             std::vector<double> cen((size_t)ncomp * d, 0.0);
@@ -795,54 +981,58 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         if (mx > 1e-8f)
             for (auto& r : meta)
                 for (float& x : r) x /= mx;
+        // largest |raw| of each spectral component
+        std::vector<float> ma(ncomp, 0.0f);
+        if (nl) {
+            DevBuf<float> amax(nch);
+            hipLaunchKernelGGL(k_chunk_absmax, dim3(nch), dim3(kWg), 0, s, dchunks.p, raw.p,
+                               n_comp, amax.p);
+            MRS_CHECK_LAUNCH();
+            std::vector<float> ha(nch);
+            MRS_HIP(hipMemcpyAsync(ha.data(), amax.p, sizeof(float) * nch, hipMemcpyDeviceToHost, s));
+            MRS_HIP(hipStreamSynchronize(s));
+            for (uint32_t q = 0; q < nl; q++)
+                for (uint32_t c2 = chunk0[q]; c2 < chunk0[q + 1]; c2++)
+                    ma[lc_comp[q]] = std::max(ma[lc_comp[q]], ha[c2]);
+        }
         // per component placement around its centroid (multi_component_init :439-484)
+        std::vector<float> hmeta((size_t)ncomp * n_comp), dr(ncomp), expansion(ncomp, 1.0f);
         for (uint32_t c = 0; c < ncomp; c++) {
-            float dr = 0.0f;  // meta_data_range: half the largest centroid distance
+            float r = 0.0f;  // meta_data_range: half the largest centroid distance
             for (uint32_t o = 0; o < ncomp; o++) {
                 float d2 = 0;
                 for (int dd = 0; dd < n_comp; dd++) {
                     float df = meta[c][dd] - meta[o][dd];
                     d2 += df * df;
                 }
-                dr = std::max(dr, std::sqrt(d2));
+                r = std::max(r, std::sqrt(d2));
             }
-            dr /= 2.0f;
-            if (!spectral[c]) {
-                for (uint32_t q = start[c]; q < start[c + 1]; q++) {
-                    const uint32_t i = perm[q];
-                    for (int dd = 0; dd < n_comp; dd++) {
-                        const float u = (float)(2.0 * u01(seed, 19, (uint64_t)i * n_comp + dd) - 1.0);
-                        emb[(size_t)i * n_comp + dd] = meta[c][dd] + u * dr;
-                    }
-                }
-                continue;
-            }
-            float ma = 0.0f;
-            for (uint32_t q = start[c]; q < start[c + 1]; q++)
-                for (int dd = 0; dd < n_comp; dd++)
-                    ma = std::max(ma, std::fabs(raw[(size_t)perm[q] * n_comp + dd]));
-            const float expansion = ma > 1e-8f ? dr / ma : 1.0f;
-            for (uint32_t q = start[c]; q < start[c + 1]; q++) {
-                const uint32_t i = perm[q];
-                for (int dd = 0; dd < n_comp; dd++)
-                    emb[(size_t)i * n_comp + dd] =
-                        meta[c][dd] + raw[(size_t)i * n_comp + dd] * expansion;
-            }
+            dr[c] = r / 2.0f;
+            if (spectral[c]) expansion[c] = ma[c] > 1e-8f ? dr[c] / ma[c] : 1.0f;
+            for (int dd = 0; dd < n_comp; dd++) hmeta[(size_t)c * n_comp + dd] = meta[c][dd];
         }
-        for (int dd = 0; dd < n_comp; dd++) {
-            double m = 0;
-            for (uint32_t i = 0; i < n; i++) m += emb[(size_t)i * n_comp + dd];
-            const float mean = (float)(m / n);
-            for (uint32_t i = 0; i < n; i++) emb[(size_t)i * n_comp + dd] -= mean;
-        }
-        float m2 = 0.0f;
-        for (float x : emb) m2 = std::max(m2, std::fabs(x));
-        if (m2 > 1e-8f) {
-            const float sc = range / m2;
-            for (float& x : emb) x *= sc;
-        }
+        DevBuf<float> dmeta(hmeta.size()), ddr(ncomp), dexp(ncomp);
+        DevBuf<uint8_t> dspec(ncomp);
+        MRS_HIP(hipMemcpyAsync(dmeta.p, hmeta.data(), sizeof(float) * hmeta.size(), hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(ddr.p, dr.data(), sizeof(float) * ncomp, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dexp.p, expansion.data(), sizeof(float) * ncomp, hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dspec.p, spectral.data(), ncomp, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_place, gn, dim3(kWg), 0, s, raw.p, dstart.p, ncomp, dperm.p, dmeta.p,
+                           dexp.p, ddr.p, dspec.p, n, n_comp, seed, emb.p);
+        MRS_CHECK_LAUNCH();
+        // the host vectors above are read by the copies: finish before they go
+        MRS_HIP(hipStreamSynchronize(s));
     }
-    MRS_HIP(hipMemcpyAsync(out, emb.data(), sizeof(float) * emb.size(), hipMemcpyHostToDevice, s));
+    // centre, scale max |x| to the range (and, for a single component, add
+    // the noise), written back in the caller's vertex order
+    DevBuf<double> colp((size_t)kFinBlocks * n_comp);
+    DevBuf<float> amaxp(kFinBlocks);
+    hipLaunchKernelGGL(k_col_sums, dim3(kFinBlocks), dim3(kWg), 0, s, emb.p, n, n_comp, colp.p);
+    hipLaunchKernelGGL(k_center_absmax, dim3(kFinBlocks), dim3(kWg), 0, s, emb.p, n, n_comp,
+                       colp.p, amaxp.p);
+    hipLaunchKernelGGL(k_finish_rows, gn, dim3(kWg), 0, s, emb.p, n, n_comp, amaxp.p, range,
+                       noise, seed, dperm.p, out);
+    MRS_CHECK_LAUNCH();
     MRS_HIP(hipStreamSynchronize(s));
 }
 

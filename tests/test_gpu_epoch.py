@@ -363,10 +363,12 @@ def test_driver_node_exact_bit_identical_rerun_and_oracle(ctx):
     assert np.all(np.isfinite(a)) and float(np.median(d)) < 0.1
 
 
-def test_driver_edge_atomic_matches_oracle(ctx):
+@pytest.mark.parametrize("neg_rate", [5, 3])
+def test_driver_edge_atomic_matches_oracle(ctx, neg_rate):
     embd, adj = _random_graph(n=1500, seed=43)
     p = M.UmapOptimParams.default_2d()
     p.n_epochs = 30
+    p.neg_sample_rate = neg_rate
     _teacher_forced(ctx, embd, adj, p, M.EPOCH_EDGE_ATOMIC, 3e-5)
     got = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_EDGE_ATOMIC)
     want = orc.optimise_adam_gpu(embd, *adj, p, 42)

@@ -131,6 +131,77 @@ def test_spectral_layout_properties(ctx):
     assert np.array_equal(emb, g.spectral_layout(2, seed=1, data=x))
 
 
+def _rectangle(n, aspect, d, seed):
+    """Uniform points in [0, aspect] x [0, 1] rotated into d dimensions."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(size=(n, 2)) * np.array([aspect, 1.0])
+    q, _ = np.linalg.qr(rng.normal(size=(d, d)))
+    return (pts @ q[:2]).astype(np.float32)
+
+
+def _dense_m(n, rows, cols, vals):
+    """M = -D^-1/2 W D^-1/2 with a zero diagonal (the matrix of
+    graph_to_negative_normalised_adjacency, src/data/init.rs:210-256)."""
+    W = np.zeros((n, n))
+    W[rows, cols] = vals
+    deg = W.sum(1)
+    dinv = np.where(deg > 1e-8, 1.0 / np.sqrt(deg), 0.0)
+    M_ = -(dinv[:, None] * W * dinv[None, :])
+    np.fill_diagonal(M_, 0.0)
+    return M_
+
+
+@pytest.mark.parametrize("n_dim", [2, 3])
+def test_spectral_layout_spans_smallest_eigenvectors(ctx, n_dim):
+    # One connected component of 40 vertices: the Lanczos budget
+    # (4 * (n_dim + 1) + 30 >= 40 iterations, capped at the component size)
+    # spans the whole space, so the Ritz vectors are the exact eigenvectors.
+    # The layout is s * (v - mean v) + N(0, 1e-4) noise per column, with v the
+    # Ritz vectors 1..n_dim of M: each column must lie in the span of the dense
+    # solver's eigenvectors 1..n_dim plus the constant vector.
+    x = _rectangle(40, 4.0, 8, seed=3)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    g = M.FuzzyGraph(ctx, idx, dist)
+    emb = g.spectral_layout(n_dim, seed=5, data=x).astype(np.float64)
+    r, c, v = g.coo()
+    Mx = _dense_m(g.n, r, c, v)
+    w, U = np.linalg.eigh(Mx)
+    assert w[n_dim + 1] - w[n_dim] > 1e-3, "test graph: eigenvalues not separated"
+    basis = np.column_stack([U[:, 1:n_dim + 1], np.ones(g.n)])
+    for dd in range(n_dim):
+        coef, *_ = np.linalg.lstsq(basis, emb[:, dd], rcond=None)
+        resid = emb[:, dd] - basis @ coef
+        r2 = 1.0 - resid.var() / emb[:, dd].var()
+        assert r2 > 0.9999, f"column {dd}: R^2 {r2:.6f} against eigenvectors 1..{n_dim}"
+    assert np.isclose(np.max(np.abs(emb)), 10.0, atol=1e-3)
+
+
+def test_spectral_layout_multi_component_places_each_component(ctx):
+    # Three far-apart rectangles are three components: each gets its own
+    # Lanczos, so within a component the layout is an affine image of that
+    # component's own eigenvectors 1..2 (meta offset + raw * expansion).
+    parts = [_rectangle(40, 4.0, 8, seed=s) + 100.0 * s for s in (1, 2, 3)]
+    x = np.concatenate(parts).astype(np.float32)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    g = M.FuzzyGraph(ctx, idx, dist)
+    emb = g.spectral_layout(2, seed=5, data=x).astype(np.float64)
+    r, c, v = g.coo()
+    for k in range(3):
+        sl = slice(40 * k, 40 * (k + 1))
+        keep = (r >= sl.start) & (r < sl.stop)
+        assert np.all((c[keep] >= sl.start) & (c[keep] < sl.stop)), "components leak"
+        Mk = _dense_m(40, r[keep] - sl.start, c[keep] - sl.start, v[keep])
+        wk, U = np.linalg.eigh(Mk)
+        assert wk[3] - wk[2] > 1e-3, "test graph: eigenvalues not separated"
+        basis = np.column_stack([U[:, 1:3], np.ones(40)])
+        for dd in range(2):
+            y = emb[sl, dd]
+            coef, *_ = np.linalg.lstsq(basis, y, rcond=None)
+            r2 = 1.0 - (y - basis @ coef).var() / y.var()
+            assert r2 > 0.9999, f"component {k} column {dd}: R^2 {r2:.6f}"
+    assert np.isclose(np.max(np.abs(emb)), 10.0, atol=1e-3)
+
+
 def test_densmap_lambda_zero_matches_umap(ctx):
     # densmap_gpu_integration_02 (tests/densmap_gpu_tests.rs:173-219)
     x, _ = diagnostic_data(40, 10, 42)
