@@ -12,6 +12,10 @@ Prints ONE JSON line.  Workloads:
            k=15-style graph): one step = one full 500-epoch optimisation.
   serve    the served stream: Poisson arrivals of uneven fits into the fit
            server, fits/s and p50/p99 latency against sequential fits.
+  fp32     the plain GPU form in FP32: n=100,000, d=50, exact FP32 kNN, 500
+           epochs; trustworthiness@15 against the CPU optimisers.
+  densmap  densMAP: n=500,000, d=50, k=30; local-radius Spearman against the
+           CPU densMAP.
 
 The CPU baseline is the C restatement of the reference's parallel Adam
 (optimise_embedding_adam_parallel) timed on this host's cores on a bounded
@@ -348,6 +352,218 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
                 knn_s=t_knn, graph_s=t_graph, adam_s=t_adam)
 
 
+def fit_steps(M, ctx, xd, p, args, dens=None):
+    """W untimed and K timed umap_gpu/densmap_gpu fits with the data resident in
+    HBM: (mean ms per fit, mean stage timings, the last output)."""
+    import torch
+
+    for _ in range(args.warmup):
+        M.umap_gpu(ctx, xd, p, seed=42, dens=dens)
+    torch.cuda.synchronize()
+    times, stages, out = [], [], None
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t_s = time.perf_counter()
+        out = M.umap_gpu(ctx, xd, p, seed=42, dens=dens)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t_s) * 1e3)
+        stages.append(M.last_fit_timings())
+        log(f"[bench] fit {times[-1]:.1f} ms: {stages[-1]}")
+    st = {kk: float(np.mean([s_[kk] for s_ in stages])) for kk in stages[0]}
+    return float(np.mean(times)), st, out
+
+
+def oracle_graph(orc, knn, n, k, n_epochs):
+    """The oracle's host graph stages on a kNN: (rows, cols, vals) filtered COO."""
+    idx, dist = knn
+    sg, rh = orc.smooth_knn_dist(dist, k)
+    r, c, v = orc.knn_to_coo(idx, dist, sg, rh)
+    r, c, v = orc.symmetrise(n, r, c, v, 1.0)
+    return orc.filter_weak_edges(r, c, v, n_epochs)
+
+
+def sample_trustworthiness(x, emb, m=5000, k=15, seed=11):
+    """trustworthiness@k on a seeded m-point sample (the sample's own
+    neighbourhoods in both spaces; the same sample for every embedding)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from metrics import trustworthiness
+
+    rows = np.sort(np.random.default_rng(seed).choice(x.shape[0], size=min(m, x.shape[0]),
+                                                      replace=False))
+    return trustworthiness(x[rows], np.asarray(emb)[rows], k)
+
+
+def bench_fp32(args):
+    """The plain GPU form, FP32 throughout: n=100,000, d=50 (20 anisotropic
+    Gaussians + 5 % uniform background), exact FP32 kNN k=15, 500 Adam epochs,
+    2-D output.  Quality: trustworthiness@15 (5 000-point sample) of the GPU
+    fit, of the CPU parallel Adam (the reference's CPU optimiser, restated) and
+    of the CPU restatement of adam_gpu (identical hashed negatives), both
+    started from the fit's own graph and spectral layout."""
+    import torch
+
+    import mrs_amd as M
+    import synthetic
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import pyoracle as orc
+    from helpers import coo_to_adjacency
+
+    n, d, k, n_epochs = 100_000, 50, 15, 500
+    x, _ = synthetic.anisotropic(n=n, d=d)
+    ctx = M.Context(0)
+    xd = torch.from_numpy(x).cuda()
+    p = M.UmapParamsGpu.default()
+    p.k = k
+    p.knn_precision = M.KNN_FP32
+    p.optim_params.n_epochs = n_epochs
+    ms, st, out = fit_steps(M, ctx, xd, p, args)
+    e_gpu = out.T.cpu().numpy()
+    _, _, kt = M.knn_exhaustive(ctx, xd, k, M.KNN_FP32, 0, timed=True)
+    flops = 2.0 * n * n * d
+    achieved = flops / (kt["shortlist_ms"] * 1e-3) / 1e12
+    # the fit's graph and layout, then the CPU optimisers from them
+    idx, dist = M.knn_exhaustive(ctx, x, k, M.KNN_FP32)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
+    init = g.spectral_layout(2, seed=42, data=x)
+    adj = coo_to_adjacency(n, *g.coo())
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = n_epochs
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    e_par = orc.optimise_adam_parallel(init, *adj, q, 42, use_lut=True, n_threads=cores)
+    t_par = time.perf_counter() - t0
+    e_hash = orc.optimise_adam_gpu(init, *adj, q, 42)
+    tw = dict(gpu=sample_trustworthiness(x, e_gpu), cpu_parallel_adam=sample_trustworthiness(x, e_par),
+              cpu_adam_gpu_restated=sample_trustworthiness(x, e_hash))
+    tw["gpu_minus_cpu_parallel"] = tw["gpu"] - tw["cpu_parallel_adam"]
+    log(f"[bench] trustworthiness@15: {tw}")
+    t0 = time.perf_counter()
+    Q = args.cpu_queries
+    orc.knn_exact(x, k, 0, Q)
+    t_knn = (time.perf_counter() - t0) * n / Q
+    t0 = time.perf_counter()
+    oracle_graph(orc, (idx, dist), n, k, n_epochs)
+    t_graph = time.perf_counter() - t0
+    return dict(
+        metric="plain FP32 GPU fit sec @100k×50 k=15 (trustworthiness@15 vs CPU parallel Adam)",
+        value=ms / 1e3, unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms, higher_is_better=False, scaling="weak", vs_baseline=None,
+        dtype="f32-mfma/f32", data="synthetic",
+        config=dict(workload="umap_gpu fit, n=100,000 x d=50 f32 (20 anisotropic Gaussians, "
+                    "5% uniform background), exhaustive FP32 kNN k=15, spectral init, 500 Adam "
+                    "epochs, 2-D output", n=n, d=d, k=k, n_epochs=n_epochs),
+        stages_ms=st,
+        roofline=dict(bound="mfma", kernel="k_shortlist<f32>", achieved=achieved, peak=157.3,
+                      unit="TFLOP/s", frac=achieved / 157.3, traffic=None,
+                      avg_launch_ms=kt["shortlist_ms"], algorithmic_flops_per_launch=flops),
+        trustworthiness_at_15=tw,
+        cpu_baseline=dict(value=t_knn + t_graph + t_par, unit="s", cores=cores, kind="port",
+                          sample=f"kNN: {Q} of {n} query rows x all {n} (scaled x{n / Q:g}); "
+                                 f"graph: full; parallel Adam: all {n_epochs} epochs; spectral "
+                                 f"init not included",
+                          knn_s=t_knn, graph_s=t_graph, adam_s=t_par),
+    )
+
+
+def log_radii(emb, rows, cols, a, b):
+    """The embedding's local log radii of densMAP (umap_dens_radii,
+    src/training/umap_optimiser_gpu.rs:487-544, in float64): ln(1e-8 +
+    sum_j phi_ij d_ij^2 / sum_j phi_ij), phi = 1 / (1 + a d^(2b)), over the
+    graph's edges."""
+    e = np.asarray(emb, np.float64)
+    d2 = ((e[rows] - e[cols]) ** 2).sum(1)
+    phi = 1.0 / (1.0 + a * d2 ** b)
+    n = e.shape[0]
+    num = np.bincount(rows, weights=phi * d2, minlength=n)
+    den = np.maximum(np.bincount(rows, weights=phi, minlength=n), np.finfo(np.float32).eps)
+    return np.log(1e-8 + num / den)
+
+
+def bench_densmap(args):
+    """densMAP on the GPU: n=500,000, d=50 (30 Gaussians with per-cluster sd
+    log-uniform in [0.2, 5]), k=30, lambda 2.0, frac 0.3, var_shift 0.1, 500
+    Adam epochs, BF16 kNN + exact re-rank.  Reports the fit time and the
+    Spearman correlation of the original and embedded local radii (R_o from
+    the density state, R_e from the final layout over the graph); the same
+    correlation for the GPU and for the CPU densMAP (the oracle's parallel Adam
+    with the density term) on a 20,000-point draw of the same distribution."""
+    import torch
+
+    import mrs_amd as M
+    import synthetic
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import pyoracle as orc
+    from helpers import coo_to_adjacency
+    from metrics import spearman
+
+    k, n_epochs = 30, 500
+    dp = M.DensParams.densmap_default()
+    p = M.UmapParamsGpu.default()
+    p.k = k
+    p.knn_precision = M.KNN_BF16_RERANK
+    p.optim_params.n_epochs = n_epochs
+    a, b = p.optim_params.a, p.optim_params.b
+    ctx = M.Context(0)
+
+    def spearman_of(x, emb, idx, dist):
+        g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
+        r_o, _, _ = g.dens_state(idx, dist)
+        rows, cols, _ = g.coo()
+        return spearman(r_o, log_radii(emb, rows, cols, a, b)), g
+
+    n = 500_000
+    x, _ = synthetic.uneven_density(n=n)
+    xd = torch.from_numpy(x).cuda()
+    ms, st, out = fit_steps(M, ctx, xd, p, args, dens=dp)
+    idx, dist = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 0)
+    rho_full, _ = spearman_of(x, out.T.cpu().numpy(), idx, dist)
+    plain = M.umap_gpu(ctx, xd, p, seed=42).T.cpu().numpy()
+    rho_plain, _ = spearman_of(x, plain, idx, dist)
+    log(f"[bench] Spearman(R_o, R_e) at n={n}: densMAP {rho_full:.4f}, plain UMAP {rho_plain:.4f}")
+    # GPU vs CPU densMAP on a smaller draw
+    ns = 20_000
+    xs, _ = synthetic.uneven_density(n=ns, seed=7)
+    e_gpu = M.umap_gpu(ctx, xs, p, seed=42, dens=dp).T
+    sidx, sdist = M.knn_exhaustive(ctx, xs, k, M.KNN_FP32)
+    rho_gpu_s, gs = spearman_of(xs, e_gpu, sidx, sdist)
+    rr, cc, vv = oracle_graph(orc, (sidx, sdist), ns, k, n_epochs)
+    ok, R, _, mu_tot = orc.dens_state(ns, rr, cc, vv, sidx, sdist)
+    init = gs.spectral_layout(2, seed=42, data=xs)
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = n_epochs
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    e_cpu = orc.optimise_adam_parallel(init, *coo_to_adjacency(ns, rr, cc, vv), q, 42,
+                                       n_threads=cores, dens_r=R,
+                                       dens_params=(dp.lambda_, dp.frac, dp.var_shift),
+                                       mu_tot=mu_tot)
+    t_cpu = time.perf_counter() - t0
+    rho_cpu_s, _ = spearman_of(xs, e_cpu, sidx, sdist)
+    log(f"[bench] Spearman at n={ns}: GPU {rho_gpu_s:.4f}, CPU densMAP {rho_cpu_s:.4f}")
+    return dict(
+        metric="densMAP GPU fit sec @500k×50 k=30 (local-radius Spearman vs CPU densMAP)",
+        value=ms / 1e3, unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms, higher_is_better=False, scaling="weak", vs_baseline=None,
+        dtype="bf16-mfma/f32, f64 reductions", data="synthetic",
+        config=dict(workload="densmap_gpu fit, n=500,000 x d=50 f32 (30 Gaussians, sd "
+                    "log-uniform in [0.2, 5]), k=30, lambda 2.0, frac 0.3, var_shift 0.1, "
+                    "BF16 kNN + exact re-rank, spectral init, 500 Adam epochs",
+                    n=n, d=50, k=k, n_epochs=n_epochs),
+        stages_ms=st,
+        local_radius_spearman=dict(densmap_gpu=rho_full, plain_umap_gpu=rho_plain,
+                                   sample_n=ns, sample_gpu=rho_gpu_s,
+                                   sample_cpu_densmap=rho_cpu_s),
+        cpu_baseline=dict(value=t_cpu * n / ns, unit="s", cores=cores, kind="port",
+                          sample=f"parallel Adam with the density term, {n_epochs} epochs on a "
+                                 f"{ns}-point draw, scaled by n ({n / ns:g}x); kNN, graph and "
+                                 f"init not included", adam_s_sample=t_cpu),
+    )
+
+
 def copy_probe(mib: int) -> None:
     """One device-to-device copy of `mib` MiB by a plain elementwise kernel: it
     reads and writes exactly mib MiB, so HBM byte counters read on it give the
@@ -497,6 +713,10 @@ def main():
         res = bench_epochs(args)
     elif args.workload == "serve":
         res = bench_serve(args)
+    elif args.workload == "fp32":
+        res = bench_fp32(args)
+    elif args.workload == "densmap":
+        res = bench_densmap(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     print(json.dumps(res))

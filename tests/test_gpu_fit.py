@@ -229,6 +229,39 @@ def test_densmap_finite_and_preserves_density(ctx):
     assert corr_d > corr_p
 
 
+def test_densmap_agrees_with_cpu_densmap(ctx):
+    # densmap_gpu_integration_05 (tests/densmap_gpu_tests.rs:313-360): the GPU
+    # and CPU density paths differ in their negative samples, so coordinates
+    # never match; both must preserve density.  Per-cluster log-radius Pearson
+    # correlation with the data: > 0.9 on the GPU and within 0.15 of the CPU
+    # densMAP's (the oracle's parallel Adam with the density term, from the
+    # oracle's graph and density state and that graph's spectral layout).
+    x, lab = variable_density(60, 10, [0.2, 1.0, 4.0], 42)
+    dp = M.DensParams.densmap_default()
+    gpu = M.umap_gpu(ctx, x, params(), seed=42, dens=dp).T
+    idx, dist, (r, c, v), adj = umap_graph_from_data(x, 15)
+    st, R, _, mu_tot = orc.dens_state(x.shape[0], r, c, v, idx, dist)
+    assert st == 0
+    g = M.FuzzyGraph(ctx, idx, dist)
+    init = g.spectral_layout(2, seed=42, data=x)
+    q = M.UmapOptimParams.default_2d()
+    cpu = orc.optimise_adam_parallel(init, *adj, q, 42, dens_r=R,
+                                     dens_params=(dp.lambda_, dp.frac, dp.var_shift),
+                                     mu_tot=mu_tot)
+
+    def radii(e):
+        return np.array([np.linalg.norm(e[lab == k] - e[lab == k].mean(0), axis=1).mean()
+                         for k in range(3)])
+
+    orig = np.log(radii(x.astype(np.float64)))
+    corr_gpu = float(np.corrcoef(orig, np.log(radii(gpu)))[0, 1])
+    corr_cpu = float(np.corrcoef(orig, np.log(radii(cpu)))[0, 1])
+    print(f"log-radius correlation: GPU {corr_gpu:.4f}, CPU {corr_cpu:.4f}")
+    assert np.all(np.isfinite(gpu)) and np.all(np.isfinite(cpu))
+    assert corr_gpu > 0.9
+    assert abs(corr_gpu - corr_cpu) < 0.15
+
+
 def test_fit_errors(ctx):
     x = np.zeros((10, 3), np.float32)
     p = params(optimiser=b"sgd")
