@@ -40,12 +40,21 @@ HIP_FLAGS = [
     "-Wno-unused-parameter",
 ]
 
+# Per-file extras.  knn.hip: MFMA results in ordinary VGPRs (the top-k
+# epilogue reads every accumulator element, which from AGPRs costs one move
+# each), and no IEEE NaN quieting on min/max (scores of finite data are never
+# NaN; IEEE mode makes every fmaxf canonicalise its inputs first).
+FILE_FLAGS = {
+    "knn.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form", "-mno-amdgpu-ieee", "-fno-honor-nans"],
+}
+
 SOURCES = sorted(CSRC.glob("*.hip"))
 
 
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src.name, []), "-I", str(ROOT / "include"), "-c",
+           str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")

@@ -12,15 +12,17 @@
 //   beats it) costs 15 maxes + 1 compare + 1 ballot per 16 candidates.
 //   Precision FP32 (v_mfma_f32_32x32x2_f32) or BF16 (v_mfma_f32_32x32x16_bf16,
 //   FP32 accumulate).
-//   Each workgroup holds 8 waves x 32 queries; database tiles of 64 rows are
-//   staged through LDS (rows padded by 16 B against bank conflicts) and shared
-//   by the 8 waves.  Candidates beating the query's threshold go to a per-query
-//   LDS queue (64 entries); a queue that could overflow on the next tile is
-//   merged into the query's sorted top-S list in global memory by the whole
-//   wavefront (bitonic sort + merge on 64-bit keys), which also refreshes the
-//   threshold.  Large problems use one database split; small ones split the
-//   database so the grid still fills 256 CUs, and a merge kernel joins the
-//   split lists.
+//   Each workgroup holds 4 waves x 32 queries; database stages of 32-128 rows
+//   are staged through two LDS buffers (rows padded by 16 B against bank
+//   conflicts), prefetched one stage ahead in registers and shared by the 4
+//   waves.  Candidates beating the query's threshold go to a per-query queue
+//   (64 entries, in global memory: it is touched only on the rare path); a
+//   queue that could overflow on the next tile is merged into the query's
+//   sorted top-S list by the whole wavefront (bitonic sort + merge on 64-bit
+//   keys), which also refreshes the threshold.  The query's own row is
+//   dropped at insertion.  Large problems use one database split; small ones
+//   split the database so the grid still fills 256 CUs, and a merge kernel
+//   joins the split lists.
 //
 // Stage 2 -- exact re-rank: one wavefront per query recomputes the S candidate
 //   distances in FP32 as fmaf((x_q[t]-x_c[t])^2) over t in order (the same
@@ -240,20 +242,22 @@ template <int E, int QC>
 __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h, int col,
                                 int lane, float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
                                 uint64_t* __restrict__ myq) {
-    if (q - rowbase < 32u) {  // the query's own row is in this tile: exclude it
-#pragma unroll
-        for (int r = 0; r < 16; r++)
-            if (rowbase + (r >> 2) * 8 + h * 4 + (r & 3) == q) acc[r] = -INFINITY;
-    }
     float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
 #pragma unroll
     for (int g = 0; g < 4; g++)
         gm[g] = fmaxf(fmaxf(acc[g * 4], acc[g * 4 + 1]), fmaxf(acc[g * 4 + 2], acc[g * 4 + 3]));
     const float m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
     if (__ballot(m > thr) == 0ull) return;
+    // The query's own row scores highest and may pass the test above; it is
+    // never queued.  Excluding it here, on the rare path, keeps the common
+    // path free of per-element index compares.
+    const uint32_t self = q - rowbase - (uint32_t)h * 4u;  // q's offset among this lane's rows
+    auto take = [&](int r) {
+        return acc[r] > thr && (uint32_t)((r >> 2) * 8 + (r & 3)) != self;
+    };
     uint32_t c = 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) c += acc[r] > thr ? 1u : 0u;
+    for (int r = 0; r < 16; r++) c += take(r) ? 1u : 0u;
     const uint32_t co = __shfl_xor(c, 32, 64);
     uint32_t pos = qn + (h ? co : 0u);
     qn += c + co;
@@ -264,7 +268,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h,
         if (gm[g] > thr) {
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                if (acc[g * 4 + j] > thr)
+                if (take(g * 4 + j))
                     dst[pos++] = score_key(acc[g * 4 + j], rowbase + g * 8 + h * 4 + j);
         }
     uint64_t need = __ballot(qn > (uint32_t)(QC - 32)) & 0xFFFFFFFFull;  // lanes 0..31: one per query
@@ -312,8 +316,10 @@ __device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned cha
 // The distance GEMM + top-S filter.  One workgroup = W waves x 32 queries; the
 // database split [db0, db1) streams through two LDS stage buffers of RT rows
 // (double buffering: one barrier per stage), each stage prefetched into
-// registers one stage ahead.  QC = per-query queue capacity.
-template <bool BF16, int E, int RT, int QC, int W>
+// registers one stage ahead.  QC = per-query queue capacity.  STEPS > 0
+// compiles the K-step count in (BF16: KP / 16), so the query fragments take
+// only the registers they need and the MFMA chain is straight-line code.
+template <bool BF16, int E, int RT, int QC, int W, int STEPS>
 __global__ __launch_bounds__(W * 64) void k_shortlist(ShortlistArgs a) {
     constexpr int CPT = (kStageChunks * kWavesMax) / W;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -328,7 +334,8 @@ __global__ __launch_bounds__(W * 64) void k_shortlist(ShortlistArgs a) {
     float* negh1 = negh0 + RT;
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63, h = lane >> 5, col = lane & 31;
+    const int lane = tid & 63, h = lane >> 5, col = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const uint32_t qbase = (blockIdx.x * W + wave) * kQPerWave;
     const uint32_t split = blockIdx.y;
     const uint32_t db0 = split * a.db_per_split;
@@ -339,8 +346,8 @@ __global__ __launch_bounds__(W * 64) void k_shortlist(ShortlistArgs a) {
     for (int i = lane; i < kQPerWave * S; i += 64) lists[i] = kInfKey;
     // B fragments: this lane's query (column) over its half of K
     const uint32_t q = qbase + col;
-    constexpr int MAXSTEPS = BF16 ? 8 : 64;  // KP <= 128
-    const int steps = BF16 ? KP / 16 : KP / 2;
+    constexpr int MAXSTEPS = STEPS > 0 ? STEPS : (BF16 ? 8 : 64);  // KP <= 128
+    const int steps = STEPS > 0 ? STEPS : (BF16 ? KP / 16 : KP / 2);
     bf16x8 bq[BF16 ? MAXSTEPS : 1];
     float bqf[BF16 ? 1 : MAXSTEPS];
     if (BF16) {
@@ -644,13 +651,34 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     };
     MRS_HIP(hipEventRecord(ev[1], s));
     bool launched = false;
+    // BF16 kernels are compiled per K-step count for the shapes that take it
+    // (128-row stages: KP <= 64; 64-row stages: KP 80..128)
+    const int ksteps = bf16 ? KP / 16 : 0;
     auto pick = [&](auto b16, auto e, auto rt, auto qc, auto w) {
         constexpr bool B = decltype(b16)::value;
         constexpr int EE = decltype(e)::value, RTT = decltype(rt)::value;
         constexpr int QCC = decltype(qc)::value, WW = decltype(w)::value;
         if (launched || B != bf16 || EE != E || RTT != sh.RT || QCC != sh.QC || WW != sh.W) return;
-        launch(k_shortlist<B, EE, RTT, QCC, WW>, WW);
-        launched = true;
+        if constexpr (B) {
+            auto st = [&](auto stc) {
+                constexpr int ST = decltype(stc)::value;
+                if (launched || ST != ksteps) return;
+                launch(k_shortlist<B, EE, RTT, QCC, WW, ST>, WW);
+                launched = true;
+            };
+            using std::integral_constant;
+            if constexpr (RTT == 128) {
+                st(integral_constant<int, 1>{}); st(integral_constant<int, 2>{});
+                st(integral_constant<int, 3>{}); st(integral_constant<int, 4>{});
+            } else if constexpr (RTT == 64) {
+                st(integral_constant<int, 5>{}); st(integral_constant<int, 6>{});
+                st(integral_constant<int, 7>{}); st(integral_constant<int, 8>{});
+            }
+        }
+        if (!launched) {
+            launch(k_shortlist<B, EE, RTT, QCC, WW, 0>, WW);
+            launched = true;
+        }
     };
     using std::integral_constant;
     using T = integral_constant<bool, true>;
