@@ -319,8 +319,13 @@ __device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned cha
 // registers one stage ahead.  QC = per-query queue capacity.  STEPS > 0
 // compiles the K-step count in (BF16: KP / 16), so the query fragments take
 // only the registers they need and the MFMA chain is straight-line code.
+//   The loop is latency-bound (waves mostly wait on LDS, the stage barrier
+// and the prefetch), so occupancy pays: the BF16 kernels with the step count
+// compiled in are held to 128 registers (four waves per SIMD; at most a few
+// spilled dwords, none for the 128-row stages), the FP32 ones to 168 (three).
 template <bool BF16, int E, int RT, int QC, int W, int STEPS>
-__global__ __launch_bounds__(W * 64) void k_shortlist(ShortlistArgs a) {
+__global__ __launch_bounds__(W * 64)
+__attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) void k_shortlist(ShortlistArgs a) {
     constexpr int CPT = (kStageChunks * kWavesMax) / W;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int S = 64 * E;
