@@ -59,9 +59,10 @@ int guarded(F&& body) {
 }
 
 // The stream the calling thread works on inside the library (set for the
-// duration of an ABI call by StreamScope).  Device buffers are allocated and
-// freed in that stream's order from the device's memory pool: hipFree would
-// wait for every stream of the device, which serialises concurrent fits.
+// duration of an ABI call by StreamScope).  With stream-ordered allocation on,
+// device buffers are allocated and freed in that stream's order from the
+// device's memory pool (hipFree waits for every stream of the device, which
+// serialises concurrent fits).
 struct StreamScope {
     hipStream_t prev;
     bool prev_set;
@@ -156,7 +157,9 @@ struct DevView {
             p = staged.p;
             host = caller;
             if (upload)
-                MRS_HIP(hipMemcpyAsync(p, caller, sizeof(T) * count, hipMemcpyHostToDevice, s));
+                // the runtime picks the direction from the address: a pointer
+                // the lookup above failed to recognise is still copied right
+                MRS_HIP(hipMemcpyAsync(p, caller, sizeof(T) * count, hipMemcpyDefault, s));
         }
     }
     bool is_staged() const { return host != nullptr; }
@@ -164,7 +167,7 @@ struct DevView {
     void download(hipStream_t s) {
         if (host && n)
             MRS_HIP(hipMemcpyAsync(const_cast<void*>(host), p, sizeof(T) * n,
-                                   hipMemcpyDeviceToHost, s));
+                                   hipMemcpyDefault, s));
     }
 };
 

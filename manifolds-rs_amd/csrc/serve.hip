@@ -232,9 +232,8 @@ void worker_loop(mrs_server* sv, int w) {
         try {
             StreamScope scope(ctx->stream);
             FitTimings t;
-            DevView<float> xv(r->data, (size_t)r->n * r->d, ctx->stream);
-            r->st = fit_prepare(ctx, xv.p, r->n, r->d, nullptr, nullptr, 0, &r->params, nullptr,
-                                r->seed, 0, t);
+            r->st = fit_prepare(ctx, r->data, r->n, r->d, nullptr, nullptr, 0, &r->params,
+                                nullptr, r->seed, 0, t);
             MRS_HIP(hipStreamSynchronize(ctx->stream));
         } catch (const Error& e) {
             status = e.status;
@@ -373,21 +372,30 @@ void batcher_loop(mrs_server* sv) {
             }
             MRS_HIP(hipEventRecord(sv->tick_ev[slot], s));
             for (mrs_server_request* r : active) r->epoch++;
-            // retire finished fits: [n_dim][n] output on the device, state freed in stream order
-            std::vector<mrs_server_request*> keep;
-            for (mrs_server_request* r : active) {
-                if (r->epoch < r->st->params.n_epochs) {
-                    keep.push_back(r);
-                    continue;
-                }
-                r->out.alloc((size_t)r->st->n * r->st->dim);
-                fit_output_async(r->st.get(), r->out.p);
-                MRS_HIP(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
-                MRS_HIP(hipEventRecord(r->done_ev, s));
-                r->st.reset();
-                finish(sv, r, MRS_OK, "");
-            }
+            // retire finished fits: [n_dim][n] output on the device, state freed
+            // in stream order.  A finished request leaves `active` before it is
+            // handed back (a waiter may free it at once), and each is finished
+            // exactly once, with its own error if its retirement fails.
+            std::vector<mrs_server_request*> keep, done;
+            for (mrs_server_request* r : active)
+                (r->epoch < r->st->params.n_epochs ? keep : done).push_back(r);
             active.swap(keep);
+            for (mrs_server_request* r : done) {
+                int status = MRS_OK;
+                std::string err;
+                try {
+                    r->out.alloc((size_t)r->st->n * r->st->dim);
+                    fit_output_async(r->st.get(), r->out.p);
+                    MRS_HIP(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
+                    MRS_HIP(hipEventRecord(r->done_ev, s));
+                } catch (const std::exception& e) {
+                    const Error* me = dynamic_cast<const Error*>(&e);
+                    status = me ? me->status : MRS_DEVICE_ERROR;
+                    err = e.what();
+                }
+                r->st.reset();
+                finish(sv, r, status, err);
+            }
             tick++;
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->ticks = tick;

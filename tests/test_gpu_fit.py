@@ -47,6 +47,57 @@ def params(**kw):
     return p
 
 
+def test_knn_diagnostic_data_correctness(ctx):
+    # umap_gpu_integration_01 (tests/umap_gpu_tests.rs:14-60): no self in the
+    # neighbour lists, > 80 % of neighbours in the point's own cluster, every
+    # distance > 0
+    x, lab = diagnostic_data(50, 10, 42)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    assert not np.any(idx == np.arange(x.shape[0])[:, None])
+    assert float(np.mean(lab[idx] == lab[:, None])) > 0.8
+    assert float(dist.min()) > 0.0
+
+
+def test_ann_dispatch_all_types(ctx):
+    # umap_gpu_integration_02 (tests/umap_gpu_tests.rs:62-88): every ann_type
+    # string dispatches and returns [n_dim, n]; the approximate types run the
+    # exact search here, so all three fits are the same fit
+    x, _ = diagnostic_data(50, 10, 42)
+    outs = []
+    for ann in (b"exhaustive_gpu", b"ivf_gpu", b"nndescent_gpu"):
+        p = params(epoch_strategy=M.EPOCH_NODE_EXACT)
+        p.ann_type = ann
+        p.optim_params.n_epochs = 50
+        e = M.umap_gpu(ctx, x, p, seed=42)
+        assert e.shape == (2, x.shape[0]) and np.all(np.isfinite(e))
+        outs.append(e)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_graph_clusters_connected(ctx):
+    # umap_gpu_integration_03 (tests/umap_gpu_tests.rs:90-158): inside every
+    # cluster of the diagnostic data the fuzzy graph is connected
+    x, lab = diagnostic_data(50, 10, 42)
+    idx, dist = M.knn_exhaustive(ctx, x, 15)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r, c, _ = g.coo()
+    for cl in range(5):
+        members = set(np.flatnonzero(lab == cl).tolist())
+        keep = np.array([a in members and b in members for a, b in zip(r, c)])
+        adj = {m: [] for m in members}
+        for a, b in zip(r[keep], c[keep]):
+            adj[int(a)].append(int(b))
+        start = min(members)
+        seen, todo = {start}, [start]
+        while todo:
+            v = todo.pop()
+            for w in adj[v]:
+                if w not in seen:
+                    seen.add(w)
+                    todo.append(w)
+        assert len(seen) == len(members), f"cluster {cl}: {len(seen)}/{len(members)} reachable"
+
+
 def test_fit_finite_and_separated(ctx):
     # umap_gpu_integration_04, tests/umap_gpu_tests.rs:161-231
     x, lab = diagnostic_data(50, 10, 123)
