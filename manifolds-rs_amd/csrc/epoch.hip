@@ -223,6 +223,54 @@ __global__ __launch_bounds__(kWg) void k_dens_radii(const float* __restrict__ em
     phi_sum[node] = sum_phi;
 }
 
+// The same radii with 16 lanes per node (lane l takes the row's entries l,
+// l+16, ...; the partial sums meet in a fixed xor tree, so the result is
+// identical run to run).  A node's ~2k entries are random gathers: one thread
+// per node leaves them mostly waiting in sequence, 16 lanes overlap them.
+// Used by the edge-parallel strategy; the node-exact one keeps the reference's
+// sequential order above.
+constexpr int kRadiiLanes = 16;
+template <int DIM>
+__global__ __launch_bounds__(kWg) void k_dens_radii_lanes(const float* __restrict__ embd,
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ other_node,
+                                                          float* __restrict__ re,
+                                                          float* __restrict__ phi_sum, uint32_t n,
+                                                          float a, float b, float dens_eps,
+                                                          float tiny) {
+    const uint32_t node = (blockIdx.x * kWg + threadIdx.x) / kRadiiLanes;
+    const int sub = threadIdx.x & (kRadiiLanes - 1);
+    float sum_sq = 0.0f, sum_phi = 0.0f;
+    if (node < n) {
+        float ys[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ys[d] = embd[(size_t)node * DIM + d];
+        for (uint32_t pos = off[node] + sub; pos < off[node + 1]; pos += kRadiiLanes) {
+            const uint32_t o = other_node[pos];
+            float dist_sq = 0.0f;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                float diff = ys[d] - embd[(size_t)o * DIM + d];
+                dist_sq += diff * diff;
+            }
+            float dsb = powf(dist_sq, b);
+            float phi = 1.0f / (1.0f + a * dsb);
+            sum_sq += phi * dist_sq;
+            sum_phi += phi;
+        }
+    }
+#pragma unroll
+    for (int o = kRadiiLanes / 2; o >= 1; o >>= 1) {
+        sum_sq += __shfl_xor(sum_sq, o, kRadiiLanes);
+        sum_phi += __shfl_xor(sum_phi, o, kRadiiLanes);
+    }
+    if (node < n && sub == 0) {
+        float denom = sum_phi < tiny ? tiny : sum_phi;
+        re[node] = logf(dens_eps + sum_sq / denom);
+        phi_sum[node] = sum_phi;
+    }
+}
+
 __device__ inline double block_sum_f64(double v, double* sh) {
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -411,15 +459,23 @@ void launch_schedule(mrs_umap_state* st, int32_t epoch) {
     MRS_CHECK_LAUNCH();
 }
 
-void launch_dens_update_dev(mrs_umap_state* st) {
+void launch_dens_update_dev(mrs_umap_state* st, bool reference_order) {
     if (!st->dens_enabled) throw Error(MRS_INVALID_ARGUMENT, "state has no densMAP buffers");
     const uint32_t n = (uint32_t)st->n;
     hipStream_t s = st->ctx->stream;
     dispatch_dim(st->dim, [&](auto dd) {
         constexpr int D = decltype(dd)::value;
-        hipLaunchKernelGGL(k_dens_radii<D>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
-                           st->off.p, st->csr_other.p, st->dens_re.p, st->dens_phi.p, n, st->dim,
-                           st->params.a, st->params.b, (float)kDensEps, FLT_EPSILON);
+        if (D > 0 && !reference_order) {
+            constexpr int DL = D > 0 ? D : 1;
+            hipLaunchKernelGGL(k_dens_radii_lanes<DL>, dim3(div_up((uint64_t)n * kRadiiLanes, kWg)),
+                               dim3(kWg), 0, s, st->embd.p, st->off.p, st->csr_other.p,
+                               st->dens_re.p, st->dens_phi.p, n, st->params.a, st->params.b,
+                               (float)kDensEps, FLT_EPSILON);
+        } else {
+            hipLaunchKernelGGL(k_dens_radii<D>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
+                               st->off.p, st->csr_other.p, st->dens_re.p, st->dens_phi.p, n,
+                               st->dim, st->params.a, st->params.b, (float)kDensEps, FLT_EPSILON);
+        }
     });
     MRS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_dens_sum, dim3(kDensBlocks), dim3(kWg), 0, s, st->dens_re.p, n,
@@ -539,7 +595,7 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
             bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
             if (dens) {
                 ProfScope ps(st, 5);
-                launch_dens_update_dev(st);
+                launch_dens_update_dev(st, strategy == MRS_EPOCH_NODE_EXACT);
             }
             if (strategy == MRS_EPOCH_NODE_EXACT) {
                 { ProfScope ps(st, 2); launch_grad_accum_node(st, e, seed, dens); }

@@ -145,7 +145,7 @@ int mrs_launch_grad_accum(mrs_umap_state* st, int32_t epoch, uint32_t seed,
 int mrs_launch_dens_update(mrs_umap_state* st, float* coeff_out) {
     return mrs::guarded([&] {
         if (!st) throw Error(MRS_INVALID_ARGUMENT, "state is NULL");
-        mrs::launch_dens_update_dev(st);
+        mrs::launch_dens_update_dev(st, true);
         float sc[4];
         MRS_HIP(hipMemcpyAsync(sc, st->dens_scalars.p, sizeof(sc), hipMemcpyDeviceToHost,
                                st->ctx->stream));

@@ -88,7 +88,9 @@ bool dens_is_active(const mrs_dens_params& dp, int32_t epoch, int32_t n_epochs);
 void launch_grad_accum_node(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
 void launch_adam_elem(mrs_umap_state* st, int32_t epoch);
 void launch_schedule(mrs_umap_state* st, int32_t epoch);
-void launch_dens_update_dev(mrs_umap_state* st);  // radii + FP64 stats + weights, coeff on device
+// radii + FP64 stats + weights, coeff on device; reference_order: the radii
+// summed per node in the reference's sequential order (else 16 lanes per node)
+void launch_dens_update_dev(mrs_umap_state* st, bool reference_order);
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
 void launch_adam_clear(mrs_umap_state* st, int32_t epoch, float* norm_partial);
 void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed, int strategy);

@@ -392,6 +392,29 @@ def test_driver_densmap_matches_oracle(ctx):
     assert np.max(np.abs(got - want)) < 1e-3
 
 
+@pytest.mark.parametrize("strategy", [M.EPOCH_EDGE_ATOMIC, M.EPOCH_NODE_EXACT])
+def test_dens_radii_match_oracle(ctx, strategy):
+    # umap_dens_radii (umap_optimiser_gpu.rs:487-544) at the start of a density
+    # epoch, from the same embedding as the oracle.  The edge-parallel strategy
+    # sums each node's row in 16 lanes (another order), the node-exact one in
+    # the reference's; device and host powf differ in the last place either
+    # way: phi sums within 1e-5 relative, log radii within 1e-5.
+    embd, adj = _random_graph(n=1200, seed=47)
+    rng = np.random.default_rng(47)
+    r = rng.normal(size=embd.shape[0]).astype(np.float32)
+    ms = np.ones_like(r)
+    ds = M.DensState(params=M.DensParams(lambda_=2.0, frac=1.0, var_shift=0.1),
+                     r=r.ctypes.data, mu_sum=ms.ctypes.data, mu_tot=float(adj[2].sum()))
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 10
+    st, _ = make_state(ctx, embd, adj, p, ds)
+    st.run_epochs(0, 1, 42, strategy)  # frac 1: epoch 0 is a density epoch
+    want_re, want_phi = orc.dens_radii(embd, oracle_csr(adj), p.a, p.b)
+    got_re, got_phi = st.read(M.BUF_DENS_RE), st.read(M.BUF_DENS_PHI)
+    np.testing.assert_allclose(got_phi, want_phi, rtol=1e-5, atol=0)
+    np.testing.assert_allclose(got_re, want_re, rtol=0, atol=1e-5)
+
+
 @pytest.mark.parametrize("dens_on", [False, True])
 def test_graph_replay_matches_eager(ctx, dens_on):
     # run_epochs(count >= 16) replays a captured HIP graph; it must equal the
