@@ -617,8 +617,16 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     const uint32_t npad = ((n + 255) / 256) * 256;
     hipStream_t s = ctx->stream;
 
-    hipEvent_t ev[4];
-    for (auto& e : ev) MRS_HIP(hipEventCreate(&e));
+    // stage timing events, destroyed on every exit (including a throw)
+    struct Events {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t* ev = evs.e;
+    for (int i = 0; i < 4; i++) MRS_HIP(hipEventCreate(&ev[i]));
     MRS_HIP(hipEventRecord(ev[0], s));
 
     DevBuf<unsigned char> P((size_t)npad * KP * (bf16 ? 2 : 4));
@@ -738,7 +746,6 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         timings->splits = splits;
         timings->shortlist = S;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 }  // namespace mrs

@@ -725,11 +725,13 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
     // [start[c], start[c+1]) and every vector access is a contiguous stream.
     std::vector<uint32_t> root_id(n, 0xFFFFFFFFu);
     std::vector<uint32_t> roots;
-    for (uint32_t i = 0; i < n; i++)
+    for (uint32_t i = 0; i < n; i++) {
+        if (hl[i] >= n) throw Error(MRS_DEVICE_ERROR, "connected-component label out of range");
         if (hl[i] == i) {
             root_id[i] = (uint32_t)roots.size();
             roots.push_back(i);
         }
+    }
     const uint32_t ncomp = (uint32_t)roots.size();
     if (ncomp == 0 || ncomp > n) throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
     std::vector<uint32_t> comp(n), size(ncomp, 0);
