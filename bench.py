@@ -16,6 +16,9 @@ Prints ONE JSON line.  Workloads:
            epochs; trustworthiness@15 against the CPU optimisers.
   densmap  densMAP: n=500,000, d=50, k=30; local-radius Spearman against the
            CPU densMAP.
+  plain    the reference's CPU-runnable case, n=10,000, d=50, k=15, 500
+           epochs: the GPU fit beside the whole CPU run (kNN, graph, SGD),
+           whose trustworthiness@15 is the reference.
 
 The CPU baseline is the C restatement of the reference's parallel Adam
 (optimise_embedding_adam_parallel) timed on this host's cores on a bounded
@@ -467,6 +470,94 @@ def bench_fp32(args):
     )
 
 
+def bench_plain(args):
+    """The reference's CPU-runnable case beside the GPU fit: n=10,000, d=50 (10
+    isotropic Gaussians, centres ~N(0, 25 I)), exhaustive kNN k=15, spectral
+    layout, 500 epochs, neg rate 5, a and b fitted to min_dist 0.1 / spread 1.0.
+    The CPU run (the oracle's exhaustive kNN, host graph and plain SGD,
+    optimise_embedding_sgd, src/training/umap_optimisers.rs:653-865, from the
+    same spectral layout) is the trustworthiness reference for the GPU fit's
+    Adam.  At this size the GPU fit is bound by launch latency: the epoch
+    kernel's roofline is reported with its per-launch time."""
+    import torch
+
+    import mrs_amd as M
+    import synthetic
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import pyoracle as orc
+    from helpers import coo_to_adjacency
+
+    n, d, k, n_epochs = 10_000, 50, 15, 500
+    x, _ = synthetic.plain_cpu(n=n, d=d)
+    a, b = orc.fit_params(0.1, 1.0)
+    ctx = M.Context(0)
+    xd = torch.from_numpy(x).cuda()
+    p = M.UmapParamsGpu.default()
+    p.k = k
+    p.knn_precision = M.KNN_FP32
+    p.optim_params.n_epochs = n_epochs
+    p.optim_params.a, p.optim_params.b = a, b
+    ms, st, out = fit_steps(M, ctx, xd, p, args)
+    e_gpu = out.T.cpu().numpy()
+    # epoch kernel: algorithmic bytes per launch from the exact schedule replay
+    idx, dist = M.knn_exhaustive(ctx, x, k, M.KNN_FP32)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
+    init = g.spectral_layout(2, seed=42, data=x)
+    stt = M.UmapGpuState(ctx, torch.from_numpy(init).cuda(), params=p.optim_params, graph=g)
+    A_total = int(M.schedule_active_counts(stt.read(M.BUF_PER), n_epochs).sum())
+    E = stt.n_edges
+    stt.set_profiling(True)
+    stt.run_epochs(0, n_epochs, 42, M.EPOCH_EDGE_ATOMIC)
+    stt.set_profiling(False)
+    edge_ms, edge_launches = stt.kernel_times()["epoch_edges"]
+    stt.close()
+    r = p.optim_params.neg_sample_rate
+    bytes_edge = (16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)) / n_epochs
+    epoch_gbs = bytes_edge / (edge_ms / edge_launches * 1e-3) / 1e9
+    # the reference's CPU run, whole: kNN, graph, SGD (same layout to start from)
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    cidx, cdist = orc.knn_exact(x, k)
+    t_knn = time.perf_counter() - t0
+    knn_rows_equal = float(np.mean(np.all(cidx == idx, axis=1)))
+    t0 = time.perf_counter()
+    rr, cc, vv = oracle_graph(orc, (cidx, cdist), n, k, n_epochs)
+    t_graph = time.perf_counter() - t0
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = n_epochs
+    q.a, q.b = a, b
+    t0 = time.perf_counter()
+    e_sgd = orc.optimise_sgd(init, *coo_to_adjacency(n, rr, cc, vv), q, 42)
+    t_sgd = time.perf_counter() - t0
+    tw = dict(gpu=sample_trustworthiness(x, e_gpu), cpu_sgd=sample_trustworthiness(x, e_sgd))
+    tw["gpu_minus_cpu_sgd"] = tw["gpu"] - tw["cpu_sgd"]
+    log(f"[bench] trustworthiness@15: {tw}")
+    return dict(
+        metric="plain UMAP GPU fit sec @10k×50 k=15 (trustworthiness@15 vs the CPU SGD run)",
+        value=ms / 1e3, unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms, higher_is_better=False, scaling="weak", vs_baseline=None,
+        dtype="f32-mfma/f32", data="synthetic",
+        config=dict(workload="umap_gpu fit, n=10,000 x d=50 f32 (10 isotropic Gaussians, "
+                    "centres ~N(0, 25 I)), exhaustive FP32 kNN k=15, spectral init, 500 Adam "
+                    "epochs, a/b from min_dist 0.1 spread 1.0, 2-D output",
+                    n=n, d=d, k=k, n_epochs=n_epochs, n_edges=E, edge_updates=A_total),
+        stages_ms=st,
+        roofline=dict(bound="hbm", kernel="k_epoch_edges", achieved=epoch_gbs,
+                      peak=HBM_PEAK_GBS, unit="GB/s", frac=epoch_gbs / HBM_PEAK_GBS,
+                      traffic=None, avg_launch_us=1e3 * edge_ms / edge_launches,
+                      algorithmic_bytes_per_launch=bytes_edge),
+        trustworthiness_at_15=tw,
+        knn_rows_equal_to_oracle=knn_rows_equal,
+        cpu_baseline=dict(value=t_knn + t_graph + t_sgd, unit="s", cores=cores, kind="port",
+                          sample=f"whole run: exhaustive kNN of all {n} rows, graph, SGD "
+                                 f"{n_epochs} epochs (single thread); spectral init not "
+                                 f"included",
+                          knn_s=t_knn, graph_s=t_graph, sgd_s=t_sgd),
+    )
+
+
 def log_radii(emb, rows, cols, a, b):
     """The embedding's local log radii of densMAP (umap_dens_radii,
     src/training/umap_optimiser_gpu.rs:487-544, in float64): ln(1e-8 +
@@ -717,6 +808,8 @@ def main():
         res = bench_fp32(args)
     elif args.workload == "densmap":
         res = bench_densmap(args)
+    elif args.workload == "plain":
+        res = bench_plain(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     print(json.dumps(res))

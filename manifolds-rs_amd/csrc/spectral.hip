@@ -100,6 +100,136 @@ __global__ __launch_bounds__(kWg) void k_cc_jump(uint32_t* __restrict__ lab, uin
     lab[i] = r;
 }
 
+// ---- component order on the device
+// Components are numbered in ascending order of their smallest vertex, which
+// is the root the labelling converges to; position q of the component order
+// runs over the components one after another, each in ascending vertex order.
+
+// flag[i] = 1 where i is a root (lab[i] == i).  *bad is raised where a label is
+// out of range or does not point at a root (the labelling has not converged).
+__global__ __launch_bounds__(kWg) void k_root_flags(const uint32_t* __restrict__ lab, uint32_t n,
+                                                    uint32_t* __restrict__ flag,
+                                                    uint32_t* __restrict__ bad) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = lab[i];
+    if (r >= n || lab[r] != r) {
+        *bad = 1u;
+        flag[i] = 0u;
+        return;
+    }
+    flag[i] = r == i ? 1u : 0u;
+}
+
+// comp[i] = the rank of i's root among the roots (rid = exclusive scan of the
+// root flags).
+__global__ __launch_bounds__(kWg) void k_comp_ids(const uint32_t* __restrict__ lab,
+                                                  const uint32_t* __restrict__ rid, uint32_t n,
+                                                  uint32_t* __restrict__ comp) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i < n) comp[i] = rid[lab[i]];
+}
+
+// Stable LSD radix sort of the component ids (8-bit digits), carrying the
+// vertex ids: the sorted vertex ids are the component order.  A workgroup owns
+// a tile of kSortTile consecutive keys.
+constexpr int kSortRounds = 8;
+constexpr uint32_t kSortTile = (uint32_t)kWg * kSortRounds;
+static_assert(kWg == 256, "one thread per digit value");
+
+// counts[d * tiles + t] = number of keys of tile t whose digit is d
+__global__ __launch_bounds__(kWg) void k_digit_counts(const uint32_t* __restrict__ key, uint32_t n,
+                                                      int shift, uint32_t tiles,
+                                                      uint32_t* __restrict__ counts) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; r++) {
+        const uint32_t i = base + (uint32_t)r * kWg + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One stable scatter pass.  A key's place is its digit's offset for the tile
+// (offs = exclusive scan of counts, digit-major) plus the number of earlier
+// keys of the tile with the same digit.  Keys are taken in index order (round,
+// then wave, then lane); the lanes of a wave that share a digit are found with
+// one ballot per digit bit.  val_in == nullptr stands for the identity (i).
+__global__ __launch_bounds__(kWg) void k_digit_scatter(const uint32_t* __restrict__ key_in,
+                                                       const uint32_t* __restrict__ val_in,
+                                                       uint32_t n, int shift, uint32_t tiles,
+                                                       const uint32_t* __restrict__ offs,
+                                                       uint32_t* __restrict__ key_out,
+                                                       uint32_t* __restrict__ val_out) {
+    constexpr int kWaves = kWg / 64;
+    __shared__ uint32_t run[256];           // keys of each digit placed so far in this tile
+    __shared__ uint32_t wpre[kWaves][256];  // per round: wave counts, then wave offsets
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    run[tid] = 0u;
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; r++) {
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) wpre[w][tid] = 0u;
+        __syncthreads();
+        const uint32_t i = base + (uint32_t)r * kWg + tid;
+        const bool valid = i < n;
+        const uint32_t k = valid ? key_in[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0) wpre[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {  // thread tid owns digit tid: wave offsets on top of the tile's running count
+            uint32_t acc = run[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) {
+                const uint32_t c = wpre[w][tid];
+                wpre[w][tid] = acc;
+                acc += c;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = offs[(size_t)d * tiles + blockIdx.x] + wpre[wave][d] + rank;
+            key_out[pos] = k;
+            val_out[pos] = val_in ? val_in[i] : i;
+        }
+        __syncthreads();
+    }
+}
+
+// start[c] = the first position of component c in the sorted ids; start[ncomp] = n
+__global__ __launch_bounds__(kWg) void k_comp_starts(const uint32_t* __restrict__ skey, uint32_t n,
+                                                     uint32_t* __restrict__ start) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    if (q == 0 || skey[q] != skey[q - 1]) start[skey[q]] = q;
+    if (q == n - 1) start[skey[q] + 1] = n;
+}
+
+__global__ __launch_bounds__(kWg) void k_iota(uint32_t* __restrict__ a, uint32_t n) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q < n) a[q] = q;
+}
+
+// inv[perm[q]] = q
+__global__ __launch_bounds__(kWg) void k_inverse(const uint32_t* __restrict__ perm, uint32_t n,
+                                                 uint32_t* __restrict__ inv) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q < n) inv[perm[q]] = q;
+}
+
 // Degree of each vertex in component order: deg[q] = row length of perm[q].
 __global__ __launch_bounds__(kWg) void k_perm_degree(const uint32_t* __restrict__ off,
                                                      const uint32_t* __restrict__ perm, uint32_t n,
@@ -190,21 +320,30 @@ __global__ __launch_bounds__(kWg) void k_multidot(const Chunk* __restrict__ chun
         partial[(size_t)blockIdx.x * m + k0 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// h[comp][k] = sum of the component's chunk partials, in chunk order.  The
-// projection on the newest basis vector (k = m-1) is the Lanczos diagonal
-// coefficient: written to alpha (pass 0) or added to it (pass 1).
+// h[comp][k] = sum of the component's chunk partials, one wavefront per
+// (comp, k): lane l adds chunks l, l+64, ... in order, then a fixed xor tree
+// (the result does not depend on scheduling).  The projection on the newest
+// basis vector (k = m-1) is the Lanczos diagonal coefficient: written to alpha
+// (pass 0) or added to it (pass 1).
+constexpr int kWavesPerWg = kWg / 64;
 __global__ __launch_bounds__(kWg) void k_multidot_finish(const double* __restrict__ partial,
                                                          const uint32_t* __restrict__ comp_chunk0,
                                                          uint32_t n_comp, int m,
                                                          double* __restrict__ h,
                                                          double* __restrict__ alpha, int pass) {
-    const uint32_t t = blockIdx.x * kWg + threadIdx.x;
-    if (t >= n_comp * (uint32_t)m) return;
+    const uint32_t t = blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (t >= n_comp * (uint32_t)m) return;  // uniform per wavefront
     const uint32_t c = t / m, k = t % m;
     double s = 0.0;
-    for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) s += partial[(size_t)q * m + k];
-    h[t] = s;
-    if ((int)k == m - 1) alpha[c] = pass ? alpha[c] + s : s;
+    for (uint32_t q = comp_chunk0[c] + lane; q < comp_chunk0[c + 1]; q += 64)
+        s += partial[(size_t)q * m + k];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        h[t] = s;
+        if ((int)k == m - 1) alpha[c] = pass ? alpha[c] + s : s;
+    }
 }
 
 // w -= sum_k V_k h[comp][k] over one chunk of a live component (one block per
@@ -265,16 +404,20 @@ __global__ __launch_bounds__(kWg) void k_seg_dot(const Chunk* __restrict__ chunk
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// out[comp] = (sqrt?) sum of the comp's chunk partials, in chunk order
+// out[comp] = (sqrt?) sum of the comp's chunk partials, one wavefront per
+// component, in the fixed order of k_multidot_finish
 __global__ __launch_bounds__(kWg) void k_seg_finish(const double* __restrict__ partial,
                                                     const uint32_t* __restrict__ comp_chunk0,
                                                     uint32_t n_comp, int do_sqrt,
                                                     double* __restrict__ out) {
-    uint32_t c = blockIdx.x * kWg + threadIdx.x;
-    if (c >= n_comp) return;
+    const uint32_t c = blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (c >= n_comp) return;  // uniform per wavefront
     double t = 0.0;
-    for (uint32_t q = comp_chunk0[c]; q < comp_chunk0[c + 1]; q++) t += partial[q];
-    out[c] = do_sqrt ? sqrt(t) : t;
+    for (uint32_t q = comp_chunk0[c] + lane; q < comp_chunk0[c + 1]; q += 64) t += partial[q];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
+    if (lane == 0) out[c] = do_sqrt ? sqrt(t) : t;
 }
 
 // v = w / beta_j and the next basis column V_{j+1} = v (live components)
@@ -715,44 +858,65 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipStreamSynchronize(s));
         if (!ch) break;
     }
-    std::vector<uint32_t> hl(n);
-    MRS_HIP(hipMemcpyAsync(hl.data(), lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    // Component order, built on the device (see k_digit_scatter): perm[q] is
+    // the vertex at position q and inv its inverse, so a component is the
+    // contiguous range [start[c], start[c+1]) and every vector access is a
+    // contiguous stream.
+    DevBuf<uint32_t> rid((size_t)n + 1), bad(1);
+    MRS_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_root_flags, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, n, rid.p, bad.p);
+    MRS_CHECK_LAUNCH();
+    exclusive_scan_u32(rid.p, rid.p, n, rid.p + n, s);
+    uint32_t hdr[2] = {0, 0};
+    MRS_HIP(hipMemcpyAsync(&hdr[0], rid.p + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MRS_HIP(hipMemcpyAsync(&hdr[1], bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     MRS_HIP(hipStreamSynchronize(s));
-    // Components in ascending order of their smallest vertex.  Everything below
-    // works in component order: position q runs over the components one after
-    // another, each in ascending vertex order; perm[q] is the vertex at q and
-    // inv its inverse, so a component is the contiguous range
-    // [start[c], start[c+1]) and every vector access is a contiguous stream.
-    std::vector<uint32_t> root_id(n, 0xFFFFFFFFu);
-    std::vector<uint32_t> roots;
-    for (uint32_t i = 0; i < n; i++) {
-        if (hl[i] >= n) throw Error(MRS_DEVICE_ERROR, "connected-component label out of range");
-        if (hl[i] == i) {
-            root_id[i] = (uint32_t)roots.size();
-            roots.push_back(i);
-        }
-    }
-    const uint32_t ncomp = (uint32_t)roots.size();
-    if (ncomp == 0 || ncomp > n) throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
-    std::vector<uint32_t> comp(n), size(ncomp, 0);
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t c = root_id[hl[i]];
-        if (c == 0xFFFFFFFFu) throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
-        comp[i] = c;
-        size[c]++;
-    }
+    const uint32_t ncomp = hdr[0];
+    if (hdr[1] || ncomp == 0 || ncomp > n)
+        throw Error(MRS_DEVICE_ERROR, "connected components did not converge");
+    DevBuf<uint32_t> dperm(n), dinvp(n), dstart((size_t)ncomp + 1);
     std::vector<uint32_t> start(ncomp + 1, 0);
-    for (uint32_t c = 0; c < ncomp; c++) start[c + 1] = start[c] + size[c];
-    std::vector<uint32_t> perm(n), inv(n), cur(start.begin(), start.end() - 1);
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t q = cur[comp[i]]++;
-        perm[q] = i;
-        inv[i] = q;
+    if (ncomp == 1) {
+        hipLaunchKernelGGL(k_iota, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dperm.p, n);
+        hipLaunchKernelGGL(k_iota, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dinvp.p, n);
+        start[1] = n;
+        MRS_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, s));
+        MRS_CHECK_LAUNCH();
+    } else {
+        DevBuf<uint32_t> ka(n), kb(n), vb(n);
+        hipLaunchKernelGGL(k_comp_ids, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, rid.p, n, ka.p);
+        MRS_CHECK_LAUNCH();
+        const int bits = 32 - __builtin_clz(ncomp - 1);
+        const int passes = (bits + 7) / 8;
+        const uint32_t tiles = div_up(n, kSortTile);
+        DevBuf<uint32_t> counts((size_t)256 * tiles);
+        // keys alternate between ka and kb; the carried vertex ids alternate
+        // between vb and dperm so that the last pass writes dperm
+        uint32_t *kin = ka.p, *kout = kb.p, *vin = nullptr;
+        for (int pass = 0; pass < passes; pass++) {
+            uint32_t* vout = ((passes - 1 - pass) & 1) == 0 ? dperm.p : vb.p;
+            hipLaunchKernelGGL(k_digit_counts, dim3(tiles), dim3(kWg), 0, s, kin, n, 8 * pass,
+                               tiles, counts.p);
+            MRS_CHECK_LAUNCH();
+            exclusive_scan_u32(counts.p, counts.p, (uint64_t)256 * tiles, nullptr, s);
+            hipLaunchKernelGGL(k_digit_scatter, dim3(tiles), dim3(kWg), 0, s, kin, vin, n,
+                               8 * pass, tiles, counts.p, kout, vout);
+            MRS_CHECK_LAUNCH();
+            vin = vout;
+            std::swap(kin, kout);
+        }
+        hipLaunchKernelGGL(k_comp_starts, dim3(div_up(n, kWg)), dim3(kWg), 0, s, kin, n, dstart.p);
+        hipLaunchKernelGGL(k_inverse, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dperm.p, n, dinvp.p);
+        MRS_CHECK_LAUNCH();
+        MRS_HIP(hipMemcpyAsync(start.data(), dstart.p, sizeof(uint32_t) * (ncomp + 1),
+                               hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipStreamSynchronize(s));
     }
-    DevBuf<uint32_t> dperm(n), dstart((size_t)ncomp + 1);
-    MRS_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-    MRS_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(uint32_t) * (ncomp + 1),
-                           hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> size(ncomp);
+    for (uint32_t c = 0; c < ncomp; c++) {
+        if (start[c + 1] <= start[c]) throw Error(MRS_DEVICE_ERROR, "empty connected component");
+        size[c] = start[c + 1] - start[c];
+    }
 
     // which components take the spectral branch
     const bool single = ncomp == 1;
@@ -790,11 +954,10 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             n_iter_c[q] = (int)std::min<uint32_t>(n_eig * 4 + 30, size[lc_comp[q]]);
             max_iter = std::max(max_iter, n_iter_c[q]);
         }
-        DevBuf<uint32_t> dchunk0(nl + 1), dc2l(ncomp), dlc(n), dinvp(n), poff((size_t)n + 1);
+        DevBuf<uint32_t> dchunk0(nl + 1), dc2l(ncomp), dlc(n), poff((size_t)n + 1);
         MRS_HIP(hipMemcpyAsync(dchunks.p, chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dchunk0.p, chunk0.data(), sizeof(uint32_t) * (nl + 1), hipMemcpyHostToDevice, s));
         MRS_HIP(hipMemcpyAsync(dc2l.p, comp_to_lc.data(), sizeof(uint32_t) * ncomp, hipMemcpyHostToDevice, s));
-        MRS_HIP(hipMemcpyAsync(dinvp.p, inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_fill_lcq, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dstart.p, ncomp,
                            dc2l.p, n, dlc.p);
         // M = -D^-1/2 W D^-1/2 in component order (rows keep their entry order,
@@ -819,10 +982,10 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemsetAsync(live.p, 1, nl, s));
         MRS_HIP(hipMemsetAsync(dn_done.p, 0, dn_done.bytes(), s));
         MRS_HIP(hipMemcpyAsync(dn_iter.p, n_iter_c.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
-        const dim3 gn(div_up(n, kWg)), gq(div_up(nl, kWg));
+        const dim3 gn(div_up(n, kWg)), gq(div_up(nl, kWg)), gqw(div_up(nl, kWavesPerWg));
         auto seg = [&](const double* a, const double* b, double* outp, int do_sqrt) {
             hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, a, b, partial.p);
-            hipLaunchKernelGGL(k_seg_finish, gq, dim3(kWg), 0, s, partial.p, dchunk0.p, nl,
+            hipLaunchKernelGGL(k_seg_finish, gqw, dim3(kWg), 0, s, partial.p, dchunk0.p, nl,
                                do_sqrt, outp);
         };
         hipLaunchKernelGGL(k_start_vec, gn, dim3(kWg), 0, s, v.p, dperm.p, n, seed);
@@ -846,13 +1009,13 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             for (int pass = 0; pass < 2; pass++) {
                 hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
                                    dchunks.p, V.p, n, w.p, m, mpart.p);
-                hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWg)),
+                hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWavesPerWg)),
                                    dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p, alpha_j,
                                    pass);
                 hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, w.p, V.p,
                                    n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
             }
-            hipLaunchKernelGGL(k_seg_finish, gq, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
+            hipLaunchKernelGGL(k_seg_finish, gqw, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
                                beta_j);
             hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p,
                                j + 1 < max_iter ? V.p + (size_t)(j + 1) * n : nullptr, w.p, beta_j,

@@ -146,6 +146,36 @@ def test_fit_trustworthiness_matches_cpu_optimiser(ctx, precision):
     assert t_gpu >= t_cpu - 0.005, (t_gpu, t_cpu)
 
 
+def test_plain_cpu_case_matches_cpu_sgd_run(ctx):
+    # The reference's CPU-runnable case: 10 000 x 50 (10 isotropic Gaussians),
+    # exhaustive kNN k=15, 500 epochs, a/b fitted to min_dist 0.1, spread 1.0.
+    # Its CPU run (kNN, graph, plain SGD, src/training/umap_optimisers.rs:653-865)
+    # is the trustworthiness reference for the GPU fit (Adam), both from the
+    # same spectral layout; trustworthiness@15 on a seeded 3 000-point sample.
+    import synthetic
+
+    n, k, n_epochs = 10_000, 15, 500
+    x, _ = synthetic.plain_cpu(n=n, d=50)
+    a, b = orc.fit_params(0.1, 1.0)
+    p = params(knn_precision=M.KNN_FP32, k=k)
+    p.optim_params.n_epochs = n_epochs
+    p.optim_params.a, p.optim_params.b = a, b
+    e_gpu = M.umap_gpu(ctx, x, p, seed=42).T
+    idx, dist, (r, c, v), adj = umap_graph_from_data(x, k, n_epochs=n_epochs)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=n_epochs)
+    init = g.spectral_layout(2, seed=42, data=x)
+    q = M.UmapOptimParams.default_2d()
+    q.n_epochs = n_epochs
+    q.a, q.b = a, b
+    e_sgd = orc.optimise_sgd(init, *adj, q, 42)
+    rows = np.sort(np.random.default_rng(11).choice(n, size=3000, replace=False))
+    t_gpu = trustworthiness(x[rows], e_gpu[rows], k)
+    t_sgd = trustworthiness(x[rows], e_sgd[rows], k)
+    print(f"trustworthiness@15: GPU {t_gpu:.4f}, CPU SGD {t_sgd:.4f}")
+    assert np.all(np.isfinite(e_gpu))
+    assert t_gpu >= t_sgd - 0.005, (t_gpu, t_sgd)
+
+
 def test_fit_gpu_replay_matches_oracle_driver(ctx):
     # The fit is the composition kNN -> graph -> spectral layout -> driver:
     # with the deterministic strategy it equals, bit for bit, the driver
