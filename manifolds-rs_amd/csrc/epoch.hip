@@ -360,6 +360,18 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 
 // ------------------------------------------------------ edge-parallel epoch
 
+// One edge-parallel epoch, with the firing edges compacted per wavefront.
+// About one edge in five fires in an epoch, scattered over the edge list, so
+// with one thread per edge nearly every wavefront would have a firing lane and
+// run the whole negative-sampling body for all 64 lanes.  Here each wavefront owns
+// kEdgeSpan consecutive edges: it first advances their schedule cursors and
+// queues the firing ones in its own LDS slice, in edge order (so runs of equal
+// i stay together for the segmented sum), then works through the queue 64
+// edges at a time.  The queue is built from ballots: no counter is shared
+// between wavefronts.
+constexpr int kEdgeSpan = 256;  // 256 and 512 measured alike on 1M nodes; 1024, 2048 slower
+static_assert(kEdgeSpan % 64 == 0, "whole wavefront rounds");
+
 template <int DIM, bool DENS>
 __global__ __launch_bounds__(kWg) void k_epoch_edges(
     const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
@@ -369,9 +381,35 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
     const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
-    edge_epoch<DIM, DENS>(blockIdx.x * kWg + threadIdx.x, edge_ij, edge_loc, per, next, embd, grad,
-                          flag, n_edges, epoch, epoch_f, seed, fm, a, b, two_a_b, two_gamma_b,
-                          neg_rate, mu, dens_re, dens_phi, dens_w, dens_scalars);
+    constexpr int kWaves = kWg / 64;
+    __shared__ uint32_t queue[kWaves][kEdgeSpan];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t base = ((uint64_t)blockIdx.x * kWaves + wave) * kEdgeSpan;
+    uint32_t count = 0;  // wavefront-uniform
+    for (int r = 0; r < kEdgeSpan / 64; r++) {
+        const uint64_t e = base + (uint64_t)r * 64 + lane;
+        bool fire = false;
+        if (e < n_edges) {
+            const float nx = next[e];
+            if (nx <= epoch_f) {
+                fire = true;
+                next[e] = nx + per[e];
+            }
+        }
+        const uint64_t m = __ballot(fire);
+        if (fire) queue[wave][count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
+        count += (uint32_t)__popcll(m);
+    }
+    __syncthreads();  // the queue is read by other lanes than the ones that wrote it
+    for (uint32_t q0 = 0; q0 < count; q0 += 64) {
+        const uint32_t slot = q0 + lane;
+        const bool active = slot < count;
+        const uint32_t e = active ? queue[wave][slot] : 0u;
+        edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a,
+                             b, two_a_b, two_gamma_b, neg_rate, mu, dens_re, dens_phi, dens_w,
+                             dens_scalars);
+    }
 }
 
 template <int DIM>
@@ -504,8 +542,10 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     FastMod fm = make_fastmod((uint32_t)st->n);
     const uint32_t ne = (uint32_t)st->n_edges;
     hipStream_t s = st->ctx->stream;
+    // one wavefront per kEdgeSpan edges
+    const unsigned blocks = div_up(div_up(ne, kEdgeSpan), kWg / 64);
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(div_up(ne, kWg)), dim3(kWg), 0, s, st->edge_ij.p,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edge_ij.p,
                            st->edge_loc.p, st->per.p, st->next.p, st->embd.p, st->grad.p,
                            st->has_update.p, ne, (uint32_t)epoch, (float)epoch, seed, fm, p.a,
                            p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,

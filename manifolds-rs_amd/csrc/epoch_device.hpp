@@ -41,31 +41,21 @@ __device__ inline bool seg_reduce(uint32_t key, float (&v)[D]) {
     return head;
 }
 
-// One undirected edge (i < j) of one Adam epoch, for thread-per-edge kernels:
-// e is this thread's edge (e >= n_edges: an idle lane that still joins the
-// wavefront's reduction).  DIM is the compile-time embedding dimension.
-// Reads next/per coalesced; an active edge gathers y_i, y_j and 2*neg_rate
-// negatives, advances its schedule cursor in place, scatters the larger
-// endpoint's gradient with FP32 atomics and the smaller endpoint's after a
-// segmented sum over the wavefront (edges are sorted by i).
+// The work of one firing undirected edge (i < j) of one Adam epoch: gathers
+// y_i, y_j and 2*neg_rate negatives, scatters the larger endpoint's gradient
+// with FP32 atomics and the smaller endpoint's after a segmented sum over the
+// wavefront (edges arrive sorted by i).  Every lane of the wavefront calls it
+// (the reduction is wavefront-wide); lanes with active == false only join the
+// reduction.  DIM is the compile-time embedding dimension.
 template <int DIM, bool DENS>
-__device__ __forceinline__ void edge_epoch(uint32_t e,
+__device__ __forceinline__ void edge_work(bool active, uint32_t e,
     const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
-    const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
-    float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
-    float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
+    const float* __restrict__ embd, float* __restrict__ grad, uint32_t* __restrict__ flag,
+    uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
     const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
     const float two = 2.0f, one = 1.0f;
-    bool active = false;
-    if (e < n_edges) {
-        float nx = next[e];
-        if (nx <= epoch_f) {
-            active = true;
-            next[e] = nx + per[e];
-        }
-    }
     uint32_t key = 0xFFFFFFFFu;
     float gi[DIM], gj[DIM];
 #pragma unroll
@@ -162,6 +152,32 @@ __device__ __forceinline__ void edge_epoch(uint32_t e,
         for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)key * DIM + d], gi[d]);
         flag[key] = 1u;
     }
+}
+
+// One undirected edge of one Adam epoch, for thread-per-edge kernels: e is
+// this thread's edge (e >= n_edges: an idle lane that still joins the
+// wavefront's reduction).  Reads next/per coalesced and advances a firing
+// edge's schedule cursor in place, then does the edge's work.
+template <int DIM, bool DENS>
+__device__ __forceinline__ void edge_epoch(uint32_t e,
+    const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
+    const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
+    float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
+    float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
+    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
+    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
+    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    bool active = false;
+    if (e < n_edges) {
+        float nx = next[e];
+        if (nx <= epoch_f) {
+            active = true;
+            next[e] = nx + per[e];
+        }
+    }
+    edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a, b,
+                         two_a_b, two_gamma_b, neg_rate, mu, dens_re, dens_phi, dens_w,
+                         dens_scalars);
 }
 
 // Adam step of one node if flagged (umap_adam_update, umap_optimiser_gpu.rs:887-923),
