@@ -104,6 +104,11 @@ int mrs_ctx_destroy(mrs_ctx *ctx);
 int mrs_ctx_synchronize(mrs_ctx *ctx);
 /* the context's HIP stream (hipStream_t) so callers can order their own work */
 void *mrs_ctx_stream(mrs_ctx *ctx);
+/* Exhaustive kNN on this context skips database stages that provably hold no
+ * candidate (on = 1, the default) or sweeps every stage (on = 0).  Both give
+ * the same results; the switch exists for comparisons.  No counterpart in the
+ * reference. */
+int mrs_ctx_set_knn_pruning(mrs_ctx *ctx, int32_t on);
 
 /* parse_umap_optimiser_gpu, src/training/umap_optimiser_gpu.rs:44-52.
  * 0 adam_gpu, 1 adam_parallel, 2 adam, 3 sgd; -1 if unknown. */
@@ -152,9 +157,11 @@ int mrs_last_fit_timings(double *out);
  * shortlist of max(shortlist, k) rounded up to 64 or 128 candidates). */
 int mrs_knn_exhaustive(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                        int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
-/* Same, returning stage times in ms_out[0..4]: pack, shortlist, merge+re-rank,
- * database splits used, shortlist size (no counterpart in the reference; used
- * by the benchmark). */
+/* Same, returning in ms_out[0..6]: the stage times in ms of pack (with the
+ * cell layout when pruning), shortlist (every pass), merge+re-rank; database
+ * splits used; shortlist size; the fraction of (query block, database stage)
+ * pairs swept (1 without pruning); cells (0 without pruning).  No counterpart
+ * in the reference; used by the benchmark. */
 int mrs_knn_exhaustive_timed(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                              int32_t precision, int32_t shortlist, int32_t *idx, float *dist,
                              double *ms_out);

@@ -113,6 +113,85 @@ __global__ void k_copy_total(const uint32_t* __restrict__ scanned,
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + last_in[0];
 }
 
+// Stable LSD radix sort of u32 keys (8-bit digits) carrying u32 values.  A
+// workgroup of kSortWg threads owns a tile of kSortTile consecutive keys.
+constexpr int kSortWg = 256;
+constexpr int kSortRounds = 8;
+constexpr uint32_t kSortTile = (uint32_t)kSortWg * kSortRounds;
+static_assert(kSortWg == 256, "one thread per digit value");
+
+// counts[d * tiles + t] = number of keys of tile t whose digit is d
+__global__ __launch_bounds__(kSortWg) void k_digit_counts(const uint32_t* __restrict__ key, uint32_t n,
+                                                      int shift, uint32_t tiles,
+                                                      uint32_t* __restrict__ counts) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; r++) {
+        const uint32_t i = base + (uint32_t)r * kSortWg + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One stable scatter pass.  A key's place is its digit's offset for the tile
+// (offs = exclusive scan of counts, digit-major) plus the number of earlier
+// keys of the tile with the same digit.  Keys are taken in index order (round,
+// then wave, then lane); the lanes of a wave that share a digit are found with
+// one ballot per digit bit.  val_in == nullptr stands for the identity (i).
+__global__ __launch_bounds__(kSortWg) void k_digit_scatter(const uint32_t* __restrict__ key_in,
+                                                       const uint32_t* __restrict__ val_in,
+                                                       uint32_t n, int shift, uint32_t tiles,
+                                                       const uint32_t* __restrict__ offs,
+                                                       uint32_t* __restrict__ key_out,
+                                                       uint32_t* __restrict__ val_out) {
+    constexpr int kWaves = kSortWg / 64;
+    __shared__ uint32_t run[256];           // keys of each digit placed so far in this tile
+    __shared__ uint32_t wpre[kWaves][256];  // per round: wave counts, then wave offsets
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    run[tid] = 0u;
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; r++) {
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) wpre[w][tid] = 0u;
+        __syncthreads();
+        const uint32_t i = base + (uint32_t)r * kSortWg + tid;
+        const bool valid = i < n;
+        const uint32_t k = valid ? key_in[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0) wpre[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {  // thread tid owns digit tid: wave offsets on top of the tile's running count
+            uint32_t acc = run[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) {
+                const uint32_t c = wpre[w][tid];
+                wpre[w][tid] = acc;
+                acc += c;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = offs[(size_t)d * tiles + blockIdx.x] + wpre[wave][d] + rank;
+            key_out[pos] = k;
+            val_out[pos] = val_in ? val_in[i] : i;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev,
@@ -143,6 +222,34 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t*
     // temporaries are freed on return: unless that free is stream-ordered it
     // is immediate, so the queued work must be finished first
     if (!sums.ordered) MRS_HIP(hipStreamSynchronize(s));
+}
+
+void radix_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
+                          int key_bits, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s) {
+    if (n == 0) return;
+    const int passes = std::max(1, (key_bits + 7) / 8);
+    const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+    DevBuf<uint32_t> counts((size_t)256 * tiles), kt(n), vt(n);
+    // the last pass writes (keys_out, vals_out); earlier ones alternate with
+    // the temporaries so that no pass reads the buffer it writes
+    const uint32_t* kin = keys_in;
+    const uint32_t* vin = vals_in;
+    for (int pass = 0; pass < passes; pass++) {
+        const bool last = ((passes - 1 - pass) & 1) == 0;
+        uint32_t* kout = last ? keys_out : kt.p;
+        uint32_t* vout = last ? vals_out : vt.p;
+        hipLaunchKernelGGL(k_digit_counts, dim3(tiles), dim3(kSortWg), 0, s, kin, n, 8 * pass,
+                           tiles, counts.p);
+        MRS_CHECK_LAUNCH();
+        exclusive_scan_u32(counts.p, counts.p, (uint64_t)256 * tiles, nullptr, s);
+        hipLaunchKernelGGL(k_digit_scatter, dim3(tiles), dim3(kSortWg), 0, s, kin, vin, n,
+                           8 * pass, tiles, counts.p, kout, vout);
+        MRS_CHECK_LAUNCH();
+        kin = kout;
+        vin = vout;
+    }
+    // temporaries are freed on return
+    if (!counts.ordered) MRS_HIP(hipStreamSynchronize(s));
 }
 
 void ensure_dynamic_lds(const void* kern, size_t bytes) {
@@ -208,6 +315,13 @@ int mrs_ctx_synchronize(mrs_ctx* ctx) {
 }
 
 void* mrs_ctx_stream(mrs_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mrs_ctx_set_knn_pruning(mrs_ctx* ctx, int32_t on) {
+    return mrs::guarded([&] {
+        if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
+        ctx->knn_prune = on ? 1 : 0;
+    });
+}
 
 static std::string lower(const char* s) {
     std::string r = s ? s : "";

@@ -369,10 +369,12 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 // i stay together for the segmented sum), then works through the queue 64
 // edges at a time.  The queue is built from ballots: no counter is shared
 // between wavefronts.
-constexpr int kEdgeSpan = 256;  // 256 and 512 measured alike on 1M nodes; 1024, 2048 slower
-static_assert(kEdgeSpan % 64 == 0, "whole wavefront rounds");
+// Span: 256 edges per wavefront (256 and 512 measured alike on the 1M fit
+// graph; 1024 and 2048 slower).  Below kWideSpanEdges edges that leaves too few
+// wavefronts to fill the chip, and spans of 64 are used.
+constexpr uint32_t kWideSpanEdges = 1u << 20;
 
-template <int DIM, bool DENS>
+template <int DIM, bool DENS, int kEdgeSpan>
 __global__ __launch_bounds__(kWg) void k_epoch_edges(
     const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
     const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
@@ -381,6 +383,7 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
     const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    static_assert(kEdgeSpan % 64 == 0, "whole wavefront rounds");
     constexpr int kWaves = kWg / 64;
     __shared__ uint32_t queue[kWaves][kEdgeSpan];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -542,8 +545,9 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     FastMod fm = make_fastmod((uint32_t)st->n);
     const uint32_t ne = (uint32_t)st->n_edges;
     hipStream_t s = st->ctx->stream;
-    // one wavefront per kEdgeSpan edges
-    const unsigned blocks = div_up(div_up(ne, kEdgeSpan), kWg / 64);
+    // one wavefront per span of edges
+    const bool wide = ne >= kWideSpanEdges;
+    const unsigned blocks = div_up(div_up(ne, wide ? 256 : 64), kWg / 64);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edge_ij.p,
                            st->edge_loc.p, st->per.p, st->next.p, st->embd.p, st->grad.p,
@@ -553,11 +557,26 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
                            st->dens_scalars.p);
     };
     switch (st->dim) {
-        case 1: dens ? go(k_epoch_edges<1, true>) : go(k_epoch_edges<1, false>); break;
-        case 2: dens ? go(k_epoch_edges<2, true>) : go(k_epoch_edges<2, false>); break;
-        case 3: dens ? go(k_epoch_edges<3, true>) : go(k_epoch_edges<3, false>); break;
-        case 4: dens ? go(k_epoch_edges<4, true>) : go(k_epoch_edges<4, false>); break;
-        case 8: dens ? go(k_epoch_edges<8, true>) : go(k_epoch_edges<8, false>); break;
+        case 1:
+            if (wide) dens ? go(k_epoch_edges<1, true, 256>) : go(k_epoch_edges<1, false, 256>);
+            else dens ? go(k_epoch_edges<1, true, 64>) : go(k_epoch_edges<1, false, 64>);
+            break;
+        case 2:
+            if (wide) dens ? go(k_epoch_edges<2, true, 256>) : go(k_epoch_edges<2, false, 256>);
+            else dens ? go(k_epoch_edges<2, true, 64>) : go(k_epoch_edges<2, false, 64>);
+            break;
+        case 3:
+            if (wide) dens ? go(k_epoch_edges<3, true, 256>) : go(k_epoch_edges<3, false, 256>);
+            else dens ? go(k_epoch_edges<3, true, 64>) : go(k_epoch_edges<3, false, 64>);
+            break;
+        case 4:
+            if (wide) dens ? go(k_epoch_edges<4, true, 256>) : go(k_epoch_edges<4, false, 256>);
+            else dens ? go(k_epoch_edges<4, true, 64>) : go(k_epoch_edges<4, false, 64>);
+            break;
+        case 8:
+            if (wide) dens ? go(k_epoch_edges<8, true, 256>) : go(k_epoch_edges<8, false, 256>);
+            else dens ? go(k_epoch_edges<8, true, 64>) : go(k_epoch_edges<8, false, 64>);
+            break;
         default:
             throw Error(MRS_INVALID_ARGUMENT,
                         "edge-parallel epoch supports n_dim in {1,2,3,4,8}; use node-exact");

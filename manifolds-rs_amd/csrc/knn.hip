@@ -208,12 +208,20 @@ struct ShortlistArgs {
     uint32_t n, npad;
     int KP;
     uint32_t db_per_split;  // multiple of kSplitAlign
+    // Pruned sweep (one split): rows are in cell order, rowmap[p] is the
+    // caller's row at position p (list keys carry caller rows), and workgroup
+    // b sweeps the blk[b].y stages stage_ids[blk[b].x + t].  cont: continue
+    // from the lists of an earlier pass.
+    const uint32_t* rowmap;
+    const uint2* blk;
+    const uint32_t* stage_ids;
+    int cont;
 };
 
 // Wave-wide merge of query b's queue (c entries) into its sorted global list;
-// returns the list's new threshold (the score of its last entry).
+// returns the list's new threshold key (its last entry).
 template <int E, int QC>
-__device__ inline float merge_query(int b, uint32_t c, int lane, uint64_t* __restrict__ lists,
+__device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __restrict__ lists,
                                     const uint64_t* __restrict__ myq) {
     constexpr int S = 64 * E;
     // the queue entries were stored by this wave's lanes: complete those
@@ -229,31 +237,39 @@ __device__ inline float merge_query(int b, uint32_t c, int lane, uint64_t* __res
     for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[e];
     const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v[E - 1] >> 32), 63);
     const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v[E - 1], 63);
-    return key_score(((uint64_t)hi << 32) | lo);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Epilogue of one 32-row MFMA tile: the lane's 16 scores (one query, rows
-// rowbase + (r/4)*8 + h*4 + r%4) against the query's threshold.  A query's
-// queue count qn and threshold thr live in registers, identical in its two
-// lanes (h = 0, 1); the halves learn each other's insert count with one lane
-// swap, so queueing costs no LDS round trip.  Most tiles leave on the
-// max/ballot test; a queue that could overflow on the next tile is merged.
-template <int E, int QC>
+// rowbase + (r/4)*8 + h*4 + r%4) against the query's threshold thr (the score
+// of its list's last entry).  A query's queue count qn and threshold live in
+// registers, identical in its two lanes (h = 0, 1); the halves learn each
+// other's insert count with one lane swap, so queueing costs no LDS round
+// trip.  Most tiles leave on the max/ballot test.  A queue that could overflow
+// on the next tile is merged.
+//   Full sweep (PR = false): the stages come in row order, so a score equal to
+// the threshold can only belong to a larger row and is not queued.  Pruned
+// sweep (PR = true): stages come in cell order, so an equal score is queued
+// too and the merge, which keeps the S smallest (score, row) keys, decides the
+// tie by row; keys carry the caller's row (rowmap).
+template <int E, int QC, bool PR>
 __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h, int col,
                                 int lane, float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
-                                uint64_t* __restrict__ myq) {
+                                uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap) {
     float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
 #pragma unroll
     for (int g = 0; g < 4; g++)
         gm[g] = fmaxf(fmaxf(acc[g * 4], acc[g * 4 + 1]), fmaxf(acc[g * 4 + 2], acc[g * 4 + 3]));
     const float m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
-    if (__ballot(m > thr) == 0ull) return;
+    auto reach = [&](float v) { return PR ? v >= thr : v > thr; };
+    if (__ballot(reach(m)) == 0ull) return;
     // The query's own row scores highest and may pass the test above; it is
     // never queued.  Excluding it here, on the rare path, keeps the common
-    // path free of per-element index compares.
+    // path free of per-element index compares.  Padding rows score -inf.
     const uint32_t self = q - rowbase - (uint32_t)h * 4u;  // q's offset among this lane's rows
     auto take = [&](int r) {
-        return acc[r] > thr && (uint32_t)((r >> 2) * 8 + (r & 3)) != self;
+        return reach(acc[r]) && (!PR || acc[r] > -INFINITY) &&
+               (uint32_t)((r >> 2) * 8 + (r & 3)) != self;
     };
     uint32_t c = 0;
 #pragma unroll
@@ -262,23 +278,25 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h,
     uint32_t pos = qn + (h ? co : 0u);
     qn += c + co;
     uint64_t* dst = myq + col * QC;
-    // a group no lane of the wave passes is skipped as a whole
+    // a group no element of which reaches the threshold is skipped as a whole
 #pragma unroll
     for (int g = 0; g < 4; g++)
-        if (gm[g] > thr) {
+        if (reach(gm[g])) {
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                if (take(g * 4 + j))
-                    dst[pos++] = score_key(acc[g * 4 + j], rowbase + g * 8 + h * 4 + j);
+                if (take(g * 4 + j)) {
+                    const uint32_t row = rowbase + g * 8 + h * 4 + j;
+                    dst[pos++] = score_key(acc[g * 4 + j], PR ? rowmap[row] : row);
+                }
         }
     uint64_t need = __ballot(qn > (uint32_t)(QC - 32)) & 0xFFFFFFFFull;  // lanes 0..31: one per query
     while (need) {
         const int b = __builtin_ctzll(need);
         need &= need - 1;
-        const float t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
+        const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
         if (col == b) {
             qn = 0;
-            thr = t;
+            thr = key_score(t);
         }
     }
 }
@@ -323,7 +341,7 @@ __device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned cha
 // and the prefetch), so occupancy pays: the BF16 kernels with the step count
 // compiled in are held to 128 registers (four waves per SIMD; at most a few
 // spilled dwords, none for the 128-row stages), the FP32 ones to 168 (three).
-template <bool BF16, int E, int RT, int QC, int W, int STEPS>
+template <bool BF16, int E, int RT, int QC, int W, int STEPS, bool PR>
 __global__ __launch_bounds__(W * 64)
 __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) void k_shortlist(ShortlistArgs a) {
     constexpr int CPT = (kStageChunks * kWavesMax) / W;
@@ -348,7 +366,8 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     uint64_t* lists = a.lists + ((size_t)split * a.npad + qbase) * S;
     uint64_t* myq = a.queue + ((size_t)split * a.npad + qbase) * QC;
 
-    for (int i = lane; i < kQPerWave * S; i += 64) lists[i] = kInfKey;
+    if (!PR || !a.cont)
+        for (int i = lane; i < kQPerWave * S; i += 64) lists[i] = kInfKey;
     // B fragments: this lane's query (column) over its half of K
     const uint32_t q = qbase + col;
     constexpr int MAXSTEPS = STEPS > 0 ? STEPS : (BF16 ? 8 : 64);  // KP <= 128
@@ -368,24 +387,34 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
             if (s < steps) bqf[s] = src[s];
     }
     float thr = -INFINITY;
+    if (PR && a.cont) thr = key_score(lists[(size_t)col * S + S - 1]);  // left by the earlier pass
     uint32_t qn = 0;
 
     const unsigned char* P = reinterpret_cast<const unsigned char*>(a.P);
     const int row_elems_bytes = KP * elem;
     const int cpr = row_elems_bytes / 16;  // 16-byte chunks per row
     const int chunks = RT * cpr;
-    const uint32_t nst = (db1 - db0) / RT;
+    // the stages this workgroup sweeps: the split's consecutive stages, or
+    // (pruned sweep) its list
+    uint32_t nst = (db1 - db0) / RT;
+    const uint32_t* sid = nullptr;
+    if (PR) {
+        const uint2 bl = a.blk[blockIdx.x];
+        nst = bl.y;
+        sid = a.stage_ids + bl.x;
+    }
+    auto srow = [&](uint32_t t) -> uint32_t { return PR ? sid[t] * (uint32_t)RT : db0 + t * RT; };
     u32x4 R[CPT];
     float Rn = 0.0f;
     if (nst > 0) {
-        stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0, row_elems_bytes, chunks, tid);
+        stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(0), row_elems_bytes, chunks, tid);
         stage_store<CPT, RT, W>(R, Rn, tile0, negh0, row_bytes, cpr, chunks, tid);
     }
     // everything loaded so far (query fragments, stage 0) is complete here, so
     // inside the loop only the one-stage-ahead prefetch is ever in flight and
     // the MFMAs do not wait on it
     __builtin_amdgcn_s_waitcnt(0);
-    if (nst > 1) stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0 + RT, row_elems_bytes, chunks, tid);
+    if (nst > 1) stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(1), row_elems_bytes, chunks, tid);
     __syncthreads();
 
     for (uint32_t st = 0; st < nst; st++) {
@@ -398,9 +427,8 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
             stage_store<CPT, RT, W>(R, Rn, odd ? tile0 : tile1, odd ? negh0 : negh1, row_bytes, cpr,
                                     chunks, tid);
         if (st + 2 < nst)
-            stage_load<CPT, RT, W>(R, Rn, P, a.negh, db0 + (st + 2) * RT, row_elems_bytes, chunks,
-                                   tid);
-        const uint32_t r0 = db0 + st * RT;
+            stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(st + 2), row_elems_bytes, chunks, tid);
+        const uint32_t r0 = srow(st);
 #pragma unroll
         for (int sp = 0; sp < RT / 32; sp += 2) {
             // Two independent accumulation chains: the stage's two 32-row MFMA
@@ -465,8 +493,10 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
             }
-            epilogue<E, QC>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq);
-            if (kPair) epilogue<E, QC>(acc1, r0 + (sp + 1) * 32, q, h, col, lane, thr, qn, lists, myq);
+            epilogue<E, QC, PR>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq, a.rowmap);
+            if (kPair)
+                epilogue<E, QC, PR>(acc1, r0 + (sp + 1) * 32, q, h, col, lane, thr, qn, lists, myq,
+                                    a.rowmap);
         }
         __syncthreads();
     }
@@ -503,7 +533,8 @@ __global__ __launch_bounds__(256) void k_merge_splits(uint64_t* __restrict__ lis
     for (int e = 0; e < E; e++) lists[(size_t)q * S + e * 64 + lane] = v[e];
 }
 
-// Exact FP32 re-rank of the shortlist; one wave per query, lane = candidate.
+// Exact FP32 re-rank of the shortlist; one wave per query, lane = candidate
+// (qpos: the position of each query's list, when the lists are in cell order).
 // The wave first stages its 64 candidate rows into LDS with coalesced row
 // loads (one row per load instruction); each lane then accumulates its own
 // distance in dimension order with fmaf, the oracle's arithmetic.  Rows are
@@ -513,6 +544,7 @@ template <int E>
 __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __restrict__ x,
                                                               uint32_t n, int d,
                                                               const uint64_t* __restrict__ lists,
+                                                              const uint32_t* __restrict__ qpos,
                                                               int k, int32_t* __restrict__ out_idx,
                                                               float* __restrict__ out_dist) {
     extern __shared__ float rsh[];  // per wave: 64 rows of d+1 floats, then the query row
@@ -524,10 +556,11 @@ __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __res
     float* rows = rsh + (size_t)wave * (64 * ld + d);
     float* xq = rows + 64 * ld;
     for (int t = lane; t < d; t += 64) xq[t] = x[(size_t)q * d + t];
+    const uint64_t* ql = lists + (size_t)(qpos ? qpos[q] : q) * S;  // the query's list
     uint64_t key[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const uint64_t lk = lists[(size_t)q * S + e * 64 + lane];
+        const uint64_t lk = ql[e * 64 + lane];
         const uint32_t c = (uint32_t)lk;
         const bool ok = lk != kInfKey;
         for (int r = 0; r < 64; r++) {
@@ -561,6 +594,332 @@ __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __res
             out_dist[(size_t)q * k + g] = ok ? __uint_as_float((uint32_t)(key[e] >> 32)) : INFINITY;
         }
     }
+}
+
+// ------------------------------------------------- pruned sweep (large n)
+//
+// For large n the database is first put in cell order: rows are assigned to
+// the nearest of m seed rows, stably sorted by cell, and each cell is padded
+// to whole 128-row query blocks (padding rows score -inf).  Every 64- or
+// 128-row group then has a centroid c and radius r (in the packed, rounded
+// coordinates the MFMA scores use).  Pass 1 sweeps, for each query block, the
+// stages of its own cell.  If a query's list is then full, its S-th distance
+// D_q bounds every candidate that can still enter it; a stage whose lower
+// bound |c_b - c_s| - r_b - r_s exceeds sqrt(max_q D_q) (with a margin that
+// covers the FP32 accumulation error of the scores) can hold no candidate for
+// any query of the block and is skipped.  Pass 2 sweeps the remaining stages.
+// Skipped rows would never have entered a list, and the threshold compares
+// (score, row) keys, so the lists equal those of the full sweep.
+
+constexpr uint32_t kBlockRows = 128;  // queries per workgroup (4 waves x 32)
+constexpr uint32_t kCellRows = 1024;  // rows per cell on average
+constexpr uint32_t kMaxCells = 1024;
+constexpr int kSeedChunk = 64;        // seeds staged in LDS at a time
+constexpr int kListBlocks = 4;        // query blocks per workgroup of k_stage_lists
+constexpr uint32_t kPruneMinRows = 65536;  // smaller problems sweep every stage
+
+__device__ inline float packed_at(const void* P, int KP, uint32_t row, int t, bool bf16) {
+    if (t >= KP) return 0.0f;
+    return bf16 ? bf16_to_f32(reinterpret_cast<const u16*>(P)[(size_t)row * KP + t])
+                : reinterpret_cast<const float*>(P)[(size_t)row * KP + t];
+}
+
+// seed c = packed row floor(c n / m), as f32, and its -0.5 |s|^2
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_seed_rows(const void* __restrict__ P, int KP, uint32_t n,
+                                                   uint32_t m, const float* __restrict__ negh,
+                                                   float* __restrict__ seeds,
+                                                   float* __restrict__ snegh) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t row = (uint32_t)(((uint64_t)c * n) / m);
+    for (int t = threadIdx.x; t < KP; t += 256) seeds[(size_t)c * KP + t] = packed_at(P, KP, row, t, BF16);
+    if (threadIdx.x == 0) snegh[c] = negh[row];
+}
+
+// cell[i] = the seed of largest score p_i . s - 0.5 |s|^2 (the nearest; the
+// first on ties); count[c] = rows per cell.  One thread per row, its packed
+// row in registers; seeds staged through LDS kSeedChunk at a time.
+template <bool BF16, int KPM>
+__global__ __launch_bounds__(256) void k_assign_cells(const void* __restrict__ P, int KP, uint32_t n,
+                                                      const float* __restrict__ seeds,
+                                                      const float* __restrict__ snegh, uint32_t m,
+                                                      uint32_t* __restrict__ cell,
+                                                      uint32_t* __restrict__ count) {
+    __shared__ __align__(16) float ss[kSeedChunk * KPM];
+    __shared__ float sn[kSeedChunk];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float xr[KPM];
+#pragma unroll
+    for (int t = 0; t < KPM; t++) xr[t] = i < n ? packed_at(P, KP, i, t, BF16) : 0.0f;
+    float best = -INFINITY;
+    uint32_t bi = 0;
+    for (uint32_t c0 = 0; c0 < m; c0 += kSeedChunk) {
+        const uint32_t cn = min((uint32_t)kSeedChunk, m - c0);
+        __syncthreads();  // the previous chunk is no longer read
+        for (uint32_t e = threadIdx.x; e < cn * KPM; e += 256) {
+            const uint32_t c = e / KPM, t = e % KPM;
+            ss[e] = (int)t < KP ? seeds[(size_t)(c0 + c) * KP + t] : 0.0f;
+        }
+        if (threadIdx.x < cn) sn[threadIdx.x] = snegh[c0 + threadIdx.x];
+        __syncthreads();
+        for (uint32_t c = 0; c < cn; c++) {
+            const float4* sv = reinterpret_cast<const float4*>(ss + c * KPM);
+            float dot = sn[c];
+#pragma unroll
+            for (int t4 = 0; t4 < KPM / 4; t4++) {
+                const float4 v = sv[t4];
+                dot = fmaf(xr[4 * t4 + 0], v.x, dot);
+                dot = fmaf(xr[4 * t4 + 1], v.y, dot);
+                dot = fmaf(xr[4 * t4 + 2], v.z, dot);
+                dot = fmaf(xr[4 * t4 + 3], v.w, dot);
+            }
+            if (dot > best) {
+                best = dot;
+                bi = c0 + c;
+            }
+        }
+    }
+    if (i < n) {
+        cell[i] = bi;
+        atomicAdd(&count[bi], 1u);
+    }
+}
+
+// position of sorted row r: its cell's padded start plus its rank in the cell
+__global__ __launch_bounds__(256) void k_cell_layout(const uint32_t* __restrict__ cell_sorted,
+                                                     const uint32_t* __restrict__ order, uint32_t n,
+                                                     const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ pstart,
+                                                     uint32_t* __restrict__ rowmap,
+                                                     uint32_t* __restrict__ posmap) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t c = cell_sorted[r];
+    const uint32_t p = pstart[c] + (r - start[c]);
+    const uint32_t i = order[r];
+    rowmap[p] = i;
+    posmap[i] = p;
+}
+
+// packed rows in cell order (padding rows zero, score -inf)
+__global__ __launch_bounds__(256) void k_gather_rows(const unsigned char* __restrict__ P,
+                                                     int row_bytes, const float* __restrict__ negh,
+                                                     const uint32_t* __restrict__ rowmap,
+                                                     uint32_t npos, unsigned char* __restrict__ P2,
+                                                     float* __restrict__ negh2) {
+    const int cpr = row_bytes / 16;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (uint64_t)npos * cpr) return;
+    const uint32_t p = (uint32_t)(t / cpr);
+    const int o = (int)(t - (uint64_t)p * cpr);
+    const uint32_t src = rowmap[p];
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (src != 0xFFFFFFFFu) v = reinterpret_cast<const u32x4*>(P + (size_t)src * row_bytes)[o];
+    reinterpret_cast<u32x4*>(P2 + (size_t)p * row_bytes)[o] = v;
+    if (o == 0) negh2[p] = src != 0xFFFFFFFFu ? negh[src] : -INFINITY;
+}
+
+// Bounds of each G-row group in cell order (one wavefront per group; lane t
+// holds dimensions t and t + 64): centroid, radius (largest distance of a row
+// to it), largest squared row norm, row count.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_group_bounds(const void* __restrict__ P2, int KP,
+                                                      const uint32_t* __restrict__ rowmap, int G,
+                                                      uint32_t ngroups, float* __restrict__ cen,
+                                                      float* __restrict__ rad,
+                                                      float* __restrict__ gnorm,
+                                                      uint32_t* __restrict__ cnt) {
+    const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= ngroups) return;  // whole wavefronts
+    float c0 = 0.0f, c1 = 0.0f;
+    uint32_t k = 0;
+    for (int r = 0; r < G; r++) {
+        const uint32_t p = g * (uint32_t)G + r;
+        if (rowmap[p] == 0xFFFFFFFFu) continue;  // uniform
+        k++;
+        c0 += packed_at(P2, KP, p, lane, BF16);
+        c1 += packed_at(P2, KP, p, lane + 64, BF16);
+    }
+    float r2 = 0.0f, n2 = 0.0f;
+    if (k) {
+        c0 /= (float)k;
+        c1 /= (float)k;
+        for (int r = 0; r < G; r++) {
+            const uint32_t p = g * (uint32_t)G + r;
+            if (rowmap[p] == 0xFFFFFFFFu) continue;
+            const float v0 = packed_at(P2, KP, p, lane, BF16), v1 = packed_at(P2, KP, p, lane + 64, BF16);
+            float dd = (v0 - c0) * (v0 - c0) + (v1 - c1) * (v1 - c1);
+            float nn = v0 * v0 + v1 * v1;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                dd += __shfl_xor(dd, o, 64);
+                nn += __shfl_xor(nn, o, 64);
+            }
+            r2 = fmaxf(r2, dd);
+            n2 = fmaxf(n2, nn);
+        }
+    }
+    if (lane < KP) cen[(size_t)g * KP + lane] = c0;
+    if (lane + 64 < KP) cen[(size_t)g * KP + lane + 64] = c1;
+    if (lane == 0) {
+        rad[g] = sqrtf(r2);
+        gnorm[g] = n2;
+        cnt[g] = k;
+    }
+}
+
+// Pass-1 range of each query block: the stages of its own cell.
+__global__ __launch_bounds__(256) void k_block_ranges(const uint32_t* __restrict__ pstart,
+                                                      uint32_t m, uint32_t nblk, uint32_t RT,
+                                                      uint2* __restrict__ range) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nblk) return;
+    const uint32_t p = b * kBlockRows;
+    if (p >= pstart[m]) {  // padding past the last cell
+        range[b] = make_uint2(0u, 0u);
+        return;
+    }
+    uint32_t lo = 0, hi = m;  // pstart[lo] <= p < pstart[hi]; empty cells are passed over
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pstart[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    range[b] = make_uint2(pstart[lo] / RT, (pstart[lo + 1] - pstart[lo]) / RT);
+}
+
+// Pass-2 stage lists.  A workgroup takes kListBlocks query blocks; each
+// thread tests one stage against all of them per round.  A block's bound D is
+// the largest distance^2 threshold of its queries after pass 1 (|x|^2 - 2 thr,
+// +inf while a list is not full).  WRITE = false counts the kept stages per
+// block; WRITE = true writes them, in ascending order, at offs[b] (never more
+// than the count pass found).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_stage_lists(
+    const float* __restrict__ bcen, const float* __restrict__ brad,
+    const float* __restrict__ bnorm, const uint32_t* __restrict__ bcnt,
+    const float* __restrict__ scen, const float* __restrict__ srad,
+    const float* __restrict__ snorm, const uint32_t* __restrict__ scnt,
+    const uint2* __restrict__ own, const float* __restrict__ negh2,
+    const uint64_t* __restrict__ lists, int S, int KP, uint32_t nblk, uint32_t nst,
+    uint32_t* __restrict__ counts, const uint32_t* __restrict__ offs, uint32_t* __restrict__ ids) {
+    // WRITE reads counts (the count pass's totals) as a bound
+    __shared__ __align__(16) float cb[kListBlocks][128];
+    __shared__ float dmax[kListBlocks];
+    __shared__ uint32_t wcnt[kListBlocks][4];
+    __shared__ uint32_t total[kListBlocks];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t b0 = blockIdx.x * kListBlocks;
+    if (tid < kListBlocks) {
+        dmax[tid] = 0.0f;
+        total[tid] = 0u;
+    }
+    for (int e = tid; e < kListBlocks * 128; e += 256) {
+        const uint32_t b = b0 + e / 128;
+        const int t = e % 128;
+        cb[e / 128][t] = (b < nblk && t < KP) ? bcen[(size_t)b * KP + t] : 0.0f;
+    }
+    __syncthreads();
+    // D of each block: 512 queries, two per thread; max of non-negative floats
+    // by their bit patterns
+    for (int e = tid; e < kListBlocks * (int)kBlockRows; e += 256) {
+        const uint32_t b = b0 + e / kBlockRows;
+        if (b >= nblk) continue;
+        const uint32_t q = b * kBlockRows + e % kBlockRows;
+        const float nh = negh2[q];
+        if (nh == -INFINITY) continue;  // padding row
+        const float thr = key_score(lists[(size_t)q * S + S - 1]);
+        float dq = thr == -INFINITY ? INFINITY : fmaxf(-2.0f * nh - 2.0f * thr, 0.0f);
+        atomicMax(reinterpret_cast<uint32_t*>(&dmax[e / kBlockRows]), __float_as_uint(dq));
+    }
+    __syncthreads();
+    float D[kListBlocks], rb[kListBlocks], nb[kListBlocks];
+    uint2 ob[kListBlocks];
+    bool vb[kListBlocks];
+#pragma unroll
+    for (int j = 0; j < kListBlocks; j++) {
+        const uint32_t b = b0 + j;
+        vb[j] = b < nblk && bcnt[b] > 0;
+        D[j] = dmax[j];
+        rb[j] = vb[j] ? brad[b] : 0.0f;
+        nb[j] = vb[j] ? sqrtf(bnorm[b]) : 0.0f;
+        ob[j] = vb[j] ? own[b] : make_uint2(0u, 0u);
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t s0 = 0; s0 < nst; s0 += 256) {
+        const uint32_t sidx = s0 + tid;
+        const bool live = sidx < nst && scnt[sidx] > 0;
+        float d2[kListBlocks];
+#pragma unroll
+        for (int j = 0; j < kListBlocks; j++) d2[j] = 0.0f;
+        float rs = 0.0f, ns = 0.0f;
+        if (live) {
+            const float4* cs = reinterpret_cast<const float4*>(scen + (size_t)sidx * KP);
+            for (int t4 = 0; t4 < KP / 4; t4++) {
+                const float4 v = cs[t4];
+#pragma unroll
+                for (int j = 0; j < kListBlocks; j++) {
+                    const float4 w = *reinterpret_cast<const float4*>(&cb[j][4 * t4]);
+                    const float e0 = w.x - v.x, e1 = w.y - v.y, e2 = w.z - v.z, e3 = w.w - v.w;
+                    d2[j] += e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
+                }
+            }
+            rs = srad[sidx];
+            ns = sqrtf(snorm[sidx]);
+        }
+#pragma unroll
+        for (int j = 0; j < kListBlocks; j++) {
+            bool keep = live && vb[j] && !(sidx >= ob[j].x && sidx < ob[j].x + ob[j].y);
+            if (keep) {
+                const float scale = nb[j] + ns;
+                const float lb = sqrtf(d2[j]) * (1.0f - 1e-5f) - (rb[j] + rs) * (1.0f + 1e-5f) -
+                                 1e-6f * scale;
+                const float margin = 2e-4f * scale * scale + 1e-5f * D[j];
+                keep = !(lb > 0.0f && lb * lb > D[j] + margin);
+            }
+            const uint64_t bal = __ballot(keep);
+            if (!WRITE) {
+                if (lane == 0 && bal) atomicAdd(&total[j], (uint32_t)__popcll(bal));
+            } else {
+                if (lane == 0) wcnt[j][wave] = (uint32_t)__popcll(bal);
+                __syncthreads();
+                uint32_t before = total[j];
+                for (int w = 0; w < wave; w++) before += wcnt[j][w];
+                const uint32_t at = before + (uint32_t)__popcll(bal & below);
+                if (keep && at < counts[b0 + j]) ids[offs[b0 + j] + at] = sidx;
+                __syncthreads();
+                if (tid == 0) total[j] += wcnt[j][0] + wcnt[j][1] + wcnt[j][2] + wcnt[j][3];
+                __syncthreads();
+            }
+        }
+    }
+    if (!WRITE) {
+        __syncthreads();
+        if (tid < kListBlocks && b0 + tid < nblk) counts[b0 + tid] = total[tid];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_range_counts(const uint2* __restrict__ range,
+                                                      uint32_t nblk, uint32_t* __restrict__ counts) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b < nblk) counts[b] = range[b].y;
+}
+
+// the stages of block b's range, written at offs[b] (one workgroup per block)
+__global__ __launch_bounds__(256) void k_range_ids(const uint2* __restrict__ range,
+                                                   const uint32_t* __restrict__ offs,
+                                                   uint32_t* __restrict__ ids) {
+    const uint2 r = range[blockIdx.x];
+    const uint32_t o = offs[blockIdx.x];
+    for (uint32_t t = threadIdx.x; t < r.y; t += 256) ids[o + t] = r.x + t;
+}
+
+__global__ __launch_bounds__(256) void k_make_blk(const uint32_t* __restrict__ offs,
+                                                  const uint32_t* __restrict__ counts,
+                                                  uint32_t nblk, uint2* __restrict__ blk) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b < nblk) blk[b] = make_uint2(offs[b], counts[b]);
 }
 
 size_t rerank_lds_bytes(int d) {
@@ -640,101 +999,264 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     MRS_CHECK_LAUNCH();
 
     const Shape sh = shortlist_shape(bf16, KP);
-    const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
-    int splits = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
-    uint32_t db_per = ((npad + splits - 1) / splits + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
-    splits = (int)((npad + db_per - 1) / db_per);
     const int S = 64 * E;
-    DevBuf<uint64_t> lists((size_t)splits * npad * S);
-    DevBuf<uint64_t> queue((size_t)splits * npad * sh.QC);
-
-    ShortlistArgs sa;
-    sa.P = P.p;
-    sa.negh = negh.p;
-    sa.lists = lists.p;
-    sa.queue = queue.p;
-    sa.n = n;
-    sa.npad = npad;
-    sa.KP = KP;
-    sa.db_per_split = db_per;
     const size_t lds = shortlist_lds_bytes(bf16, KP, sh.RT);
-    auto launch = [&](auto kern, int W) {
-        ensure_dynamic_lds((const void*)kern, lds);
-        hipLaunchKernelGGL(kern, dim3(qblocks, splits), dim3(W * 64), lds, s, sa);
-    };
-    MRS_HIP(hipEventRecord(ev[1], s));
-    bool launched = false;
-    // BF16 kernels are compiled per K-step count for the shapes that take it
-    // (128-row stages: KP <= 64; 64-row stages: KP 80..128)
-    const int ksteps = bf16 ? KP / 16 : 0;
-    auto pick = [&](auto b16, auto e, auto rt, auto qc, auto w) {
-        constexpr bool B = decltype(b16)::value;
-        constexpr int EE = decltype(e)::value, RTT = decltype(rt)::value;
-        constexpr int QCC = decltype(qc)::value, WW = decltype(w)::value;
-        if (launched || B != bf16 || EE != E || RTT != sh.RT || QCC != sh.QC || WW != sh.W) return;
-        if constexpr (B) {
-            auto st = [&](auto stc) {
-                constexpr int ST = decltype(stc)::value;
-                if (launched || ST != ksteps) return;
-                launch(k_shortlist<B, EE, RTT, QCC, WW, ST>, WW);
+    // one launch of the shortlist kernel compiled for this (precision, E,
+    // shape); BF16 kernels are compiled per K-step count for the shapes that
+    // take it (128-row stages: KP <= 64; 64-row stages: KP 80..128)
+    auto run_shortlist = [&](const ShortlistArgs& sa, dim3 grid, bool pr) {
+        bool launched = false;
+        auto launch = [&](auto kern, int W) {
+            ensure_dynamic_lds((const void*)kern, lds);
+            hipLaunchKernelGGL(kern, grid, dim3(W * 64), lds, s, sa);
+        };
+        const int ksteps = bf16 ? KP / 16 : 0;
+        auto pick = [&](auto b16, auto e, auto rt, auto qc, auto w) {
+            constexpr bool B = decltype(b16)::value;
+            constexpr int EE = decltype(e)::value, RTT = decltype(rt)::value;
+            constexpr int QCC = decltype(qc)::value, WW = decltype(w)::value;
+            if (launched || B != bf16 || EE != E || RTT != sh.RT || QCC != sh.QC || WW != sh.W) return;
+            if constexpr (B) {
+                auto st = [&](auto stc) {
+                    constexpr int ST = decltype(stc)::value;
+                    if (launched || ST != ksteps) return;
+                    if (pr) launch(k_shortlist<B, EE, RTT, QCC, WW, ST, true>, WW);
+                    else launch(k_shortlist<B, EE, RTT, QCC, WW, ST, false>, WW);
+                    launched = true;
+                };
+                using std::integral_constant;
+                if constexpr (RTT == 128) {
+                    st(integral_constant<int, 1>{}); st(integral_constant<int, 2>{});
+                    st(integral_constant<int, 3>{}); st(integral_constant<int, 4>{});
+                } else if constexpr (RTT == 64) {
+                    st(integral_constant<int, 5>{}); st(integral_constant<int, 6>{});
+                    st(integral_constant<int, 7>{}); st(integral_constant<int, 8>{});
+                }
+            }
+            if (!launched) {
+                if (pr) launch(k_shortlist<B, EE, RTT, QCC, WW, 0, true>, WW);
+                else launch(k_shortlist<B, EE, RTT, QCC, WW, 0, false>, WW);
                 launched = true;
-            };
-            using std::integral_constant;
-            if constexpr (RTT == 128) {
-                st(integral_constant<int, 1>{}); st(integral_constant<int, 2>{});
-                st(integral_constant<int, 3>{}); st(integral_constant<int, 4>{});
-            } else if constexpr (RTT == 64) {
-                st(integral_constant<int, 5>{}); st(integral_constant<int, 6>{});
-                st(integral_constant<int, 7>{}); st(integral_constant<int, 8>{});
+            }
+        };
+        using std::integral_constant;
+        using T = integral_constant<bool, true>;
+        using F = integral_constant<bool, false>;
+        using I1 = integral_constant<int, 1>;
+        using I2 = integral_constant<int, 2>;
+        using I4 = integral_constant<int, 4>;
+        using I32 = integral_constant<int, 32>;
+        using I64 = integral_constant<int, 64>;
+        using I128 = integral_constant<int, 128>;
+        auto all = [&](auto rt, auto qc, auto w) {
+            pick(T{}, I1{}, rt, qc, w);
+            pick(T{}, I2{}, rt, qc, w);
+            pick(F{}, I1{}, rt, qc, w);
+            pick(F{}, I2{}, rt, qc, w);
+        };
+        all(I128{}, I64{}, I4{});
+        all(I64{}, I64{}, I4{});
+        all(I32{}, I64{}, I4{});
+        if (!launched)
+            throw Error(MRS_INVALID_ARGUMENT, "no shortlist kernel for this (precision, d) shape");
+        MRS_CHECK_LAUNCH();
+    };
+    const size_t rlds = rerank_lds_bytes(d);
+    auto rerank = [&](const uint64_t* lists, const uint32_t* qpos) {
+        auto go = [&](auto kern) {
+            ensure_dynamic_lds((const void*)kern, rlds);
+            hipLaunchKernelGGL(kern, dim3(div_up(n, kRerankWaves)), dim3(kRerankWaves * 64), rlds,
+                               s, x, n, d, lists, qpos, k, out_idx, out_dist);
+        };
+        if (E == 1) go(k_rerank<1>);
+        else go(k_rerank<2>);
+        MRS_CHECK_LAUNCH();
+    };
+    int splits = 1;
+    double swept = 1.0;
+    uint32_t cells = 0;
+
+    const bool prune = ctx->knn_prune && n >= kPruneMinRows && sh.W * kQPerWave == (int)kBlockRows;
+    if (!prune) {
+        const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
+        splits = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
+        uint32_t db_per = ((npad + splits - 1) / splits + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
+        splits = (int)((npad + db_per - 1) / db_per);
+        DevBuf<uint64_t> lists((size_t)splits * npad * S);
+        DevBuf<uint64_t> queue((size_t)splits * npad * sh.QC);
+        ShortlistArgs sa{};
+        sa.P = P.p;
+        sa.negh = negh.p;
+        sa.lists = lists.p;
+        sa.queue = queue.p;
+        sa.n = n;
+        sa.npad = npad;
+        sa.KP = KP;
+        sa.db_per_split = db_per;
+        MRS_HIP(hipEventRecord(ev[1], s));
+        run_shortlist(sa, dim3(qblocks, splits), false);
+        MRS_HIP(hipEventRecord(ev[2], s));
+        if (splits > 1) {
+            if (E == 1)
+                hipLaunchKernelGGL(k_merge_splits<1>, dim3(div_up(npad, 4)), dim3(256), 0, s,
+                                   lists.p, npad, splits);
+            else
+                hipLaunchKernelGGL(k_merge_splits<2>, dim3(div_up(npad, 4)), dim3(256), 0, s,
+                                   lists.p, npad, splits);
+            MRS_CHECK_LAUNCH();
+        }
+        rerank(lists.p, nullptr);
+        MRS_HIP(hipEventRecord(ev[3], s));
+        MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
+    } else {
+        // ---- cell order (see the pruned-sweep kernels above)
+        const int elem = bf16 ? 2 : 4;
+        const uint32_t m = std::min<uint32_t>(kMaxCells, std::max<uint32_t>(1u, n / kCellRows));
+        cells = m;
+        DevBuf<float> seeds((size_t)m * KP), snegh(m);
+        DevBuf<uint32_t> cell(n), ccount(m), csorted(n), order(n);
+        MRS_HIP(hipMemsetAsync(ccount.p, 0, ccount.bytes(), s));
+        if (bf16)
+            hipLaunchKernelGGL(k_seed_rows<true>, dim3(m), dim3(256), 0, s, (const void*)P.p, KP, n,
+                               m, negh.p, seeds.p, snegh.p);
+        else
+            hipLaunchKernelGGL(k_seed_rows<false>, dim3(m), dim3(256), 0, s, (const void*)P.p, KP,
+                               n, m, negh.p, seeds.p, snegh.p);
+        {
+            const dim3 ga(div_up(n, 256));
+            const void* pp = P.p;
+            if (KP <= 64) {
+                if (bf16)
+                    hipLaunchKernelGGL((k_assign_cells<true, 64>), ga, dim3(256), 0, s, pp, KP, n,
+                                       seeds.p, snegh.p, m, cell.p, ccount.p);
+                else
+                    hipLaunchKernelGGL((k_assign_cells<false, 64>), ga, dim3(256), 0, s, pp, KP, n,
+                                       seeds.p, snegh.p, m, cell.p, ccount.p);
+            } else {
+                if (bf16)
+                    hipLaunchKernelGGL((k_assign_cells<true, 128>), ga, dim3(256), 0, s, pp, KP, n,
+                                       seeds.p, snegh.p, m, cell.p, ccount.p);
+                else
+                    hipLaunchKernelGGL((k_assign_cells<false, 128>), ga, dim3(256), 0, s, pp, KP, n,
+                                       seeds.p, snegh.p, m, cell.p, ccount.p);
             }
         }
-        if (!launched) {
-            launch(k_shortlist<B, EE, RTT, QCC, WW, 0>, WW);
-            launched = true;
-        }
-    };
-    using std::integral_constant;
-    using T = integral_constant<bool, true>;
-    using F = integral_constant<bool, false>;
-    using I1 = integral_constant<int, 1>;
-    using I2 = integral_constant<int, 2>;
-    using I4 = integral_constant<int, 4>;
-    using I32 = integral_constant<int, 32>;
-    using I64 = integral_constant<int, 64>;
-    using I128 = integral_constant<int, 128>;
-    auto all = [&](auto rt, auto qc, auto w) {
-        pick(T{}, I1{}, rt, qc, w);
-        pick(T{}, I2{}, rt, qc, w);
-        pick(F{}, I1{}, rt, qc, w);
-        pick(F{}, I2{}, rt, qc, w);
-    };
-    all(I128{}, I64{}, I4{});
-    all(I64{}, I64{}, I4{});
-    all(I32{}, I64{}, I4{});
-    if (!launched)
-        throw Error(MRS_INVALID_ARGUMENT, "no shortlist kernel for this (precision, d) shape");
-    MRS_CHECK_LAUNCH();
-    MRS_HIP(hipEventRecord(ev[2], s));
-    if (splits > 1) {
-        if (E == 1)
-            hipLaunchKernelGGL(k_merge_splits<1>, dim3(div_up(npad, 4)), dim3(256), 0, s, lists.p,
-                               npad, splits);
-        else
-            hipLaunchKernelGGL(k_merge_splits<2>, dim3(div_up(npad, 4)), dim3(256), 0, s, lists.p,
-                               npad, splits);
         MRS_CHECK_LAUNCH();
+        radix_sort_pairs_u32(cell.p, nullptr, n, m > 1 ? 32 - __builtin_clz(m - 1) : 1, csorted.p,
+                             order.p, s);
+        std::vector<uint32_t> hc(m), start(m + 1, 0), pstart(m + 1, 0);
+        MRS_HIP(hipMemcpyAsync(hc.data(), ccount.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipStreamSynchronize(s));
+        uint64_t pairs1 = 0;
+        for (uint32_t c = 0; c < m; c++) {
+            const uint32_t padded = (hc[c] + kBlockRows - 1) / kBlockRows * kBlockRows;
+            start[c + 1] = start[c] + hc[c];
+            pstart[c + 1] = pstart[c] + padded;
+            pairs1 += (uint64_t)(padded / kBlockRows) * (padded / (uint32_t)sh.RT);
+        }
+        if (start[m] != n) throw Error(MRS_DEVICE_ERROR, "kNN cell counts do not add up");
+        const uint32_t npos = (pstart[m] + 255u) / 256u * 256u;
+        DevBuf<uint32_t> dstart(m + 1), dpstart(m + 1), rowmap(npos), posmap(n);
+        MRS_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(uint32_t) * (m + 1), hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemcpyAsync(dpstart.p, pstart.data(), sizeof(uint32_t) * (m + 1), hipMemcpyHostToDevice, s));
+        MRS_HIP(hipMemsetAsync(rowmap.p, 0xFF, rowmap.bytes(), s));
+        hipLaunchKernelGGL(k_cell_layout, dim3(div_up(n, 256)), dim3(256), 0, s, csorted.p, order.p,
+                           n, dstart.p, dpstart.p, rowmap.p, posmap.p);
+        MRS_CHECK_LAUNCH();
+        const int row_bytes = KP * elem;
+        DevBuf<unsigned char> P2((size_t)npos * row_bytes);
+        DevBuf<float> negh2(npos);
+        hipLaunchKernelGGL(k_gather_rows, dim3(div_up((uint64_t)npos * (row_bytes / 16), 256)),
+                           dim3(256), 0, s, P.p, row_bytes, negh.p, rowmap.p, npos, P2.p, negh2.p);
+        MRS_CHECK_LAUNCH();
+        const uint32_t nblk = npos / kBlockRows, nst = npos / (uint32_t)sh.RT;
+        DevBuf<float> bcen((size_t)nblk * KP), brad(nblk), bnorm(nblk);
+        DevBuf<uint32_t> bcnt(nblk);
+        DevBuf<float> scen_own, srad_own, snorm_own;
+        DevBuf<uint32_t> scnt_own;
+        auto bounds = [&](int G, uint32_t ng, float* cen, float* rad, float* gn, uint32_t* cnt) {
+            if (bf16)
+                hipLaunchKernelGGL(k_group_bounds<true>, dim3(div_up(ng, 4)), dim3(256), 0, s,
+                                   (const void*)P2.p, KP, rowmap.p, G, ng, cen, rad, gn, cnt);
+            else
+                hipLaunchKernelGGL(k_group_bounds<false>, dim3(div_up(ng, 4)), dim3(256), 0, s,
+                                   (const void*)P2.p, KP, rowmap.p, G, ng, cen, rad, gn, cnt);
+            MRS_CHECK_LAUNCH();
+        };
+        bounds((int)kBlockRows, nblk, bcen.p, brad.p, bnorm.p, bcnt.p);
+        const float *scen = bcen.p, *srad = brad.p, *snorm = bnorm.p;
+        const uint32_t* scnt = bcnt.p;
+        if ((uint32_t)sh.RT != kBlockRows) {
+            scen_own.alloc((size_t)nst * KP);
+            srad_own.alloc(nst);
+            snorm_own.alloc(nst);
+            scnt_own.alloc(nst);
+            bounds(sh.RT, nst, scen_own.p, srad_own.p, snorm_own.p, scnt_own.p);
+            scen = scen_own.p;
+            srad = srad_own.p;
+            snorm = snorm_own.p;
+            scnt = scnt_own.p;
+        }
+        DevBuf<uint2> range(nblk), blk2(nblk);
+        hipLaunchKernelGGL(k_block_ranges, dim3(div_up(nblk, 256)), dim3(256), 0, s, dpstart.p, m,
+                           nblk, (uint32_t)sh.RT, range.p);
+        MRS_CHECK_LAUNCH();
+        DevBuf<uint64_t> lists((size_t)npos * S);
+        DevBuf<uint64_t> queue((size_t)npos * sh.QC);
+        ShortlistArgs sa{};
+        sa.P = P2.p;
+        sa.negh = negh2.p;
+        sa.lists = lists.p;
+        sa.queue = queue.p;
+        sa.n = n;
+        sa.npad = npos;
+        sa.KP = KP;
+        sa.db_per_split = npos;
+        sa.rowmap = rowmap.p;
+        // pass 1: each query block's own cell, as an explicit stage list
+        DevBuf<uint32_t> counts1(nblk), offs1((size_t)nblk + 1), ids1(pairs1 ? pairs1 : 1);
+        DevBuf<uint2> blk1(nblk);
+        hipLaunchKernelGGL(k_range_counts, dim3(div_up(nblk, 256)), dim3(256), 0, s, range.p, nblk,
+                           counts1.p);
+        MRS_CHECK_LAUNCH();
+        exclusive_scan_u32(counts1.p, offs1.p, nblk, offs1.p + nblk, s);
+        hipLaunchKernelGGL(k_range_ids, dim3(nblk), dim3(256), 0, s, range.p, offs1.p, ids1.p);
+        hipLaunchKernelGGL(k_make_blk, dim3(div_up(nblk, 256)), dim3(256), 0, s, offs1.p,
+                           counts1.p, nblk, blk1.p);
+        MRS_CHECK_LAUNCH();
+        sa.blk = blk1.p;
+        sa.stage_ids = ids1.p;
+        MRS_HIP(hipEventRecord(ev[1], s));
+        run_shortlist(sa, dim3(nblk, 1), true);
+        // pass 2: the stages the bounds cannot rule out
+        DevBuf<uint32_t> counts(nblk), offs((size_t)nblk + 1);
+        const dim3 gl(div_up(nblk, kListBlocks));
+        hipLaunchKernelGGL(k_stage_lists<false>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
+                           bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
+                           nst, counts.p, nullptr, nullptr);
+        MRS_CHECK_LAUNCH();
+        exclusive_scan_u32(counts.p, offs.p, nblk, offs.p + nblk, s);
+        uint32_t total = 0;
+        MRS_HIP(hipMemcpyAsync(&total, offs.p + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        MRS_HIP(hipStreamSynchronize(s));
+        DevBuf<uint32_t> ids(total ? total : 1);
+        hipLaunchKernelGGL(k_stage_lists<true>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
+                           bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
+                           nst, counts.p, offs.p, ids.p);
+        hipLaunchKernelGGL(k_make_blk, dim3(div_up(nblk, 256)), dim3(256), 0, s, offs.p, counts.p,
+                           nblk, blk2.p);
+        MRS_CHECK_LAUNCH();
+        sa.blk = blk2.p;
+        sa.stage_ids = ids.p;
+        sa.cont = 1;
+        run_shortlist(sa, dim3(nblk, 1), true);
+        MRS_HIP(hipEventRecord(ev[2], s));
+        rerank(lists.p, posmap.p);
+        MRS_HIP(hipEventRecord(ev[3], s));
+        MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
+        const double full = (double)(pstart[m] / kBlockRows) * (double)(pstart[m] / (uint32_t)sh.RT);
+        swept = full > 0 ? ((double)pairs1 + (double)total) / full : 1.0;
     }
-    const size_t rlds = rerank_lds_bytes(d);
-    auto rerank = [&](auto kern) {
-        ensure_dynamic_lds((const void*)kern, rlds);
-        hipLaunchKernelGGL(kern, dim3(div_up(n, kRerankWaves)), dim3(kRerankWaves * 64), rlds, s, x,
-                           n, d, lists.p, k, out_idx, out_dist);
-    };
-    if (E == 1) rerank(k_rerank<1>);
-    else rerank(k_rerank<2>);
-    MRS_CHECK_LAUNCH();
-    MRS_HIP(hipEventRecord(ev[3], s));
-    MRS_HIP(hipEventSynchronize(ev[3]));
     if (timings) {
         float a = 0, b = 0, c = 0;
         MRS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -745,6 +1267,8 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         timings->rerank_ms = c;
         timings->splits = splits;
         timings->shortlist = S;
+        timings->swept = swept;
+        timings->cells = (int)cells;
     }
 }
 
@@ -789,6 +1313,8 @@ int mrs_knn_exhaustive_timed(mrs_ctx* ctx, const float* x, int64_t n, int32_t d,
             ms_out[2] = t.rerank_ms;
             ms_out[3] = t.splits;
             ms_out[4] = t.shortlist;
+            ms_out[5] = t.swept;
+            ms_out[6] = t.cells;
         }
     });
 }

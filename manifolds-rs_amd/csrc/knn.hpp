@@ -8,6 +8,8 @@ namespace mrs {
 struct KnnTimings {
     float pack_ms = 0, shortlist_ms = 0, rerank_ms = 0;
     int splits = 0, shortlist = 0;
+    double swept = 1.0;  // fraction of (query block, stage) pairs swept
+    int cells = 0;       // cells of the pruned sweep (0: full sweep)
 };
 
 // x: device [n][d] f32; out_idx/out_dist: device [n][k].

@@ -178,6 +178,12 @@ inline unsigned div_up(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) /
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev,
                         hipStream_t s);
 
+// Stable sort of u32 keys by their low key_bits bits (LSD radix, 8-bit
+// digits), carrying u32 values (vals_in == nullptr: the identity 0..n-1).
+// keys_out / vals_out must not alias the inputs.
+void radix_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
+                          int key_bits, uint32_t* keys_out, uint32_t* vals_out, hipStream_t s);
+
 // Raise kernel `kern`'s dynamic-LDS limit on the current device to at least
 // `bytes`.  The limit is per function and shared by every thread, so it only
 // ever grows: a caller never lowers it under a launch another thread is about
@@ -192,4 +198,5 @@ struct mrs_ctx {
     hipStream_t stream = nullptr;
     size_t hbm_budget = 0;
     int verbose = 0;
+    int knn_prune = 1;  // exhaustive kNN skips stages that provably hold no candidate
 };

@@ -130,85 +130,6 @@ __global__ __launch_bounds__(kWg) void k_comp_ids(const uint32_t* __restrict__ l
     if (i < n) comp[i] = rid[lab[i]];
 }
 
-// Stable LSD radix sort of the component ids (8-bit digits), carrying the
-// vertex ids: the sorted vertex ids are the component order.  A workgroup owns
-// a tile of kSortTile consecutive keys.
-constexpr int kSortRounds = 8;
-constexpr uint32_t kSortTile = (uint32_t)kWg * kSortRounds;
-static_assert(kWg == 256, "one thread per digit value");
-
-// counts[d * tiles + t] = number of keys of tile t whose digit is d
-__global__ __launch_bounds__(kWg) void k_digit_counts(const uint32_t* __restrict__ key, uint32_t n,
-                                                      int shift, uint32_t tiles,
-                                                      uint32_t* __restrict__ counts) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0u;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; r++) {
-        const uint32_t i = base + (uint32_t)r * kWg + threadIdx.x;
-        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// One stable scatter pass.  A key's place is its digit's offset for the tile
-// (offs = exclusive scan of counts, digit-major) plus the number of earlier
-// keys of the tile with the same digit.  Keys are taken in index order (round,
-// then wave, then lane); the lanes of a wave that share a digit are found with
-// one ballot per digit bit.  val_in == nullptr stands for the identity (i).
-__global__ __launch_bounds__(kWg) void k_digit_scatter(const uint32_t* __restrict__ key_in,
-                                                       const uint32_t* __restrict__ val_in,
-                                                       uint32_t n, int shift, uint32_t tiles,
-                                                       const uint32_t* __restrict__ offs,
-                                                       uint32_t* __restrict__ key_out,
-                                                       uint32_t* __restrict__ val_out) {
-    constexpr int kWaves = kWg / 64;
-    __shared__ uint32_t run[256];           // keys of each digit placed so far in this tile
-    __shared__ uint32_t wpre[kWaves][256];  // per round: wave counts, then wave offsets
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
-    run[tid] = 0u;
-    const uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; r++) {
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) wpre[w][tid] = 0u;
-        __syncthreads();
-        const uint32_t i = base + (uint32_t)r * kWg + tid;
-        const bool valid = i < n;
-        const uint32_t k = valid ? key_in[i] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const uint32_t rank = (uint32_t)__popcll(peers & below);
-        if (valid && rank == 0) wpre[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {  // thread tid owns digit tid: wave offsets on top of the tile's running count
-            uint32_t acc = run[tid];
-#pragma unroll
-            for (int w = 0; w < kWaves; w++) {
-                const uint32_t c = wpre[w][tid];
-                wpre[w][tid] = acc;
-                acc += c;
-            }
-            run[tid] = acc;
-        }
-        __syncthreads();
-        if (valid) {
-            const uint32_t pos = offs[(size_t)d * tiles + blockIdx.x] + wpre[wave][d] + rank;
-            key_out[pos] = k;
-            val_out[pos] = val_in ? val_in[i] : i;
-        }
-        __syncthreads();
-    }
-}
-
 // start[c] = the first position of component c in the sorted ids; start[ncomp] = n
 __global__ __launch_bounds__(kWg) void k_comp_starts(const uint32_t* __restrict__ skey, uint32_t n,
                                                      uint32_t* __restrict__ start) {
@@ -858,7 +779,8 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipStreamSynchronize(s));
         if (!ch) break;
     }
-    // Component order, built on the device (see k_digit_scatter): perm[q] is
+    // Component order, built on the device (a stable sort of the component
+    // ids carrying the vertex ids): perm[q] is
     // the vertex at position q and inv its inverse, so a component is the
     // contiguous range [start[c], start[c+1]) and every vector access is a
     // contiguous stream.
@@ -883,28 +805,11 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, s));
         MRS_CHECK_LAUNCH();
     } else {
-        DevBuf<uint32_t> ka(n), kb(n), vb(n);
+        DevBuf<uint32_t> ka(n), kb(n);
         hipLaunchKernelGGL(k_comp_ids, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, rid.p, n, ka.p);
         MRS_CHECK_LAUNCH();
-        const int bits = 32 - __builtin_clz(ncomp - 1);
-        const int passes = (bits + 7) / 8;
-        const uint32_t tiles = div_up(n, kSortTile);
-        DevBuf<uint32_t> counts((size_t)256 * tiles);
-        // keys alternate between ka and kb; the carried vertex ids alternate
-        // between vb and dperm so that the last pass writes dperm
-        uint32_t *kin = ka.p, *kout = kb.p, *vin = nullptr;
-        for (int pass = 0; pass < passes; pass++) {
-            uint32_t* vout = ((passes - 1 - pass) & 1) == 0 ? dperm.p : vb.p;
-            hipLaunchKernelGGL(k_digit_counts, dim3(tiles), dim3(kWg), 0, s, kin, n, 8 * pass,
-                               tiles, counts.p);
-            MRS_CHECK_LAUNCH();
-            exclusive_scan_u32(counts.p, counts.p, (uint64_t)256 * tiles, nullptr, s);
-            hipLaunchKernelGGL(k_digit_scatter, dim3(tiles), dim3(kWg), 0, s, kin, vin, n,
-                               8 * pass, tiles, counts.p, kout, vout);
-            MRS_CHECK_LAUNCH();
-            vin = vout;
-            std::swap(kin, kout);
-        }
+        radix_sort_pairs_u32(ka.p, nullptr, n, 32 - __builtin_clz(ncomp - 1), kb.p, dperm.p, s);
+        const uint32_t* kin = kb.p;
         hipLaunchKernelGGL(k_comp_starts, dim3(div_up(n, kWg)), dim3(kWg), 0, s, kin, n, dstart.p);
         hipLaunchKernelGGL(k_inverse, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dperm.p, n, dinvp.p);
         MRS_CHECK_LAUNCH();

@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
         "mrs_ctx_destroy": (C.c_int, [vp]),
         "mrs_ctx_synchronize": (C.c_int, [vp]),
         "mrs_ctx_stream": (vp, [vp]),
+        "mrs_ctx_set_knn_pruning": (C.c_int, [vp, i32]),
         "mrs_parse_umap_optimiser_gpu": (i32, [C.c_char_p]),
         "mrs_parse_ann_search_gpu": (i32, [C.c_char_p]),
         "mrs_hash_neg": (u32, [u32, u32, u32, u32, u32, u32]),
@@ -252,6 +253,11 @@ class Context:
     def stream(self) -> int:
         return lib().mrs_ctx_stream(self.handle)
 
+    def set_knn_pruning(self, on: bool) -> None:
+        """Exhaustive kNN skips database stages that provably hold no candidate
+        (default) or sweeps every stage; both give the same results."""
+        _check(lib().mrs_ctx_set_knn_pruning(self.handle, 1 if on else 0))
+
     def close(self) -> None:
         if self.handle:
             for obj in list(self._deps):
@@ -306,11 +312,12 @@ def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist
         idx = torch.empty((n, k), dtype=torch.int32, device=x.device)
         dist = torch.empty((n, k), dtype=torch.float32, device=x.device)
     if timed:
-        ms = (C.c_double * 5)()
+        ms = (C.c_double * 7)()
         _check(lib().mrs_knn_exhaustive_timed(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                               _ptr(idx), _ptr(dist), ms))
         return idx, dist, dict(pack_ms=ms[0], shortlist_ms=ms[1], rerank_ms=ms[2],
-                               splits=int(ms[3]), shortlist=int(ms[4]))
+                               splits=int(ms[3]), shortlist=int(ms[4]), swept=ms[5],
+                               cells=int(ms[6]))
     _check(lib().mrs_knn_exhaustive(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                     _ptr(idx), _ptr(dist)))
     return idx, dist

@@ -70,3 +70,67 @@ def test_knn_device_tensors(ctx):
     widx, wdist = orc.knn_exact(x, 15)
     assert np.array_equal(idx.cpu().numpy(), widx)
     np.testing.assert_allclose(dist.cpu().numpy(), wdist, rtol=1e-5)
+
+
+# ---- pruned sweep (n >= 65536): rows in cell order, stages ruled out by bounds
+
+
+def _both_sweeps(ctx, x, k, precision, shortlist):
+    """(idx, dist, timings) with pruning, and (idx, dist) of the full sweep."""
+    got = M.knn_exhaustive(ctx, x, k, precision, shortlist, timed=True)
+    ctx.set_knn_pruning(False)
+    try:
+        full = M.knn_exhaustive(ctx, x, k, precision, shortlist, timed=True)
+    finally:
+        ctx.set_knn_pruning(True)
+    assert full[2]["swept"] == 1.0 and full[2]["cells"] == 0
+    return got, full
+
+
+def _check_pruned(ctx, x, k, precision, shortlist=0):
+    (idx, dist, t), (fidx, fdist, _) = _both_sweeps(ctx, x, k, precision, shortlist)
+    assert t["cells"] > 0, "the pruned sweep did not run"
+    # the same lists as the full sweep, so the same result, bit for bit
+    bad = np.nonzero(np.any(idx != fidx, axis=1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ from the full sweep, first {bad[:5]}"
+    assert np.array_equal(dist.view(np.uint32), fdist.view(np.uint32))
+    # and the oracle's brute force on slices of rows (ties by index)
+    n = x.shape[0]
+    for q0, q1 in ((0, 300), (n // 2, n // 2 + 300), (n - 300, n)):
+        widx, wdist = orc.knn_exact(x, k, q0, q1)
+        assert np.array_equal(idx[q0:q1], widx), (q0, q1)
+        np.testing.assert_allclose(dist[q0:q1], wdist, rtol=1e-5, atol=0)
+    return t
+
+
+@pytest.mark.parametrize("precision", [M.KNN_BF16_RERANK, M.KNN_FP32])
+def test_pruned_sweep_clustered(ctx, precision):
+    import synthetic
+
+    x, _ = synthetic.subspace(n=70_000, d=64, seed=3)
+    t = _check_pruned(ctx, x, 15, precision, shortlist=60)
+    print(f"cells {t['cells']}, swept {t['swept']:.3f}")
+    assert t["swept"] < 0.9, t  # the bounds rule out stages on clustered data
+
+
+def test_pruned_sweep_wide_shortlist(ctx):
+    import synthetic
+
+    x, _ = synthetic.subspace(n=68_000, d=50, n_clusters=30, seed=9)
+    _check_pruned(ctx, x, 30, M.KNN_BF16_RERANK, shortlist=120)  # 128 candidates
+
+
+def test_pruned_sweep_uniform(ctx):
+    # no structure: few stages can be ruled out, the result is still exact
+    x = np.random.default_rng(5).uniform(-1, 1, size=(66_000, 8)).astype(np.float32)
+    _check_pruned(ctx, x, 15, M.KNN_BF16_RERANK, shortlist=60)
+
+
+@pytest.mark.parametrize("precision", [M.KNN_BF16_RERANK, M.KNN_FP32])
+def test_pruned_sweep_lattice_ties(ctx, precision):
+    # integer lattice points: exact duplicates and many equal distances, so
+    # the lists are decided by row order at every threshold
+    rng = np.random.default_rng(11)
+    x = np.zeros((68_000, 16), np.float32)
+    x[:, :3] = rng.integers(0, 30, size=(68_000, 3))
+    _check_pruned(ctx, x, 15, precision, shortlist=60)
