@@ -533,8 +533,7 @@ __global__ __launch_bounds__(256) void k_merge_splits(uint64_t* __restrict__ lis
     for (int e = 0; e < E; e++) lists[(size_t)q * S + e * 64 + lane] = v[e];
 }
 
-// Exact FP32 re-rank of the shortlist; one wave per query, lane = candidate
-// (qpos: the position of each query's list, when the lists are in cell order).
+// Exact FP32 re-rank of the shortlist; one wave per query, lane = candidate.
 // The wave first stages its 64 candidate rows into LDS with coalesced row
 // loads (one row per load instruction); each lane then accumulates its own
 // distance in dimension order with fmaf, the oracle's arithmetic.  Rows are
@@ -544,7 +543,6 @@ template <int E>
 __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __restrict__ x,
                                                               uint32_t n, int d,
                                                               const uint64_t* __restrict__ lists,
-                                                              const uint32_t* __restrict__ qpos,
                                                               int k, int32_t* __restrict__ out_idx,
                                                               float* __restrict__ out_dist) {
     extern __shared__ float rsh[];  // per wave: 64 rows of d+1 floats, then the query row
@@ -556,7 +554,7 @@ __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __res
     float* rows = rsh + (size_t)wave * (64 * ld + d);
     float* xq = rows + 64 * ld;
     for (int t = lane; t < d; t += 64) xq[t] = x[(size_t)q * d + t];
-    const uint64_t* ql = lists + (size_t)(qpos ? qpos[q] : q) * S;  // the query's list
+    const uint64_t* ql = lists + (size_t)q * S;  // the query's list
     uint64_t key[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -596,6 +594,74 @@ __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __res
     }
 }
 
+// The re-rank of the pruned sweep: lists and xs (X in cell order, rows of dp
+// floats, dp a multiple of 4) are in position order.  Wave w takes position w
+// (the caller's row rowmap[w]; none for padding); lane = candidate, reading its
+// own row xs[posmap[c]] with 16-byte loads.  A query's candidates lie mostly
+// in its own cell, so those rows are L2 hits, and without LDS staging many
+// waves are resident to hide the remaining latency.  The distance is the
+// oracle's fmaf over t in order, as in k_rerank.
+template <int E>
+__global__ __launch_bounds__(256) void k_rerank_cells(const float* __restrict__ xs, int dp, int d,
+                                                      uint32_t npos,
+                                                      const uint64_t* __restrict__ lists,
+                                                      const uint32_t* __restrict__ rowmap,
+                                                      const uint32_t* __restrict__ posmap, int k,
+                                                      int32_t* __restrict__ out_idx,
+                                                      float* __restrict__ out_dist) {
+    __shared__ float xqs[4][128];  // d <= 128
+    constexpr int S = 64 * E;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * 4 + wave;
+    if (w >= npos) return;  // whole waves leave; nothing below is block-wide
+    const uint32_t q = rowmap[w];
+    if (q == 0xFFFFFFFFu) return;
+    float* xq = xqs[wave];
+    for (int t = lane; t < d; t += 64) xq[t] = xs[(size_t)w * dp + t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int d4 = d & ~3;
+    uint64_t key[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint64_t lk = lists[(size_t)w * S + e * 64 + lane];
+        key[e] = kInfKey;
+        if (lk != kInfKey) {
+            const uint32_t c = (uint32_t)lk;
+            const float* row = xs + (size_t)posmap[c] * dp;
+            const float4* r4 = reinterpret_cast<const float4*>(row);
+            float acc = 0.0f;
+            for (int t = 0; t < d4; t += 4) {
+                const float4 v = r4[t >> 2];
+                float df = xq[t] - v.x;
+                acc = fmaf(df, df, acc);
+                df = xq[t + 1] - v.y;
+                acc = fmaf(df, df, acc);
+                df = xq[t + 2] - v.z;
+                acc = fmaf(df, df, acc);
+                df = xq[t + 3] - v.w;
+                acc = fmaf(df, df, acc);
+            }
+            for (int t = d4; t < d; t++) {
+                const float df = xq[t] - row[t];
+                acc = fmaf(df, df, acc);
+            }
+            key[e] = ((uint64_t)__float_as_uint(acc) << 32) | c;
+        }
+    }
+    wave_sort<E>(key);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int g = e * 64 + lane;
+        if (g < k) {
+            const bool ok = key[e] != kInfKey;
+            out_idx[(size_t)q * k + g] = ok ? (int32_t)(uint32_t)key[e] : -1;
+            out_dist[(size_t)q * k + g] = ok ? __uint_as_float((uint32_t)(key[e] >> 32)) : INFINITY;
+        }
+    }
+}
+
 // ------------------------------------------------- pruned sweep (large n)
 //
 // For large n the database is first put in cell order: rows are assigned to
@@ -614,8 +680,7 @@ __global__ __launch_bounds__(kRerankWaves * 64) void k_rerank(const float* __res
 constexpr uint32_t kBlockRows = 128;  // queries per workgroup (4 waves x 32)
 constexpr uint32_t kCellRows = 1024;  // rows per cell on average
 constexpr uint32_t kMaxCells = 1024;
-constexpr int kSeedChunk = 64;        // seeds staged in LDS at a time
-constexpr int kListBlocks = 4;        // query blocks per workgroup of k_stage_lists
+constexpr int kListBlocks = 8;        // query blocks per workgroup of k_stage_lists
 constexpr uint32_t kPruneMinRows = 65536;  // smaller problems sweep every stage
 
 __device__ inline float packed_at(const void* P, int KP, uint32_t row, int t, bool bf16) {
@@ -624,62 +689,89 @@ __device__ inline float packed_at(const void* P, int KP, uint32_t row, int t, bo
                 : reinterpret_cast<const float*>(P)[(size_t)row * KP + t];
 }
 
-// seed c = packed row floor(c n / m), as f32, and its -0.5 |s|^2
+// Seed c = packed row floor(c n / m), as BF16 (K padded to KS, a multiple of
+// 16), and its -0.5 |s|^2 from the rounded values; padding seeds up to mpad
+// (a multiple of 32) score -inf.
 template <bool BF16>
-__global__ __launch_bounds__(256) void k_seed_rows(const void* __restrict__ P, int KP, uint32_t n,
-                                                   uint32_t m, const float* __restrict__ negh,
-                                                   float* __restrict__ seeds,
-                                                   float* __restrict__ snegh) {
-    const uint32_t c = blockIdx.x;
-    const uint32_t row = (uint32_t)(((uint64_t)c * n) / m);
-    for (int t = threadIdx.x; t < KP; t += 256) seeds[(size_t)c * KP + t] = packed_at(P, KP, row, t, BF16);
-    if (threadIdx.x == 0) snegh[c] = negh[row];
+__global__ __launch_bounds__(64) void k_seed_rows(const void* __restrict__ P, int KP, int KS,
+                                                  uint32_t n, uint32_t m,
+                                                  u16* __restrict__ seeds,
+                                                  float* __restrict__ snegh) {
+    const uint32_t c = blockIdx.x;  // < mpad
+    const int lane = threadIdx.x;
+    const uint32_t row = c < m ? (uint32_t)(((uint64_t)c * n) / m) : 0u;
+    float part = 0.0f;
+    for (int t = lane; t < KS; t += 64) {
+        const float v = c < m ? packed_at(P, KP, row, t, BF16) : 0.0f;
+        const u16 h = f32_to_bf16_rne(v);
+        seeds[(size_t)c * KS + t] = h;
+        const float r = bf16_to_f32(h);
+        part += r * r;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) snegh[c] = c < m ? -0.5f * part : -INFINITY;
 }
 
 // cell[i] = the seed of largest score p_i . s - 0.5 |s|^2 (the nearest; the
-// first on ties); count[c] = rows per cell.  One thread per row, its packed
-// row in registers; seeds staged through LDS kSeedChunk at a time.
-template <bool BF16, int KPM>
-__global__ __launch_bounds__(256) void k_assign_cells(const void* __restrict__ P, int KP, uint32_t n,
-                                                      const float* __restrict__ seeds,
-                                                      const float* __restrict__ snegh, uint32_t m,
-                                                      uint32_t* __restrict__ cell,
+// smallest seed on ties), count[c] = rows per cell.  A BF16 MFMA GEMM with an
+// argmax epilogue: one wave per 32 rows (the B operand, as in the shortlist),
+// 32 seeds per A tile read straight from the L2-resident seed matrix.  The
+// cells only group rows for the pruned sweep, whose result does not depend on
+// them, so BF16 rounding here is harmless.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_assign_cells(const void* __restrict__ P, int KP, int KS,
+                                                      uint32_t n, const u16* __restrict__ seeds,
+                                                      const float* __restrict__ snegh,
+                                                      uint32_t mpad, uint32_t* __restrict__ cell,
                                                       uint32_t* __restrict__ count) {
-    __shared__ __align__(16) float ss[kSeedChunk * KPM];
-    __shared__ float sn[kSeedChunk];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    float xr[KPM];
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const uint32_t i = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32 + col;  // this lane's row
+    const int steps = KS / 16;
+    bf16x8 bq[8];
 #pragma unroll
-    for (int t = 0; t < KPM; t++) xr[t] = i < n ? packed_at(P, KP, i, t, BF16) : 0.0f;
+    for (int st = 0; st < 8; st++) {
+        if (st >= steps) break;
+        union {
+            u16 h16[8];
+            bf16x8 v;
+        } f;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int t = h * (KS / 2) + st * 8 + e;
+            f.h16[e] = f32_to_bf16_rne(i < n ? packed_at(P, KP, i, t, BF16) : 0.0f);
+        }
+        bq[st] = f.v;
+    }
     float best = -INFINITY;
     uint32_t bi = 0;
-    for (uint32_t c0 = 0; c0 < m; c0 += kSeedChunk) {
-        const uint32_t cn = min((uint32_t)kSeedChunk, m - c0);
-        __syncthreads();  // the previous chunk is no longer read
-        for (uint32_t e = threadIdx.x; e < cn * KPM; e += 256) {
-            const uint32_t c = e / KPM, t = e % KPM;
-            ss[e] = (int)t < KP ? seeds[(size_t)(c0 + c) * KP + t] : 0.0f;
-        }
-        if (threadIdx.x < cn) sn[threadIdx.x] = snegh[c0 + threadIdx.x];
-        __syncthreads();
-        for (uint32_t c = 0; c < cn; c++) {
-            const float4* sv = reinterpret_cast<const float4*>(ss + c * KPM);
-            float dot = sn[c];
+    for (uint32_t c0 = 0; c0 < mpad; c0 += 32) {
+        f32x16 acc;
 #pragma unroll
-            for (int t4 = 0; t4 < KPM / 4; t4++) {
-                const float4 v = sv[t4];
-                dot = fmaf(xr[4 * t4 + 0], v.x, dot);
-                dot = fmaf(xr[4 * t4 + 1], v.y, dot);
-                dot = fmaf(xr[4 * t4 + 2], v.z, dot);
-                dot = fmaf(xr[4 * t4 + 3], v.w, dot);
-            }
-            if (dot > best) {
-                best = dot;
-                bi = c0 + c;
-            }
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[g * 4 + j] = snegh[c0 + g * 8 + h * 4 + j];
+        const bf16x8* arow =
+            reinterpret_cast<const bf16x8*>(seeds + (size_t)(c0 + col) * KS + h * (KS / 2));
+#pragma unroll
+        for (int st = 0; st < 8; st++) {
+            if (st >= steps) break;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arow[st], bq[st], acc, 0, 0, 0);
         }
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (acc[g * 4 + j] > best) {
+                    best = acc[g * 4 + j];
+                    bi = c0 + g * 8 + h * 4 + j;
+                }
     }
-    if (i < n) {
+    // the two lanes of a row hold the two halves of each seed tile's rows
+    const float ob = __shfl_xor(best, 32, 64);
+    const uint32_t oi = __shfl_xor(bi, 32, 64);
+    if (ob > best || (ob == best && oi < bi)) bi = oi;
+    if (h == 0 && i < n) {
         cell[i] = bi;
         atomicAdd(&count[bi], 1u);
     }
@@ -717,6 +809,18 @@ __global__ __launch_bounds__(256) void k_gather_rows(const unsigned char* __rest
     if (src != 0xFFFFFFFFu) v = reinterpret_cast<const u32x4*>(P + (size_t)src * row_bytes)[o];
     reinterpret_cast<u32x4*>(P2 + (size_t)p * row_bytes)[o] = v;
     if (o == 0) negh2[p] = src != 0xFFFFFFFFu ? negh[src] : -INFINITY;
+}
+
+// X rows in cell order, dp >= d floats per row (padding rows and columns zero)
+__global__ __launch_bounds__(256) void k_gather_x(const float* __restrict__ x, int d, int dp,
+                                                  const uint32_t* __restrict__ rowmap, uint32_t npos,
+                                                  float* __restrict__ xs) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (uint64_t)npos * dp) return;
+    const uint32_t p = (uint32_t)(t / dp);
+    const int f = (int)(t - (uint64_t)p * dp);
+    const uint32_t src = rowmap[p];
+    xs[t] = (src != 0xFFFFFFFFu && f < d) ? x[(size_t)src * d + f] : 0.0f;
 }
 
 // Bounds of each G-row group in cell order (one wavefront per group; lane t
@@ -1062,11 +1166,11 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         MRS_CHECK_LAUNCH();
     };
     const size_t rlds = rerank_lds_bytes(d);
-    auto rerank = [&](const uint64_t* lists, const uint32_t* qpos) {
+    auto rerank = [&](const uint64_t* lists) {
         auto go = [&](auto kern) {
             ensure_dynamic_lds((const void*)kern, rlds);
             hipLaunchKernelGGL(kern, dim3(div_up(n, kRerankWaves)), dim3(kRerankWaves * 64), rlds,
-                               s, x, n, d, lists, qpos, k, out_idx, out_dist);
+                               s, x, n, d, lists, k, out_idx, out_dist);
         };
         if (E == 1) go(k_rerank<1>);
         else go(k_rerank<2>);
@@ -1105,7 +1209,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                                    lists.p, npad, splits);
             MRS_CHECK_LAUNCH();
         }
-        rerank(lists.p, nullptr);
+        rerank(lists.p);
         MRS_HIP(hipEventRecord(ev[3], s));
         MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
     } else {
@@ -1113,33 +1217,24 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         const int elem = bf16 ? 2 : 4;
         const uint32_t m = std::min<uint32_t>(kMaxCells, std::max<uint32_t>(1u, n / kCellRows));
         cells = m;
-        DevBuf<float> seeds((size_t)m * KP), snegh(m);
-        DevBuf<uint32_t> cell(n), ccount(m), csorted(n), order(n);
+        const int KS = (KP + 15) / 16 * 16;
+        const uint32_t mpad = (m + 31) / 32 * 32;
+        DevBuf<u16> seeds((size_t)mpad * KS);
+        DevBuf<float> snegh(mpad);
+        DevBuf<uint32_t> cell(n), ccount(mpad), csorted(n), order(n);
         MRS_HIP(hipMemsetAsync(ccount.p, 0, ccount.bytes(), s));
-        if (bf16)
-            hipLaunchKernelGGL(k_seed_rows<true>, dim3(m), dim3(256), 0, s, (const void*)P.p, KP, n,
-                               m, negh.p, seeds.p, snegh.p);
-        else
-            hipLaunchKernelGGL(k_seed_rows<false>, dim3(m), dim3(256), 0, s, (const void*)P.p, KP,
-                               n, m, negh.p, seeds.p, snegh.p);
-        {
-            const dim3 ga(div_up(n, 256));
-            const void* pp = P.p;
-            if (KP <= 64) {
-                if (bf16)
-                    hipLaunchKernelGGL((k_assign_cells<true, 64>), ga, dim3(256), 0, s, pp, KP, n,
-                                       seeds.p, snegh.p, m, cell.p, ccount.p);
-                else
-                    hipLaunchKernelGGL((k_assign_cells<false, 64>), ga, dim3(256), 0, s, pp, KP, n,
-                                       seeds.p, snegh.p, m, cell.p, ccount.p);
-            } else {
-                if (bf16)
-                    hipLaunchKernelGGL((k_assign_cells<true, 128>), ga, dim3(256), 0, s, pp, KP, n,
-                                       seeds.p, snegh.p, m, cell.p, ccount.p);
-                else
-                    hipLaunchKernelGGL((k_assign_cells<false, 128>), ga, dim3(256), 0, s, pp, KP, n,
-                                       seeds.p, snegh.p, m, cell.p, ccount.p);
-            }
+        const void* pp = P.p;
+        const dim3 ga(div_up(n, 128));
+        if (bf16) {
+            hipLaunchKernelGGL(k_seed_rows<true>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
+                               seeds.p, snegh.p);
+            hipLaunchKernelGGL(k_assign_cells<true>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
+                               snegh.p, mpad, cell.p, ccount.p);
+        } else {
+            hipLaunchKernelGGL(k_seed_rows<false>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
+                               seeds.p, snegh.p);
+            hipLaunchKernelGGL(k_assign_cells<false>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
+                               snegh.p, mpad, cell.p, ccount.p);
         }
         MRS_CHECK_LAUNCH();
         radix_sort_pairs_u32(cell.p, nullptr, n, m > 1 ? 32 - __builtin_clz(m - 1) : 1, csorted.p,
@@ -1251,7 +1346,19 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.cont = 1;
         run_shortlist(sa, dim3(nblk, 1), true);
         MRS_HIP(hipEventRecord(ev[2], s));
-        rerank(lists.p, posmap.p);
+        // the re-rank in cell order, from X in cell order
+        const int dp = (d + 3) / 4 * 4;
+        DevBuf<float> xs((size_t)npos * dp);
+        hipLaunchKernelGGL(k_gather_x, dim3(div_up((uint64_t)npos * dp, 256)), dim3(256), 0, s, x,
+                           d, dp, rowmap.p, npos, xs.p);
+        const dim3 gr(div_up(npos, 4));
+        if (E == 1)
+            hipLaunchKernelGGL(k_rerank_cells<1>, gr, dim3(256), 0, s, xs.p, dp, d, npos, lists.p,
+                               rowmap.p, posmap.p, k, out_idx, out_dist);
+        else
+            hipLaunchKernelGGL(k_rerank_cells<2>, gr, dim3(256), 0, s, xs.p, dp, d, npos, lists.p,
+                               rowmap.p, posmap.p, k, out_idx, out_dist);
+        MRS_CHECK_LAUNCH();
         MRS_HIP(hipEventRecord(ev[3], s));
         MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
         const double full = (double)(pstart[m] / kBlockRows) * (double)(pstart[m] / (uint32_t)sh.RT);
