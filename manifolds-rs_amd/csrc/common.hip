@@ -17,17 +17,80 @@ static thread_local std::string g_last_error;
 static thread_local hipStream_t t_stream = nullptr;
 static thread_local bool t_stream_set = false;
 
-StreamScope::StreamScope(hipStream_t s) : prev(t_stream), prev_set(t_stream_set) {
+static thread_local std::shared_ptr<DevicePool> t_pool;
+
+StreamScope::StreamScope(hipStream_t s, std::shared_ptr<DevicePool> pool)
+    : prev(t_stream), prev_set(t_stream_set), prev_pool(t_pool) {
     t_stream = s;
     t_stream_set = true;
+    t_pool = std::move(pool);
 }
 StreamScope::~StreamScope() {
     t_stream = prev;
     t_stream_set = prev_set;
+    t_pool = std::move(prev_pool);
 }
 bool current_stream(hipStream_t* s) {
     if (t_stream_set) *s = t_stream;
     return t_stream_set;
+}
+std::shared_ptr<DevicePool> current_pool() { return t_pool; }
+
+// ------------------------------------------------------------ memory pool
+
+DevicePool::~DevicePool() {
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device_);
+    for (auto& kv : free_) (void)hipFree(kv.second);  // hipFree waits for the device
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+void* DevicePool::get(size_t bytes, size_t* got) {
+    bytes = (bytes + 255) / 256 * 256;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            *got = it->first;
+            cached_ -= it->first;
+            free_.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        MRS_HIP(hipMalloc(&p, bytes));
+    }
+    *got = bytes;
+    return p;
+}
+
+void DevicePool::put(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.emplace(bytes, p);
+    cached_ += bytes;
+}
+
+void DevicePool::trim() {
+    std::multimap<size_t, void*> blocks;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        blocks.swap(free_);
+        cached_ = 0;
+    }
+    if (blocks.empty()) return;
+    // a cached block may still be read by queued work of its stream
+    (void)hipDeviceSynchronize();
+    for (auto& kv : blocks) (void)hipFree(kv.second);
+}
+
+size_t DevicePool::cached_bytes() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
 }
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -221,7 +284,7 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t*
     }
     // temporaries are freed on return: unless that free is stream-ordered it
     // is immediate, so the queued work must be finished first
-    if (!sums.ordered) MRS_HIP(hipStreamSynchronize(s));
+    if (sums.freed_at_once()) MRS_HIP(hipStreamSynchronize(s));
 }
 
 void radix_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
@@ -249,7 +312,7 @@ void radix_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint
         vin = vout;
     }
     // temporaries are freed on return
-    if (!counts.ordered) MRS_HIP(hipStreamSynchronize(s));
+    if (counts.freed_at_once()) MRS_HIP(hipStreamSynchronize(s));
 }
 
 void ensure_dynamic_lds(const void* kern, size_t bytes) {
@@ -289,6 +352,7 @@ int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx** out) {
         auto* c = new mrs_ctx();
         c->device = device;
         c->hbm_budget = hbm_budget;
+        c->pool = std::make_shared<mrs::DevicePool>(device);
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;

@@ -681,6 +681,11 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
     }
     const int64_t key[4] = {first, count, (int64_t)seed, strategy};
     const bool hit = st->graph_exec && std::equal(key, key + 4, st->graph_key);
+    if (!hit && !std::equal(key, key + 4, st->last_key)) {
+        std::copy(key, key + 4, st->last_key);
+        launches();
+        return;
+    }
     if (!hit) {
         if (st->graph_exec) {
             (void)hipGraphExecDestroy(st->graph_exec);

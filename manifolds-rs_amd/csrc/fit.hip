@@ -268,7 +268,7 @@ int mrs_umap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const in
     return mrs::guarded([&] {
         if (!ctx || !out) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         mrs::umap_fit(ctx, data, n, d, knn_idx, knn_dist, knn_k, params, nullptr, seed, verbose,
                       out);
     });
@@ -281,7 +281,7 @@ int mrs_densmap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d,
     return mrs::guarded([&] {
         if (!ctx || !out || !dens) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         mrs::umap_fit(ctx, data, n, d, knn_idx, knn_dist, knn_k, params, dens, seed, verbose,
                       out);
     });

@@ -678,7 +678,7 @@ int mrs_fuzzy_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_dist,
                     mrs_graph** out) {
     return mrs::guarded([&] {
         if (!ctx || !out || !gp) throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         if (n <= 0 || !knn_idx || !knn_dist) throw Error(MRS_NO_DATA, "UMAP: empty kNN (NoData)");
         MRS_HIP(hipSetDevice(ctx->device));
         const size_t nk = (size_t)n * (size_t)k;
@@ -693,7 +693,7 @@ int mrs_graph_destroy(mrs_graph* g) {
     return mrs::guarded([&] {
         if (!g) return;
         (void)hipStreamSynchronize(g->ctx->stream);
-        mrs::StreamScope scope(g->ctx->stream);
+        mrs::StreamScope scope(g->ctx->stream, g->ctx->pool);
         delete g;
     });
 }
@@ -737,7 +737,7 @@ int mrs_dens_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const int32_t* k
     return mrs::guarded([&] {
         if (!ctx || !g || !r_out || !mu_sum_out)
             throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         const size_t nk = (size_t)g->n * (size_t)g->k;
         mrs::DevView<int32_t> idx(knn_idx, nk, ctx->stream);
         mrs::DevView<float> dist(knn_dist, nk, ctx->stream);
@@ -758,7 +758,7 @@ int mrs_umap_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const float* emb
     return mrs::guarded([&] {
         if (!ctx || !g || !embd || !params || !out)
             throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         if (n_dim <= 0 || n_dim > 16) throw Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
         if (params->n_epochs <= 0) throw Error(MRS_INVALID_ARGUMENT, "n_epochs must be positive");
         if (g->n_edges == 0) throw Error(MRS_NO_GRAPH_EDGES, "UMAP: no edges to optimise (NoGraphEdges)");

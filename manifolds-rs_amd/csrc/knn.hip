@@ -1388,7 +1388,7 @@ int mrs_knn_exhaustive(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32
     return mrs::guarded([&] {
         if (!ctx || !x || !idx || !dist) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         mrs::DevView<float> xv(x, (size_t)n * d, ctx->stream);
         mrs::DevView<int32_t> iv(idx, (size_t)n * k, ctx->stream, false);
         mrs::DevView<float> dv(dist, (size_t)n * k, ctx->stream, false);
@@ -1405,7 +1405,7 @@ int mrs_knn_exhaustive_timed(mrs_ctx* ctx, const float* x, int64_t n, int32_t d,
     return mrs::guarded([&] {
         if (!ctx || !x || !idx || !dist) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         MRS_HIP(hipSetDevice(ctx->device));
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         mrs::DevView<float> xv(x, (size_t)n * d, ctx->stream);
         mrs::DevView<int32_t> iv(idx, (size_t)n * k, ctx->stream, false);
         mrs::DevView<float> dv(dist, (size_t)n * k, ctx->stream, false);

@@ -12,6 +12,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -58,20 +61,47 @@ int guarded(F&& body) {
     }
 }
 
+// Device memory kept for reuse by one context (one stream).  A block a DevBuf
+// releases goes back here instead of to hipFree, and a later DevBuf of the
+// same context takes it: every user of a context's blocks works on that
+// context's stream, so stream order alone keeps a block's old and new users
+// apart, and a fit of the same shape allocates nothing after the first.
+// Blocks are reused for requests of at least half their size.  When hipMalloc
+// fails, the cache is returned to the device and the allocation tried again.
+class DevicePool {
+  public:
+    explicit DevicePool(int device) : device_(device) {}
+    ~DevicePool();  // frees every cached block
+    DevicePool(const DevicePool&) = delete;
+    DevicePool& operator=(const DevicePool&) = delete;
+    void* get(size_t bytes, size_t* got);
+    void put(void* p, size_t bytes);
+    void trim();  // waits for the device, then frees the cached blocks
+    size_t cached_bytes();
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    size_t cached_ = 0;
+    int device_;
+};
+
 // The stream the calling thread works on inside the library (set for the
-// duration of an ABI call by StreamScope).  With stream-ordered allocation on,
+// duration of an ABI call by StreamScope), and the memory pool of its context.  With stream-ordered allocation on,
 // device buffers are allocated and freed in that stream's order from the
 // device's memory pool (hipFree waits for every stream of the device, which
 // serialises concurrent fits).
 struct StreamScope {
     hipStream_t prev;
     bool prev_set;
-    explicit StreamScope(hipStream_t s);
+    std::shared_ptr<DevicePool> prev_pool;
+    explicit StreamScope(hipStream_t s, std::shared_ptr<DevicePool> pool = nullptr);
     ~StreamScope();
     StreamScope(const StreamScope&) = delete;
     StreamScope& operator=(const StreamScope&) = delete;
 };
 bool current_stream(hipStream_t* s);  // false outside any StreamScope
+std::shared_ptr<DevicePool> current_pool();  // null outside a pooled StreamScope
 
 // Stream-ordered allocation (hipMallocAsync / hipFreeAsync on the StreamScope
 // stream) is switched off: a device memory fault seen while several fits ran
@@ -81,20 +111,23 @@ bool current_stream(hipStream_t* s);  // false outside any StreamScope
 // that used the buffer.
 constexpr bool kStreamOrderedAlloc = false;
 
-// Owning device allocation (no copy, movable).  With kStreamOrderedAlloc and
-// inside a StreamScope it is stream-ordered (freed on the thread's stream at
-// release, after the work queued there so far); otherwise hipMalloc/hipFree.
+// Owning device allocation (no copy, movable).  Inside a StreamScope with a
+// pool it comes from (and returns to) that context's DevicePool; with
+// kStreamOrderedAlloc it would be stream-ordered; otherwise hipMalloc/hipFree.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
     bool ordered = false;
     hipStream_t home = nullptr;  // stream of the allocation
+    std::shared_ptr<DevicePool> pool;  // set: the block returns there
+    size_t cap = 0;                    // bytes of the pool block
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), ordered(o.ordered), home(o.home) {
+    DevBuf(DevBuf&& o) noexcept
+        : p(o.p), n(o.n), ordered(o.ordered), home(o.home), pool(std::move(o.pool)), cap(o.cap) {
         o.p = nullptr;
         o.n = 0;
     }
@@ -102,6 +135,7 @@ struct DevBuf {
         if (this != &o) {
             release();
             p = o.p; n = o.n; ordered = o.ordered; home = o.home;
+            pool = std::move(o.pool); cap = o.cap;
             o.p = nullptr; o.n = 0;
         }
         return *this;
@@ -112,6 +146,11 @@ struct DevBuf {
         n = count;
         if (!count) return;
         hipStream_t s;
+        ordered = false;
+        if ((pool = current_pool())) {
+            p = static_cast<T*>(pool->get(sizeof(T) * count, &cap));
+            return;
+        }
         ordered = kStreamOrderedAlloc && current_stream(&s);
         if (ordered) {
             home = s;
@@ -122,7 +161,9 @@ struct DevBuf {
     }
     void release() {
         if (p) {
-            if (ordered) {
+            if (pool) {
+                pool->put(p, cap);
+            } else if (ordered) {
                 hipStream_t s;
                 if (!current_stream(&s)) s = home;
                 (void)hipFreeAsync(p, s);
@@ -130,10 +171,14 @@ struct DevBuf {
                 (void)hipFree(p);
             }
         }
+        pool.reset();
         p = nullptr;
         n = 0;
     }
     size_t bytes() const { return sizeof(T) * n; }
+    // true if releasing the block returns it to the device at once (hipFree),
+    // so work still queued on it must finish first
+    bool freed_at_once() const { return p && !pool && !ordered; }
 };
 
 // True if `ptr` is device (or managed) memory visible to the current device.
@@ -196,6 +241,7 @@ void ensure_dynamic_lds(const void* kern, size_t bytes);
 struct mrs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::shared_ptr<mrs::DevicePool> pool;  // this context's reusable device memory
     size_t hbm_budget = 0;
     int verbose = 0;
     int knn_prune = 1;  // exhaustive kNN skips stages that provably hold no candidate

@@ -78,7 +78,7 @@ int mrs_umap_state_create(mrs_ctx* ctx, const float* embd, int64_t n, int32_t n_
     return mrs::guarded([&] {
         if (!ctx || !out) throw Error(MRS_INVALID_ARGUMENT, "ctx/out is NULL");
         check_params(params);
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         if (n <= 0 || !embd) throw Error(MRS_NO_DATA, "UMAP: empty embedding (NoData)");
         if (n_dim <= 0 || n_dim > 16) throw Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
         if (n > (int64_t)INT32_MAX) throw Error(MRS_INVALID_ARGUMENT, "n too large");
@@ -93,7 +93,7 @@ int mrs_umap_state_destroy(mrs_umap_state* st) {
     return mrs::guarded([&] {
         if (!st) return;
         (void)hipStreamSynchronize(st->ctx->stream);
-        mrs::StreamScope scope(st->ctx->stream);
+        mrs::StreamScope scope(st->ctx->stream, st->ctx->pool);
         delete st;
     });
 }

@@ -1112,7 +1112,7 @@ extern "C" int mrs_spectral_layout(mrs_ctx* ctx, const mrs_graph* g, int32_t n_d
                                    float range, const float* data, int32_t d, float* out) {
     return mrs::guarded([&] {
         if (!ctx || !g || !out) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
-        mrs::StreamScope scope(ctx->stream);
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
         if (n_dim <= 0 || n_dim > 16) throw mrs::Error(MRS_INVALID_ARGUMENT, "n_dim must be 1..16");
         hipStream_t s = ctx->stream;
         mrs::DevView<float> xv(data, data ? (size_t)g->n * d : 0, s);

@@ -45,11 +45,15 @@ struct mrs_umap_state {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     double prof_ms[8] = {0};
     int64_t prof_count[8] = {0};
-    // the epoch loop captured as a HIP graph (one launch per fit); the key is
-    // the (first, count, seed, strategy) it was captured for
+    // The epoch loop captured as a HIP graph, replayed with one launch.  A
+    // loop is captured the second time the same (first, count, seed,
+    // strategy) is run on this state: capturing costs as much host time as
+    // launching directly, and the device waits through it, so a loop run
+    // once (one fit) is launched directly.
     bool use_graphs = true;
     hipGraphExec_t graph_exec = nullptr;
     int64_t graph_key[4] = {-1, -1, -1, -1};
+    int64_t last_key[4] = {-1, -1, -1, -1};  // the last loop launched directly
     ~mrs_umap_state();
 };
 
