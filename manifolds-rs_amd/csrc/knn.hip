@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
+#include <vector>
 #include <type_traits>
 #include <cfloat>
 #include <cmath>
@@ -216,6 +218,9 @@ struct ShortlistArgs {
     const uint2* blk;
     const uint32_t* stage_ids;
     int cont;
+    // workgroup g sweeps query block order[g] (null: block g); the longest
+    // lists go first, so the last workgroups to start are short ones
+    const uint32_t* order;
 };
 
 // Wave-wide merge of query b's queue (c entries) into its sorted global list;
@@ -359,7 +364,8 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     const int tid = threadIdx.x;
     const int lane = tid & 63, h = lane >> 5, col = lane & 31;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const uint32_t qbase = (blockIdx.x * W + wave) * kQPerWave;
+    const uint32_t blk_id = (PR && a.order) ? a.order[blockIdx.x] : blockIdx.x;
+    const uint32_t qbase = (blk_id * W + wave) * kQPerWave;
     const uint32_t split = blockIdx.y;
     const uint32_t db0 = split * a.db_per_split;
     const uint32_t db1 = min(db0 + a.db_per_split, a.npad);
@@ -399,7 +405,7 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     uint32_t nst = (db1 - db0) / RT;
     const uint32_t* sid = nullptr;
     if (PR) {
-        const uint2 bl = a.blk[blockIdx.x];
+        const uint2 bl = a.blk[blk_id];
         nst = bl.y;
         sid = a.stage_ids + bl.x;
     }
@@ -1308,8 +1314,22 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.KP = KP;
         sa.db_per_split = npos;
         sa.rowmap = rowmap.p;
+        // the order in which a pass starts its query blocks: longest stage
+        // list first, so the last workgroups to start are short ones
+        auto longest_first = [&](const uint32_t* dcounts, uint32_t* dord) {
+            std::vector<uint32_t> hcnt(nblk), ord(nblk);
+            MRS_HIP(hipMemcpyAsync(hcnt.data(), dcounts, sizeof(uint32_t) * nblk,
+                                   hipMemcpyDeviceToHost, s));
+            MRS_HIP(hipStreamSynchronize(s));
+            std::iota(ord.begin(), ord.end(), 0u);
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](uint32_t a_, uint32_t b_) { return hcnt[a_] > hcnt[b_]; });
+            MRS_HIP(hipMemcpyAsync(dord, ord.data(), sizeof(uint32_t) * nblk,
+                                   hipMemcpyHostToDevice, s));
+            MRS_HIP(hipStreamSynchronize(s));  // ord is read by the copy
+        };
         // pass 1: each query block's own cell, as an explicit stage list
-        DevBuf<uint32_t> counts1(nblk), offs1((size_t)nblk + 1), ids1(pairs1 ? pairs1 : 1);
+        DevBuf<uint32_t> counts1(nblk), offs1((size_t)nblk + 1), ids1(pairs1 ? pairs1 : 1), dorder1(nblk);
         DevBuf<uint2> blk1(nblk);
         hipLaunchKernelGGL(k_range_counts, dim3(div_up(nblk, 256)), dim3(256), 0, s, range.p, nblk,
                            counts1.p);
@@ -1319,8 +1339,10 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         hipLaunchKernelGGL(k_make_blk, dim3(div_up(nblk, 256)), dim3(256), 0, s, offs1.p,
                            counts1.p, nblk, blk1.p);
         MRS_CHECK_LAUNCH();
+        longest_first(counts1.p, dorder1.p);
         sa.blk = blk1.p;
         sa.stage_ids = ids1.p;
+        sa.order = dorder1.p;
         MRS_HIP(hipEventRecord(ev[1], s));
         run_shortlist(sa, dim3(nblk, 1), true);
         // pass 2: the stages the bounds cannot rule out
@@ -1334,7 +1356,8 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         uint32_t total = 0;
         MRS_HIP(hipMemcpyAsync(&total, offs.p + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         MRS_HIP(hipStreamSynchronize(s));
-        DevBuf<uint32_t> ids(total ? total : 1);
+        DevBuf<uint32_t> ids(total ? total : 1), dorder(nblk);
+        longest_first(counts.p, dorder.p);
         hipLaunchKernelGGL(k_stage_lists<true>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
                            bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
                            nst, counts.p, offs.p, ids.p);
@@ -1344,6 +1367,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.blk = blk2.p;
         sa.stage_ids = ids.p;
         sa.cont = 1;
+        sa.order = dorder.p;
         run_shortlist(sa, dim3(nblk, 1), true);
         MRS_HIP(hipEventRecord(ev[2], s));
         // the re-rank in cell order, from X in cell order
