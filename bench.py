@@ -304,7 +304,8 @@ def bench_fit(args):
                       algorithmic_bytes_per_launch=bytes_edge),
         knn_kernel=dict(bound="mfma", kernel="k_shortlist<bf16> (pruned sweep, both passes)",
                         achieved=achieved, peak=2500.0, unit="TFLOP/s", frac=achieved / 2500.0,
-                        traffic=sl_traffic, traffic_source=sl_src,
+                        traffic=None, traffic_per_dispatch_mean=sl_traffic,
+                        traffic_source=sl_src,
                         avg_launch_ms=kt["shortlist_ms"], swept=kt["swept"],
                         algorithmic_flops_per_launch=kt["swept"] * flops,
                         brute_force_flops=flops),

@@ -20,9 +20,15 @@
 //   queue that could overflow on the next tile is merged into the query's
 //   sorted top-S list by the whole wavefront (bitonic sort + merge on 64-bit
 //   keys), which also refreshes the threshold.  The query's own row is
-//   dropped at insertion.  Large problems use one database split; small ones
-//   split the database so the grid still fills 256 CUs, and a merge kernel
-//   joins the split lists.
+//   dropped at insertion.  Problems below 65 536 rows split the database so
+//   the grid still fills 256 CUs, and a merge kernel joins the split lists.
+//
+// Pruned sweep (65 536 rows and more; the result is the same, bit for bit):
+//   rows are put in cell order (nearest of n/1024 seed rows, stable radix sort
+//   by cell, cells padded to whole query blocks), each query block first
+//   sweeps its own cell, and then only the stages whose centroid/radius bound
+//   cannot rule them out for any of its queries (see the pruned-sweep kernels
+//   below).  The re-rank then runs in cell order as well.
 //
 // Stage 2 -- exact re-rank: one wavefront per query recomputes the S candidate
 //   distances in FP32 as fmaf((x_q[t]-x_c[t])^2) over t in order (the same
