@@ -134,3 +134,12 @@ def test_pruned_sweep_lattice_ties(ctx, precision):
     x = np.zeros((68_000, 16), np.float32)
     x[:, :3] = rng.integers(0, 30, size=(68_000, 3))
     _check_pruned(ctx, x, 15, precision, shortlist=60)
+
+
+def test_pruned_sweep_wide_rows(ctx):
+    # d = 100: 64-row stages (separate stage and block bounds) and the 7-step
+    # BF16 kernel
+    import synthetic
+
+    x, _ = synthetic.subspace(n=66_000, d=100, n_clusters=40, seed=13)
+    _check_pruned(ctx, x, 15, M.KNN_BF16_RERANK, shortlist=60)
