@@ -177,6 +177,17 @@ int mrs_knn_exhaustive_timed(mrs_ctx *ctx, const float *x, int64_t n, int32_t d,
 int mrs_fuzzy_graph(mrs_ctx *ctx, const int32_t *knn_idx, const float *knn_dist, int64_t n,
                     int32_t k, const mrs_umap_graph_params *params, int32_t n_epochs,
                     mrs_graph **out);
+/* A graph from a caller's CoordinateList (src/data/structures.rs:22-31): the
+ * symmetric fuzzy graph symmetrise_graph / filter_weak_edges produce
+ * (src/data/graph.rs:684-694, :776-853), entries in any order, host or device
+ * pointers.  It is filtered again with n_epochs (which keeps an already
+ * filtered graph as it is) and laid out as mrs_fuzzy_graph does, so
+ * mrs_graph_export_coo returns it in (row, col) order.  Entries out of range,
+ * on the diagonal, repeated, or without their mirror (c, r) are rejected with
+ * MRS_INVALID_ARGUMENT.  The graph has no kNN behind it: mrs_graph_sigma_rho
+ * and mrs_dens_state_from_graph refuse it. */
+int mrs_graph_from_coo(mrs_ctx *ctx, int64_t n, const int32_t *rows, const int32_t *cols,
+                       const float *vals, int64_t nnz, int32_t n_epochs, mrs_graph **out);
 int mrs_graph_destroy(mrs_graph *g);
 /* n, number of COO entries after filtering (both directions), undirected edges */
 int mrs_graph_info(const mrs_graph *g, int64_t *n, int64_t *nnz, int64_t *n_edges);

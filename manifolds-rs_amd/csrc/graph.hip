@@ -478,6 +478,233 @@ uint32_t read_u32(const uint32_t* p, hipStream_t s) {
 
 }  // namespace
 
+// filter_weak_edges on the fuzzy union in g->u_* (m entries, rows grouped,
+// cols ascending, symmetric pattern), then the filtered COO, the CSR under both
+// endpoints and the per-edge arrays of the epoch kernels.
+void finish_graph(mrs_graph* g, uint32_t m) {
+    const uint32_t n = (uint32_t)g->n;
+    const int32_t n_epochs = g->n_epochs;
+    hipStream_t s = g->ctx->stream;
+    const size_t ma = m ? m : 1;
+    // filter flags
+    DevBuf<uint32_t> max_bits(2);
+    MRS_HIP(hipMemsetAsync(max_bits.p, 0, max_bits.bytes(), s));
+    if (m) {
+        hipLaunchKernelGGL(k_max_pos, dim3(std::min<unsigned>(div_up(m, kWg), 1024)),
+                           dim3(kWg), 0, s, g->u_w.p, (uint64_t)m, max_bits.p);
+        MRS_CHECK_LAUNCH();
+    }
+    g->u_keep.alloc(ma);
+    DevBuf<uint32_t> mirror(ma), upper(ma), present(ma), keep_scan(ma), up_scan(ma),
+        present_scan(ma);
+    if (m) {
+        hipLaunchKernelGGL(k_flags, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
+                           g->u_col.p, g->u_w.p, g->u_off.p, (uint64_t)m, max_bits.p,
+                           (float)n_epochs, g->u_keep.p, mirror.p, upper.p);
+        MRS_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_present, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
+                           g->u_col.p, g->u_keep.p, mirror.p, (uint64_t)m, present.p);
+        MRS_CHECK_LAUNCH();
+    }
+    DevBuf<uint32_t> totals(3);
+    exclusive_scan_u32(g->u_keep.p, keep_scan.p, m, totals.p + 0, s);
+    exclusive_scan_u32(upper.p, up_scan.p, m, totals.p + 1, s);
+    exclusive_scan_u32(present.p, present_scan.p, m, totals.p + 2, s);
+    uint32_t tot[3];
+    MRS_HIP(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+    MRS_HIP(hipStreamSynchronize(s));
+    // the totals size every output array below: check them against the union
+    if (tot[0] > m || tot[1] > m || tot[2] > m)
+        throw Error(MRS_DEVICE_ERROR, "graph scan totals exceed the union size");
+    g->nnz = tot[0];
+    g->n_edges = tot[1];
+    if ((uint64_t)tot[2] != 2ull * tot[1])
+        throw Error(MRS_DEVICE_ERROR, "graph CSR size mismatch (2E != present entries)");
+    const size_t nn = g->nnz ? g->nnz : 1, ne = g->n_edges ? g->n_edges : 1;
+    g->coo_row.alloc(nn);
+    g->coo_col.alloc(nn);
+    g->coo_val.alloc(nn);
+    g->off.alloc((size_t)n + 1);
+    g->csr_other.alloc(2 * ne);
+    g->csr_edge_idx.alloc(2 * ne);
+    g->edge_ij.alloc(2 * ne);
+    g->edge_loc.alloc(ne);
+    g->edge_w.alloc(ne);
+    g->per.alloc(ne);
+    hipLaunchKernelGGL(k_offsets, dim3(div_up((uint64_t)n + 1, kWg)), dim3(kWg), 0, s,
+                       g->u_off.p, present_scan.p, tot[2], (uint64_t)m, n, g->off.p);
+    MRS_CHECK_LAUNCH();
+    DevBuf<uint32_t> bad(1);
+    MRS_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), s));
+    if (m) {
+        hipLaunchKernelGGL(k_emit, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
+                           g->u_col.p, g->u_w.p, g->u_keep.p, mirror.p, upper.p, keep_scan.p,
+                           up_scan.p, present_scan.p, g->off.p, (uint64_t)m, g->coo_row.p,
+                           g->coo_col.p, g->coo_val.p, g->csr_other.p, g->csr_edge_idx.p,
+                           g->edge_ij.p, g->edge_loc.p, g->edge_w.p, bad.p);
+        MRS_CHECK_LAUNCH();
+    }
+    if (g->n_edges) {
+        hipLaunchKernelGGL(k_max_pos, dim3(std::min<unsigned>(div_up(g->n_edges, kWg), 1024)),
+                           dim3(kWg), 0, s, g->edge_w.p, (uint64_t)g->n_edges,
+                           max_bits.p + 1);
+        MRS_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_periods, dim3(div_up(g->n_edges, kWg)), dim3(kWg), 0, s,
+                           g->edge_w.p, (uint64_t)g->n_edges, max_bits.p + 1, g->per.p);
+        MRS_CHECK_LAUNCH();
+    }
+    g->loc_packed = read_u32(bad.p, s) == 0;
+}
+
+// ---- import of a CoordinateList (src/data/structures.rs:22-31)
+
+// keys of the caller's entries, with the checks: indices in [0, n), no
+// diagonal (bad = 1)
+__global__ __launch_bounds__(kWg) void k_coo_keys(const int32_t* __restrict__ rows,
+                                                  const int32_t* __restrict__ cols, uint64_t m,
+                                                  uint32_t n, uint32_t* __restrict__ krow,
+                                                  uint32_t* __restrict__ kcol,
+                                                  uint32_t* __restrict__ bad) {
+    const uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (p >= m) return;
+    const int32_t r = rows[p], c = cols[p];
+    if (r < 0 || c < 0 || (uint32_t)r >= n || (uint32_t)c >= n || r == c) {
+        *bad = 1u;
+        krow[p] = 0u;
+        kcol[p] = 0u;
+        return;
+    }
+    krow[p] = (uint32_t)r;
+    kcol[p] = (uint32_t)c;
+}
+
+// out[q] = in[idx[q]]
+__global__ __launch_bounds__(kWg) void k_gather_u32(const uint32_t* __restrict__ in,
+                                                    const uint32_t* __restrict__ idx, uint64_t m,
+                                                    uint32_t* __restrict__ out) {
+    const uint64_t q = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (q < m) out[q] = in[idx[q]];
+}
+
+// The union arrays in (row, col) order; a repeated (row, col) sets bad = 2.
+__global__ __launch_bounds__(kWg) void k_coo_union(const uint32_t* __restrict__ srow,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const uint32_t* __restrict__ kcol,
+                                                   const float* __restrict__ vals, uint64_t m,
+                                                   uint32_t* __restrict__ u_row,
+                                                   uint32_t* __restrict__ u_col,
+                                                   float* __restrict__ u_w,
+                                                   uint32_t* __restrict__ bad) {
+    const uint64_t q = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (q >= m) return;
+    const uint32_t e = idx[q];
+    u_row[q] = srow[q];
+    u_col[q] = kcol[e];
+    u_w[q] = vals[e];
+    if (q > 0) {
+        const uint32_t e0 = idx[q - 1];
+        if (srow[q - 1] == srow[q] && kcol[e0] == kcol[e]) *bad = 2u;
+    }
+}
+
+// off[v] = first entry of row v in the row-sorted list (off[n] = m)
+__global__ __launch_bounds__(kWg) void k_sorted_row_offsets(const uint32_t* __restrict__ row,
+                                                            uint64_t m, uint32_t n,
+                                                            uint32_t* __restrict__ off) {
+    const uint32_t v = blockIdx.x * kWg + threadIdx.x;
+    if (v > n) return;
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (row[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    off[v] = (uint32_t)lo;
+}
+
+// every (r, c) needs its mirror (c, r): bad = 3 otherwise
+__global__ __launch_bounds__(kWg) void k_coo_symmetric(const uint32_t* __restrict__ u_row,
+                                                       const uint32_t* __restrict__ u_col,
+                                                       const uint32_t* __restrict__ u_off,
+                                                       uint64_t m, uint32_t* __restrict__ bad) {
+    const uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t r = u_row[p], c = u_col[p];
+    const uint32_t q = find_in_row(u_col, u_off[c], u_off[c + 1], r);
+    if (!(q < u_off[c + 1] && u_col[q] == r)) *bad = 3u;
+}
+
+// A graph from a caller's CoordinateList: the symmetric fuzzy graph that
+// symmetrise_graph (+ filter_weak_edges) produces, in any entry order.  The
+// entries are sorted by (row, col) and then go through the same filter and
+// layout as a graph built from kNN (re-filtering an already filtered graph
+// with its own n_epochs keeps it as it is).  The graph has no kNN behind it:
+// no sigma/rho and no densMAP state.
+mrs_graph* graph_from_coo(mrs_ctx* ctx, int64_t n64, const int32_t* rows, const int32_t* cols,
+                          const float* vals, int64_t m64, int32_t n_epochs) {
+    if (n64 <= 0) throw Error(MRS_NO_DATA, "UMAP: empty graph (NoData)");
+    if (n64 > (int64_t)UINT32_MAX - 1 || m64 < 0 || m64 >= (int64_t)UINT32_MAX)
+        throw Error(MRS_INVALID_ARGUMENT, "graph too large for 32-bit indices");
+    if (n_epochs <= 0) throw Error(MRS_INVALID_ARGUMENT, "n_epochs must be positive");
+    const uint32_t n = (uint32_t)n64, m = (uint32_t)m64;
+    hipStream_t s = ctx->stream;
+    auto* g = new mrs_graph();
+    try {
+        g->ctx = ctx;
+        g->n = n64;
+        g->k = 0;
+        g->n_epochs = n_epochs;
+        g->n_union = m;
+        const size_t ma = m ? m : 1;
+        g->u_off.alloc((size_t)n + 1);
+        g->u_row.alloc(ma);
+        g->u_col.alloc(ma);
+        g->u_w.alloc(ma);
+        DevBuf<uint32_t> bad(1);
+        MRS_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+        if (m) {
+            DevView<int32_t> rv(rows, m, s), cv(cols, m, s);
+            DevView<float> vv(vals, m, s);
+            DevBuf<uint32_t> krow(m), kcol(m), t_key(m), idx1(m), row1(m), srow(m), idx2(m);
+            const dim3 gm(div_up(m, kWg));
+            hipLaunchKernelGGL(k_coo_keys, gm, dim3(kWg), 0, s, rv.p, cv.p, (uint64_t)m, n,
+                               krow.p, kcol.p, bad.p);
+            MRS_CHECK_LAUNCH();
+            const int bits = n > 1 ? 32 - __builtin_clz(n - 1) : 1;
+            // stable by column, then stable by row: (row, col) order
+            radix_sort_pairs_u32(kcol.p, nullptr, m, bits, t_key.p, idx1.p, s);
+            hipLaunchKernelGGL(k_gather_u32, gm, dim3(kWg), 0, s, krow.p, idx1.p, (uint64_t)m,
+                               row1.p);
+            MRS_CHECK_LAUNCH();
+            radix_sort_pairs_u32(row1.p, idx1.p, m, bits, srow.p, idx2.p, s);
+            hipLaunchKernelGGL(k_coo_union, gm, dim3(kWg), 0, s, srow.p, idx2.p, kcol.p, vv.p,
+                               (uint64_t)m, g->u_row.p, g->u_col.p, g->u_w.p, bad.p);
+            MRS_CHECK_LAUNCH();
+        }
+        hipLaunchKernelGGL(k_sorted_row_offsets, dim3(div_up((uint64_t)n + 1, kWg)), dim3(kWg), 0,
+                           s, g->u_row.p, (uint64_t)m, n, g->u_off.p);
+        MRS_CHECK_LAUNCH();
+        uint32_t b = read_u32(bad.p, s);
+        if (b == 0 && m) {
+            hipLaunchKernelGGL(k_coo_symmetric, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
+                               g->u_col.p, g->u_off.p, (uint64_t)m, bad.p);
+            MRS_CHECK_LAUNCH();
+            b = read_u32(bad.p, s);
+        }
+        if (b == 1)
+            throw Error(MRS_INVALID_ARGUMENT, "graph entry out of range or on the diagonal");
+        if (b == 2) throw Error(MRS_INVALID_ARGUMENT, "graph entry (row, col) repeated");
+        if (b == 3)
+            throw Error(MRS_INVALID_ARGUMENT,
+                        "graph is not symmetric: an entry (r, c) has no (c, r)");
+        finish_graph(g, m);
+    } catch (...) {
+        delete g;
+        throw;
+    }
+    return g;
+}
+
 mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_dist, int64_t n64,
                        int32_t k, float bandwidth, float local_connectivity, float mix_weight,
                        int32_t n_epochs) {
@@ -552,75 +779,8 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
                            raw_off.p, n, nullptr, g->u_off.p, g->u_row.p, g->u_col.p, g->u_w.p,
                            mix_weight, 1.0f - mix_weight);
         MRS_CHECK_LAUNCH();
-        // filter flags
-        DevBuf<uint32_t> max_bits(2);
-        MRS_HIP(hipMemsetAsync(max_bits.p, 0, max_bits.bytes(), s));
-        if (m) {
-            hipLaunchKernelGGL(k_max_pos, dim3(std::min<unsigned>(div_up(m, kWg), 1024)),
-                               dim3(kWg), 0, s, g->u_w.p, (uint64_t)m, max_bits.p);
-            MRS_CHECK_LAUNCH();
-        }
-        g->u_keep.alloc(ma);
-        DevBuf<uint32_t> mirror(ma), upper(ma), present(ma), keep_scan(ma), up_scan(ma),
-            present_scan(ma);
-        if (m) {
-            hipLaunchKernelGGL(k_flags, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
-                               g->u_col.p, g->u_w.p, g->u_off.p, (uint64_t)m, max_bits.p,
-                               (float)n_epochs, g->u_keep.p, mirror.p, upper.p);
-            MRS_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_present, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
-                               g->u_col.p, g->u_keep.p, mirror.p, (uint64_t)m, present.p);
-            MRS_CHECK_LAUNCH();
-        }
-        DevBuf<uint32_t> totals(3);
-        exclusive_scan_u32(g->u_keep.p, keep_scan.p, m, totals.p + 0, s);
-        exclusive_scan_u32(upper.p, up_scan.p, m, totals.p + 1, s);
-        exclusive_scan_u32(present.p, present_scan.p, m, totals.p + 2, s);
-        uint32_t tot[3];
-        MRS_HIP(hipMemcpyAsync(tot, totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
-        MRS_HIP(hipStreamSynchronize(s));
-        raw.release();  // the union read it; the stream is idle here
-        // the totals size every output array below: check them against the union
-        if (tot[0] > m || tot[1] > m || tot[2] > m)
-            throw Error(MRS_DEVICE_ERROR, "graph scan totals exceed the union size");
-        g->nnz = tot[0];
-        g->n_edges = tot[1];
-        if ((uint64_t)tot[2] != 2ull * tot[1])
-            throw Error(MRS_DEVICE_ERROR, "graph CSR size mismatch (2E != present entries)");
-        const size_t nn = g->nnz ? g->nnz : 1, ne = g->n_edges ? g->n_edges : 1;
-        g->coo_row.alloc(nn);
-        g->coo_col.alloc(nn);
-        g->coo_val.alloc(nn);
-        g->off.alloc((size_t)n + 1);
-        g->csr_other.alloc(2 * ne);
-        g->csr_edge_idx.alloc(2 * ne);
-        g->edge_ij.alloc(2 * ne);
-        g->edge_loc.alloc(ne);
-        g->edge_w.alloc(ne);
-        g->per.alloc(ne);
-        hipLaunchKernelGGL(k_offsets, dim3(div_up((uint64_t)n + 1, kWg)), dim3(kWg), 0, s,
-                           g->u_off.p, present_scan.p, tot[2], (uint64_t)m, n, g->off.p);
-        MRS_CHECK_LAUNCH();
-        DevBuf<uint32_t> bad(1);
-        MRS_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), s));
-        if (m) {
-            hipLaunchKernelGGL(k_emit, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
-                               g->u_col.p, g->u_w.p, g->u_keep.p, mirror.p, upper.p, keep_scan.p,
-                               up_scan.p, present_scan.p, g->off.p, (uint64_t)m, g->coo_row.p,
-                               g->coo_col.p, g->coo_val.p, g->csr_other.p, g->csr_edge_idx.p,
-                               g->edge_ij.p, g->edge_loc.p, g->edge_w.p, bad.p);
-            MRS_CHECK_LAUNCH();
-        }
-        if (g->n_edges) {
-            hipLaunchKernelGGL(k_max_pos, dim3(std::min<unsigned>(div_up(g->n_edges, kWg), 1024)),
-                               dim3(kWg), 0, s, g->edge_w.p, (uint64_t)g->n_edges,
-                               max_bits.p + 1);
-            MRS_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_periods, dim3(div_up(g->n_edges, kWg)), dim3(kWg), 0, s,
-                               g->edge_w.p, (uint64_t)g->n_edges, max_bits.p + 1, g->per.p);
-            MRS_CHECK_LAUNCH();
-        }
-        g->loc_packed = read_u32(bad.p, s) == 0;
+        raw.release();  // the union kernels that read it are queued before any reuse
+        finish_graph(g, m);
     } catch (...) {
         delete g;
         throw;
@@ -689,6 +849,17 @@ int mrs_fuzzy_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_dist,
     });
 }
 
+int mrs_graph_from_coo(mrs_ctx* ctx, int64_t n, const int32_t* rows, const int32_t* cols,
+                       const float* vals, int64_t nnz, int32_t n_epochs, mrs_graph** out) {
+    return mrs::guarded([&] {
+        if (!ctx || !out || (nnz > 0 && (!rows || !cols || !vals)))
+            throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        MRS_HIP(hipSetDevice(ctx->device));
+        mrs::StreamScope scope(ctx->stream, ctx->pool);
+        *out = mrs::graph_from_coo(ctx, n, rows, cols, vals, nnz, n_epochs);
+    });
+}
+
 int mrs_graph_destroy(mrs_graph* g) {
     return mrs::guarded([&] {
         if (!g) return;
@@ -724,6 +895,7 @@ int mrs_graph_export_coo(const mrs_graph* g, int32_t* rows, int32_t* cols, float
 int mrs_graph_sigma_rho(const mrs_graph* g, float* sigma, float* rho) {
     return mrs::guarded([&] {
         if (!g) throw Error(MRS_INVALID_ARGUMENT, "graph is NULL");
+        if (g->k == 0) throw Error(MRS_INVALID_ARGUMENT, "an imported graph has no sigma/rho");
         hipStream_t s = g->ctx->stream;
         if (sigma) MRS_HIP(hipMemcpyAsync(sigma, g->sigma.p, g->sigma.bytes(), hipMemcpyDefault, s));
         if (rho) MRS_HIP(hipMemcpyAsync(rho, g->rho.p, g->rho.bytes(), hipMemcpyDefault, s));
@@ -737,6 +909,8 @@ int mrs_dens_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const int32_t* k
     return mrs::guarded([&] {
         if (!ctx || !g || !r_out || !mu_sum_out)
             throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (g->k == 0)
+            throw Error(MRS_INVALID_ARGUMENT, "densMAP needs a graph built from kNN");
         mrs::StreamScope scope(ctx->stream, ctx->pool);
         const size_t nk = (size_t)g->n * (size_t)g->k;
         mrs::DevView<int32_t> idx(knn_idx, nk, ctx->stream);

@@ -176,6 +176,7 @@ def lib() -> C.CDLL:
                                                C.POINTER(C.c_double)]),
         "mrs_fuzzy_graph": (C.c_int, [vp, vp, vp, i64, i32, C.POINTER(UmapGraphParams), i32,
                                       C.POINTER(vp)]),
+        "mrs_graph_from_coo": (C.c_int, [vp, i64, vp, vp, vp, i64, i32, C.POINTER(vp)]),
         "mrs_graph_destroy": (C.c_int, [vp]),
         "mrs_graph_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
         "mrs_graph_export_coo": (C.c_int, [vp, vp, vp, vp]),
@@ -342,6 +343,26 @@ class FuzzyGraph:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().mrs_graph_info(h, C.byref(a), C.byref(b), C.byref(c)))
         self.n, self.nnz, self.n_edges = a.value, b.value, c.value
+
+    @classmethod
+    def from_coo(cls, ctx: Context, n: int, rows, cols, vals, n_epochs: int = 500) -> "FuzzyGraph":
+        """A graph from a CoordinateList (symmetric, any entry order): the
+        import counterpart of coo()."""
+        if isinstance(rows, np.ndarray):
+            rows = np.ascontiguousarray(rows, dtype=np.int32)
+            cols = np.ascontiguousarray(cols, dtype=np.int32)
+            vals = np.ascontiguousarray(vals, dtype=np.float32)
+        self = cls.__new__(cls)
+        self.ctx, self.k = ctx, 0
+        h = C.c_void_p()
+        _check(lib().mrs_graph_from_coo(ctx.handle, n, _ptr(rows), _ptr(cols), _ptr(vals),
+                                        len(rows), n_epochs, C.byref(h)))
+        self.handle = h
+        ctx._adopt(self)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().mrs_graph_info(h, C.byref(a), C.byref(b), C.byref(c)))
+        self.n, self.nnz, self.n_edges = a.value, b.value, c.value
+        return self
 
     def coo(self):
         """(rows, cols, vals) of the filtered CoordinateList."""

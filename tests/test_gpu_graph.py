@@ -125,3 +125,81 @@ def test_dens_state_degenerate(ctx):
     with pytest.raises(M.ManifoldsError) as e:
         g.dens_state(idx, dist, True)
     assert e.value.status == M.MRS_DEGENERATE_LOCAL_RADII
+
+
+# ---- import of a CoordinateList (the counterpart of export)
+
+
+def test_graph_from_coo_round_trips(ctx):
+    # Export, shuffle the entries, import: the same filtered COO, the same
+    # epoch layout, the same spectral layout and the same epochs, bit for bit.
+    x, _ = blobs(2500, 8, 5, seed=21)
+    idx, dist = orc.knn_exact(x, 15)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r, c, v = g.coo()
+    perm = np.random.default_rng(3).permutation(r.size)
+    h = M.FuzzyGraph.from_coo(ctx, g.n, r[perm], c[perm], v[perm], n_epochs=500)
+    assert (h.n, h.nnz, h.n_edges) == (g.n, g.nnz, g.n_edges)
+    r2, c2, v2 = h.coo()
+    assert np.array_equal(r2, r) and np.array_equal(c2, c)
+    assert np.array_equal(v2.view(np.uint32), v.view(np.uint32))
+    assert np.array_equal(h.spectral_layout(2, seed=7, data=x), g.spectral_layout(2, seed=7, data=x))
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 40
+    e0 = g.spectral_layout(2, seed=7, data=x)
+    sg = M.UmapGpuState(ctx, e0, params=p, graph=g)
+    sh = M.UmapGpuState(ctx, e0, params=p, graph=h)
+    for b in (M.BUF_OFFSETS, M.BUF_EDGE_IDX, M.BUF_OTHER, M.BUF_PER, M.BUF_EDGE_IJ):
+        assert np.array_equal(sg.read(b), sh.read(b)), b
+    sg.run_epochs(0, 40, 42, M.EPOCH_NODE_EXACT)
+    sh.run_epochs(0, 40, 42, M.EPOCH_NODE_EXACT)
+    assert np.array_equal(sg.read(M.BUF_EMBD), sh.read(M.BUF_EMBD))
+
+
+def test_graph_from_coo_device_pointers(ctx):
+    import torch
+
+    idx, dist = _knn(seed=8)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=200)
+    r, c, v = g.coo()
+    h = M.FuzzyGraph.from_coo(ctx, g.n, torch.from_numpy(r).cuda(), torch.from_numpy(c).cuda(),
+                              torch.from_numpy(v).cuda(), n_epochs=200)
+    torch.cuda.synchronize()
+    r2, c2, v2 = h.coo()
+    assert np.array_equal(r2, r) and np.array_equal(c2, c) and np.array_equal(v2, v)
+
+
+def test_graph_from_coo_filters_with_its_epochs(ctx):
+    # a graph filtered for 500 epochs, imported for 50: filter_weak_edges again
+    idx, dist = _knn(seed=9)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r, c, v = g.coo()
+    h = M.FuzzyGraph.from_coo(ctx, g.n, r, c, v, n_epochs=50)
+    wr, wc, wv = orc.filter_weak_edges(r, c, v, 50)
+    r2, c2, v2 = h.coo()
+    assert np.array_equal(r2, wr) and np.array_equal(c2, wc) and np.array_equal(v2, wv)
+
+
+def test_graph_from_coo_rejects_bad_lists(ctx):
+    idx, dist = _knn(n=600, seed=10)
+    g = M.FuzzyGraph(ctx, idx, dist, n_epochs=500)
+    r, c, v = g.coo()
+    n = g.n
+
+    def refused(rr, cc, vv, status=M.MRS_INVALID_ARGUMENT):
+        with pytest.raises(M.ManifoldsError) as e:
+            M.FuzzyGraph.from_coo(ctx, n, rr, cc, vv)
+        assert e.value.status == status, e.value
+
+    refused(r[1:], c[1:], v[1:])                       # (r, c) without (c, r)
+    refused(np.r_[r, r[:1]], np.r_[c, c[:1]], np.r_[v, v[:1]])  # repeated entry
+    refused(np.r_[r, 5], np.r_[c, 5], np.r_[v, 1.0])   # diagonal
+    bad = c.copy()
+    bad[0] = n                                          # out of range
+    refused(r, bad, v)
+    with pytest.raises(M.ManifoldsError) as e:
+        M.FuzzyGraph.from_coo(ctx, 0, r[:0], c[:0], v[:0])
+    assert e.value.status == M.MRS_NO_DATA
+    h = M.FuzzyGraph.from_coo(ctx, n, r, c, v)
+    with pytest.raises(M.ManifoldsError):
+        h.sigma_rho()
