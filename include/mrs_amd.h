@@ -104,6 +104,14 @@ int mrs_ctx_destroy(mrs_ctx *ctx);
 int mrs_ctx_synchronize(mrs_ctx *ctx);
 /* the context's HIP stream (hipStream_t) so callers can order their own work */
 void *mrs_ctx_stream(mrs_ctx *ctx);
+/* Device memory this context keeps for reuse: every buffer a call allocates
+ * comes from the context's cache and returns to it, so repeated fits of one
+ * shape allocate nothing after the first.  mrs_ctx_cached_bytes reports the
+ * cache's size; mrs_ctx_release_cache waits for the device and returns the
+ * cache to it (buffers held by live graphs and states are not affected).  No
+ * counterpart in the reference. */
+size_t mrs_ctx_cached_bytes(mrs_ctx *ctx);
+int mrs_ctx_release_cache(mrs_ctx *ctx);
 /* Exhaustive kNN on this context skips database stages that provably hold no
  * candidate (on = 1, the default) or sweeps every stage (on = 0).  Both give
  * the same results; the switch exists for comparisons.  No counterpart in the

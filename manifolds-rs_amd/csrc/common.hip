@@ -380,6 +380,18 @@ int mrs_ctx_synchronize(mrs_ctx* ctx) {
 
 void* mrs_ctx_stream(mrs_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+size_t mrs_ctx_cached_bytes(mrs_ctx* ctx) {
+    return ctx && ctx->pool ? ctx->pool->cached_bytes() : 0;
+}
+
+int mrs_ctx_release_cache(mrs_ctx* ctx) {
+    return mrs::guarded([&] {
+        if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
+        MRS_HIP(hipSetDevice(ctx->device));
+        if (ctx->pool) ctx->pool->trim();
+    });
+}
+
 int mrs_ctx_set_knn_pruning(mrs_ctx* ctx, int32_t on) {
     return mrs::guarded([&] {
         if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");

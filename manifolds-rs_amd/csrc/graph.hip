@@ -385,40 +385,53 @@ __global__ __launch_bounds__(kWg) void k_periods(const float* __restrict__ w, ui
 // ------------------------------------------------------------ DensState
 
 // edge_distances_from_knn + the row-wise f64 accumulation of original_log_radii
-// (src/utils/density.rs:309-398), one thread per row in column order.
+// (src/utils/density.rs:309-398).  kDensLanes lanes per row: lane l takes the
+// row's entries l, l + kDensLanes, ... and the lanes' f64 sums meet in a fixed
+// xor tree, so the result does not depend on scheduling.  An entry's distance
+// is looked up in row i's kNN list, else in row j's.
+constexpr int kDensLanes = 16;
 __global__ __launch_bounds__(kWg) void k_dens_rows(
     const uint32_t* __restrict__ u_off, const uint32_t* __restrict__ u_col,
     const float* __restrict__ u_w, const uint32_t* __restrict__ keep,
     const int32_t* __restrict__ knn_idx, const float* __restrict__ knn_dist, int k, uint32_t n,
     int squared, double* __restrict__ ro, float* __restrict__ mu_sum) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = (blockIdx.x * kWg + threadIdx.x) / kDensLanes;
+    const uint32_t sub = threadIdx.x & (kDensLanes - 1);
     double acc = 0.0, ms = 0.0;
-    for (uint32_t p = u_off[i]; p < u_off[i + 1]; p++) {
-        if (!keep[p]) continue;
-        const uint32_t j = u_col[p];
-        float raw = 0.0f;
-        bool found = false;
-        for (int t = 0; t < k; t++)
-            if ((uint32_t)knn_idx[(size_t)i * k + t] == j) {
-                raw = knn_dist[(size_t)i * k + t];
-                found = true;
-                break;
-            }
-        if (!found)
+    if (i < n) {
+        for (uint32_t p = u_off[i] + sub; p < u_off[i + 1]; p += kDensLanes) {
+            if (!keep[p]) continue;
+            const uint32_t j = u_col[p];
+            float raw = 0.0f;
+            bool found = false;
             for (int t = 0; t < k; t++)
-                if ((uint32_t)knn_idx[(size_t)j * k + t] == i) {
-                    raw = knn_dist[(size_t)j * k + t];
+                if ((uint32_t)knn_idx[(size_t)i * k + t] == j) {
+                    raw = knn_dist[(size_t)i * k + t];
+                    found = true;
                     break;
                 }
-        const float dsq = squared ? raw : raw * raw;
-        const double mu = (double)u_w[p];
-        acc += mu * (double)dsq;
-        ms += mu;
+            if (!found)
+                for (int t = 0; t < k; t++)
+                    if ((uint32_t)knn_idx[(size_t)j * k + t] == i) {
+                        raw = knn_dist[(size_t)j * k + t];
+                        break;
+                    }
+            const float dsq = squared ? raw : raw * raw;
+            const double mu = (double)u_w[p];
+            acc += mu * (double)dsq;
+            ms += mu;
+        }
     }
-    const double den = ms > DBL_EPSILON ? ms : DBL_EPSILON;
-    ro[i] = log(1e-8 + acc / den);
-    mu_sum[i] = (float)ms;
+#pragma unroll
+    for (int o = kDensLanes / 2; o >= 1; o >>= 1) {
+        acc += __shfl_xor(acc, o, kDensLanes);
+        ms += __shfl_xor(ms, o, kDensLanes);
+    }
+    if (i < n && sub == 0) {
+        const double den = ms > DBL_EPSILON ? ms : DBL_EPSILON;
+        ro[i] = log(1e-8 + acc / den);
+        mu_sum[i] = (float)ms;
+    }
 }
 
 constexpr int kRedBlocks = 256;
@@ -794,7 +807,7 @@ float dens_state_on_device(mrs_graph* g, const int32_t* knn_idx, const float* kn
     if (n == 0) throw Error(MRS_NO_DATA, "densMAP: empty graph (NoData)");
     hipStream_t s = g->ctx->stream;
     DevBuf<double> ro(n), partials(kRedBlocks), scal(3);
-    hipLaunchKernelGGL(k_dens_rows, dim3(div_up(n, kWg)), dim3(kWg), 0, s, g->u_off.p,
+    hipLaunchKernelGGL(k_dens_rows, dim3(div_up((uint64_t)n * kDensLanes, kWg)), dim3(kWg), 0, s, g->u_off.p,
                        g->u_col.p, g->u_w.p, g->u_keep.p, knn_idx, knn_dist, g->k, n,
                        distances_squared ? 1 : 0, ro.p, mu_sum_dev);
     MRS_CHECK_LAUNCH();

@@ -148,6 +148,8 @@ def lib() -> C.CDLL:
         "mrs_ctx_synchronize": (C.c_int, [vp]),
         "mrs_ctx_stream": (vp, [vp]),
         "mrs_ctx_set_knn_pruning": (C.c_int, [vp, i32]),
+        "mrs_ctx_cached_bytes": (C.c_size_t, [vp]),
+        "mrs_ctx_release_cache": (C.c_int, [vp]),
         "mrs_parse_umap_optimiser_gpu": (i32, [C.c_char_p]),
         "mrs_parse_ann_search_gpu": (i32, [C.c_char_p]),
         "mrs_hash_neg": (u32, [u32, u32, u32, u32, u32, u32]),
@@ -253,6 +255,14 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().mrs_ctx_stream(self.handle)
+
+    def cached_bytes(self) -> int:
+        """Device memory the context keeps for reuse by later calls."""
+        return lib().mrs_ctx_cached_bytes(self.handle)
+
+    def release_cache(self) -> None:
+        """Wait for the device and return the context's cached memory to it."""
+        _check(lib().mrs_ctx_release_cache(self.handle))
 
     def set_knn_pruning(self, on: bool) -> None:
         """Exhaustive kNN skips database stages that provably hold no candidate

@@ -352,3 +352,21 @@ def test_fit_errors(ctx):
     with pytest.raises(M.ManifoldsError) as e:
         M.umap_gpu(ctx, np.zeros((0, 3), np.float32), params())
     assert e.value.status == M.MRS_NO_DATA
+
+
+def test_context_cache_reuses_and_releases(ctx):
+    # a second fit of one shape takes its buffers from the context's cache;
+    # releasing the cache empties it and later fits still work
+    x, _ = diagnostic_data(80, 8, 17)
+    p = params()
+    p.optim_params.n_epochs = 30
+    ctx.release_cache()
+    a = M.umap_gpu(ctx, x, p, seed=42)
+    cached = ctx.cached_bytes()
+    assert cached > 0  # the fit's buffers came back to the cache
+    b = M.umap_gpu(ctx, x, p, seed=42)
+    assert ctx.cached_bytes() <= 2 * cached  # the second fit reused them
+    ctx.release_cache()
+    assert ctx.cached_bytes() == 0
+    c = M.umap_gpu(ctx, x, p, seed=42)
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)) and np.all(np.isfinite(c))
