@@ -736,7 +736,10 @@ __global__ __launch_bounds__(256) void k_assign_cells(const void* __restrict__ P
                                                       uint32_t n, const u16* __restrict__ seeds,
                                                       const float* __restrict__ snegh,
                                                       uint32_t mpad, uint32_t* __restrict__ cell,
-                                                      uint32_t* __restrict__ count) {
+                                                      uint32_t* __restrict__ count,
+                                                      const float* __restrict__ negh,
+                                                      float* __restrict__ dist2,
+                                                      double* __restrict__ cell_d2) {
     const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
     const uint32_t i = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32 + col;  // this lane's row
     const int steps = KS / 16;
@@ -786,7 +789,26 @@ __global__ __launch_bounds__(256) void k_assign_cells(const void* __restrict__ P
     if (h == 0 && i < n) {
         cell[i] = bi;
         atomicAdd(&count[bi], 1u);
+        // squared distance to the seed, |p|^2 - 2 score (BF16 scores: a rank, not a distance)
+        const float d2 = fmaxf(-2.0f * negh[i] - 2.0f * fmaxf(best, ob), 0.0f);
+        dist2[i] = d2;
+        atomicAdd(&cell_d2[bi], (double)d2);
     }
+}
+
+// Sort key of row i: its cell, then 1 for a far row (squared distance to the
+// seed above twice the cell's mean), so each cell's far rows (outliers,
+// uniform background) go to its last blocks and its other blocks stay compact.
+__global__ __launch_bounds__(256) void k_cell_keys(const uint32_t* __restrict__ cell,
+                                                   const float* __restrict__ dist2,
+                                                   const double* __restrict__ cell_d2,
+                                                   const uint32_t* __restrict__ count, uint32_t n,
+                                                   uint32_t* __restrict__ key) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cell[i];
+    const double mean = cell_d2[c] / (double)count[c];
+    key[i] = (c << 1) | ((double)dist2[i] > 2.0 * mean ? 1u : 0u);
 }
 
 // position of sorted row r: its cell's padded start plus its rank in the cell
@@ -798,7 +820,7 @@ __global__ __launch_bounds__(256) void k_cell_layout(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ posmap) {
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
     if (r >= n) return;
-    const uint32_t c = cell_sorted[r];
+    const uint32_t c = cell_sorted[r] >> 1;  // sort key: cell, then far rows last
     const uint32_t p = pstart[c] + (r - start[c]);
     const uint32_t i = order[r];
     rowmap[p] = i;
@@ -1233,24 +1255,30 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         const uint32_t mpad = (m + 31) / 32 * 32;
         DevBuf<u16> seeds((size_t)mpad * KS);
         DevBuf<float> snegh(mpad);
-        DevBuf<uint32_t> cell(n), ccount(mpad), csorted(n), order(n);
+        DevBuf<uint32_t> cell(n), ccount(mpad), csorted(n), order(n), ckey(n);
+        DevBuf<float> dist2(n);
+        DevBuf<double> cell_d2(mpad);
         MRS_HIP(hipMemsetAsync(ccount.p, 0, ccount.bytes(), s));
+        MRS_HIP(hipMemsetAsync(cell_d2.p, 0, cell_d2.bytes(), s));
         const void* pp = P.p;
         const dim3 ga(div_up(n, 128));
         if (bf16) {
             hipLaunchKernelGGL(k_seed_rows<true>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
                                seeds.p, snegh.p);
             hipLaunchKernelGGL(k_assign_cells<true>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
-                               snegh.p, mpad, cell.p, ccount.p);
+                               snegh.p, mpad, cell.p, ccount.p, negh.p, dist2.p, cell_d2.p);
         } else {
             hipLaunchKernelGGL(k_seed_rows<false>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
                                seeds.p, snegh.p);
             hipLaunchKernelGGL(k_assign_cells<false>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
-                               snegh.p, mpad, cell.p, ccount.p);
+                               snegh.p, mpad, cell.p, ccount.p, negh.p, dist2.p, cell_d2.p);
         }
         MRS_CHECK_LAUNCH();
-        radix_sort_pairs_u32(cell.p, nullptr, n, m > 1 ? 32 - __builtin_clz(m - 1) : 1, csorted.p,
-                             order.p, s);
+        hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, s, cell.p, dist2.p,
+                           cell_d2.p, ccount.p, n, ckey.p);
+        MRS_CHECK_LAUNCH();
+        radix_sort_pairs_u32(ckey.p, nullptr, n, (m > 1 ? 32 - __builtin_clz(m - 1) : 1) + 1,
+                             csorted.p, order.p, s);
         std::vector<uint32_t> hc(m), start(m + 1, 0), pstart(m + 1, 0);
         MRS_HIP(hipMemcpyAsync(hc.data(), ccount.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, s));
         MRS_HIP(hipStreamSynchronize(s));
