@@ -165,11 +165,12 @@ int mrs_last_fit_timings(double *out);
  * shortlist of max(shortlist, k) rounded up to 64 or 128 candidates). */
 int mrs_knn_exhaustive(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                        int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
-/* Same, returning in ms_out[0..6]: the stage times in ms of pack (with the
+/* Same, returning in ms_out[0..8]: the stage times in ms of pack (with the
  * cell layout when pruning), shortlist (every pass), merge+re-rank; database
  * splits used; shortlist size; the fraction of (query block, database stage)
- * pairs swept (1 without pruning); cells (0 without pruning).  No counterpart
- * in the reference; used by the benchmark. */
+ * pairs swept (1 without pruning); cells (0 without pruning); the longest and
+ * the mean second-pass stage list of a query block.  No counterpart in the
+ * reference; used by the benchmark. */
 int mrs_knn_exhaustive_timed(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                              int32_t precision, int32_t shortlist, int32_t *idx, float *dist,
                              double *ms_out);

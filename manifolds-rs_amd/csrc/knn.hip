@@ -1211,8 +1211,8 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         MRS_CHECK_LAUNCH();
     };
     int splits = 1;
-    double swept = 1.0;
-    uint32_t cells = 0;
+    double swept = 1.0, list_mean = 0.0;
+    uint32_t cells = 0, list_max = 0;
 
     const bool prune = ctx->knn_prune && n >= kPruneMinRows && sh.W * kQPerWave == (int)kBlockRows;
     if (!prune) {
@@ -1350,8 +1350,9 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.rowmap = rowmap.p;
         // the order in which a pass starts its query blocks: longest stage
         // list first, so the last workgroups to start are short ones
+        std::vector<uint32_t> hcnt(nblk);  // the last pass's list lengths
         auto longest_first = [&](const uint32_t* dcounts, uint32_t* dord) {
-            std::vector<uint32_t> hcnt(nblk), ord(nblk);
+            std::vector<uint32_t> ord(nblk);
             MRS_HIP(hipMemcpyAsync(hcnt.data(), dcounts, sizeof(uint32_t) * nblk,
                                    hipMemcpyDeviceToHost, s));
             MRS_HIP(hipStreamSynchronize(s));
@@ -1421,6 +1422,8 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
         const double full = (double)(pstart[m] / kBlockRows) * (double)(pstart[m] / (uint32_t)sh.RT);
         swept = full > 0 ? ((double)pairs1 + (double)total) / full : 1.0;
+        list_max = hcnt.empty() ? 0 : *std::max_element(hcnt.begin(), hcnt.end());
+        list_mean = nblk ? (double)total / nblk : 0.0;
     }
     if (timings) {
         float a = 0, b = 0, c = 0;
@@ -1434,6 +1437,8 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         timings->shortlist = S;
         timings->swept = swept;
         timings->cells = (int)cells;
+        timings->list_max = list_max;
+        timings->list_mean = list_mean;
     }
 }
 
@@ -1480,6 +1485,8 @@ int mrs_knn_exhaustive_timed(mrs_ctx* ctx, const float* x, int64_t n, int32_t d,
             ms_out[4] = t.shortlist;
             ms_out[5] = t.swept;
             ms_out[6] = t.cells;
+            ms_out[7] = t.list_max;
+            ms_out[8] = t.list_mean;
         }
     });
 }

@@ -10,6 +10,8 @@ struct KnnTimings {
     int splits = 0, shortlist = 0;
     double swept = 1.0;  // fraction of (query block, stage) pairs swept
     int cells = 0;       // cells of the pruned sweep (0: full sweep)
+    uint32_t list_max = 0;   // longest pass-2 stage list of a query block
+    double list_mean = 0.0;  // mean pass-2 stage list
 };
 
 // x: device [n][d] f32; out_idx/out_dist: device [n][k].

@@ -323,12 +323,12 @@ def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist
         idx = torch.empty((n, k), dtype=torch.int32, device=x.device)
         dist = torch.empty((n, k), dtype=torch.float32, device=x.device)
     if timed:
-        ms = (C.c_double * 7)()
+        ms = (C.c_double * 9)()
         _check(lib().mrs_knn_exhaustive_timed(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                               _ptr(idx), _ptr(dist), ms))
         return idx, dist, dict(pack_ms=ms[0], shortlist_ms=ms[1], rerank_ms=ms[2],
                                splits=int(ms[3]), shortlist=int(ms[4]), swept=ms[5],
-                               cells=int(ms[6]))
+                               cells=int(ms[6]), list_max=int(ms[7]), list_mean=ms[8])
     _check(lib().mrs_knn_exhaustive(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                     _ptr(idx), _ptr(dist)))
     return idx, dist
