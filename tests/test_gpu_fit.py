@@ -370,3 +370,20 @@ def test_context_cache_reuses_and_releases(ctx):
     assert ctx.cached_bytes() == 0
     c = M.umap_gpu(ctx, x, p, seed=42)
     assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)) and np.all(np.isfinite(c))
+
+
+def test_fit_same_with_and_without_knn_pruning(ctx):
+    # n >= 65 536 takes the pruned kNN sweep: the whole fit (graph, spectral
+    # layout, deterministic epochs) is the same as with the full sweep
+    import synthetic
+
+    x, _ = synthetic.subspace(n=70_000, d=32, n_clusters=30, seed=23)
+    p = params(epoch_strategy=M.EPOCH_NODE_EXACT)
+    p.optim_params.n_epochs = 20
+    a = M.umap_gpu(ctx, x, p, seed=42)
+    ctx.set_knn_pruning(False)
+    try:
+        b = M.umap_gpu(ctx, x, p, seed=42)
+    finally:
+        ctx.set_knn_pruning(True)
+    assert np.array_equal(a, b)

@@ -143,3 +143,14 @@ def test_pruned_sweep_wide_rows(ctx):
 
     x, _ = synthetic.subspace(n=66_000, d=100, n_clusters=40, seed=13)
     _check_pruned(ctx, x, 15, M.KNN_BF16_RERANK, shortlist=60)
+
+
+def test_pruned_sweep_two_dimensional(ctx):
+    # d = 2, dense 2-D blobs: neighbours lie ~0.05 apart at coordinates up to
+    # 50, below BF16's resolution there (8 significant bits: steps of 0.25), so
+    # this runs in FP32, whose shortlist is exact; the sweep is the same code
+    rng = np.random.default_rng(17)
+    centres = rng.uniform(-50, 50, size=(40, 2))
+    x = (centres[rng.integers(0, 40, size=67_000)] + rng.normal(size=(67_000, 2))).astype(np.float32)
+    t = _check_pruned(ctx, x, 15, M.KNN_FP32)
+    assert t["swept"] < 0.5, t
