@@ -218,16 +218,21 @@ struct ShortlistArgs {
     uint32_t db_per_split;  // multiple of kSplitAlign
     // Pruned sweep (one split): rows are in cell order, rowmap[p] is the
     // caller's row at position p (list keys carry caller rows), and workgroup
-    // b sweeps the blk[b].y stages stage_ids[blk[b].x + t].  cont: continue
-    // from the lists of an earlier pass.
+    // g does work item items[g] = (query block b, list offset o, count c, aux
+    // slot a): it sweeps the c stages stage_ids[o + t] for block b.  With
+    // a == kMainList it keeps the block's own lists (continuing them when
+    // cont); otherwise it fills the aux lists of slot a from empty, above the
+    // floor thresholds (the block's thresholds when the pass started), and
+    // k_merge_aux later folds them into the block's lists.
     const uint32_t* rowmap;
-    const uint2* blk;
+    const uint4* items;
     const uint32_t* stage_ids;
     int cont;
-    // workgroup g sweeps query block order[g] (null: block g); the longest
-    // lists go first, so the last workgroups to start are short ones
-    const uint32_t* order;
+    uint64_t* aux_lists;   // [slots][128][S]
+    uint64_t* aux_queue;   // [slots][128][QC]
+    const float* floor;    // [npad] or null
 };
+constexpr uint32_t kMainList = 0xFFFFFFFFu;
 
 // Wave-wide merge of query b's queue (c entries) into its sorted global list;
 // returns the list's new threshold key (its last entry).
@@ -253,7 +258,7 @@ __device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __
 
 // Epilogue of one 32-row MFMA tile: the lane's 16 scores (one query, rows
 // rowbase + (r/4)*8 + h*4 + r%4) against the query's threshold thr (the score
-// of its list's last entry).  A query's queue count qn and threshold live in
+// of its list's last entry, never below floor).  A query's queue count qn and threshold live in
 // registers, identical in its two lanes (h = 0, 1); the halves learn each
 // other's insert count with one lane swap, so queueing costs no LDS round
 // trip.  Most tiles leave on the max/ballot test.  A queue that could overflow
@@ -266,7 +271,8 @@ __device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __
 template <int E, int QC, bool PR>
 __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h, int col,
                                 int lane, float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
-                                uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap) {
+                                uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap,
+                                float floor) {
     float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
 #pragma unroll
     for (int g = 0; g < 4; g++)
@@ -307,7 +313,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h,
         const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
         if (col == b) {
             qn = 0;
-            thr = key_score(t);
+            thr = fmaxf(key_score(t), floor);  // an aux list never goes below its floor
         }
     }
 }
@@ -370,15 +376,23 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     const int tid = threadIdx.x;
     const int lane = tid & 63, h = lane >> 5, col = lane & 31;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const uint32_t blk_id = (PR && a.order) ? a.order[blockIdx.x] : blockIdx.x;
+    uint4 item = make_uint4(blockIdx.x, 0u, 0u, kMainList);
+    if (PR) item = a.items[blockIdx.x];
+    const uint32_t blk_id = item.x;
+    const bool aux = PR && item.w != kMainList;
     const uint32_t qbase = (blk_id * W + wave) * kQPerWave;
     const uint32_t split = blockIdx.y;
     const uint32_t db0 = split * a.db_per_split;
     const uint32_t db1 = min(db0 + a.db_per_split, a.npad);
     uint64_t* lists = a.lists + ((size_t)split * a.npad + qbase) * S;
     uint64_t* myq = a.queue + ((size_t)split * a.npad + qbase) * QC;
+    if (aux) {
+        const size_t r0 = (size_t)item.w * (W * kQPerWave) + (size_t)wave * kQPerWave;
+        lists = a.aux_lists + r0 * S;
+        myq = a.aux_queue + r0 * QC;
+    }
 
-    if (!PR || !a.cont)
+    if (!PR || !a.cont || aux)
         for (int i = lane; i < kQPerWave * S; i += 64) lists[i] = kInfKey;
     // B fragments: this lane's query (column) over its half of K
     const uint32_t q = qbase + col;
@@ -398,8 +412,11 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
         for (int s = 0; s < MAXSTEPS; s++)
             if (s < steps) bqf[s] = src[s];
     }
+    // a pass that continues: the thresholds left by the earlier pass (an aux
+    // item starts from the floor, its lists being empty)
+    const float floor = (PR && a.floor) ? a.floor[q] : -INFINITY;
     float thr = -INFINITY;
-    if (PR && a.cont) thr = key_score(lists[(size_t)col * S + S - 1]);  // left by the earlier pass
+    if (PR && a.cont) thr = aux ? floor : key_score(lists[(size_t)col * S + S - 1]);
     uint32_t qn = 0;
 
     const unsigned char* P = reinterpret_cast<const unsigned char*>(a.P);
@@ -411,9 +428,8 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     uint32_t nst = (db1 - db0) / RT;
     const uint32_t* sid = nullptr;
     if (PR) {
-        const uint2 bl = a.blk[blk_id];
-        nst = bl.y;
-        sid = a.stage_ids + bl.x;
+        nst = item.z;
+        sid = a.stage_ids + item.y;
     }
     auto srow = [&](uint32_t t) -> uint32_t { return PR ? sid[t] * (uint32_t)RT : db0 + t * RT; };
     u32x4 R[CPT];
@@ -505,10 +521,11 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
             }
-            epilogue<E, QC, PR>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq, a.rowmap);
+            epilogue<E, QC, PR>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq, a.rowmap,
+                                floor);
             if (kPair)
                 epilogue<E, QC, PR>(acc1, r0 + (sp + 1) * 32, q, h, col, lane, thr, qn, lists, myq,
-                                    a.rowmap);
+                                    a.rowmap, floor);
         }
         __syncthreads();
     }
@@ -1038,6 +1055,41 @@ __global__ __launch_bounds__(256) void k_stage_lists(
     }
 }
 
+// floor[q] = the score of query position q's S-th list entry (a pass's start)
+__global__ __launch_bounds__(256) void k_list_floor(const uint64_t* __restrict__ lists, int S,
+                                                    uint32_t npos, float* __restrict__ floor) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q < npos) floor[q] = key_score(lists[(size_t)q * S + S - 1]);
+}
+
+// Fold the aux lists of split work items into their block's lists: merges[m]
+// = (block, first aux slot, aux count); one wavefront per query of the block,
+// merging the slots in order (disjoint candidates, so the result is the S
+// smallest keys of the union).
+template <int E>
+__global__ __launch_bounds__(256) void k_merge_aux(uint64_t* __restrict__ lists,
+                                                   const uint64_t* __restrict__ aux_lists,
+                                                   const uint4* __restrict__ merges,
+                                                   uint32_t n_merges) {
+    constexpr int S = 64 * E;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= n_merges * kBlockRows) return;  // whole wavefronts
+    const uint4 mg = merges[w / kBlockRows];
+    const uint32_t j = w % kBlockRows;
+    uint64_t* dst = lists + ((size_t)mg.x * kBlockRows + j) * S;
+    uint64_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = dst[e * 64 + lane];
+    for (uint32_t a = 0; a < mg.z; a++) {
+        const uint64_t* src = aux_lists + ((size_t)(mg.y + a) * kBlockRows + j) * S;
+#pragma unroll
+        for (int e = 0; e < E; e++) wave_merge<E>(v, src[e * 64 + lane]);
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) dst[e * 64 + lane] = v[e];
+}
+
 __global__ __launch_bounds__(256) void k_range_counts(const uint2* __restrict__ range,
                                                       uint32_t nblk, uint32_t* __restrict__ counts) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
@@ -1053,12 +1105,6 @@ __global__ __launch_bounds__(256) void k_range_ids(const uint2* __restrict__ ran
     for (uint32_t t = threadIdx.x; t < r.y; t += 256) ids[o + t] = r.x + t;
 }
 
-__global__ __launch_bounds__(256) void k_make_blk(const uint32_t* __restrict__ offs,
-                                                  const uint32_t* __restrict__ counts,
-                                                  uint32_t nblk, uint2* __restrict__ blk) {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b < nblk) blk[b] = make_uint2(offs[b], counts[b]);
-}
 
 size_t rerank_lds_bytes(int d) {
     return (size_t)kRerankWaves * (64 * (d + 1) + d) * sizeof(float);
@@ -1332,7 +1378,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
             snorm = snorm_own.p;
             scnt = scnt_own.p;
         }
-        DevBuf<uint2> range(nblk), blk2(nblk);
+        DevBuf<uint2> range(nblk);
         hipLaunchKernelGGL(k_block_ranges, dim3(div_up(nblk, 256)), dim3(256), 0, s, dpstart.p, m,
                            nblk, (uint32_t)sh.RT, range.p);
         MRS_CHECK_LAUNCH();
@@ -1348,38 +1394,64 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.KP = KP;
         sa.db_per_split = npos;
         sa.rowmap = rowmap.p;
-        // the order in which a pass starts its query blocks: longest stage
-        // list first, so the last workgroups to start are short ones
-        std::vector<uint32_t> hcnt(nblk);  // the last pass's list lengths
-        auto longest_first = [&](const uint32_t* dcounts, uint32_t* dord) {
-            std::vector<uint32_t> ord(nblk);
+        // Work items of a pass: (block, list offset, count, aux slot).  A block
+        // whose list is longer than chunk stages is split over several items
+        // (the first keeps the block's lists, the others fill aux lists that
+        // k_merge_aux folds in), and items start longest first, so the last
+        // workgroups to start are short ones.
+        std::vector<uint32_t> hcnt(nblk), hoff(nblk);
+        auto fetch_counts = [&](const uint32_t* dcounts) {
             MRS_HIP(hipMemcpyAsync(hcnt.data(), dcounts, sizeof(uint32_t) * nblk,
                                    hipMemcpyDeviceToHost, s));
             MRS_HIP(hipStreamSynchronize(s));
-            std::iota(ord.begin(), ord.end(), 0u);
-            std::stable_sort(ord.begin(), ord.end(),
-                             [&](uint32_t a_, uint32_t b_) { return hcnt[a_] > hcnt[b_]; });
-            MRS_HIP(hipMemcpyAsync(dord, ord.data(), sizeof(uint32_t) * nblk,
-                                   hipMemcpyHostToDevice, s));
-            MRS_HIP(hipStreamSynchronize(s));  // ord is read by the copy
+            uint32_t acc = 0;  // the exclusive scan the device made of the same counts
+            for (uint32_t b = 0; b < nblk; b++) {
+                hoff[b] = acc;
+                acc += hcnt[b];
+            }
         };
-        // pass 1: each query block's own cell, as an explicit stage list
-        DevBuf<uint32_t> counts1(nblk), offs1((size_t)nblk + 1), ids1(pairs1 ? pairs1 : 1), dorder1(nblk);
-        DevBuf<uint2> blk1(nblk);
+        std::vector<uint4> hitems, hmerges;
+        uint32_t n_aux = 0;
+        auto make_items = [&](uint32_t chunk) {
+            hitems.clear();
+            hmerges.clear();
+            n_aux = 0;
+            for (uint32_t b = 0; b < nblk; b++) {
+                const uint32_t c = hcnt[b];
+                if (c == 0) continue;
+                const uint32_t parts = (uint32_t)(((uint64_t)c + chunk - 1) / chunk);
+                if (parts > 1) hmerges.push_back(make_uint4(b, n_aux, parts - 1, 0u));
+                for (uint32_t j = 0; j < parts; j++) {
+                    const uint32_t len = std::min(chunk, c - j * chunk);
+                    hitems.push_back(make_uint4(b, hoff[b] + j * chunk, len, j ? n_aux++ : kMainList));
+                }
+            }
+            std::stable_sort(hitems.begin(), hitems.end(),
+                             [](const uint4& x_, const uint4& y_) { return x_.z > y_.z; });
+        };
+        DevBuf<uint4> items1, items2, dmerges;
+        auto upload = [&](DevBuf<uint4>& dst, const std::vector<uint4>& src) {
+            dst.alloc(src.size() ? src.size() : 1);
+            if (!src.empty())
+                MRS_HIP(hipMemcpyAsync(dst.p, src.data(), sizeof(uint4) * src.size(),
+                                       hipMemcpyHostToDevice, s));
+        };
+        // pass 1: each query block's own cell, as an explicit stage list, one item per block
+        DevBuf<uint32_t> counts1(nblk), offs1((size_t)nblk + 1), ids1(pairs1 ? pairs1 : 1);
         hipLaunchKernelGGL(k_range_counts, dim3(div_up(nblk, 256)), dim3(256), 0, s, range.p, nblk,
                            counts1.p);
         MRS_CHECK_LAUNCH();
         exclusive_scan_u32(counts1.p, offs1.p, nblk, offs1.p + nblk, s);
         hipLaunchKernelGGL(k_range_ids, dim3(nblk), dim3(256), 0, s, range.p, offs1.p, ids1.p);
-        hipLaunchKernelGGL(k_make_blk, dim3(div_up(nblk, 256)), dim3(256), 0, s, offs1.p,
-                           counts1.p, nblk, blk1.p);
         MRS_CHECK_LAUNCH();
-        longest_first(counts1.p, dorder1.p);
-        sa.blk = blk1.p;
+        fetch_counts(counts1.p);
+        make_items(0xFFFFFFFFu);
+        upload(items1, hitems);
+        sa.items = items1.p;
         sa.stage_ids = ids1.p;
-        sa.order = dorder1.p;
+        MRS_HIP(hipStreamSynchronize(s));  // hitems is read by the copy
         MRS_HIP(hipEventRecord(ev[1], s));
-        run_shortlist(sa, dim3(nblk, 1), true);
+        if (!hitems.empty()) run_shortlist(sa, dim3((unsigned)hitems.size(), 1), true);
         // pass 2: the stages the bounds cannot rule out
         DevBuf<uint32_t> counts(nblk), offs((size_t)nblk + 1);
         const dim3 gl(div_up(nblk, kListBlocks));
@@ -1388,22 +1460,42 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                            nst, counts.p, nullptr, nullptr);
         MRS_CHECK_LAUNCH();
         exclusive_scan_u32(counts.p, offs.p, nblk, offs.p + nblk, s);
-        uint32_t total = 0;
-        MRS_HIP(hipMemcpyAsync(&total, offs.p + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        MRS_HIP(hipStreamSynchronize(s));
-        DevBuf<uint32_t> ids(total ? total : 1), dorder(nblk);
-        longest_first(counts.p, dorder.p);
+        fetch_counts(counts.p);
+        const uint32_t total = nblk ? hoff[nblk - 1] + hcnt[nblk - 1] : 0;
+        DevBuf<uint32_t> ids(total ? total : 1);
         hipLaunchKernelGGL(k_stage_lists<true>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
                            bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
                            nst, counts.p, offs.p, ids.p);
-        hipLaunchKernelGGL(k_make_blk, dim3(div_up(nblk, 256)), dim3(256), 0, s, offs.p, counts.p,
-                           nblk, blk2.p);
         MRS_CHECK_LAUNCH();
-        sa.blk = blk2.p;
+        // chunks: a tenth of a round of workgroups' share, at least 1024 stages
+        const uint32_t chunk = std::max<uint32_t>(1024u, (uint32_t)((uint64_t)total / 2048u));
+        make_items(chunk);
+        upload(items2, hitems);
+        upload(dmerges, hmerges);
+        DevBuf<uint64_t> aux_lists((size_t)std::max<uint32_t>(n_aux, 1u) * kBlockRows * S);
+        DevBuf<uint64_t> aux_queue((size_t)std::max<uint32_t>(n_aux, 1u) * kBlockRows * sh.QC);
+        DevBuf<float> floor(npos);
+        hipLaunchKernelGGL(k_list_floor, dim3(div_up(npos, 256)), dim3(256), 0, s, lists.p, S,
+                           npos, floor.p);
+        MRS_CHECK_LAUNCH();
+        MRS_HIP(hipStreamSynchronize(s));  // hitems and hmerges are read by the copies
+        sa.items = items2.p;
         sa.stage_ids = ids.p;
         sa.cont = 1;
-        sa.order = dorder.p;
-        run_shortlist(sa, dim3(nblk, 1), true);
+        sa.aux_lists = aux_lists.p;
+        sa.aux_queue = aux_queue.p;
+        sa.floor = floor.p;
+        if (!hitems.empty()) run_shortlist(sa, dim3((unsigned)hitems.size(), 1), true);
+        if (!hmerges.empty()) {
+            const dim3 gmg(div_up((uint64_t)hmerges.size() * kBlockRows, 4));
+            if (E == 1)
+                hipLaunchKernelGGL(k_merge_aux<1>, gmg, dim3(256), 0, s, lists.p, aux_lists.p,
+                                   dmerges.p, (uint32_t)hmerges.size());
+            else
+                hipLaunchKernelGGL(k_merge_aux<2>, gmg, dim3(256), 0, s, lists.p, aux_lists.p,
+                                   dmerges.p, (uint32_t)hmerges.size());
+            MRS_CHECK_LAUNCH();
+        }
         MRS_HIP(hipEventRecord(ev[2], s));
         // the re-rank in cell order, from X in cell order
         const int dp = (d + 3) / 4 * 4;
