@@ -79,6 +79,11 @@ __global__ __launch_bounds__(kWg) void k_cc_init(uint32_t* __restrict__ lab, uin
     if (i < n) lab[i] = i;
 }
 
+// One hooking sweep over the filtered COO entries in row order.  (Sweeping
+// only the undirected edge list, i < j, halves the gathers but took 2.2x as
+// long at 1M nodes: in row order an entry (i, j < i) pulls j's label into
+// row i, labels spread within a sweep, and most later entries see equal
+// labels and skip their atomic.)
 __global__ __launch_bounds__(kWg) void k_cc_hook(const uint32_t* __restrict__ row,
                                                  const uint32_t* __restrict__ col, uint64_t nnz,
                                                  uint32_t* __restrict__ lab,
@@ -151,57 +156,69 @@ __global__ __launch_bounds__(kWg) void k_inverse(const uint32_t* __restrict__ pe
     if (q < n) inv[perm[q]] = q;
 }
 
-// Degree of each vertex in component order: deg[q] = row length of perm[q].
+// Degree of each vertex in component order, deg[q] = row length of perm[q],
+// and its D^-1/2 in component order, dinv_p[q] = dinv[perm[q]].
 __global__ __launch_bounds__(kWg) void k_perm_degree(const uint32_t* __restrict__ off,
+                                                     const double* __restrict__ dinv,
                                                      const uint32_t* __restrict__ perm, uint32_t n,
-                                                     uint32_t* __restrict__ deg) {
+                                                     uint32_t* __restrict__ deg,
+                                                     double* __restrict__ dinv_p) {
     const uint32_t q = blockIdx.x * kWg + threadIdx.x;
     if (q >= n) return;
     const uint32_t i = perm[q];
     deg[q] = off[i + 1] - off[i];
+    dinv_p[q] = dinv[i];
 }
 
-// M in component order: row q is row perm[q] with its columns renamed through
-// inv (= perm^-1) and M's values mval = -d_i^-1/2 w d_j^-1/2 (0 on the
-// diagonal), entries kept in their row order.  Every Lanczos vector then lives
-// in component order, so a component's vertices are one contiguous range.
+constexpr int kRowLanes = 16;  // lanes per CSR row in the row kernels below
+
+// W in component order: row q is row perm[q] with its columns renamed through
+// inv (= perm^-1), its weights kept in FP32 and the diagonal set to 0, entries
+// in their row order.  Every Lanczos vector then lives in component order, so
+// a component's vertices are one contiguous range.  16 lanes per row, each
+// copying entries sub, sub + 16, ... of its row.
 __global__ __launch_bounds__(kWg) void k_perm_rows(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ col,
                                                    const float* __restrict__ val,
-                                                   const double* __restrict__ dinv,
                                                    const uint32_t* __restrict__ perm,
                                                    const uint32_t* __restrict__ inv,
                                                    const uint32_t* __restrict__ poff, uint32_t n,
                                                    uint32_t* __restrict__ pcol,
-                                                   double* __restrict__ pval) {
-    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+                                                   float* __restrict__ pw) {
+    const uint64_t t = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint64_t q = t / kRowLanes;
+    const uint32_t sub = (uint32_t)(t % kRowLanes);
     if (q >= n) return;
     const uint32_t i = perm[q];
-    const double di = dinv[i];
-    uint32_t o = poff[q];
-    for (uint32_t p = off[i]; p < off[i + 1]; p++, o++) {
+    const uint32_t b = off[i], e = off[i + 1], o = poff[q];
+    for (uint32_t p = b + sub; p < e; p += kRowLanes) {
         const uint32_t j = col[p];
-        pcol[o] = inv[j];
-        pval[o] = j == i ? 0.0 : -di * (double)val[p] * dinv[j];
+        pcol[o + (p - b)] = inv[j];
+        pw[o + (p - b)] = j == i ? 0.0f : val[p];
     }
 }
 
-// y = M x, 16 lanes per row (strided partial sums + a fixed xor tree: the
-// result does not depend on scheduling).
-constexpr int kRowLanes = 16;
+// y = M x for M = -D^-1/2 W D^-1/2 (zero diagonal), from u = D^-1/2 x:
+// y_q = -dinv_p[q] * sum_p w_p u[col_p].  The row streams 8 bytes per entry
+// (FP32 weight, column) where precomputed FP64 products of M would take 12.
+// 16 lanes per row, strided partial sums and a fixed xor tree: the result does
+// not depend on scheduling.
 __global__ __launch_bounds__(kWg) void k_spmv_rows(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ col,
-                                                   const double* __restrict__ mval,
-                                                   const double* __restrict__ x, uint32_t n,
+                                                   const float* __restrict__ w,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ dinv_p, uint32_t n,
                                                    double* __restrict__ y) {
-    const uint32_t row = (blockIdx.x * kWg + threadIdx.x) / kRowLanes;
-    const int sub = threadIdx.x & (kRowLanes - 1);
+    const uint64_t t = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint64_t row = t / kRowLanes;
+    const int sub = (int)(t % kRowLanes);
     double acc = 0.0;
     if (row < n)
-        for (uint32_t p = off[row] + sub; p < off[row + 1]; p += kRowLanes) acc += mval[p] * x[col[p]];
+        for (uint32_t p = off[row] + sub; p < off[row + 1]; p += kRowLanes)
+            acc += (double)w[p] * u[col[p]];
 #pragma unroll
     for (int o = kRowLanes / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
-    if (row < n && sub == 0) y[row] = acc;
+    if (row < n && sub == 0) y[row] = -dinv_p[row] * acc;
 }
 
 struct Chunk {
@@ -341,10 +358,12 @@ __global__ __launch_bounds__(kWg) void k_seg_finish(const double* __restrict__ p
     if (lane == 0) out[c] = do_sqrt ? sqrt(t) : t;
 }
 
-// v = w / beta_j and the next basis column V_{j+1} = v (live components)
-__global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double* __restrict__ vnext,
+// v_{j+1} = w / beta_j into the next basis column V_{j+1}, and the next
+// product's input u = D^-1/2 v_{j+1} (live components only)
+__global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ u, double* __restrict__ vnext,
                                                  const double* __restrict__ w,
                                                  const double* __restrict__ beta,
+                                                 const double* __restrict__ dinv_p,
                                                  const uint32_t* __restrict__ comp_of,
                                                  const uint8_t* __restrict__ live, uint32_t n) {
     uint32_t i = blockIdx.x * kWg + threadIdx.x;
@@ -352,8 +371,16 @@ __global__ __launch_bounds__(kWg) void k_advance(double* __restrict__ v, double*
     const uint32_t c = comp_of[i];
     if (c == 0xFFFFFFFFu || !live[c]) return;
     const double x = w[i] / beta[c];
-    v[i] = x;
+    u[i] = x * dinv_p[i];
     if (vnext) vnext[i] = x;
+}
+
+// u = D^-1/2 v (the first product's input; u may be v itself)
+__global__ __launch_bounds__(kWg) void k_mul_dinv(const double* v,
+                                                  const double* __restrict__ dinv_p, uint32_t n,
+                                                  double* u) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i < n) u[i] = v[i] * dinv_p[i];
 }
 
 // Start of Lanczos step j: a component past its iteration budget, or whose
@@ -865,17 +892,18 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(dc2l.p, comp_to_lc.data(), sizeof(uint32_t) * ncomp, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_fill_lcq, dim3(div_up(n, kWg)), dim3(kWg), 0, s, dstart.p, ncomp,
                            dc2l.p, n, dlc.p);
-        // M = -D^-1/2 W D^-1/2 in component order (rows keep their entry order,
-        // so every row sum below adds the same products in the same order)
-        hipLaunchKernelGGL(k_perm_degree, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, dperm.p,
-                           n, poff.p);
+        // W and D^-1/2 in component order for M = -D^-1/2 W D^-1/2 (rows keep
+        // their entry order, so every row sum adds its terms in a fixed order)
+        DevBuf<double> dinv_p(n);
+        hipLaunchKernelGGL(k_perm_degree, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p, dinv.p,
+                           dperm.p, n, poff.p, dinv_p.p);
         MRS_CHECK_LAUNCH();
         exclusive_scan_u32(poff.p, poff.p, n, poff.p + n, s);
         DevBuf<uint32_t> pcol(nnz ? nnz : 1);
-        DevBuf<double> mval(nnz ? nnz : 1);
-        hipLaunchKernelGGL(k_perm_rows, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p,
-                           g->coo_col.p, g->coo_val.p, dinv.p, dperm.p, dinvp.p, poff.p, n,
-                           pcol.p, mval.p);
+        DevBuf<float> pw(nnz ? nnz : 1);
+        hipLaunchKernelGGL(k_perm_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
+                           s, off.p, g->coo_col.p, g->coo_val.p, dperm.p, dinvp.p, poff.p, n,
+                           pcol.p, pw.p);
         MRS_CHECK_LAUNCH();
         DevBuf<double> v(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
             alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl),
@@ -897,6 +925,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         seg(v.p, v.p, sc.p, 1);
         hipLaunchKernelGGL(k_scale_seg, gn, dim3(kWg), 0, s, v.p, sc.p, dlc.p, n);
         MRS_HIP(hipMemcpyAsync(V.p, v.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_mul_dinv, gn, dim3(kWg), 0, s, v.p, dinv_p.p, n, v.p);
         MRS_CHECK_LAUNCH();
         for (int j = 0; j < max_iter; j++) {
             double* alpha_j = alpha_d.p + (size_t)j * nl;
@@ -904,7 +933,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             hipLaunchKernelGGL(k_lanczos_gate, gq, dim3(kWg), 0, s, live.p, dn_done.p, dn_iter.p,
                                j > 0 ? beta_j - nl : nullptr, nl, j);
             hipLaunchKernelGGL(k_spmv_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
-                               s, poff.p, pcol.p, mval.p, v.p, n, w.p);
+                               s, poff.p, pcol.p, pw.p, v.p, dinv_p.p, n, w.p);
             // Orthogonalise w = M v_j against V_0..V_j by classical Gram-Schmidt
             // applied twice.  The first pass's projections on v_j and v_{j-1}
             // are alpha_j and beta_{j-1}, so it also performs the three-term
@@ -924,7 +953,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
                                beta_j);
             hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p,
                                j + 1 < max_iter ? V.p + (size_t)(j + 1) * n : nullptr, w.p, beta_j,
-                               dlc.p, live.p, n);
+                               dinv_p.p, dlc.p, live.p, n);
             MRS_CHECK_LAUNCH();
         }
         MRS_HIP(hipMemcpyAsync(n_done.data(), dn_done.p, sizeof(int) * nl, hipMemcpyDeviceToHost, s));
