@@ -179,7 +179,7 @@ __global__ void k_copy_total(const uint32_t* __restrict__ scanned,
 // Stable LSD radix sort of u32 keys (8-bit digits) carrying u32 values.  A
 // workgroup of kSortWg threads owns a tile of kSortTile consecutive keys.
 constexpr int kSortWg = 256;
-constexpr int kSortRounds = 8;
+constexpr int kSortRounds = 16;
 constexpr uint32_t kSortTile = (uint32_t)kSortWg * kSortRounds;
 static_assert(kSortWg == 256, "one thread per digit value");
 
@@ -199,11 +199,15 @@ __global__ __launch_bounds__(kSortWg) void k_digit_counts(const uint32_t* __rest
     counts[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// One stable scatter pass.  A key's place is its digit's offset for the tile
-// (offs = exclusive scan of counts, digit-major) plus the number of earlier
-// keys of the tile with the same digit.  Keys are taken in index order (round,
-// then wave, then lane); the lanes of a wave that share a digit are found with
-// one ballot per digit bit.  val_in == nullptr stands for the identity (i).
+// One stable scatter pass.  A key's place in its tile's digit-sorted order is
+// the digit's start in the tile plus the number of earlier keys of the tile
+// with the same digit.  Keys are taken in index order (round, then wave, then
+// lane); the lanes of a wave that share a digit are found with one ballot per
+// digit bit.  The tile is first sorted into LDS, then written out run by run,
+// so consecutive threads store consecutive places of a digit's run (a direct
+// scatter stored each lane's key to its own place, up to 64 places per
+// wavefront store).  A digit's runs lie at offs (exclusive scan of counts,
+// digit-major).  val_in == nullptr stands for the identity (i).
 __global__ __launch_bounds__(kSortWg) void k_digit_scatter(const uint32_t* __restrict__ key_in,
                                                        const uint32_t* __restrict__ val_in,
                                                        uint32_t n, int shift, uint32_t tiles,
@@ -213,10 +217,30 @@ __global__ __launch_bounds__(kSortWg) void k_digit_scatter(const uint32_t* __res
     constexpr int kWaves = kSortWg / 64;
     __shared__ uint32_t run[256];           // keys of each digit placed so far in this tile
     __shared__ uint32_t wpre[kWaves][256];  // per round: wave counts, then wave offsets
+    __shared__ uint32_t lstart[256];        // start of each digit's run in the tile
+    __shared__ uint32_t gstart[256];        // start of that run in the output
+    __shared__ uint32_t sk[kSortTile], sv[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    run[tid] = 0u;
     const uint32_t base = blockIdx.x * kSortTile;
+    const uint32_t len = n - base < kSortTile ? n - base : kSortTile;
+    {  // this tile's count of digit tid, from consecutive scanned offsets
+        const size_t idx = (size_t)tid * tiles + blockIdx.x;
+        const uint32_t o = offs[idx];
+        const uint32_t nx = idx + 1 < (size_t)256 * tiles ? offs[idx + 1] : n;
+        gstart[tid] = o;
+        lstart[tid] = nx - o;
+        run[tid] = 0u;
+    }
+    __syncthreads();
+    const uint32_t own = lstart[tid];
+    for (int off = 1; off < 256; off <<= 1) {  // inclusive scan of the tile's counts
+        const uint32_t t = tid >= off ? lstart[tid - off] : 0u;
+        __syncthreads();
+        lstart[tid] += t;
+        __syncthreads();
+    }
+    lstart[tid] -= own;
     for (int r = 0; r < kSortRounds; r++) {
 #pragma unroll
         for (int w = 0; w < kWaves; w++) wpre[w][tid] = 0u;
@@ -247,11 +271,18 @@ __global__ __launch_bounds__(kSortWg) void k_digit_scatter(const uint32_t* __res
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = offs[(size_t)d * tiles + blockIdx.x] + wpre[wave][d] + rank;
-            key_out[pos] = k;
-            val_out[pos] = val_in ? val_in[i] : i;
+            const uint32_t lp = lstart[d] + wpre[wave][d] + rank;
+            sk[lp] = k;
+            sv[lp] = val_in ? val_in[i] : i;
         }
         __syncthreads();
+    }
+    for (uint32_t q = tid; q < len; q += kSortWg) {
+        const uint32_t k = sk[q];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = gstart[d] + (q - lstart[d]);
+        key_out[pos] = k;
+        val_out[pos] = sv[q];
     }
 }
 

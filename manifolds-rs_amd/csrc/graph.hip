@@ -7,7 +7,7 @@
 //                     its source row (out) and its target row (in); rows are
 //                     sorted by (column, direction) with per-row bitonic sorts
 //                     (registers / LDS / global for the rare long row) and
-//                     merged by one thread per row into the fuzzy union
+//                     merged by one wavefront per row into the fuzzy union
 //   filter_weak_edges (:776-853)                   device max + threshold flags
 //   UmapCsrGraph::from_graph (src/training/umap_optimiser_gpu.rs:132-209)
 //                     edges are the kept upper-triangle entries in row-major
@@ -16,8 +16,8 @@
 //                     neighbour row; ranks come from three exclusive scans.
 //
 // The union keeps every (row, col) of out(row) U in(row) -- a pattern that is
-// symmetric by construction -- so the mirror (col, row) of any entry is found by
-// binary search, and no host adjacency list is ever built.
+// symmetric by construction -- so the mirror (col, row) of every entry comes
+// from one stable sort of the columns, and no host adjacency list is ever built.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -230,41 +230,55 @@ __global__ __launch_bounds__(kWg) void k_sort_rows_long(uint64_t* __restrict__ r
     bitonic_block(raw + base, L, npad, threadIdx.x, kWg);
 }
 
-// merge_row / union_len (src/data/graph.rs:278-363) over a sorted raw row.
-// pass 0 counts distinct columns; pass 1 writes (row, col, combine(w_ij, w_ji)).
+// merge_row / union_len (src/data/graph.rs:278-363) over a sorted raw row,
+// one wavefront per row.  An entry heads its column's group when the entry
+// before it has another column; the heads are counted with ballots, and in the
+// write pass each head folds its group (the last entry of each direction
+// wins, as in a serial walk of the sorted row) and writes (row, col,
+// combine(w_ij, w_ji)) at the row's offset plus the heads before it.
+// Pass 0 counts the distinct columns of each row.
 template <bool WRITE>
 __global__ __launch_bounds__(kWg) void k_union_rows(
     const uint64_t* __restrict__ raw, const uint32_t* __restrict__ raw_off, uint32_t n,
     uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ u_off, uint32_t* __restrict__ u_row,
     uint32_t* __restrict__ u_col, float* __restrict__ u_w, float mix, float directed) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    uint32_t p = raw_off[i];
-    const uint32_t end = raw_off[i + 1];
-    uint32_t o = WRITE ? u_off[i] : 0u;
+    const uint32_t row = blockIdx.x * (kWg / 64) + (threadIdx.x >> 6);  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t beg = raw_off[row], end = raw_off[row + 1];
+    const uint32_t o = WRITE ? u_off[row] : 0u;
     uint32_t cnt = 0;
-    while (p < end) {
-        const uint32_t col = (uint32_t)(raw[p] >> 33);
-        float wij = 0.0f, wji = 0.0f;
-        while (p < end && (uint32_t)(raw[p] >> 33) == col) {
-            const uint64_t e = raw[p];
-            const float w = __uint_as_float((uint32_t)e);
-            if ((e >> 32) & 1ull) wji = w;
-            else wij = w;
-            p++;
+    for (uint32_t c0 = beg; c0 < end; c0 += 64) {
+        const uint32_t p = c0 + (uint32_t)lane;
+        bool head = false;
+        uint32_t col = 0;
+        if (p < end) {
+            col = (uint32_t)(raw[p] >> 33);
+            head = p == beg || (uint32_t)(raw[p - 1] >> 33) != col;
         }
-        if (WRITE) {
+        const uint64_t heads = __ballot(head);
+        if (WRITE && head) {
+            float wij = 0.0f, wji = 0.0f;
+            for (uint32_t q = p; q < end; q++) {
+                const uint64_t e = raw[q];
+                if ((uint32_t)(e >> 33) != col) break;
+                const float w = __uint_as_float((uint32_t)e);
+                if ((e >> 32) & 1ull) wji = w;
+                else wij = w;
+            }
             // f32, unfused: union = w_ij + w_ji - w_ij*w_ji; mix*union + (1-mix)*w_ij
-            const float s = __fadd_rn(wij, wji);
-            const float uni = __fsub_rn(s, __fmul_rn(wij, wji));
+            const float sum = __fadd_rn(wij, wji);
+            const float uni = __fsub_rn(sum, __fmul_rn(wij, wji));
             const float wsym = __fadd_rn(__fmul_rn(mix, uni), __fmul_rn(directed, wij));
-            u_row[o + cnt] = i;
-            u_col[o + cnt] = col;
-            u_w[o + cnt] = wsym;
+            const uint32_t at = o + cnt + (uint32_t)__popcll(heads & below);
+            u_row[at] = row;
+            u_col[at] = col;
+            u_w[at] = wsym;
         }
-        cnt++;
+        cnt += (uint32_t)__popcll(heads);
     }
-    if (!WRITE) ucnt[i] = cnt;
+    if (!WRITE && lane == 0) ucnt[row] = cnt;
 }
 
 __global__ __launch_bounds__(kWg) void k_max_pos(const float* __restrict__ w, uint64_t m,
@@ -281,22 +295,18 @@ __global__ __launch_bounds__(kWg) void k_max_pos(const float* __restrict__ w, ui
     atomicMax(max_bits, best);
 }
 
-__device__ inline uint32_t find_in_row(const uint32_t* __restrict__ u_col, uint32_t lo,
-                                       uint32_t hi, uint32_t key) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (u_col[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// filter_weak_edges flags, mirrors and the three flag vectors for the scans.
+// filter_weak_edges flags and the flag vectors for the scans.  The mirror of
+// entry p = (r, c) is the position of (c, r).  The union is grouped by row with
+// columns ascending and its pattern is symmetric, so the entries taken in
+// (column, row) order are the same (row, column) list: a stable sort of the
+// columns (values = entry index) puts p at position mirror[p], and the sorted
+// columns equal u_row element by element.  That equality is checked here
+// (*bad is raised if the pattern is not symmetric).
 __global__ __launch_bounds__(kWg) void k_flags(
     const uint32_t* __restrict__ u_row, const uint32_t* __restrict__ u_col,
-    const float* __restrict__ u_w, const uint32_t* __restrict__ u_off, uint64_t m,
+    const float* __restrict__ u_w, const uint32_t* __restrict__ sorted_col, uint64_t m,
     const uint32_t* __restrict__ max_bits, float n_epochs_f, uint32_t* __restrict__ keep,
-    uint32_t* __restrict__ mirror, uint32_t* __restrict__ upper) {
+    uint32_t* __restrict__ upper, uint32_t* __restrict__ bad) {
     uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
     if (p >= m) return;
     const float thr = __uint_as_float(*max_bits) / n_epochs_f;
@@ -305,10 +315,7 @@ __global__ __launch_bounds__(kWg) void k_flags(
     const uint32_t kp = (w > 0.0f && w >= thr) ? 1u : 0u;
     keep[p] = kp;
     upper[p] = (kp && c > r) ? 1u : 0u;
-    // the union pattern is symmetric, so (c, r) exists; fall back to p itself
-    // rather than index past row c if that invariant were ever broken
-    const uint32_t q = find_in_row(u_col, u_off[c], u_off[c + 1], r);
-    mirror[p] = (q < u_off[c + 1] && u_col[q] == r) ? q : (uint32_t)p;
+    if (sorted_col[p] != r) *bad = 1u;
 }
 
 __global__ __launch_bounds__(kWg) void k_present(const uint32_t* __restrict__ u_row,
@@ -511,10 +518,17 @@ void finish_graph(mrs_graph* g, uint32_t m) {
     DevBuf<uint32_t> mirror(ma), upper(ma), present(ma), keep_scan(ma), up_scan(ma),
         present_scan(ma);
     if (m) {
+        // mirrors: a stable sort of the columns carrying the entry index
+        DevBuf<uint32_t> sorted_col(m), asym(1);
+        radix_sort_pairs_u32(g->u_col.p, nullptr, m, n > 1 ? 32 - __builtin_clz(n - 1) : 1,
+                             sorted_col.p, mirror.p, s);
+        MRS_HIP(hipMemsetAsync(asym.p, 0, sizeof(uint32_t), s));
         hipLaunchKernelGGL(k_flags, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
-                           g->u_col.p, g->u_w.p, g->u_off.p, (uint64_t)m, max_bits.p,
-                           (float)n_epochs, g->u_keep.p, mirror.p, upper.p);
+                           g->u_col.p, g->u_w.p, sorted_col.p, (uint64_t)m, max_bits.p,
+                           (float)n_epochs, g->u_keep.p, upper.p, asym.p);
         MRS_CHECK_LAUNCH();
+        if (read_u32(asym.p, s))
+            throw Error(MRS_DEVICE_ERROR, "fuzzy union pattern is not symmetric");
         hipLaunchKernelGGL(k_present, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
                            g->u_col.p, g->u_keep.p, mirror.p, (uint64_t)m, present.p);
         MRS_CHECK_LAUNCH();
@@ -633,6 +647,16 @@ __global__ __launch_bounds__(kWg) void k_sorted_row_offsets(const uint32_t* __re
         else hi = mid;
     }
     off[v] = (uint32_t)lo;
+}
+
+__device__ inline uint32_t find_in_row(const uint32_t* __restrict__ u_col, uint32_t lo,
+                                       uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u_col[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
 }
 
 // every (r, c) needs its mirror (c, r): bad = 3 otherwise
@@ -776,7 +800,7 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         // fuzzy union
         DevBuf<uint32_t> ucnt(n + 1);
         MRS_HIP(hipMemsetAsync(ucnt.p, 0, ucnt.bytes(), s));
-        hipLaunchKernelGGL(k_union_rows<false>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, raw.p,
+        hipLaunchKernelGGL(k_union_rows<false>, dim3(div_up(n, kWg / 64)), dim3(kWg), 0, s, raw.p,
                            raw_off.p, n, ucnt.p, nullptr, nullptr, nullptr, nullptr, mix_weight,
                            1.0f - mix_weight);
         MRS_CHECK_LAUNCH();
@@ -788,7 +812,7 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         g->u_row.alloc(ma);
         g->u_col.alloc(ma);
         g->u_w.alloc(ma);
-        hipLaunchKernelGGL(k_union_rows<true>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, raw.p,
+        hipLaunchKernelGGL(k_union_rows<true>, dim3(div_up(n, kWg / 64)), dim3(kWg), 0, s, raw.p,
                            raw_off.p, n, nullptr, g->u_off.p, g->u_row.p, g->u_col.p, g->u_w.p,
                            mix_weight, 1.0f - mix_weight);
         MRS_CHECK_LAUNCH();
