@@ -63,6 +63,26 @@ def pmc_traffic(workload: str, kernel_prefix: str):
     return None, None
 
 
+def request_floor(ctx, n: int, firing_per_launch: float, neg_rate: int, dim: int,
+                  avg_launch_us: float):
+    """The epoch kernel's bound in random requests.  Per firing edge it makes
+    y_j and 2*neg_rate negative gathers, the larger endpoint's `dim` FP32 atomics and
+    flag store, and about one request for the smaller endpoint's run-summed
+    atomics (y_i is an L2 hit: edges are sorted by i).  Each of these touches a
+    random line of an n-row table that does not fit one XCD's L2, so the
+    kernel can go no faster than that many requests at the measured rate of
+    uniform random 8-byte gathers from an n-row table."""
+    rate = ctx.probe_random_gather(n, 1 << 24, 10)
+    per_edge = 1 + 2 * neg_rate + dim + 1 + 1
+    requests = firing_per_launch * per_edge
+    floor_us = requests / rate * 1e6
+    return dict(probe="uniform random float2 gathers, n-row table, 2^24 per launch, "
+                      "8 in flight per thread (mrs_probe_random_gather)",
+                rate_g_per_s=rate / 1e9, requests_per_firing_edge=per_edge,
+                requests_per_launch=requests, floor_us=floor_us,
+                achieved_us=avg_launch_us, frac=floor_us / avg_launch_us)
+
+
 def synthetic_knn_graph(n: int, k: int, n_epochs: int, seed: int = 42):
     """A UMAP-shaped graph for the epoch workload: exact kNN (k) of n points drawn
     from 50 2-D Gaussian blobs, membership weights exp(-(d - rho)/sigma) with
@@ -171,6 +191,7 @@ def bench_epochs(args):
                       traffic=traffic, traffic_source=traffic_src,
                       avg_launch_us=edge_avg_ms * 1e3,
                       algorithmic_bytes_per_launch=bytes_per_launch),
+        request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2, edge_avg_ms * 1e3),
         wall_s=wall,
     )
     if not args.no_cpu:
@@ -302,6 +323,8 @@ def bench_fit(args):
                       traffic=ep_traffic, traffic_source=ep_src,
                       avg_launch_us=1e3 * edge_ms / edge_launches,
                       algorithmic_bytes_per_launch=bytes_edge),
+        request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2,
+                                    1e3 * edge_ms / edge_launches),
         knn_kernel=dict(bound="mfma", kernel="k_shortlist<bf16> (pruned sweep, both passes)",
                         achieved=achieved, peak=2500.0, unit="TFLOP/s", frac=achieved / 2500.0,
                         traffic=None, traffic_per_dispatch_mean=sl_traffic,

@@ -117,6 +117,14 @@ int mrs_ctx_release_cache(mrs_ctx *ctx);
  * the same results; the switch exists for comparisons.  No counterpart in the
  * reference. */
 int mrs_ctx_set_knn_pruning(mrs_ctx *ctx, int32_t on);
+/* Measurement hook: the rate (gathers per second, written to *per_s) of
+ * uniform random 8-byte gathers from a table of `rows` float2 rows, the
+ * request pattern of the epoch kernel's negative samples and scattered
+ * endpoints.  Times `reps` launches of `gathers` gathers each (8 in flight per
+ * thread) on the context's stream after two warm-up launches.  No counterpart
+ * in the reference. */
+int mrs_probe_random_gather(mrs_ctx *ctx, int64_t rows, int64_t gathers, int32_t reps,
+                            double *per_s);
 
 /* parse_umap_optimiser_gpu, src/training/umap_optimiser_gpu.rs:44-52.
  * 0 adam_gpu, 1 adam_parallel, 2 adam, 3 sgd; -1 if unknown. */

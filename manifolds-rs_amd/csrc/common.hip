@@ -176,6 +176,40 @@ __global__ void k_copy_total(const uint32_t* __restrict__ scanned,
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + last_in[0];
 }
 
+constexpr int kProbeU = 8;  // gathers in flight per thread in the probe below
+
+// Random-request rate probe: each thread gathers kProbeU float2 rows of a
+// `rows`-row table at uniform pseudo-random indices (a 32-bit mix of the
+// gather's number, scaled to [0, rows) by a multiply-high), all of them in
+// flight together, and stores their sum so that no load is dropped.
+__global__ __launch_bounds__(256) void k_probe_gather(const float2* __restrict__ table,
+                                                      uint32_t rows, uint64_t threads,
+                                                      uint32_t salt, float2* __restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= threads) return;
+    uint32_t id[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; u++) {
+        uint32_t h = (uint32_t)(t * kProbeU + u) * 0x9E3779B1u ^ salt * 0x85EBCA77u;
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        h *= 0x846CA68Bu;
+        h ^= h >> 16;
+        id[u] = (uint32_t)(((uint64_t)h * rows) >> 32);
+    }
+    float2 v[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; u++) v[u] = table[id[u]];
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kProbeU; u++) {
+        acc.x += v[u].x;
+        acc.y += v[u].y;
+    }
+    out[t] = acc;
+}
+
 // Stable LSD radix sort of u32 keys (8-bit digits) carrying u32 values.  A
 // workgroup of kSortWg threads owns a tile of kSortTile consecutive keys.
 constexpr int kSortWg = 256;
@@ -420,6 +454,44 @@ int mrs_ctx_release_cache(mrs_ctx* ctx) {
         if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
         MRS_HIP(hipSetDevice(ctx->device));
         if (ctx->pool) ctx->pool->trim();
+    });
+}
+
+int mrs_probe_random_gather(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t reps,
+                            double* per_s) {
+    return mrs::guarded([&] {
+        if (!ctx || !per_s) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (rows < 1 || rows > (int64_t)UINT32_MAX)
+            throw mrs::Error(MRS_INVALID_ARGUMENT, "rows must be in [1, 2^32)");
+        if (gathers < mrs::kProbeU || gathers > ((int64_t)1 << 36) || reps < 1)
+            throw mrs::Error(MRS_INVALID_ARGUMENT, "gathers must be in [8, 2^36], reps >= 1");
+        MRS_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        mrs::StreamScope scope(s, ctx->pool);
+        const uint64_t threads = (uint64_t)gathers / mrs::kProbeU;
+        mrs::DevBuf<float2> table((size_t)rows), out((size_t)threads);
+        MRS_HIP(hipMemsetAsync(table.p, 0, table.bytes(), s));
+        const dim3 grid((unsigned)((threads + 255) / 256));
+        for (int w = 0; w < 2; w++)
+            hipLaunchKernelGGL(mrs::k_probe_gather, grid, dim3(256), 0, s, table.p, (uint32_t)rows,
+                               threads, (uint32_t)w, out.p);
+        MRS_CHECK_LAUNCH();
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        MRS_HIP(hipEventCreate(&e0));
+        MRS_HIP(hipEventCreate(&e1));
+        float ms = 0.0f;
+        hipError_t err = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && err == hipSuccess; r++)
+            hipLaunchKernelGGL(mrs::k_probe_gather, grid, dim3(256), 0, s, table.p,
+                               (uint32_t)rows, threads, (uint32_t)(r + 2), out.p);
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err == hipSuccess) err = hipEventRecord(e1, s);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        MRS_HIP(err);
+        *per_s = ms > 0.0f ? (double)threads * mrs::kProbeU * reps / (ms * 1e-3) : 0.0;
     });
 }
 

@@ -64,11 +64,14 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
     const double sentinel = (double)FLT_MAX;
     double lo = 0.0, hi = sentinel, mid = 1.0;
     for (int it = 0; it < n_iter; it++) {
+        // the reciprocal is taken once per pass and multiplied in, as the
+        // reference does (graph.rs:538-540)
+        const double inv_mid = 1.0 / mid;
         double s = 0.0;
         for (int j = 0; j < k; j++) {
             float a = d[j] - r;
             a = a > 0.0f ? a : 0.0f;
-            s += exp(-(double)a / mid);
+            s += exp(-(double)a * inv_mid);
         }
         if (fabs(s - target) < tol) break;
         if (s > target) {

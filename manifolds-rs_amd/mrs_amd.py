@@ -148,6 +148,7 @@ def lib() -> C.CDLL:
         "mrs_ctx_synchronize": (C.c_int, [vp]),
         "mrs_ctx_stream": (vp, [vp]),
         "mrs_ctx_set_knn_pruning": (C.c_int, [vp, i32]),
+        "mrs_probe_random_gather": (C.c_int, [vp, i64, i64, i32, C.POINTER(C.c_double)]),
         "mrs_ctx_cached_bytes": (C.c_size_t, [vp]),
         "mrs_ctx_release_cache": (C.c_int, [vp]),
         "mrs_parse_umap_optimiser_gpu": (i32, [C.c_char_p]),
@@ -268,6 +269,14 @@ class Context:
         """Exhaustive kNN skips database stages that provably hold no candidate
         (default) or sweeps every stage; both give the same results."""
         _check(lib().mrs_ctx_set_knn_pruning(self.handle, 1 if on else 0))
+
+    def probe_random_gather(self, rows: int, gathers: int = 1 << 24, reps: int = 10) -> float:
+        """Uniform random 8-byte gathers per second from a `rows`-row float2
+        table (the epoch kernel's request pattern), measured on this context."""
+        out = C.c_double(0.0)
+        _check(lib().mrs_probe_random_gather(self.handle, int(rows), int(gathers), int(reps),
+                                             C.byref(out)))
+        return out.value
 
     def close(self) -> None:
         if self.handle:
