@@ -79,6 +79,31 @@ __global__ __launch_bounds__(kWg) void k_cc_init(uint32_t* __restrict__ lab, uin
     if (i < n) lab[i] = i;
 }
 
+// Hook the components of u and v together: the larger of their labels is
+// lowered to the smaller one (a root's label is itself after a jump pass).
+__device__ inline void cc_hook_pair(uint32_t u, uint32_t v, uint32_t* __restrict__ lab,
+                                    uint32_t* __restrict__ changed) {
+    uint32_t a = lab[u], b = lab[v];
+    if (a == b) return;
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    uint32_t old = atomicMin(&lab[hi], lo);
+    if (old > lo) *changed = 1u;
+}
+
+// Sampling sweep: every vertex hooks to its first kCcSample neighbours.  On a
+// kNN graph this joins most of each component at a tenth of a full sweep's
+// gathers, so the full sweeps below mostly confirm.
+constexpr int kCcSample = 2;
+__global__ __launch_bounds__(kWg) void k_cc_sample(const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ col, uint32_t n,
+                                                   uint32_t* __restrict__ lab,
+                                                   uint32_t* __restrict__ changed) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = off[i], e = off[i + 1];
+    for (uint32_t p = b; p < e && p < b + kCcSample; p++) cc_hook_pair(i, col[p], lab, changed);
+}
+
 // One hooking sweep over the filtered COO entries in row order.  (Sweeping
 // only the undirected edge list, i < j, halves the gathers but took 2.2x as
 // long at 1M nodes: in row order an entry (i, j < i) pulls j's label into
@@ -90,11 +115,7 @@ __global__ __launch_bounds__(kWg) void k_cc_hook(const uint32_t* __restrict__ ro
                                                  uint32_t* __restrict__ changed) {
     uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
     if (p >= nnz) return;
-    uint32_t a = lab[row[p]], b = lab[col[p]];
-    if (a == b) return;
-    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    uint32_t old = atomicMin(&lab[hi], lo);
-    if (old > lo) *changed = 1u;
+    cc_hook_pair(row[p], col[p], lab, changed);
 }
 
 __global__ __launch_bounds__(kWg) void k_cc_jump(uint32_t* __restrict__ lab, uint32_t n) {
@@ -794,6 +815,12 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
     // ---- connected components
     DevBuf<uint32_t> lab(n), changed(1);
     hipLaunchKernelGGL(k_cc_init, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, n);
+    if (nnz) {
+        hipLaunchKernelGGL(k_cc_sample, dim3(div_up(n, kWg)), dim3(kWg), 0, s, off.p,
+                           g->coo_col.p, n, lab.p, changed.p);
+        hipLaunchKernelGGL(k_cc_jump, dim3(div_up(n, kWg)), dim3(kWg), 0, s, lab.p, n);
+        MRS_CHECK_LAUNCH();
+    }
     for (int it = 0; it < 10000; it++) {
         MRS_HIP(hipMemsetAsync(changed.p, 0, sizeof(uint32_t), s));
         if (nnz)
