@@ -244,10 +244,14 @@ __device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __
     // stores before any lane reads another lane's entry
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     uint64_t one[1] = {(uint32_t)lane < c ? myq[b * QC + lane] : kInfKey};
-    wave_sort<1>(one);
+    // the list load is issued before the queue sort, so its latency overlaps
+    // the sort instead of following it (each lane reads back only the list
+    // entries it wrote itself)
     uint64_t v[E];
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = lists[(size_t)b * S + e * 64 + lane];
+    __builtin_amdgcn_sched_barrier(0);  // keep the load ahead of the sort
+    wave_sort<1>(one);
     wave_merge<E>(v, one[0]);
 #pragma unroll
     for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[e];
