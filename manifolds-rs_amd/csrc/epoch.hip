@@ -234,6 +234,7 @@ template <int DIM>
 __global__ __launch_bounds__(kWg) void k_dens_radii_lanes(const float* __restrict__ embd,
                                                           const uint32_t* __restrict__ off,
                                                           const uint32_t* __restrict__ other_node,
+                                                          const uint32_t* __restrict__ outmap,
                                                           float* __restrict__ re,
                                                           float* __restrict__ phi_sum, uint32_t n,
                                                           float a, float b, float dens_eps,
@@ -245,18 +246,42 @@ __global__ __launch_bounds__(kWg) void k_dens_radii_lanes(const float* __restric
         float ys[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) ys[d] = embd[(size_t)node * DIM + d];
-        for (uint32_t pos = off[node] + sub; pos < off[node + 1]; pos += kRadiiLanes) {
-            const uint32_t o = other_node[pos];
+        auto add = [&](const float (&yo)[DIM]) {
             float dist_sq = 0.0f;
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                float diff = ys[d] - embd[(size_t)o * DIM + d];
+                float diff = ys[d] - yo[d];
                 dist_sq += diff * diff;
             }
             float dsb = powf(dist_sq, b);
             float phi = 1.0f / (1.0f + a * dsb);
             sum_sq += phi * dist_sq;
             sum_phi += phi;
+        };
+        // a lane's entries pos, pos + 16, ... are added in that order; four at
+        // a time have their rows gathered together first, so four gathers are
+        // in flight per lane instead of one after another
+        constexpr int kU = 4;
+        const uint32_t end = off[node + 1];
+        uint32_t pos = off[node] + sub;
+        for (; pos + (kU - 1) * kRadiiLanes < end; pos += kU * kRadiiLanes) {
+            uint32_t o[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) o[u] = other_node[pos + u * kRadiiLanes];
+            float yo[kU][DIM];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) yo[u][d] = embd[(size_t)o[u] * DIM + d];
+#pragma unroll
+            for (int u = 0; u < kU; u++) add(yo[u]);
+        }
+        for (; pos < end; pos += kRadiiLanes) {
+            const uint32_t o = other_node[pos];
+            float yo[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) yo[d] = embd[(size_t)o * DIM + d];
+            add(yo);
         }
     }
 #pragma unroll
@@ -265,10 +290,106 @@ __global__ __launch_bounds__(kWg) void k_dens_radii_lanes(const float* __restric
         sum_phi += __shfl_xor(sum_phi, o, kRadiiLanes);
     }
     if (node < n && sub == 0) {
+        const uint32_t out = outmap ? outmap[node] : node;
         float denom = sum_phi < tiny ? tiny : sum_phi;
-        re[node] = logf(dens_eps + sum_sq / denom);
-        phi_sum[node] = sum_phi;
+        re[out] = logf(dens_eps + sum_sq / denom);
+        phi_sum[out] = sum_phi;
     }
+}
+
+// ---- densMAP locality order
+// The radii kernel gathers every neighbour's embedding row.  In node order
+// those rows are scattered over the whole embedding; in an order where graph
+// neighbours sit close together most of them are L2 hits.  The order is a
+// Z-order (Morton) sort of the first min(dim, 2) coordinates of the embedding
+// at the state's first density update (graph neighbours lie close together in
+// the layout by then), built once per state.  Each density epoch gathers the embedding into that
+// order and runs the radii kernel on the CSR renumbered into it, writing each
+// radius back to its node.  Every row keeps its entries in their order, so
+// every node adds the same terms in the same order: the radii are identical
+// to those of node order.
+__device__ inline uint32_t ordered_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// bbox[2d] / bbox[2d+1]: ordered-bit minimum / maximum of coordinate d < nd
+__global__ __launch_bounds__(kWg) void k_embd_bbox(const float* __restrict__ embd, uint32_t n,
+                                                   int dim, int nd, uint32_t* __restrict__ bbox) {
+    uint32_t lo[2] = {~0u, ~0u}, hi[2] = {0u, 0u};
+    for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg)
+        for (int d = 0; d < nd; d++) {
+            const uint32_t o = ordered_bits(embd[(size_t)i * dim + d]);
+            lo[d] = o < lo[d] ? o : lo[d];
+            hi[d] = o > hi[d] ? o : hi[d];
+        }
+    for (int d = 0; d < nd; d++) {
+        atomicMin(&bbox[2 * d], lo[d]);
+        atomicMax(&bbox[2 * d + 1], hi[d]);
+    }
+}
+
+__device__ inline uint32_t spread_bits16(uint32_t x) {  // bit k -> bit 2k
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// key[i] = Z-order of node i's first nd coordinates, each quantised to 16 bits
+__global__ __launch_bounds__(kWg) void k_morton_keys(const float* __restrict__ embd, uint32_t n,
+                                                     int dim, int nd, float lo0, float sc0,
+                                                     float lo1, float sc1,
+                                                     uint32_t* __restrict__ key) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    auto quant = [](float x, float lo, float sc) {
+        const float t = (x - lo) * sc;
+        return t > 0.0f ? (t < 65535.0f ? (uint32_t)t : 65535u) : 0u;  // NaN -> 0
+    };
+    const uint32_t q0 = quant(embd[(size_t)i * dim], lo0, sc0);
+    key[i] = nd > 1 ? (spread_bits16(q0) | (spread_bits16(quant(embd[(size_t)i * dim + 1], lo1, sc1)) << 1))
+                    : q0;
+}
+
+// pos[order[q]] = q and deg[q] = row length of node order[q]
+__global__ __launch_bounds__(kWg) void k_order_rows(const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ off, uint32_t n,
+                                                    uint32_t* __restrict__ pos,
+                                                    uint32_t* __restrict__ deg) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t i = order[q];
+    pos[i] = q;
+    deg[q] = off[i + 1] - off[i];
+}
+
+// row q of the renumbered CSR = row order[q] with its neighbours renamed
+// through pos, entries in their order (kRadiiLanes lanes per row)
+__global__ __launch_bounds__(kWg) void k_order_csr(const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ pos,
+                                                   const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ other,
+                                                   const uint32_t* __restrict__ doff, uint32_t n,
+                                                   uint32_t* __restrict__ dcols) {
+    const uint64_t t = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint64_t q = t / kRadiiLanes;
+    const uint32_t sub = (uint32_t)(t % kRadiiLanes);
+    if (q >= n) return;
+    const uint32_t i = order[q], b = off[i], e = off[i + 1], o = doff[q];
+    for (uint32_t p = b + sub; p < e; p += kRadiiLanes) dcols[o + (p - b)] = pos[other[p]];
+}
+
+// y[q] = embd[order[q]] (dim floats per row)
+__global__ __launch_bounds__(kWg) void k_order_embd(const float* __restrict__ embd,
+                                                    const uint32_t* __restrict__ order, uint32_t n,
+                                                    int dim, float* __restrict__ y) {
+    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
+    if (q >= n) return;
+    const size_t src = (size_t)order[q] * dim, dst = (size_t)q * dim;
+    for (int d = 0; d < dim; d++) y[dst + d] = embd[src + d];
 }
 
 __device__ inline double block_sum_f64(double v, double* sh) {
@@ -508,10 +629,23 @@ void launch_dens_update_dev(mrs_umap_state* st, bool reference_order) {
         constexpr int D = decltype(dd)::value;
         if (D > 0 && !reference_order) {
             constexpr int DL = D > 0 ? D : 1;
+            if (!st->dens_order.p) {
+                hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+                MRS_HIP(hipStreamIsCapturing(s, &cap));
+                if (cap == hipStreamCaptureStatusNone) state_dens_order(st);
+            }
+            const bool ordered = st->dens_order.p != nullptr;
+            if (ordered) {
+                hipLaunchKernelGGL(k_order_embd, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
+                                   st->dens_order.p, n, st->dim, st->dens_y.p);
+            }
             hipLaunchKernelGGL(k_dens_radii_lanes<DL>, dim3(div_up((uint64_t)n * kRadiiLanes, kWg)),
-                               dim3(kWg), 0, s, st->embd.p, st->off.p, st->csr_other.p,
-                               st->dens_re.p, st->dens_phi.p, n, st->params.a, st->params.b,
-                               (float)kDensEps, FLT_EPSILON);
+                               dim3(kWg), 0, s, ordered ? st->dens_y.p : st->embd.p,
+                               ordered ? st->dens_off.p : st->off.p,
+                               ordered ? st->dens_cols.p : st->csr_other.p,
+                               ordered ? st->dens_order.p : nullptr, st->dens_re.p,
+                               st->dens_phi.p, n, st->params.a, st->params.b, (float)kDensEps,
+                               FLT_EPSILON);
         } else {
             hipLaunchKernelGGL(k_dens_radii<D>, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p,
                                st->off.p, st->csr_other.p, st->dens_re.p, st->dens_phi.p, n,
@@ -792,6 +926,58 @@ void state_alloc_nodes(mrs_umap_state* st, const float* embd) {
         st->dens_phi.alloc(1);
         st->dens_w.alloc(1);
     }
+}
+
+// The densMAP locality order of a state (see k_morton_keys), for the
+// dimensions the lane-parallel radii kernel serves (1..4).  It is built from
+// the embedding at the state's first density update outside a stream capture
+// (by then the layout has pulled graph neighbours together), and kept.
+void state_dens_order(mrs_umap_state* st) {
+    hipStream_t s = st->ctx->stream;
+    const uint32_t n = (uint32_t)st->n;
+    const int dim = st->dim;
+    if (n < 2 || dim < 1 || dim > 4) return;
+    const int nd = dim >= 2 ? 2 : 1;
+    DevBuf<uint32_t> bbox(4), key(n), skey(n), pos(n), deg((size_t)n + 1);
+    const uint32_t init[4] = {~0u, 0u, ~0u, 0u};
+    MRS_HIP(hipMemcpyAsync(bbox.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_embd_bbox, dim3(std::min<uint32_t>(div_up(n, kWg), 1024)), dim3(kWg), 0,
+                       s, st->embd.p, n, dim, nd, bbox.p);
+    MRS_CHECK_LAUNCH();
+    uint32_t bb[4];
+    MRS_HIP(hipMemcpyAsync(bb, bbox.p, sizeof(bb), hipMemcpyDeviceToHost, s));
+    MRS_HIP(hipStreamSynchronize(s));
+    auto to_float = [](uint32_t o) {
+        const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+        float f;
+        std::memcpy(&f, &u, sizeof(f));
+        return f;
+    };
+    float lo[2] = {0.0f, 0.0f}, sc[2] = {0.0f, 0.0f};
+    for (int d = 0; d < nd; d++) {
+        lo[d] = to_float(bb[2 * d]);
+        const float span = to_float(bb[2 * d + 1]) - lo[d];
+        sc[d] = std::isfinite(span) && span > 0.0f ? 65535.0f / span : 0.0f;
+        if (!std::isfinite(lo[d])) lo[d] = 0.0f;
+    }
+    hipLaunchKernelGGL(k_morton_keys, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->embd.p, n, dim,
+                       nd, lo[0], sc[0], lo[1], sc[1], key.p);
+    MRS_CHECK_LAUNCH();
+    st->dens_order.alloc(n);
+    radix_sort_pairs_u32(key.p, nullptr, n, 32, skey.p, st->dens_order.p, s);
+    hipLaunchKernelGGL(k_order_rows, dim3(div_up(n, kWg)), dim3(kWg), 0, s, st->dens_order.p,
+                       st->off.p, n, pos.p, deg.p);
+    MRS_CHECK_LAUNCH();
+    st->dens_off.alloc((size_t)n + 1);
+    exclusive_scan_u32(deg.p, st->dens_off.p, (uint64_t)n + 1, nullptr, s);
+    const size_t nnz = 2 * (size_t)st->n_edges;
+    st->dens_cols.alloc(nnz ? nnz : 1);
+    hipLaunchKernelGGL(k_order_csr, dim3(div_up((uint64_t)n * kRadiiLanes, kWg)), dim3(kWg), 0, s,
+                       st->dens_order.p, pos.p, st->off.p, st->csr_other.p, st->dens_off.p, n,
+                       st->dens_cols.p);
+    MRS_CHECK_LAUNCH();
+    st->dens_y.alloc((size_t)n * dim);
+    MRS_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
 }
 
 void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float* r,
