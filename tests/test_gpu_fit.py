@@ -372,6 +372,17 @@ def test_context_cache_reuses_and_releases(ctx):
     assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)) and np.all(np.isfinite(c))
 
 
+def test_random_gather_probe(ctx):
+    # the measurement hook behind bench.py's request floor: a positive rate,
+    # and its arguments are checked before any launch
+    rate = ctx.probe_random_gather(1 << 16, 1 << 20, 2)
+    assert 1e8 < rate < 1e13
+    for rows, gathers, reps in ((0, 1 << 20, 1), (1 << 16, 4, 1), (1 << 16, 1 << 20, 0)):
+        with pytest.raises(M.ManifoldsError) as e:
+            ctx.probe_random_gather(rows, gathers, reps)
+        assert e.value.status == M.MRS_INVALID_ARGUMENT
+
+
 def test_fit_same_with_and_without_knn_pruning(ctx):
     # n >= 65 536 takes the pruned kNN sweep: the whole fit (graph, spectral
     # layout, deterministic epochs) is the same as with the full sweep
