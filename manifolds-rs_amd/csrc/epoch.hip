@@ -253,7 +253,8 @@ __global__ __launch_bounds__(kWg) void k_dens_radii_lanes(const float* __restric
                 float diff = ys[d] - yo[d];
                 dist_sq += diff * diff;
             }
-            float dsb = powf(dist_sq, b);
+            // a neighbour at the node's own position (dist_sq = 0) has phi = 1
+            float dsb = dist_sq > 0.0f ? pos_powf(dist_sq, b) : 0.0f;
             float phi = 1.0f / (1.0f + a * dsb);
             sum_sq += phi * dist_sq;
             sum_phi += phi;

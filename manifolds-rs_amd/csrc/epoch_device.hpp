@@ -8,6 +8,16 @@
 
 namespace mrs {
 
+// x^y for x > 0 (a normal float) from the hardware base-2 logarithm and
+// exponential: exp2(y * log2(x)).  Its relative error is about
+// ln2 * |y * log2(x)| * 2^-23 plus an ulp, i.e. below 2e-6 for the
+// arguments of the epoch and radii kernels (x = a squared distance >= 1e-8,
+// or one plus 0.001; y = b in [0.1, 2]), against ~100 instructions for the
+// correctly rounded powf.  The node-exact strategy keeps powf.
+__device__ inline float pos_powf(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 __device__ inline float clampf(float g, float c) {
     if (g > c) g = c;
     if (g < -c) g = -c;
@@ -81,7 +91,7 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
             dist_sq += diff * diff;
         }
         if (dist_sq >= kGradDistSqThreshold) {
-            float dsb = powf(dist_sq, b);
+            float dsb = pos_powf(dist_sq, b);
             float denom = one + a * dsb;
             float gc = two_a_b * dsb / (dist_sq * denom);
 #pragma unroll
@@ -124,7 +134,7 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
                     dsk += diff * diff;
                 }
                 float safe = dsk + kGradRepEps;
-                float gc = clampf(two_gamma_b / (safe * (one + a * powf(safe, b))), kGradClipVal);
+                float gc = clampf(two_gamma_b / (safe * (one + a * pos_powf(safe, b))), kGradClipVal);
 #pragma unroll
                 for (int d = 0; d < DIM; d++) gi[d] += gc * (yi[d] - yki[d]);
             }
@@ -136,7 +146,7 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
                     dsk += diff * diff;
                 }
                 float safe = dsk + kGradRepEps;
-                float gc = clampf(two_gamma_b / (safe * (one + a * powf(safe, b))), kGradClipVal);
+                float gc = clampf(two_gamma_b / (safe * (one + a * pos_powf(safe, b))), kGradClipVal);
 #pragma unroll
                 for (int d = 0; d < DIM; d++) gj[d] += gc * (yj[d] - ykj[d]);
             }
