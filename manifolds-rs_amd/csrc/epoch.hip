@@ -383,6 +383,17 @@ __global__ __launch_bounds__(kWg) void k_order_csr(const uint32_t* __restrict__ 
     for (uint32_t p = b + sub; p < e; p += kRadiiLanes) dcols[o + (p - b)] = pos[other[p]];
 }
 
+// pack[i] = (w, 1 / phi_sum, 1 / exp(re)) of node i, the per-node factors of
+// the edge kernel's density term, read as one 16-byte row per endpoint
+__global__ __launch_bounds__(kWg) void k_dens_pack(const float* __restrict__ w,
+                                                   const float* __restrict__ phi,
+                                                   const float* __restrict__ re, uint32_t n,
+                                                   float4* __restrict__ pack) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    pack[i] = make_float4(w[i], 1.0f / phi[i], 1.0f / expf(re[i]), 0.0f);
+}
+
 // y[q] = embd[order[q]] (dim floats per row)
 __global__ __launch_bounds__(kWg) void k_order_embd(const float* __restrict__ embd,
                                                     const uint32_t* __restrict__ order, uint32_t n,
@@ -503,8 +514,7 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
     float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
-    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
     static_assert(kEdgeSpan % 64 == 0, "whole wavefront rounds");
     constexpr int kWaves = kWg / 64;
     __shared__ uint32_t queue[kWaves][kEdgeSpan];
@@ -532,8 +542,7 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
         const bool active = slot < count;
         const uint32_t e = active ? queue[wave][slot] : 0u;
         edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a,
-                             b, two_a_b, two_gamma_b, neg_rate, mu, dens_re, dens_phi, dens_w,
-                             dens_scalars);
+                             b, two_a_b, two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
     }
 }
 
@@ -664,6 +673,8 @@ void launch_dens_update_dev(mrs_umap_state* st, bool reference_order) {
                        st->dens_r.p, n, st->dens_partials.p, st->dens_w.p, st->dens_scalars.p,
                        st->dens_params.var_shift, st->dens_params.lambda, st->mu_tot);
     MRS_CHECK_LAUNCH();
+    dens_pack_refresh(st);
+    MRS_CHECK_LAUNCH();
 }
 
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
@@ -688,7 +699,7 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
                            st->edge_loc.p, st->per.p, st->next.p, st->embd.p, st->grad.p,
                            st->has_update.p, ne, (uint32_t)epoch, (float)epoch, seed, fm, p.a,
                            p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,
-                           st->mu.p, st->dens_re.p, st->dens_phi.p, st->dens_w.p,
+                           st->mu.p, reinterpret_cast<const float4*>(st->dens_pack.p),
                            st->dens_scalars.p);
     };
     switch (st->dim) {
@@ -926,7 +937,19 @@ void state_alloc_nodes(mrs_umap_state* st, const float* embd) {
         st->dens_re.alloc(1);
         st->dens_phi.alloc(1);
         st->dens_w.alloc(1);
+        st->dens_pack.alloc(4);
     }
+}
+
+// Rebuild the packed per-node density factors from dens_w, dens_phi and
+// dens_re (after each density update, and after a caller writes one of them).
+void dens_pack_refresh(mrs_umap_state* st) {
+    if (!st->dens_enabled) return;
+    const uint32_t n = (uint32_t)st->n;
+    hipLaunchKernelGGL(k_dens_pack, dim3(div_up(n, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->dens_w.p, st->dens_phi.p, st->dens_re.p, n,
+                       reinterpret_cast<float4*>(st->dens_pack.p));
+    MRS_CHECK_LAUNCH();
 }
 
 // The densMAP locality order of a state (see k_morton_keys), for the
@@ -993,6 +1016,8 @@ void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float
     st->dens_phi.alloc(n);
     st->dens_w.alloc(n);
     st->dens_partials.alloc(3 * kDensBlocks);
+    st->dens_pack.alloc(4 * n);
+    MRS_HIP(hipMemsetAsync(st->dens_pack.p, 0, st->dens_pack.bytes(), s));
     MRS_HIP(hipMemcpyAsync(st->dens_r.p, r, n * sizeof(float), hipMemcpyDefault, s));
     MRS_HIP(hipMemsetAsync(st->dens_re.p, 0, st->dens_re.bytes(), s));
     MRS_HIP(hipMemsetAsync(st->dens_phi.p, 0, st->dens_phi.bytes(), s));

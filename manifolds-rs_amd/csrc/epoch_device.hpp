@@ -63,8 +63,7 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t* __restrict__ flag,
     uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
-    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
     const float two = 2.0f, one = 1.0f;
     uint32_t key = 0xFFFFFFFFu;
     float gi[DIM], gj[DIM];
@@ -104,9 +103,11 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
                 float phi = one / denom;
                 float dphi_term = a * b * dsb / dist_sq * phi;
                 float common = one - b * (one - phi);
-                float dr_i = (phi / dens_phi[i]) * (common / expf(dens_re[i]) + dphi_term);
-                float dr_j = (phi / dens_phi[j]) * (common / expf(dens_re[j]) + dphi_term);
-                float cor = dens_scalars[3] * (dens_w[i] * dr_i + dens_w[j] * dr_j) / mu[e];
+                // per node (w, 1 / phi_sum, 1 / exp(re)): one 16-byte gather per endpoint
+                const float4 pi = dens_pack[i], pj = dens_pack[j];
+                float dr_i = (phi * pi.y) * (common * pi.z + dphi_term);
+                float dr_j = (phi * pj.y) * (common * pj.z + dphi_term);
+                float cor = dens_scalars[3] * (pi.x * dr_i + pj.x * dr_j) / mu[e];
                 float two_cor = two * cor;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
@@ -175,8 +176,7 @@ __device__ __forceinline__ void edge_epoch(uint32_t e,
     float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float* __restrict__ dens_re, const float* __restrict__ dens_phi,
-    const float* __restrict__ dens_w, const float* __restrict__ dens_scalars) {
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
     bool active = false;
     if (e < n_edges) {
         float nx = next[e];
@@ -186,8 +186,7 @@ __device__ __forceinline__ void edge_epoch(uint32_t e,
         }
     }
     edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a, b,
-                         two_a_b, two_gamma_b, neg_rate, mu, dens_re, dens_phi, dens_w,
-                         dens_scalars);
+                         two_a_b, two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
 }
 
 // Adam step of one node if flagged (umap_adam_update, umap_optimiser_gpu.rs:887-923),

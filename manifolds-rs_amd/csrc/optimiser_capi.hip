@@ -123,6 +123,11 @@ int mrs_umap_state_write(mrs_umap_state* st, int32_t which, const void* in) {
         MRS_HIP(hipMemcpyAsync(b.p, in, b.bytes, hipMemcpyDefault, st->ctx->stream));
         MRS_HIP(hipStreamSynchronize(st->ctx->stream));
         if (which == 3 || which == 4) st->grad_clean = false;
+        if (which >= 11 && which <= 13) {  // a density buffer the edge kernel reads packed
+            mrs::StreamScope scope(st->ctx->stream, st->ctx->pool);
+            mrs::dens_pack_refresh(st);
+            MRS_HIP(hipStreamSynchronize(st->ctx->stream));
+        }
     });
 }
 

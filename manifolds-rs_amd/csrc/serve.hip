@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restri
     const uint32_t e = (blockIdx.x - f.blk_edge0) * kWg + threadIdx.x;
     edge_epoch<DIM, false>(e, f.edge_ij, f.edge_loc, f.per, f.next, f.embd, f.grad, f.flag,
                            f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b, f.two_a_b,
-                           f.two_gamma_b, f.neg_rate, nullptr, nullptr, nullptr, nullptr, nullptr);
+                           f.two_gamma_b, f.neg_rate, nullptr, nullptr, nullptr);
 }
 
 template <int DIM>

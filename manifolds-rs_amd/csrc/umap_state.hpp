@@ -34,6 +34,7 @@ struct mrs_umap_state {
     mrs_dens_params dens_params{};
     float mu_tot = 0.0f;
     mrs::DevBuf<float> dens_r, dens_re, dens_phi, dens_w;  // [n] (or [1] when off)
+    mrs::DevBuf<float> dens_pack;  // [n][4]: w, 1/phi_sum, 1/exp(re), 0 (edge kernel)
     // locality order of the radii kernel: node order[q] at position q, the CSR
     // renumbered into it (rows keep their entry order) and the embedding
     // gathered into it each density epoch; empty when not built
@@ -88,6 +89,7 @@ mrs_umap_state* make_state(mrs_ctx* ctx, const float* embd, int64_t n, int32_t d
 // the device (filled by the caller).  embd is device or host memory [n*dim].
 void state_alloc_nodes(mrs_umap_state* st, const float* embd);
 void state_dens_order(mrs_umap_state* st);
+void dens_pack_refresh(mrs_umap_state* st);
 void state_alloc_dens(mrs_umap_state* st, const mrs_dens_params& dp, const float* r_dev_or_host,
                       float mu_tot);
 
