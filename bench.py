@@ -23,6 +23,14 @@ Prints ONE JSON line.  Workloads:
 The CPU baseline is the C restatement of the reference's parallel Adam
 (optimise_embedding_adam_parallel) timed on this host's cores on a bounded
 sample of the same workload.
+
+`roofline` prices the dominant kernel (the epoch kernel; the FP32 shortlist
+for `fp32`) against HBM bandwidth or MFMA peak.  The epoch kernel's random
+8-byte gathers and atomics each touch a line of an embedding that does not
+fit one XCD's L2, so `request_floor` (fit and epochs) also prices it against
+the rate of uniform random gathers measured in the same run
+(mrs_probe_random_gather): requests per launch / that rate, against the
+measured launch time.
 """
 from __future__ import annotations
 
