@@ -117,6 +117,12 @@ int mrs_ctx_release_cache(mrs_ctx *ctx);
  * the same results; the switch exists for comparisons.  No counterpart in the
  * reference. */
 int mrs_ctx_set_knn_pruning(mrs_ctx *ctx, int32_t on);
+/* The pruned sweep cuts a query block's second-pass stage list longer than
+ * `stages` into work items whose partial lists a merge kernel folds back in;
+ * 0 (the default) picks max(1024, total / 2048).  The result does not depend
+ * on it: smaller values make small problems take the split path, as large
+ * ones do by default.  No counterpart in the reference. */
+int mrs_ctx_set_knn_work_chunk(mrs_ctx *ctx, int32_t stages);
 /* Measurement hook: the rate (gathers per second, written to *per_s) of
  * uniform random 8-byte gathers from a table of `rows` float2 rows, the
  * request pattern of the epoch kernel's negative samples and scattered
@@ -144,6 +150,21 @@ uint32_t mrs_hash_neg(uint32_t seed, uint32_t node, uint32_t epoch, uint32_t edg
 
 /* UmapParamsGpu::default (src/lib.rs:3244-3260) with ann_type "exhaustive_gpu". */
 void mrs_umap_params_gpu_default(mrs_umap_params_gpu *p);
+/* UmapParamsGpu::new_default_2d (src/lib.rs:3338-3346): the defaults with
+ * optim_params = from_min_dist_spread(min_dist, spread); NaN = None (0.5, 1.0).
+ * A min_dist/spread pair from_min_dist_spread refuses leaves default_2d's a, b. */
+void mrs_umap_params_gpu_default_2d(float min_dist, float spread, mrs_umap_params_gpu *p);
+
+/* UmapOptimParams::from_min_dist_spread (src/training/umap_optimisers.rs:89-122)
+ * with fit_params (:139-209): a, b fitted to min_dist and spread by the
+ * reference's 300-step gradient descent in f32.  The optional arguments take
+ * the reference's defaults when NaN (floats) or negative (ints): lr 1, gamma 1,
+ * n_epochs 500, neg_sample_rate 5, beta1 0.5, beta2 0.9, eps 1e-7.  Host only;
+ * needs no device.  spread <= 0 or a NaN min_dist is MRS_INVALID_ARGUMENT. */
+int mrs_umap_optim_params_from_min_dist_spread(float min_dist, float spread, float lr,
+                                               float gamma, int32_t n_epochs,
+                                               int32_t neg_sample_rate, float beta1, float beta2,
+                                               float eps, mrs_umap_optim_params *out);
 
 /* umap_gpu (src/lib.rs:3465-3487 -> umap_gpu_inner :3510-3708): data [n*d] f32
  * row-major; optional precomputed kNN (knn_idx/knn_dist [n*knn_k], self
@@ -173,12 +194,13 @@ int mrs_last_fit_timings(double *out);
  * shortlist of max(shortlist, k) rounded up to 64 or 128 candidates). */
 int mrs_knn_exhaustive(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                        int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
-/* Same, returning in ms_out[0..8]: the stage times in ms of pack (with the
+/* Same, returning in ms_out[0..9]: the stage times in ms of pack (with the
  * cell layout when pruning), shortlist (every pass), merge+re-rank; database
  * splits used; shortlist size; the fraction of (query block, database stage)
  * pairs swept (1 without pruning); cells (0 without pruning); the longest and
- * the mean second-pass stage list of a query block.  No counterpart in the
- * reference; used by the benchmark. */
+ * the mean second-pass stage list of a query block; the second-pass work items
+ * split off a longer list.  No counterpart in the reference; used by the
+ * benchmark and the tests. */
 int mrs_knn_exhaustive_timed(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                              int32_t precision, int32_t shortlist, int32_t *idx, float *dist,
                              double *ms_out);
@@ -254,7 +276,9 @@ int mrs_umap_state_info(const mrs_umap_state *st, int64_t *n, int32_t *n_dim,
  *  5 node_edge_offsets [n+1] u32  6 csr_edge_idx [2E] u32  7 csr_other_node [2E] u32
  *  8 epochs_per_sample [E] f32    9 epoch_of_next_sample [E] f32   10 mu [E] f32
  * 11 dens_re [n]  12 dens_phi_sum [n]  13 dens_weight [n]  14 dens_r [n]
- * 15 edge endpoints [2E] u32 (i, j interleaved, i < j, edge order) */
+ * 15 edge endpoints [2E] u32 (i, j interleaved, i < j, edge order; read-only)
+ * 16 edge CSR slots [2E] u32 (the edge's position in row i and in row j,
+ *    interleaved: the edge_local keys of its negative samples; read-only) */
 int mrs_umap_state_read(mrs_umap_state *st, int32_t which, void *out);
 int mrs_umap_state_write(mrs_umap_state *st, int32_t which, const void *in);
 

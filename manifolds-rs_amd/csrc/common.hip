@@ -502,6 +502,14 @@ int mrs_ctx_set_knn_pruning(mrs_ctx* ctx, int32_t on) {
     });
 }
 
+int mrs_ctx_set_knn_work_chunk(mrs_ctx* ctx, int32_t stages) {
+    return mrs::guarded([&] {
+        if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
+        if (stages < 0) throw mrs::Error(MRS_INVALID_ARGUMENT, "stages must be >= 0");
+        ctx->knn_chunk = (uint32_t)stages;
+    });
+}
+
 static std::string lower(const char* s) {
     std::string r = s ? s : "";
     std::transform(r.begin(), r.end(), r.begin(), [](unsigned char c) { return std::tolower(c); });

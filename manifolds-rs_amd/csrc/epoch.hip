@@ -509,8 +509,7 @@ constexpr uint32_t kWideSpanEdges = 1u << 20;
 
 template <int DIM, bool DENS, int kEdgeSpan>
 __global__ __launch_bounds__(kWg) void k_epoch_edges(
-    const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
-    const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
+    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
     float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
@@ -541,8 +540,8 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
         const uint32_t slot = q0 + lane;
         const bool active = slot < count;
         const uint32_t e = active ? queue[wave][slot] : 0u;
-        edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a,
-                             b, two_a_b, two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+        edge_work<DIM, DENS>(active, e, edges, embd, grad, flag, epoch, seed, fm, a, b, two_a_b,
+                             two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
     }
 }
 
@@ -678,9 +677,6 @@ void launch_dens_update_dev(mrs_umap_state* st, bool reference_order) {
 }
 
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
-    if (!st->loc_packed)
-        throw Error(MRS_INVALID_ARGUMENT,
-                    "a node degree exceeds 65535; use the node-exact epoch strategy");
     if (!st->grad_clean) {
         MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), st->ctx->stream));
         MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), st->ctx->stream));
@@ -695,8 +691,8 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     const bool wide = ne >= kWideSpanEdges;
     const unsigned blocks = div_up(div_up(ne, wide ? 256 : 64), kWg / 64);
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edge_ij.p,
-                           st->edge_loc.p, st->per.p, st->next.p, st->embd.p, st->grad.p,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edges.p, st->per.p,
+                           st->next.p, st->embd.p, st->grad.p,
                            st->has_update.p, ne, (uint32_t)epoch, (float)epoch, seed, fm, p.a,
                            p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,
                            st->mu.p, reinterpret_cast<const float4*>(st->dens_pack.p),
@@ -790,7 +786,6 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
     const int32_t N = st->params.n_epochs;
     if (strategy != MRS_EPOCH_EDGE_ATOMIC && strategy != MRS_EPOCH_NODE_EXACT)
         throw Error(MRS_INVALID_ARGUMENT, "unknown epoch strategy");
-    if (strategy == MRS_EPOCH_EDGE_ATOMIC && !st->loc_packed) strategy = MRS_EPOCH_NODE_EXACT;
     if (strategy == MRS_EPOCH_EDGE_ATOMIC) {
         int d = st->dim;
         if (!(d == 1 || d == 2 || d == 3 || d == 4 || d == 8)) strategy = MRS_EPOCH_NODE_EXACT;
@@ -885,8 +880,7 @@ HostCsr host_csr_from_graph(int64_t n, const int64_t* adj_off, const int32_t* ad
     for (int64_t i = 0; i < n; i++) c.off[i + 1] = c.off[i] + degree[i];
     c.csr_edge_idx.resize((size_t)(2 * ne));
     c.csr_other.resize((size_t)(2 * ne));
-    c.edge_ij.resize((size_t)(2 * ne));
-    c.edge_loc.resize((size_t)ne);
+    c.edges.resize((size_t)ne);
     c.per.resize((size_t)ne);
     c.mu.resize((size_t)ne);
     std::vector<uint32_t> cursor(c.off.begin(), c.off.end() - 1);
@@ -900,11 +894,7 @@ HostCsr host_csr_from_graph(int64_t n, const int64_t* adj_off, const int32_t* ad
             c.csr_other[pi] = (uint32_t)j;
             c.csr_edge_idx[pj] = (uint32_t)e;
             c.csr_other[pj] = (uint32_t)i;
-            uint32_t li = pi - c.off[i], lj = pj - c.off[j];
-            if (li > 0xFFFFu || lj > 0xFFFFu) c.loc_packed = false;
-            c.edge_loc[e] = (li & 0xFFFFu) | ((lj & 0xFFFFu) << 16);
-            c.edge_ij[2 * e] = (uint32_t)i;
-            c.edge_ij[2 * e + 1] = (uint32_t)j;
+            c.edges[e] = make_uint4((uint32_t)i, (uint32_t)j, pi - c.off[i], pj - c.off[j]);
             float w = adj_vals[p];
             float norm = w / max_w;
             c.per[e] = norm > 0.0f ? 1.0f / norm : 1e8f;
@@ -1046,7 +1036,6 @@ mrs_umap_state* make_state(mrs_ctx* ctx, const float* embd, int64_t n, int32_t d
         st->dim = dim;
         st->n_edges = c.n_edges;
         st->params = p;
-        st->loc_packed = c.loc_packed;
         hipStream_t s = ctx->stream;
         auto up = [&](auto& buf, const auto& vec) {
             buf.alloc(vec.size());
@@ -1055,8 +1044,7 @@ mrs_umap_state* make_state(mrs_ctx* ctx, const float* embd, int64_t n, int32_t d
         up(st->off, c.off);
         up(st->csr_edge_idx, c.csr_edge_idx);
         up(st->csr_other, c.csr_other);
-        up(st->edge_ij, c.edge_ij);
-        up(st->edge_loc, c.edge_loc);
+        up(st->edges, c.edges);
         up(st->per, c.per);
         up(st->next, c.per);  // cursors start one period from epoch 0
         up(st->mu, c.mu);

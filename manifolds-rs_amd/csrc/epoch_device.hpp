@@ -58,8 +58,7 @@ __device__ inline bool seg_reduce(uint32_t key, float (&v)[D]) {
 // (the reduction is wavefront-wide); lanes with active == false only join the
 // reduction.  DIM is the compile-time embedding dimension.
 template <int DIM, bool DENS>
-__device__ __forceinline__ void edge_work(bool active, uint32_t e,
-    const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
+__device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* __restrict__ edges,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t* __restrict__ flag,
     uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
@@ -71,12 +70,11 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
     for (int d = 0; d < DIM; d++) gi[d] = gj[d] = 0.0f;
     uint32_t j = 0;
     if (active) {
-        const uint2 ij = reinterpret_cast<const uint2*>(edge_ij)[e];
-        const uint32_t i = ij.x;
-        j = ij.y;
+        const uint4 rec = edges[e];  // {i, j, CSR slot under i, under j}
+        const uint32_t i = rec.x;
+        j = rec.y;
         key = i;
-        const uint32_t lc = edge_loc[e];
-        const uint32_t li = lc & 0xFFFFu, lj = lc >> 16;
+        const uint32_t li = rec.z, lj = rec.w;
         float yi[DIM], yj[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
@@ -170,8 +168,7 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e,
 // wavefront's reduction).  Reads next/per coalesced and advances a firing
 // edge's schedule cursor in place, then does the edge's work.
 template <int DIM, bool DENS>
-__device__ __forceinline__ void edge_epoch(uint32_t e,
-    const uint32_t* __restrict__ edge_ij, const uint32_t* __restrict__ edge_loc,
+__device__ __forceinline__ void edge_epoch(uint32_t e, const uint4* __restrict__ edges,
     const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
     float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
@@ -185,8 +182,8 @@ __device__ __forceinline__ void edge_epoch(uint32_t e,
             next[e] = nx + per[e];
         }
     }
-    edge_work<DIM, DENS>(active, e, edge_ij, edge_loc, embd, grad, flag, epoch, seed, fm, a, b,
-                         two_a_b, two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+    edge_work<DIM, DENS>(active, e, edges, embd, grad, flag, epoch, seed, fm, a, b, two_a_b,
+                         two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
 }
 
 // Adam step of one node if flagged (umap_adam_update, umap_optimiser_gpu.rs:887-923),

@@ -262,6 +262,19 @@ void mrs_umap_params_gpu_default(mrs_umap_params_gpu* p) {
     p->epoch_strategy = MRS_EPOCH_EDGE_ATOMIC;
 }
 
+void mrs_umap_params_gpu_default_2d(float min_dist, float spread, mrs_umap_params_gpu* p) {
+    if (!p) return;
+    // UmapParamsGpu::new_default_2d, src/lib.rs:3338-3346: the defaults with
+    // the optimiser's a, b fitted to min_dist (None: 0.5) and spread (None: 1.0)
+    mrs_umap_params_gpu_default(p);
+    const float md = std::isnan(min_dist) ? 0.5f : min_dist;
+    const float sp = std::isnan(spread) ? 1.0f : spread;
+    mrs_umap_optim_params o{};
+    if (mrs_umap_optim_params_from_min_dist_spread(md, sp, NAN, NAN, -1, -1, NAN, NAN, NAN, &o) ==
+        MRS_OK)
+        p->optim_params = o;
+}
+
 int mrs_umap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d, const int32_t* knn_idx,
                  const float* knn_dist, int32_t knn_k, const mrs_umap_params_gpu* params,
                  uint64_t seed, int32_t verbose, float* out) {

@@ -56,7 +56,8 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
                                                     float* __restrict__ rho,
                                                     float* __restrict__ w_out,
                                                     uint32_t* __restrict__ out_cnt,
-                                                    uint32_t* __restrict__ in_cnt) {
+                                                    uint32_t* __restrict__ in_cnt,
+                                                    uint32_t* __restrict__ bad_idx) {
     uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= n) return;
     const float* d = dist + (size_t)i * k;
@@ -95,7 +96,9 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
         a = a > 0.0f ? a : 0.0f;
         const float w = smooth ? expf(-a * inv_sigma) : (a > 0.0f ? 0.0f : 1.0f);
         w_out[(size_t)i * k + j] = w;
-        if (c >= 0 && (uint32_t)c < n && (uint32_t)c != i) {
+        if (c < 0 || (uint32_t)c >= n) {
+            atomicAdd(bad_idx, 1u);  // the caller's precomputed kNN names no such point
+        } else if ((uint32_t)c != i) {
             cnt++;
             atomicAdd(&in_cnt[c], 1u);
         }
@@ -350,8 +353,7 @@ __global__ __launch_bounds__(kWg) void k_emit(
     const uint32_t* __restrict__ present_scan, const uint32_t* __restrict__ off, uint64_t m,
     uint32_t* __restrict__ coo_row, uint32_t* __restrict__ coo_col, float* __restrict__ coo_val,
     uint32_t* __restrict__ csr_other, uint32_t* __restrict__ csr_edge_idx,
-    uint32_t* __restrict__ edge_ij, uint32_t* __restrict__ edge_loc, float* __restrict__ edge_w,
-    uint32_t* __restrict__ bad_loc) {
+    uint4* __restrict__ edges, float* __restrict__ edge_w) {
     uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
     if (p >= m) return;
     const uint32_t r = u_row[p], c = u_col[p];
@@ -370,13 +372,9 @@ __global__ __launch_bounds__(kWg) void k_emit(
     }
     if (upper[p]) {
         const uint32_t e = up_scan[p];
-        edge_ij[2 * e] = r;
-        edge_ij[2 * e + 1] = c;
         edge_w[e] = u_w[p];
-        const uint32_t li = present_scan[p] - off[r];
-        const uint32_t lj = present_scan[mp] - off[c];
-        if (li > 0xFFFFu || lj > 0xFFFFu) atomicOr(bad_loc, 1u);
-        edge_loc[e] = (li & 0xFFFFu) | ((lj & 0xFFFFu) << 16);
+        // the edge's CSR slots under r and c: the edge_local keys of its negatives
+        edges[e] = make_uint4(r, c, present_scan[p] - off[r], present_scan[mp] - off[c]);
     }
 }
 
@@ -557,21 +555,18 @@ void finish_graph(mrs_graph* g, uint32_t m) {
     g->off.alloc((size_t)n + 1);
     g->csr_other.alloc(2 * ne);
     g->csr_edge_idx.alloc(2 * ne);
-    g->edge_ij.alloc(2 * ne);
-    g->edge_loc.alloc(ne);
+    g->edges.alloc(ne);
     g->edge_w.alloc(ne);
     g->per.alloc(ne);
     hipLaunchKernelGGL(k_offsets, dim3(div_up((uint64_t)n + 1, kWg)), dim3(kWg), 0, s,
                        g->u_off.p, present_scan.p, tot[2], (uint64_t)m, n, g->off.p);
     MRS_CHECK_LAUNCH();
-    DevBuf<uint32_t> bad(1);
-    MRS_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), s));
     if (m) {
         hipLaunchKernelGGL(k_emit, dim3(div_up(m, kWg)), dim3(kWg), 0, s, g->u_row.p,
                            g->u_col.p, g->u_w.p, g->u_keep.p, mirror.p, upper.p, keep_scan.p,
                            up_scan.p, present_scan.p, g->off.p, (uint64_t)m, g->coo_row.p,
                            g->coo_col.p, g->coo_val.p, g->csr_other.p, g->csr_edge_idx.p,
-                           g->edge_ij.p, g->edge_loc.p, g->edge_w.p, bad.p);
+                           g->edges.p, g->edge_w.p);
         MRS_CHECK_LAUNCH();
     }
     if (g->n_edges) {
@@ -583,7 +578,6 @@ void finish_graph(mrs_graph* g, uint32_t m) {
                            g->edge_w.p, (uint64_t)g->n_edges, max_bits.p + 1, g->per.p);
         MRS_CHECK_LAUNCH();
     }
-    g->loc_packed = read_u32(bad.p, s) == 0;
 }
 
 // ---- import of a CoordinateList (src/data/structures.rs:22-31)
@@ -764,14 +758,21 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         g->sigma.alloc(n);
         g->rho.alloc(n);
         DevBuf<float> w((size_t)n * k);
-        DevBuf<uint32_t> out_cnt(n), in_cnt(n), raw_cnt(n + 1), raw_off(n + 1), cursor(n);
+        DevBuf<uint32_t> out_cnt(n), in_cnt(n), raw_cnt(n + 1), raw_off(n + 1), cursor(n),
+            bad_idx(1);
+        MRS_HIP(hipMemsetAsync(bad_idx.p, 0, bad_idx.bytes(), s));
         MRS_HIP(hipMemsetAsync(in_cnt.p, 0, in_cnt.bytes(), s));
         MRS_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), s));
         MRS_HIP(hipMemsetAsync(raw_cnt.p, 0, raw_cnt.bytes(), s));
         hipLaunchKernelGGL(k_smooth_knn, dim3(div_up(n, kWg)), dim3(kWg), 0, s, knn_dist, knn_idx,
                            n, k, local_connectivity, (double)bandwidth, std::log((double)k), 64,
-                           g->sigma.p, g->rho.p, w.p, out_cnt.p, in_cnt.p);
+                           g->sigma.p, g->rho.p, w.p, out_cnt.p, in_cnt.p, bad_idx.p);
         MRS_CHECK_LAUNCH();
+        // the reference indexes its point arrays with every neighbour id and
+        // stops on one out of range; so does this build, before any use of it
+        if (const uint32_t nbad = read_u32(bad_idx.p, s))
+            throw Error(MRS_INVALID_ARGUMENT, std::to_string(nbad) +
+                                                  " kNN indices outside [0, n)");
         hipLaunchKernelGGL(k_add_u32, dim3(div_up(n, kWg)), dim3(kWg), 0, s, out_cnt.p, in_cnt.p,
                            raw_cnt.p, n);
         MRS_CHECK_LAUNCH();
@@ -984,7 +985,6 @@ int mrs_umap_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const float* emb
             st->dim = n_dim;
             st->n_edges = g->n_edges;
             st->params = *params;
-            st->loc_packed = g->loc_packed;
             auto cp = [&](auto& dst, const auto& src, size_t count) {
                 dst.alloc(count);
                 MRS_HIP(hipMemcpyAsync(dst.p, src.p, dst.bytes(), hipMemcpyDeviceToDevice, s));
@@ -993,8 +993,7 @@ int mrs_umap_state_from_graph(mrs_ctx* ctx, const mrs_graph* g, const float* emb
             cp(st->off, g->off, (size_t)g->n + 1);
             cp(st->csr_edge_idx, g->csr_edge_idx, 2 * E);
             cp(st->csr_other, g->csr_other, 2 * E);
-            cp(st->edge_ij, g->edge_ij, 2 * E);
-            cp(st->edge_loc, g->edge_loc, E);
+            cp(st->edges, g->edges, E);
             cp(st->per, g->per, E);
             cp(st->next, g->per, E);
             cp(st->mu, g->edge_w, E);

@@ -30,11 +30,9 @@ struct mrs_graph {
     mrs::DevBuf<uint32_t> off;           // [n+1]
     mrs::DevBuf<uint32_t> csr_edge_idx;  // [2E]
     mrs::DevBuf<uint32_t> csr_other;     // [2E]
-    mrs::DevBuf<uint32_t> edge_ij;       // [2E]
-    mrs::DevBuf<uint32_t> edge_loc;      // [E]
+    mrs::DevBuf<uint4> edges;            // [E] {i, j, CSR slot under i, CSR slot under j}
     mrs::DevBuf<float> edge_w;           // [E]
     mrs::DevBuf<float> per;              // [E]
-    bool loc_packed = true;
 };
 
 namespace mrs {

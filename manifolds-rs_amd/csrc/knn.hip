@@ -1262,7 +1262,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     };
     int splits = 1;
     double swept = 1.0, list_mean = 0.0;
-    uint32_t cells = 0, list_max = 0;
+    uint32_t cells = 0, list_max = 0, aux_items = 0;
 
     const bool prune = ctx->knn_prune && n >= kPruneMinRows && sh.W * kQPerWave == (int)kBlockRows;
     if (!prune) {
@@ -1472,8 +1472,11 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                            nst, counts.p, offs.p, ids.p);
         MRS_CHECK_LAUNCH();
         // chunks: a tenth of a round of workgroups' share, at least 1024 stages
-        const uint32_t chunk = std::max<uint32_t>(1024u, (uint32_t)((uint64_t)total / 2048u));
+        // (or the context's chunk, which the tests lower to split small problems)
+        const uint32_t chunk = ctx->knn_chunk ? ctx->knn_chunk
+                                              : std::max<uint32_t>(1024u, (uint32_t)((uint64_t)total / 2048u));
         make_items(chunk);
+        aux_items = n_aux;
         upload(items2, hitems);
         upload(dmerges, hmerges);
         DevBuf<uint64_t> aux_lists((size_t)std::max<uint32_t>(n_aux, 1u) * kBlockRows * S);
@@ -1535,6 +1538,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         timings->cells = (int)cells;
         timings->list_max = list_max;
         timings->list_mean = list_mean;
+        timings->aux_items = aux_items;
     }
 }
 
@@ -1583,6 +1587,7 @@ int mrs_knn_exhaustive_timed(mrs_ctx* ctx, const float* x, int64_t n, int32_t d,
             ms_out[6] = t.cells;
             ms_out[7] = t.list_max;
             ms_out[8] = t.list_mean;
+            ms_out[9] = t.aux_items;
         }
     });
 }

@@ -12,6 +12,7 @@ struct KnnTimings {
     int cells = 0;       // cells of the pruned sweep (0: full sweep)
     uint32_t list_max = 0;   // longest pass-2 stage list of a query block
     double list_mean = 0.0;  // mean pass-2 stage list
+    uint32_t aux_items = 0;  // pass-2 work items that fill aux lists (k_merge_aux folds them in)
 };
 
 // x: device [n][d] f32; out_idx/out_dist: device [n][k].

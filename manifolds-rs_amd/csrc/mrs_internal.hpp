@@ -245,4 +245,5 @@ struct mrs_ctx {
     size_t hbm_budget = 0;
     int verbose = 0;
     int knn_prune = 1;  // exhaustive kNN skips stages that provably hold no candidate
+    uint32_t knn_chunk = 0;  // pass-2 work-item length in stages (0: the size-based default)
 };

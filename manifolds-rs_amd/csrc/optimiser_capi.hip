@@ -2,6 +2,8 @@
 // src/training/umap_optimiser_gpu.rs:1016-1272 and the driver of :1302-1369.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <numeric>
 #include <vector>
@@ -62,14 +64,78 @@ BufRef state_buffer(mrs_umap_state* st, int which) {
         case 12: return {st->dens_phi.p, st->dens_phi.bytes()};
         case 13: return {st->dens_w.p, st->dens_w.bytes()};
         case 14: return {st->dens_r.p, st->dens_r.bytes()};
-        case 15: return {st->edge_ij.p, st->edge_ij.bytes()};
         default: throw Error(MRS_INVALID_ARGUMENT, "unknown state buffer id");
     }
+}
+
+// UmapOptimParams::fit_params (src/training/umap_optimisers.rs:139-209): a, b of
+// 1 / (1 + a x^(2b)) fitted by plain gradient descent (step 1, n_iter
+// iterations, a clamped to [0.001, 10] and b to [0.1, 2] after each step) to
+// the target curve 1 for x < min_dist, exp(-(x - min_dist) / spread) beyond,
+// sampled at 300 points on [0, 3 spread].  Every operation is the reference's,
+// in f32 and in its order (this file is built without contraction), so the
+// result is the reference's to the last bit wherever libm's f32 powf / expf /
+// logf agree.
+void fit_curve_ab(float min_dist, float spread, int n_iter, float* a_out, float* b_out) {
+    constexpr int kPoints = 300;
+    const float two = 2.0f, one = 1.0f;
+    const float step = spread * 3.0f / (float)(kPoints - 1);
+    float xs[kPoints], ys[kPoints], lx[kPoints];
+    for (int i = 0; i < kPoints; i++) {
+        xs[i] = step * (float)i;
+        ys[i] = xs[i] < min_dist ? one : expf(-(xs[i] - min_dist) / spread);
+        lx[i] = xs[i] > 0.0f ? logf(xs[i]) : 0.0f;  // x = 0 takes no part in the fit
+    }
+    const float n_pts = (float)kPoints;
+    float a = one, b = one;
+    for (int it = 0; it < n_iter; it++) {
+        float ga = 0.0f, gb = 0.0f;
+        const float two_b = two * b;
+        for (int i = 1; i < kPoints; i++) {  // xs[0] = 0 is skipped (x <= 0)
+            const float p2b = powf(xs[i], two_b);
+            const float den = one + a * p2b;
+            const float den2 = den * den;
+            const float err = one / den - ys[i];
+            ga += err * (-p2b / den2);
+            gb += err * (-two * a * p2b * lx[i] / den2);
+        }
+        ga /= n_pts;
+        gb /= n_pts;
+        a -= 1.0f * ga;
+        b -= 1.0f * gb;
+        a = std::min(std::max(a, 0.001f), 10.0f);
+        b = std::min(std::max(b, 0.1f), 2.0f);
+    }
+    *a_out = a;
+    *b_out = b;
 }
 
 }  // namespace
 
 extern "C" {
+
+int mrs_umap_optim_params_from_min_dist_spread(float min_dist, float spread, float lr,
+                                               float gamma, int32_t n_epochs,
+                                               int32_t neg_sample_rate, float beta1, float beta2,
+                                               float eps, mrs_umap_optim_params* out) {
+    return mrs::guarded([&] {
+        if (!out) throw Error(MRS_INVALID_ARGUMENT, "out is NULL");
+        if (!(spread > 0.0f) || std::isnan(min_dist))
+            throw Error(MRS_INVALID_ARGUMENT, "spread must be positive and min_dist a number");
+        // None (NaN / negative) takes the reference's default (:99-106)
+        mrs_umap_optim_params p{};
+        fit_curve_ab(min_dist, spread, 300, &p.a, &p.b);
+        p.lr = std::isnan(lr) ? 1.0f : lr;
+        p.gamma = std::isnan(gamma) ? 1.0f : gamma;
+        p.n_epochs = n_epochs < 0 ? 500 : n_epochs;
+        p.neg_sample_rate = neg_sample_rate < 0 ? 5 : neg_sample_rate;
+        p.min_dist = min_dist;
+        p.beta1 = std::isnan(beta1) ? 0.5f : beta1;   // UMAP_BETA1, src/training/mod.rs:16
+        p.beta2 = std::isnan(beta2) ? 0.9f : beta2;   // UMAP_BETA2, :18
+        p.eps = std::isnan(eps) ? 1e-7f : eps;        // EPS, :24
+        *out = p;
+    });
+}
 
 int mrs_umap_state_create(mrs_ctx* ctx, const float* embd, int64_t n, int32_t n_dim,
                           const int64_t* adj_off, const int32_t* adj_cols,
@@ -110,6 +176,17 @@ int mrs_umap_state_info(const mrs_umap_state* st, int64_t* n, int32_t* n_dim, in
 int mrs_umap_state_read(mrs_umap_state* st, int32_t which, void* outp) {
     return mrs::guarded([&] {
         if (!st || !outp) throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (which == 15 || which == 16) {
+            // (i, j) or (slot under i, slot under j) of every 16-byte edge record
+            const size_t half = 2 * sizeof(uint32_t);
+            MRS_HIP(hipMemcpy2DAsync(outp, half,
+                                     reinterpret_cast<const char*>(st->edges.p) +
+                                         (which == 16 ? half : 0),
+                                     sizeof(uint4), half, (size_t)st->n_edges, hipMemcpyDefault,
+                                     st->ctx->stream));
+            MRS_HIP(hipStreamSynchronize(st->ctx->stream));
+            return;
+        }
         BufRef b = state_buffer(st, which);
         MRS_HIP(hipMemcpyAsync(outp, b.p, b.bytes, hipMemcpyDefault, st->ctx->stream));
         MRS_HIP(hipStreamSynchronize(st->ctx->stream));
@@ -119,6 +196,8 @@ int mrs_umap_state_read(mrs_umap_state* st, int32_t which, void* outp) {
 int mrs_umap_state_write(mrs_umap_state* st, int32_t which, const void* in) {
     return mrs::guarded([&] {
         if (!st || !in) throw Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (which == 15 || which == 16)
+            throw Error(MRS_INVALID_ARGUMENT, "the edge records follow the graph: read-only");
         BufRef b = state_buffer(st, which);
         MRS_HIP(hipMemcpyAsync(b.p, in, b.bytes, hipMemcpyDefault, st->ctx->stream));
         MRS_HIP(hipStreamSynchronize(st->ctx->stream));

@@ -56,8 +56,7 @@ namespace {
 
 // One resident edge-parallel fit in a batched epoch launch.
 struct FitDesc {
-    const uint32_t* edge_ij;
-    const uint32_t* edge_loc;
+    const uint4* edges;
     const float* per;
     float* next;
     float* embd;
@@ -94,7 +93,7 @@ __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restri
                                                          int n_fits) {
     const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x, true)];
     const uint32_t e = (blockIdx.x - f.blk_edge0) * kWg + threadIdx.x;
-    edge_epoch<DIM, false>(e, f.edge_ij, f.edge_loc, f.per, f.next, f.embd, f.grad, f.flag,
+    edge_epoch<DIM, false>(e, f.edges, f.per, f.next, f.embd, f.grad, f.flag,
                            f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b, f.two_a_b,
                            f.two_gamma_b, f.neg_rate, nullptr, nullptr, nullptr);
 }
@@ -263,8 +262,7 @@ void worker_loop(mrs_server* sv, int w) {
 void fill_desc(FitDesc& f, const mrs_umap_state* st, int32_t epoch, uint32_t seed) {
     const mrs_umap_optim_params& p = st->params;
     const AdamScalars sc = adam_scalars(p, epoch);
-    f.edge_ij = st->edge_ij.p;
-    f.edge_loc = st->edge_loc.p;
+    f.edges = st->edges.p;
     f.per = st->per.p;
     f.next = st->next.p;
     f.embd = st->embd.p;
@@ -329,8 +327,7 @@ void batcher_loop(mrs_server* sv) {
                 Group g{dim, nd, 0, 0, 0};
                 for (mrs_server_request* r : active) {
                     const mrs_umap_state* st = r->st.get();
-                    if (r->params.epoch_strategy != MRS_EPOCH_EDGE_ATOMIC || st->dim != dim ||
-                        !st->loc_packed)
+                    if (r->params.epoch_strategy != MRS_EPOCH_EDGE_ATOMIC || st->dim != dim)
                         continue;
                     FitDesc& f = hd[nd++];
                     fill_desc(f, st, r->epoch, (uint32_t)r->seed);
@@ -366,7 +363,7 @@ void batcher_loop(mrs_server* sv) {
             for (mrs_server_request* r : active) {
                 mrs_umap_state* st = r->st.get();
                 const bool batched = r->params.epoch_strategy == MRS_EPOCH_EDGE_ATOMIC &&
-                                     edge_dim_ok(st->dim) && st->loc_packed;
+                                     edge_dim_ok(st->dim);
                 if (!batched)
                     run_epochs(st, r->epoch, 1, (uint32_t)r->seed, r->params.epoch_strategy);
             }

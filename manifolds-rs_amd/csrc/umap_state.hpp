@@ -20,10 +20,9 @@ struct mrs_umap_state {
     mrs::DevBuf<uint32_t> off;           // [n+1]
     mrs::DevBuf<uint32_t> csr_edge_idx;  // [2E]
     mrs::DevBuf<uint32_t> csr_other;     // [2E]
-    // edge stream for the edge-parallel kernel, edges sorted by (i, j), i < j
-    mrs::DevBuf<uint32_t> edge_ij;   // [2E] interleaved (i, j)
-    mrs::DevBuf<uint32_t> edge_loc;  // [E] local_i | local_j << 16 (CSR slot of the edge)
-    bool loc_packed = true;          // false if a degree >= 65536 (edge kernel unavailable)
+    // edge stream for the edge-parallel kernel, edges sorted by (i, j), i < j:
+    // one 16-byte record per edge, {i, j, CSR slot of the edge under i, under j}
+    mrs::DevBuf<uint4> edges;        // [E]
     mrs::DevBuf<float> per, next, mu;  // [E]
 
     // grad/has_update are all-zero (ready for the accumulate-by-atomics kernel)
@@ -72,9 +71,9 @@ constexpr int kDensBlocks = 256;
 // arrays and local CSR slots used by the edge-parallel kernel.
 struct HostCsr {
     int64_t n = 0, n_edges = 0;
-    std::vector<uint32_t> off, csr_edge_idx, csr_other, edge_ij, edge_loc;
+    std::vector<uint32_t> off, csr_edge_idx, csr_other;
+    std::vector<uint4> edges;
     std::vector<float> per, mu;
-    bool loc_packed = true;
 };
 // Throws Error(MRS_NO_DATA / MRS_NO_GRAPH_EDGES) like the reference.
 HostCsr host_csr_from_graph(int64_t n, const int64_t* adj_off, const int32_t* adj_cols,

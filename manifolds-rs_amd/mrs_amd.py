@@ -65,6 +65,21 @@ class UmapOptimParams(C.Structure):
         return cls(a=1.5, b=0.9, lr=1.0, gamma=1.0, n_epochs=500, neg_sample_rate=5,
                    min_dist=0.1, beta1=0.5, beta2=0.9, eps=1e-7)
 
+    @classmethod
+    def from_min_dist_spread(cls, min_dist: float, spread: float, lr=None, gamma=None,
+                             n_epochs=None, neg_sample_rate=None, beta1=None, beta2=None,
+                             eps=None) -> "UmapOptimParams":
+        # UmapOptimParams::from_min_dist_spread, src/training/umap_optimisers.rs:89-122:
+        # a, b fitted to (min_dist, spread) by the library; None = the reference's default
+        nan = float("nan")
+        f = lambda v: nan if v is None else float(v)
+        i = lambda v: -1 if v is None else int(v)
+        out = cls()
+        _check(lib().mrs_umap_optim_params_from_min_dist_spread(
+            min_dist, spread, f(lr), f(gamma), i(n_epochs), i(neg_sample_rate), f(beta1),
+            f(beta2), f(eps), C.byref(out)))
+        return out
+
 
 class DensParams(C.Structure):
     """DensParams<f32> (src/utils/density.rs:69-79); defaults :38-55."""
@@ -111,6 +126,15 @@ class UmapParamsGpu(C.Structure):
         lib().mrs_umap_params_gpu_default(C.byref(p))
         return p
 
+    @classmethod
+    def new_default_2d(cls, min_dist=None, spread=None) -> "UmapParamsGpu":
+        # UmapParamsGpu::new_default_2d, src/lib.rs:3338-3346
+        p = cls()
+        nan = float("nan")
+        lib().mrs_umap_params_gpu_default_2d(nan if min_dist is None else float(min_dist),
+                                             nan if spread is None else float(spread), C.byref(p))
+        return p
+
 
 _lib = None
 
@@ -148,6 +172,12 @@ def lib() -> C.CDLL:
         "mrs_ctx_synchronize": (C.c_int, [vp]),
         "mrs_ctx_stream": (vp, [vp]),
         "mrs_ctx_set_knn_pruning": (C.c_int, [vp, i32]),
+        "mrs_ctx_set_knn_work_chunk": (C.c_int, [vp, i32]),
+        "mrs_umap_optim_params_from_min_dist_spread": (C.c_int, [
+            C.c_float, C.c_float, C.c_float, C.c_float, i32, i32, C.c_float, C.c_float,
+            C.c_float, C.POINTER(UmapOptimParams)]),
+        "mrs_umap_params_gpu_default_2d": (None, [C.c_float, C.c_float,
+                                                  C.POINTER(UmapParamsGpu)]),
         "mrs_probe_random_gather": (C.c_int, [vp, i64, i64, i32, C.POINTER(C.c_double)]),
         "mrs_ctx_cached_bytes": (C.c_size_t, [vp]),
         "mrs_ctx_release_cache": (C.c_int, [vp]),
@@ -270,6 +300,11 @@ class Context:
         (default) or sweeps every stage; both give the same results."""
         _check(lib().mrs_ctx_set_knn_pruning(self.handle, 1 if on else 0))
 
+    def set_knn_work_chunk(self, stages: int) -> None:
+        """Second-pass stage lists longer than `stages` are cut into work items
+        merged afterwards (0: the size-based default); results do not change."""
+        _check(lib().mrs_ctx_set_knn_work_chunk(self.handle, int(stages)))
+
     def probe_random_gather(self, rows: int, gathers: int = 1 << 24, reps: int = 10) -> float:
         """Uniform random 8-byte gathers per second from a `rows`-row float2
         table (the epoch kernel's request pattern), measured on this context."""
@@ -312,6 +347,7 @@ def adjacency_to_csr(graph: list[list[tuple[int, float]]]):
 BUF_EMBD, BUF_M, BUF_V, BUF_GRAD, BUF_HAS_UPDATE = 0, 1, 2, 3, 4
 BUF_OFFSETS, BUF_EDGE_IDX, BUF_OTHER, BUF_PER, BUF_NEXT, BUF_MU = 5, 6, 7, 8, 9, 10
 BUF_DENS_RE, BUF_DENS_PHI, BUF_DENS_W, BUF_DENS_R, BUF_EDGE_IJ = 11, 12, 13, 14, 15
+BUF_EDGE_SLOTS = 16
 
 
 def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist: int = 0,
@@ -332,12 +368,13 @@ def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist
         idx = torch.empty((n, k), dtype=torch.int32, device=x.device)
         dist = torch.empty((n, k), dtype=torch.float32, device=x.device)
     if timed:
-        ms = (C.c_double * 9)()
+        ms = (C.c_double * 10)()
         _check(lib().mrs_knn_exhaustive_timed(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                               _ptr(idx), _ptr(dist), ms))
         return idx, dist, dict(pack_ms=ms[0], shortlist_ms=ms[1], rerank_ms=ms[2],
                                splits=int(ms[3]), shortlist=int(ms[4]), swept=ms[5],
-                               cells=int(ms[6]), list_max=int(ms[7]), list_mean=ms[8])
+                               cells=int(ms[6]), list_max=int(ms[7]), list_mean=ms[8],
+                               aux_items=int(ms[9]))
     _check(lib().mrs_knn_exhaustive(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                     _ptr(idx), _ptr(dist)))
     return idx, dist
@@ -465,7 +502,8 @@ class UmapGpuState:
         table = {0: (n * d, np.float32), 1: (n * d, np.float32), 2: (n * d, np.float32),
                  3: (n * d, np.float32), 4: (n, np.uint32), 5: (n + 1, np.uint32),
                  6: (2 * e, np.uint32), 7: (2 * e, np.uint32), 8: (e, np.float32),
-                 9: (e, np.float32), 10: (e, np.float32), 15: (2 * e, np.uint32)}
+                 9: (e, np.float32), 10: (e, np.float32), 15: (2 * e, np.uint32),
+                 16: (2 * e, np.uint32)}
         if which in (11, 12, 13, 14):
             return (n, np.float32)
         return table[which]

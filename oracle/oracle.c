@@ -33,13 +33,24 @@
  *                           dense-matrix restatement on random small graphs
  *   orc_filter_weak_edges   closed-form threshold on random inputs
  *   orc_knn_exact_*         scipy cdist + lexsort brute force on small inputs
- *   orc_fit_params          bound/shape invariants (umap_optimisers.rs:1559-1593)
+ *   orc_fit_params          bound/shape invariants (umap_optimisers.rs:1559-1593);
+ *                           its step (orc_fit_params_grad) equals central
+ *                           differences of the stated mean squared curve error;
+ *                           n_iter steps compose as projected gradient descent
+ *   orc_lut_pow             x^b within the linear-interpolation error bound of
+ *                           each cell (h^2/8 max|f''|), exact powf from 25 up
+ *   orc_density_edge_coeff  hand values at a = b = d^2 = 1 and its max(eps)
+ *                           floors on a zero phi sum and a zero edge weight
  *   orc_optimise_adam_gpu   composition of the pinned pieces above
  *   orc_optimise_adam_parallel, orc_optimise_sgd
- *                           parity unpinned: the reference draws negatives from
- *                           rand 0.9.5 SmallRng streams that cannot be run here;
- *                           checked only through quality invariants
- *                           (cluster separation, trustworthiness).
+ *                           hand-computed float64 goldens on 2- and 3-node graphs
+ *                           (schedule, first tick drawing rate - 1 negatives,
+ *                           floor of fractional negative counts, negatives on the
+ *                           smaller endpoint only, in-place SGD with the
+ *                           attractive clip, the densMAP term); the reference's
+ *                           own negatives come from rand 0.9.5 SmallRng streams
+ *                           that cannot be run here, so these functions draw the
+ *                           hash's negatives instead (the GPU path's keys).
  */
 #include "oracle.h"
 
@@ -127,13 +138,16 @@ int orc_csr_from_graph(int64_t n, const int64_t *adj_off, const int32_t *adj_col
 
 /* ------------------------------------------------------- gradient mirror */
 
-/* cpu_grad_accum, umap_optimiser_gpu.rs:1461-1579 */
-void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
-                    const uint32_t *csr_edge_idx, const uint32_t *csr_other,
-                    const float *next_sample, int32_t epoch, uint32_t seed, int32_t neg_rate,
-                    const orc_optim_params *p, const float *mu, const float *dens_re,
-                    const float *dens_phi_sum, const float *dens_weight, float dens_coeff,
-                    float *grad, uint32_t *has_update) {
+/* cpu_grad_accum, umap_optimiser_gpu.rs:1461-1579.  exact != 0: the same f32
+ * terms, summed in double and rounded once (the exact sum of the reference's
+ * terms, the yardstick for any summation order) */
+static void grad_accum_impl(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                            const uint32_t *csr_edge_idx, const uint32_t *csr_other,
+                            const float *next_sample, int32_t epoch, uint32_t seed,
+                            int32_t neg_rate, const orc_optim_params *p, const float *mu,
+                            const float *dens_re, const float *dens_phi_sum,
+                            const float *dens_weight, float dens_coeff, float *grad,
+                            uint32_t *has_update, float *abs_out, int exact) {
     const float two = 2.0f, one = 1.0f;
     const float clip = ORC_GRAD_CLIP_VAL, rep_eps = ORC_GRAD_REP_EPS;
     const float two_a_b = two * p->a * p->b;
@@ -141,10 +155,23 @@ void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t 
     const float epoch_t = (float)epoch;
     const int dens = dens_re != NULL;
     memset(grad, 0, sizeof(float) * (size_t)n * (size_t)n_dim);
+    if (abs_out) memset(abs_out, 0, sizeof(float) * (size_t)n * (size_t)n_dim);
+    /* nodes are independent (each writes only its own gradient): parallel over nodes */
+#pragma omp parallel for schedule(dynamic, 64)
     for (int64_t node = 0; node < n; node++) {
         const float *ys = embd + node * n_dim;
         float *g = grad + node * n_dim;
+        double gx[16] = {0}; /* the exact mode's sums (n_dim <= 16) */
+        /* |term| sums per coordinate, the scale of each gradient's rounding */
+        float *ab = abs_out ? abs_out + node * n_dim : NULL;
         uint32_t active = 0, edge_local = 0;
+#define ADD_TERM(d, t)                   \
+    do {                                 \
+        float term_ = (t);               \
+        if (exact) gx[d] += (double)term_; \
+        else g[d] += term_;              \
+        if (ab) ab[d] += fabsf(term_);   \
+    } while (0)
         for (uint32_t pos = off[node]; pos < off[node + 1]; pos++, edge_local++) {
             uint32_t e = csr_edge_idx[pos];
             if (next_sample[e] > epoch_t) continue;
@@ -162,7 +189,7 @@ void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t 
                 float grad_coeff = two_a_b * dist_sq_b / (dist_sq * denom);
                 for (int d = 0; d < n_dim; d++) {
                     float delta = yo[d] - ys[d];
-                    g[d] += two * grad_coeff * delta;
+                    ADD_TERM(d, two * grad_coeff * delta);
                 }
                 if (dens) {
                     float phi = one / denom;
@@ -181,7 +208,7 @@ void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t 
                         float gg = two_cor * delta;
                         if (gg > clip) gg = clip;
                         if (gg < -clip) gg = -clip;
-                        g[d] += two * gg;
+                        ADD_TERM(d, two * gg);
                     }
                 }
             }
@@ -203,12 +230,38 @@ void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t 
                 if (gc < -clip) gc = -clip;
                 for (int d = 0; d < n_dim; d++) {
                     float delta = ys[d] - yk[d];
-                    g[d] += gc * delta;
+                    ADD_TERM(d, gc * delta);
                 }
             }
         }
+#undef ADD_TERM
+        if (exact)
+            for (int d = 0; d < n_dim; d++) g[d] = (float)gx[d];
         if (has_update) has_update[node] = active;
     }
+}
+
+void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                    const uint32_t *csr_edge_idx, const uint32_t *csr_other,
+                    const float *next_sample, int32_t epoch, uint32_t seed, int32_t neg_rate,
+                    const orc_optim_params *p, const float *mu, const float *dens_re,
+                    const float *dens_phi_sum, const float *dens_weight, float dens_coeff,
+                    float *grad, uint32_t *has_update, float *abs_out) {
+    grad_accum_impl(embd, n, n_dim, off, csr_edge_idx, csr_other, next_sample, epoch, seed,
+                    neg_rate, p, mu, dens_re, dens_phi_sum, dens_weight, dens_coeff, grad,
+                    has_update, abs_out, 0);
+}
+
+void orc_grad_accum_exact(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                          const uint32_t *csr_edge_idx, const uint32_t *csr_other,
+                          const float *next_sample, int32_t epoch, uint32_t seed,
+                          int32_t neg_rate, const orc_optim_params *p, const float *mu,
+                          const float *dens_re, const float *dens_phi_sum,
+                          const float *dens_weight, float dens_coeff, float *grad,
+                          uint32_t *has_update, float *abs_out) {
+    grad_accum_impl(embd, n, n_dim, off, csr_edge_idx, csr_other, next_sample, epoch, seed,
+                    neg_rate, p, mu, dens_re, dens_phi_sum, dens_weight, dens_coeff, grad,
+                    has_update, abs_out, 1);
 }
 
 /* ------------------------------------------------------------------- Adam */
@@ -374,7 +427,7 @@ int orc_optimise_adam_gpu(float *embd, int64_t n, int32_t n_dim, const int64_t *
             coeff = orc_dens_weights(dens_r, re, n, dens_var_shift, dens_lambda, dens_mu_tot, w);
         }
         orc_grad_accum(embd, n, n_dim, off, eidx, oth, next, e, seed32, p->neg_sample_rate, p,
-                       mu, dens_on ? re : NULL, phi, w, coeff, g, hu);
+                       mu, dens_on ? re : NULL, phi, w, coeff, g, hu, NULL);
         orc_adam_update(g, hu, m, v, embd, n, n_dim, p, e, p->n_epochs);
         orc_schedule_update(per, next, ne, e);
     }
@@ -550,80 +603,91 @@ int64_t orc_filter_weak_edges(int32_t *rows, int32_t *cols, float *vals, int64_t
 /* ------------------------------------------------------------------- kNN */
 
 #define KNN_MAXK 256
+#define KNN_QT 8 /* queries per tile: each candidate row is read once per 8 queries */
+
+/* Exhaustive kNN of the queries qrows[0..nq) (nq <= KNN_QT) against all n rows:
+ * squared distances summed over dims in order (fmaf for f32, plain in f64), the
+ * k smallest kept in an insertion list where an equal distance never displaces
+ * an earlier (smaller) index.  Per (query, candidate) pair the operations are
+ * exactly those of a scalar double loop; the tile only interleaves 8 independent
+ * pairs so the sums vectorise across queries. */
+#define KNN_TILE(NAME, T, ACC)                                                               \
+    static void NAME(const float *x, int64_t n, int32_t d, int32_t k, const int64_t *qrows,   \
+                     int nq, int32_t *idx, T *dist) {                                          \
+        T *qt = (T *)calloc((size_t)d * KNN_QT, sizeof(T));                                   \
+        T bd[KNN_QT][KNN_MAXK];                                                                \
+        int32_t bi[KNN_QT][KNN_MAXK];                                                          \
+        int cnt[KNN_QT] = {0};                                                                 \
+        for (int q = 0; q < nq; q++)                                                           \
+            for (int t = 0; t < d; t++) qt[(size_t)t * KNN_QT + q] = (T)x[qrows[q] * d + t];   \
+        for (int64_t j = 0; j < n; j++) {                                                      \
+            const float *xj = x + j * d;                                                       \
+            T s[KNN_QT];                                                                       \
+            for (int q = 0; q < KNN_QT; q++) s[q] = 0;                                         \
+            for (int t = 0; t < d; t++) {                                                      \
+                const T xt = (T)xj[t];                                                         \
+                const T *qrow = qt + (size_t)t * KNN_QT;                                       \
+                for (int q = 0; q < KNN_QT; q++) {                                             \
+                    T df = qrow[q] - xt;                                                       \
+                    s[q] = ACC(df, s[q]);                                                      \
+                }                                                                              \
+            }                                                                                  \
+            for (int q = 0; q < nq; q++) {                                                     \
+                if (j == qrows[q]) continue;                                                   \
+                if (cnt[q] == k && !(s[q] < bd[q][k - 1])) continue;                           \
+                int p = cnt[q] < k ? cnt[q] : k - 1;                                           \
+                while (p > 0 && (s[q] < bd[q][p - 1])) {                                       \
+                    bd[q][p] = bd[q][p - 1];                                                   \
+                    bi[q][p] = bi[q][p - 1];                                                   \
+                    p--;                                                                       \
+                }                                                                              \
+                bd[q][p] = s[q];                                                               \
+                bi[q][p] = (int32_t)j;                                                         \
+                if (cnt[q] < k) cnt[q]++;                                                      \
+            }                                                                                  \
+        }                                                                                      \
+        for (int q = 0; q < nq; q++)                                                           \
+            for (int t = 0; t < k; t++) {                                                      \
+                idx[(int64_t)q * k + t] = t < cnt[q] ? bi[q][t] : -1;                          \
+                dist[(int64_t)q * k + t] = t < cnt[q] ? bd[q][t] : (T)INFINITY;                \
+            }                                                                                  \
+        free(qt);                                                                              \
+    }
+
+#define ACC_F32(df, s) fmaf((df), (df), (s))
+#define ACC_F64(df, s) ((s) + (df) * (df))
+KNN_TILE(knn_tile_f32, float, ACC_F32)
+KNN_TILE(knn_tile_f64, double, ACC_F64)
 
 void orc_knn_exact_f32(const float *x, int64_t n, int32_t d, int32_t k, int64_t q0, int64_t q1,
                        int32_t *idx, float *dist) {
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t q = q0; q < q1; q++) {
-        float bd[KNN_MAXK];
-        int32_t bi[KNN_MAXK];
-        int cnt = 0;
-        const float *xq = x + q * d;
-        for (int64_t j = 0; j < n; j++) {
-            if (j == q) continue;
-            const float *xj = x + j * d;
-            float s = 0.0f;
-            for (int t = 0; t < d; t++) {
-                float df = xq[t] - xj[t];
-                s = fmaf(df, df, s);
-            }
-            if (cnt == k && !(s < bd[k - 1])) continue; /* ties: keep smaller index (seen first) */
-            int p = cnt < k ? cnt : k - 1;
-            while (p > 0 && (s < bd[p - 1])) {
-                bd[p] = bd[p - 1];
-                bi[p] = bi[p - 1];
-                p--;
-            }
-            bd[p] = s;
-            bi[p] = (int32_t)j;
-            if (cnt < k) cnt++;
-        }
-        for (int t = 0; t < k; t++) {
-            idx[(q - q0) * k + t] = t < cnt ? bi[t] : -1;
-            dist[(q - q0) * k + t] = t < cnt ? bd[t] : INFINITY;
-        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t a = q0; a < q1; a += KNN_QT) {
+        int64_t rows[KNN_QT];
+        int nq = (int)(q1 - a < KNN_QT ? q1 - a : KNN_QT);
+        for (int q = 0; q < nq; q++) rows[q] = a + q;
+        knn_tile_f32(x, n, d, k, rows, nq, idx + (a - q0) * k, dist + (a - q0) * k);
     }
 }
 
 void orc_knn_exact_f64(const float *x, int64_t n, int32_t d, int32_t k, int64_t q0, int64_t q1,
                        int32_t *idx, double *dist) {
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t q = q0; q < q1; q++) {
-        double bd[KNN_MAXK];
-        int32_t bi[KNN_MAXK];
-        int cnt = 0;
-        const float *xq = x + q * d;
-        for (int64_t j = 0; j < n; j++) {
-            if (j == q) continue;
-            const float *xj = x + j * d;
-            double s = 0.0;
-            for (int t = 0; t < d; t++) {
-                double df = (double)xq[t] - (double)xj[t];
-                s += df * df;
-            }
-            if (cnt == k && !(s < bd[k - 1])) continue;
-            int p = cnt < k ? cnt : k - 1;
-            while (p > 0 && (s < bd[p - 1])) {
-                bd[p] = bd[p - 1];
-                bi[p] = bi[p - 1];
-                p--;
-            }
-            bd[p] = s;
-            bi[p] = (int32_t)j;
-            if (cnt < k) cnt++;
-        }
-        for (int t = 0; t < k; t++) {
-            idx[(q - q0) * k + t] = t < cnt ? bi[t] : -1;
-            dist[(q - q0) * k + t] = t < cnt ? bd[t] : INFINITY;
-        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t a = q0; a < q1; a += KNN_QT) {
+        int64_t rows[KNN_QT];
+        int nq = (int)(q1 - a < KNN_QT ? q1 - a : KNN_QT);
+        for (int q = 0; q < nq; q++) rows[q] = a + q;
+        knn_tile_f64(x, n, d, k, rows, nq, idx + (a - q0) * k, dist + (a - q0) * k);
     }
 }
 
 void orc_knn_exact_f64_rows(const float *x, int64_t n, int32_t d, int32_t k,
                             const int64_t *rows, int64_t n_rows, int32_t *idx, double *dist) {
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t a = 0; a < n_rows; a++)
-        orc_knn_exact_f64(x, n, d, k, rows[a], rows[a] + 1, idx + a * k, dist + a * k);
+    for (int64_t a = 0; a < n_rows; a += KNN_QT) {
+        int nq = (int)(n_rows - a < KNN_QT ? n_rows - a : KNN_QT);
+        knn_tile_f64(x, n, d, k, rows + a, nq, idx + a * k, dist + a * k);
+    }
 }
 
 /* --------------------------------------------------------------- DensState */
@@ -696,35 +760,53 @@ int orc_dens_state(int64_t n, const int32_t *rows, const int32_t *cols, const fl
 
 /* -------------------------------------------------------------- fit_params */
 
-/* UmapOptimParams::fit_params, src/training/umap_optimisers.rs:139-209 */
-void orc_fit_params(float min_dist, float spread, int32_t n_iter, float *a_out, float *b_out) {
-    if (n_iter <= 0) n_iter = 300;
-    const int np = 300;
-    float xv[300], yv[300];
+/* the 300 (x, target) points of fit_params, src/training/umap_optimisers.rs:145-164 */
+#define ORC_FIT_POINTS 300
+static void fit_points(float min_dist, float spread, float *xv, float *yv) {
     float max_x = spread * 3.0f;
-    float step = max_x / (float)(np - 1);
-    for (int i = 0; i < np; i++) {
+    float step = max_x / (float)(ORC_FIT_POINTS - 1);
+    for (int i = 0; i < ORC_FIT_POINTS; i++) {
         float x = step * (float)i;
         xv[i] = x;
         yv[i] = x < min_dist ? 1.0f : expf(-(x - min_dist) / spread);
     }
-    float a = 1.0f, b = 1.0f;
+}
+
+/* one iteration's (grad_a, grad_b), :170-193 */
+static void fit_grad(const float *xv, const float *yv, float a, float b, float *ga_out,
+                     float *gb_out) {
     const float two = 2.0f;
+    float ga = 0.0f, gb = 0.0f;
+    for (int i = 0; i < ORC_FIT_POINTS; i++) {
+        float x = xv[i];
+        if (x <= 0.0f) continue;
+        float x2b = powf(x, two * b);
+        float denom = 1.0f + a * x2b;
+        float pred = 1.0f / denom;
+        float err = pred - yv[i];
+        ga += err * (-x2b / (denom * denom));
+        float lx = logf(x);
+        gb += err * (-two * a * x2b * lx / (denom * denom));
+    }
+    *ga_out = ga / (float)ORC_FIT_POINTS;
+    *gb_out = gb / (float)ORC_FIT_POINTS;
+}
+
+void orc_fit_params_grad(float min_dist, float spread, float a, float b, float *ga, float *gb) {
+    float xv[ORC_FIT_POINTS], yv[ORC_FIT_POINTS];
+    fit_points(min_dist, spread, xv, yv);
+    fit_grad(xv, yv, a, b, ga, gb);
+}
+
+/* UmapOptimParams::fit_params, src/training/umap_optimisers.rs:139-209 */
+void orc_fit_params(float min_dist, float spread, int32_t n_iter, float *a_out, float *b_out) {
+    if (n_iter <= 0) n_iter = 300;
+    float xv[ORC_FIT_POINTS], yv[ORC_FIT_POINTS];
+    fit_points(min_dist, spread, xv, yv);
+    float a = 1.0f, b = 1.0f;
     for (int it = 0; it < n_iter; it++) {
-        float ga = 0.0f, gb = 0.0f;
-        for (int i = 0; i < np; i++) {
-            float x = xv[i];
-            if (x <= 0.0f) continue;
-            float x2b = powf(x, two * b);
-            float denom = 1.0f + a * x2b;
-            float pred = 1.0f / denom;
-            float err = pred - yv[i];
-            ga += err * (-x2b / (denom * denom));
-            float lx = logf(x);
-            gb += err * (-two * a * x2b * lx / (denom * denom));
-        }
-        ga /= (float)np;
-        gb /= (float)np;
+        float ga, gb;
+        fit_grad(xv, yv, a, b, &ga, &gb);
         a -= 1.0f * ga;
         b -= 1.0f * gb;
         a = a < 0.001f ? 0.001f : a;
@@ -763,6 +845,38 @@ static inline float lut_get(const orc_lut *L, float x) {
     float rem = idx_f - (float)idx;
     float y0 = L->table[idx], y1 = L->table[idx + 1];
     return y0 + rem * (y1 - y0);
+}
+
+/* the parallel optimiser's LUT (65 536 entries on [0, 25]) applied to x[] */
+void orc_lut_pow(float b, const float *x, int64_t n, float *out) {
+    orc_lut L;
+    lut_init(&L, b, 25.0f, 65536);
+    for (int64_t i = 0; i < n; i++) out[i] = lut_get(&L, x[i]);
+    free(L.table);
+}
+
+/* density_edge_coeff, src/training/umap_optimisers.rs:564-592, with its
+ * max(T::epsilon()) floors on phi_sum and mu */
+static inline float dens_edge_coeff(float a, float b, float dist_sq, float dist_sq_b,
+                                    float denom, float phi_i, float phi_o, float re_i,
+                                    float re_o, float w_i, float w_o, float coeff, float mu_e) {
+    const float one = 1.0f, feps = FLT_EPSILON;
+    float phi = one / denom;
+    float dphi_term = a * b * dist_sq_b / dist_sq * phi;
+    float common = one - b * (one - phi);
+    float pi_ = phi_i > feps ? phi_i : feps;
+    float po_ = phi_o > feps ? phi_o : feps;
+    float dr_self = (phi / pi_) * (common / expf(re_i) + dphi_term);
+    float dr_other = (phi / po_) * (common / expf(re_o) + dphi_term);
+    float mue = mu_e > feps ? mu_e : feps;
+    return coeff * (w_i * dr_self + w_o * dr_other) / mue;
+}
+
+float orc_density_edge_coeff(float a, float b, float dist_sq, float dist_sq_b, float denom,
+                             float phi_sum_i, float phi_sum_o, float re_i, float re_o, float w_i,
+                             float w_o, float coeff, float mu_edge) {
+    return dens_edge_coeff(a, b, dist_sq, dist_sq_b, denom, phi_sum_i, phi_sum_o, re_i, re_o, w_i,
+                           w_o, coeff, mu_edge);
 }
 
 /* optimise_embedding_adam_parallel, src/training/umap_optimisers.rs:1170-1492,
@@ -876,15 +990,9 @@ int orc_optimise_adam_parallel_dens(float *embd, int64_t n, int32_t n_dim,
                     float gc = two_a_b * dsb / (dsq * denom);
                     for (int d = 0; d < n_dim; d++) gi[d] += two * gc * (yo[d] - yi[d]);
                     if (dens_on) {
-                        float phi = 1.0f / denom;
-                        float dphi_term = p->a * p->b * dsb / dsq * phi;
-                        float common = 1.0f - p->b * (1.0f - phi);
-                        float pi_ = phis[i] > feps ? phis[i] : feps;
-                        float po_ = phis[o] > feps ? phis[o] : feps;
-                        float dr_self = (phi / pi_) * (common / expf(re[i]) + dphi_term);
-                        float dr_other = (phi / po_) * (common / expf(re[o]) + dphi_term);
-                        float mue = mu[e] > feps ? mu[e] : feps;
-                        float cor = dcoeff * (dw[i] * dr_self + dw[o] * dr_other) / mue;
+                        float cor = dens_edge_coeff(p->a, p->b, dsq, dsb, denom, phis[i],
+                                                    phis[o], re[i], re[o], dw[i], dw[o], dcoeff,
+                                                    mu[e]);
                         float two_cor = two * cor;
                         for (int d = 0; d < n_dim; d++) {
                             float gg = two_cor * (yi[d] - yo[d]);

@@ -50,13 +50,25 @@ int orc_csr_from_graph(int64_t n, const int64_t *adj_off, const int32_t *adj_col
                        float *epochs_per_sample, float *mu, int32_t *edge_i, int32_t *edge_j);
 
 /* cpu_grad_accum, umap_optimiser_gpu.rs:1461-1579 (GPU kernel semantics :682-860).
- * dens_* may be NULL (plain UMAP).  has_update may be NULL. */
+ * dens_* may be NULL (plain UMAP).  has_update may be NULL.  abs_out (NULL
+ * allowed) receives per coordinate the sum of the absolute values of the terms
+ * the gradient adds up, the scale of its rounding error. */
 void orc_grad_accum(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
                     const uint32_t *csr_edge_idx, const uint32_t *csr_other,
                     const float *next_sample, int32_t epoch, uint32_t seed, int32_t neg_rate,
                     const orc_optim_params *p, const float *mu, const float *dens_re,
                     const float *dens_phi_sum, const float *dens_weight, float dens_coeff,
-                    float *grad_out, uint32_t *has_update);
+                    float *grad_out, uint32_t *has_update, float *abs_out);
+
+/* the same terms summed in double and rounded once: the exact sum of the
+ * reference's f32 terms, against which any summation order is measured */
+void orc_grad_accum_exact(const float *embd, int64_t n, int32_t n_dim, const uint32_t *off,
+                          const uint32_t *csr_edge_idx, const uint32_t *csr_other,
+                          const float *next_sample, int32_t epoch, uint32_t seed,
+                          int32_t neg_rate, const orc_optim_params *p, const float *mu,
+                          const float *dens_re, const float *dens_phi_sum,
+                          const float *dens_weight, float dens_coeff, float *grad_out,
+                          uint32_t *has_update, float *abs_out);
 
 /* per-epoch Adam scalars, umap_optimiser_gpu.rs:1155-1196 / :1595-1604 */
 void orc_adam_scalars(const orc_optim_params *p, int32_t epoch, int32_t n_epochs,
@@ -137,6 +149,17 @@ int orc_dens_state(int64_t n, const int32_t *rows, const int32_t *cols, const fl
 
 /* UmapOptimParams::fit_params, src/training/umap_optimisers.rs:139-209 (T = f32) */
 void orc_fit_params(float min_dist, float spread, int32_t n_iter, float *a, float *b);
+/* one iteration's normalised (grad_a, grad_b) of fit_params at (a, b), :170-193 */
+void orc_fit_params_grad(float min_dist, float spread, float a, float b, float *ga, float *gb);
+
+/* FastPowLut::new(b, 25.0, 65_536).get(x) for each x, src/training/umap_optimisers.rs:325-397 */
+void orc_lut_pow(float b, const float *x, int64_t n, float *out);
+
+/* density_edge_coeff, src/training/umap_optimisers.rs:564-592 (phi_sum and mu
+ * floored at FLT_EPSILON) */
+float orc_density_edge_coeff(float a, float b, float dist_sq, float dist_sq_b, float denom,
+                             float phi_sum_i, float phi_sum_o, float re_i, float re_o, float w_i,
+                             float w_o, float coeff, float mu_edge);
 
 /* optimise_embedding_adam_parallel, src/training/umap_optimisers.rs:1170-1492.
  * Negatives: the reference draws from per-node SmallRng streams (parity

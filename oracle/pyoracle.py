@@ -90,12 +90,16 @@ def csr_from_graph(adj_off, adj_cols, adj_vals):
     return 0, out
 
 
-def grad_accum(embd, csr, next_sample, epoch, seed, neg_rate, params, dens=None):
-    """dens = dict(re, phi_sum, weight, coeff) or None; returns (grad, has_update)."""
+def grad_accum(embd, csr, next_sample, epoch, seed, neg_rate, params, dens=None,
+               with_abs=False, exact=False):
+    """dens = dict(re, phi_sum, weight, coeff) or None; returns (grad, has_update),
+    plus, with with_abs, the per-coordinate sum of |term| of each gradient.
+    exact: the same f32 terms summed in float64 (rounded once at the end)."""
     embd = f32(embd)
     n, dim = embd.shape
     g = np.zeros((n, dim), np.float32)
     hu = np.zeros(n, np.uint32)
+    ab = np.zeros((n, dim), np.float32) if with_abs else None
     p = params_from(params)
     if dens is None:
         re = phi = w = None
@@ -103,11 +107,13 @@ def grad_accum(embd, csr, next_sample, epoch, seed, neg_rate, params, dens=None)
     else:
         re, phi, w = f32(dens["re"]), f32(dens["phi_sum"]), f32(dens["weight"])
         coeff = float(dens["coeff"])
-    lib().orc_grad_accum(P(embd), C.c_int64(n), C.c_int32(dim), P(csr["off"]),
+    fn = lib().orc_grad_accum_exact if exact else lib().orc_grad_accum
+    fn(P(embd), C.c_int64(n), C.c_int32(dim), P(csr["off"]),
                          P(csr["edge_idx"]), P(csr["other"]), P(f32(next_sample)),
                          C.c_int32(epoch), C.c_uint32(seed), C.c_int32(neg_rate), C.byref(p),
-                         P(csr["mu"]), P(re), P(phi), P(w), C.c_float(coeff), P(g), P(hu))
-    return g, hu
+                         P(csr["mu"]), P(re), P(phi), P(w), C.c_float(coeff), P(g), P(hu),
+                         P(ab))
+    return (g, hu, ab) if with_abs else (g, hu)
 
 
 def adam_update(grad, has_update, m, v, embd, params, epoch, n_epochs):
@@ -259,6 +265,30 @@ def fit_params(min_dist, spread, n_iter=300):
     lib().orc_fit_params(C.c_float(min_dist), C.c_float(spread), C.c_int32(n_iter), C.byref(a),
                          C.byref(b))
     return a.value, b.value
+
+
+def fit_params_grad(min_dist, spread, a, b):
+    """One iteration's normalised (grad_a, grad_b) of fit_params at (a, b)."""
+    ga, gb = C.c_float(), C.c_float()
+    lib().orc_fit_params_grad(C.c_float(min_dist), C.c_float(spread), C.c_float(a), C.c_float(b),
+                              C.byref(ga), C.byref(gb))
+    return ga.value, gb.value
+
+
+def lut_pow(b, x):
+    """The parallel optimiser's FastPowLut(b, 25, 65 536).get over x."""
+    x = f32(x).ravel()
+    out = np.zeros_like(x)
+    lib().orc_lut_pow(C.c_float(b), P(x), C.c_int64(x.size), P(out))
+    return out
+
+
+def density_edge_coeff(a, b, dist_sq, dist_sq_b, denom, phi_i, phi_o, re_i, re_o, w_i, w_o,
+                       coeff, mu_edge):
+    fn = lib().orc_density_edge_coeff
+    fn.restype = C.c_float
+    fn.argtypes = [C.c_float] * 13
+    return fn(a, b, dist_sq, dist_sq_b, denom, phi_i, phi_o, re_i, re_o, w_i, w_o, coeff, mu_edge)
 
 
 def optimise_adam_parallel(embd, adj_off, adj_cols, adj_vals, params, seed, use_lut=True,

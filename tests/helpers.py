@@ -85,6 +85,28 @@ def assert_scaled_close(got, want, rtol, ctx=""):
         f"scaled {d[i] / scale:.3e} > {rtol:.1e}; median diff {float(np.median(d)):.3e}")
 
 
+def assert_grad_close(got, want, absum, rtol=1e-4, ctx=""):
+    """Per coordinate |got - want| <= rtol * sum|terms| of that coordinate.
+
+    A node's gradient is a sum of many (clipped) terms that can cancel; its
+    rounding error, and the error any reordering of the sum can make, is
+    bounded by the sum of the terms' magnitudes, which the oracle returns with
+    the gradient.  A coordinate no term touches (sum 0) must be exactly 0."""
+    got = np.asarray(got, np.float64).ravel()
+    want = np.asarray(want, np.float64).ravel()
+    absum = np.asarray(absum, np.float64).ravel()
+    assert got.shape == want.shape == absum.shape, f"{ctx}: shapes"
+    assert np.all(np.isfinite(got)), f"{ctx}: non-finite output"
+    d = np.abs(got - want)
+    bar = rtol * absum
+    bad = np.flatnonzero(d > bar)
+    worst = float(np.max(d / np.maximum(absum, 1e-30))) if d.size else 0.0
+    assert bad.size == 0, (
+        f"{ctx}: {bad.size} coords over {rtol:g} * sum|terms|, first {bad[0]}: got {got[bad[0]]!r} "
+        f"want {want[bad[0]]!r} diff {d[bad[0]]:.3e} sum|terms| {absum[bad[0]]:.3e}")
+    return worst
+
+
 def diff_stats(got, want):
     d = np.abs(np.asarray(got, np.float64).ravel() - np.asarray(want, np.float64).ravel())
     return (f"max {d.max():.3e} p99 {np.percentile(d, 99):.3e} median {np.median(d):.3e} "

@@ -3,21 +3,26 @@
 Ports the reference's kernel tests (src/training/umap_optimiser_gpu.rs:1786-2505)
 and adds seeded larger graphs.  Tolerances: the reference's own (bit-exact for
 attractive-only, 1e-4 with negatives, 1e-5/1e-6 for Adam).  On larger graphs a
-node gradient is a sum of hundreds of clipped terms that can cancel, so it is
-compared against the largest gradient magnitude (max |got - want| <= rtol *
-max |want|): 1e-5 for the node kernel (same summation order as the oracle,
-only powf/expf ulps differ), 3e-5 for the edge kernel (FP32 atomics reorder the
-sum).  Whole runs are compared epoch by epoch from the same state (teacher
-forcing): Adam divides by sqrt(v), so a free-running comparison amplifies ulp
-differences in near-zero gradients into O(lr) steps within a few epochs.
+node gradient is a sum of hundreds of clipped terms that can cancel, so each
+coordinate is held to 1e-4 relative to the sum of its terms' magnitudes
+(|got - want| <= 1e-4 * sum|terms|, the oracle returning the sum), with want
+the exact (float64) sum of the oracle's f32 terms: that bounds any reordering
+of the sum (the edge kernel's FP32 atomics) and the device's powf/expf ulps,
+and a term dropped or doubled fails it.  The exact sum is the yardstick
+because a sequential f32 sum is itself off by more than the bar on a hub: on
+the 70 000-neighbour star below the oracle's own f32 sum misses the exact one
+by 3.7e-4 of sum|terms|.  Whole runs are
+compared epoch by epoch from the same state (teacher forcing): Adam divides by
+sqrt(v), so a free-running comparison amplifies ulp differences in near-zero
+gradients into O(lr) steps within a few epochs.
 """
 import numpy as np
 import pytest
 
 import mrs_amd as M
 import pyoracle as orc
-from helpers import (assert_close, assert_scaled_close, blobs, diff_stats, triangle_setup,
-                     umap_graph_from_data)
+from helpers import (assert_close, assert_grad_close, assert_scaled_close, blobs, coo_to_adjacency,
+                     diff_stats, triangle_setup, umap_graph_from_data)
 
 pytestmark = pytest.mark.gpu
 
@@ -132,17 +137,19 @@ def test_grad_random_graph_node_and_edge_kernels(ctx, epoch):
     st, _ = make_state(ctx, embd, adj, params)
     c = oracle_csr(adj)
     nxt = st.read(M.BUF_NEXT)
-    want, hu = orc.grad_accum(embd, c, nxt, epoch, 42, 5, params)
+    want, hu, absum = orc.grad_accum(embd, c, nxt, epoch, 42, 5, params, with_abs=True,
+                                     exact=True)
     st.launch_grad_accum(epoch, 42)
     g_node = st.read(M.BUF_GRAD).reshape(-1, 2)
     assert np.array_equal(st.read(M.BUF_HAS_UPDATE), hu), "node kernel has_update flags"
-    assert_scaled_close(g_node, want, 1e-5, f"node kernel epoch {epoch}")
+    w1 = assert_grad_close(g_node, want, absum, 1e-4, f"node kernel epoch {epoch}")
     # edge-parallel: same gradient by atomics, same flags, schedule advanced
     st.launch_epoch_edges(epoch, 42)
     g_edge = st.read(M.BUF_GRAD).reshape(-1, 2)
     assert np.array_equal(st.read(M.BUF_HAS_UPDATE), hu), "edge kernel has_update flags"
-    assert_scaled_close(g_edge, want, 3e-5, f"edge kernel epoch {epoch}")
+    w2 = assert_grad_close(g_edge, want, absum, 1e-4, f"edge kernel epoch {epoch}")
     assert np.array_equal(st.read(M.BUF_NEXT), orc.schedule_update(c["per"], nxt, epoch))
+    print(f"epoch {epoch}: max |diff| / sum|terms|: node {w1:.2e}, edge {w2:.2e}")
 
 
 # ------------------------------------------------------------------ Adam
@@ -283,12 +290,13 @@ def test_dens_random_graph_stats_and_grad(ctx):
     w, coeff_w = orc.dens_weights(r, re, 0.1, 2.0, 1234.5)
     assert abs(coeff - coeff_w) <= 1e-5 * abs(coeff_w), f"coeff {coeff!r} vs {coeff_w!r}"
     assert_scaled_close(st.read(M.BUF_DENS_W), w, 1e-5, "dens weights")
-    want, _ = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 7, 42, 5, params,
-                             dict(re=re, phi_sum=phi, weight=w, coeff=coeff_w))
+    want, _, absum = orc.grad_accum(embd, c, st.read(M.BUF_NEXT), 7, 42, 5, params,
+                                    dict(re=re, phi_sum=phi, weight=w, coeff=coeff_w),
+                                    with_abs=True, exact=True)
     st.launch_grad_accum(7, 42, coeff)
-    assert_scaled_close(st.read(M.BUF_GRAD).reshape(-1, 2), want, 1e-5, "dens node kernel")
+    assert_grad_close(st.read(M.BUF_GRAD).reshape(-1, 2), want, absum, 1e-4, "dens node kernel")
     st.launch_epoch_edges(7, 42, coeff)
-    assert_scaled_close(st.read(M.BUF_GRAD).reshape(-1, 2), want, 3e-5, "dens edge kernel")
+    assert_grad_close(st.read(M.BUF_GRAD).reshape(-1, 2), want, absum, 1e-4, "dens edge kernel")
 
 
 # ------------------------------------------------------------- driver
@@ -311,31 +319,33 @@ def test_driver_preserves_graph_structure(ctx):
         assert np.all(np.isfinite(out))
 
 
-def _teacher_forced(ctx, embd, adj, p, strategy, gtol, seed=42):
+def _teacher_forced(ctx, embd, adj, p, strategy, gtol=1e-4, seed=42, epochs=None):
     """Every epoch runs on the GPU (run_epochs(e, 1)) from the GPU's own state
     and is replayed on the oracle (grad_accum + adam_update + schedule_update)
     from that same state.  Schedule: exact.  m: within (1-beta1) * the gradient
-    bar.  Embedding: the gradient bar carried through Adam's step
-    fused_lr * m / (sqrt(v) + eps), i.e. |d embd| <= 4 lr gtol gscale /
-    (sqrt(v) + eps) plus rounding."""
+    bar gtol * sum|terms| of each coordinate.  Embedding: that bar carried
+    through Adam's step fused_lr * m / (sqrt(v) + eps), i.e. |d embd| <=
+    4 lr gtol sum|terms| / (sqrt(v) + eps) plus rounding."""
     st, _ = make_state(ctx, embd, adj, p)
     c = oracle_csr(adj)
     n, dim = embd.shape
     worst = 0.0
-    for e in range(p.n_epochs):
+    for e in range(p.n_epochs if epochs is None else epochs):
         e0 = st.read(M.BUF_EMBD).reshape(n, dim)
         m0, v0, nx0 = st.read(M.BUF_M), st.read(M.BUF_V), st.read(M.BUF_NEXT)
         st.run_epochs(e, 1, seed, strategy)
-        g, hu = orc.grad_accum(e0, c, nx0, e, seed, p.neg_sample_rate, p)
+        g, hu, absum = orc.grad_accum(e0, c, nx0, e, seed, p.neg_sample_rate, p, with_abs=True,
+                                      exact=True)
         ew, mw, vw = orc.adam_update(g, hu, m0, v0, e0, p, e, p.n_epochs)
         assert np.array_equal(st.read(M.BUF_NEXT), orc.schedule_update(c["per"], nx0, e)), \
             f"schedule epoch {e}"
-        gscale = float(np.max(np.abs(g))) if g.size else 0.0
-        dg = gtol * gscale
+        dg = gtol * absum.ravel()
         got_m = st.read(M.BUF_M)
         dm = np.abs(got_m - mw.ravel())
-        assert np.all(dm <= 1e-6 * (1 + np.abs(mw.ravel())) + dg), \
-            f"m epoch {e}: {diff_stats(got_m, mw)}, bar {dg:.3e}"
+        mbad = np.flatnonzero(dm > 1e-6 * (1 + np.abs(mw.ravel())) + dg)
+        assert mbad.size == 0, (f"m epoch {e}: {mbad.size} coords over the bar, first {mbad[0]}: "
+                                f"diff {dm[mbad[0]]:.3e} bar {dg[mbad[0]]:.3e}; "
+                                f"{diff_stats(got_m, mw)}")
         got = st.read(M.BUF_EMBD)
         d = np.abs(got - ew.ravel())
         bar = 1e-5 * (1 + np.abs(ew.ravel())) + 4 * p.lr * dg / (np.sqrt(vw.ravel()) + p.eps)
@@ -354,7 +364,7 @@ def test_driver_node_exact_bit_identical_rerun_and_oracle(ctx):
     a = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_NODE_EXACT)
     b = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_NODE_EXACT)
     assert np.array_equal(a, b)
-    _teacher_forced(ctx, embd, adj, p, M.EPOCH_NODE_EXACT, 1e-5)
+    _teacher_forced(ctx, embd, adj, p, M.EPOCH_NODE_EXACT, 1e-4)
     # the free-running driver against the oracle's replay of the whole driver:
     # reported, with a loose bar (ulp differences grow through Adam)
     want = orc.optimise_adam_gpu(embd, *adj, p, 42)
@@ -369,7 +379,7 @@ def test_driver_edge_atomic_matches_oracle(ctx, neg_rate):
     p = M.UmapOptimParams.default_2d()
     p.n_epochs = 30
     p.neg_sample_rate = neg_rate
-    _teacher_forced(ctx, embd, adj, p, M.EPOCH_EDGE_ATOMIC, 3e-5)
+    _teacher_forced(ctx, embd, adj, p, M.EPOCH_EDGE_ATOMIC, 1e-4)
     got = M.optimise_embedding_adam_gpu(ctx, embd, *adj, p, seed=42, strategy=M.EPOCH_EDGE_ATOMIC)
     want = orc.optimise_adam_gpu(embd, *adj, p, 42)
     print("edge-atomic free run vs oracle:", diff_stats(got, want))
@@ -452,3 +462,76 @@ def test_graph_replay_matches_eager(ctx, dens_on):
     # reported, loose bar
     print("edge-atomic graph vs eager:", diff_stats(got_a, eager_a))
     assert np.all(np.isfinite(got_a)) and float(np.median(np.abs(got_a - eager_a))) < 0.1
+
+
+# ------------------------------------------- wide spans and hub nodes
+
+
+def _random_sparse_graph(n, per_node, seed, w_lo=0.05):
+    """A random symmetric graph: every node picks per_node random partners;
+    weights uniform in (w_lo, 1].  Returns the adjacency (CSR of both
+    directions, columns ascending per row) and the undirected edge count."""
+    rng = np.random.default_rng(seed)
+    i = np.repeat(np.arange(n, dtype=np.int64), per_node)
+    j = rng.integers(0, n, size=i.size)
+    keep = i != j
+    a, b = np.minimum(i[keep], j[keep]), np.maximum(i[keep], j[keep])
+    key = np.unique(a * n + b)
+    a, b = key // n, key % n
+    w = rng.uniform(w_lo, 1.0, size=a.size).astype(np.float32)
+    rows = np.concatenate([a, b])
+    cols = np.concatenate([b, a])
+    vals = np.concatenate([w, w])
+    order = np.lexsort((cols, rows))
+    return coo_to_adjacency(n, rows[order], cols[order], vals[order]), a.size
+
+
+def test_edge_kernel_wide_span_teacher_forced(ctx):
+    # Graphs of 2^20 or more edges take the 256-edge-span instantiation of the
+    # edge kernel (every headline fit does); below that the 64-edge span runs.
+    # Eight epochs on a 1.1M-edge graph, each replayed on the oracle from the
+    # GPU's state: schedule exact, gradients and Adam within the per-node bar.
+    adj, n_edges = _random_sparse_graph(160_000, 7, seed=61)
+    assert n_edges >= 1 << 20, n_edges
+    n = adj[0].size - 1
+    embd = np.random.default_rng(61).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 8
+    st, worst = _teacher_forced(ctx, embd, adj, p, M.EPOCH_EDGE_ATOMIC, epochs=8)
+    fired = int(np.count_nonzero(st.read(M.BUF_NEXT) > st.read(M.BUF_PER)))
+    print(f"wide span: {n_edges} edges, {fired} fired at least once, max |embd diff| {worst:.3e}")
+    assert fired > n_edges // 4
+
+
+def test_edge_kernel_hub_degree_above_65535(ctx):
+    # Node 0 is joined to every other node (degree 69 999 > 65 535), the others
+    # also form a ring.  The negatives of an edge are keyed by its CSR slot
+    # under each endpoint, so the hub's slots must reach past 16 bits: the edge
+    # kernel against the oracle (which walks the CSR itself), and the slots read
+    # back from the edge records.
+    n = 70_000
+    rows, cols = [], []
+    for i in range(1, n):
+        rows += [0, i]
+        cols += [i, 0]
+        j = i + 1 if i + 1 < n else 1
+        rows += [i, j]
+        cols += [j, i]
+    rows, cols = np.array(rows), np.array(cols)
+    order = np.lexsort((cols, rows))
+    vals = np.where((rows == 0) | (cols == 0), 1.0, 0.5).astype(np.float32)
+    adj = coo_to_adjacency(n, rows[order], cols[order], vals[order])
+    embd = np.random.default_rng(63).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 4
+    st, _ = make_state(ctx, embd, adj, p)
+    c = oracle_csr(adj)
+    slots = st.read(M.BUF_EDGE_SLOTS).reshape(-1, 2)
+    ij = st.read(M.BUF_EDGE_IJ).reshape(-1, 2)
+    hub = ij[:, 0] == 0
+    assert slots[hub, 0].max() == n - 2  # the hub's last slot
+    # every slot points at its edge in the endpoint's CSR row
+    for end in (0, 1):
+        pos = c["off"][ij[:, end]] + slots[:, end]
+        assert np.array_equal(c["edge_idx"][pos], np.arange(ij.shape[0]))
+    _teacher_forced(ctx, embd, adj, p, M.EPOCH_EDGE_ATOMIC, epochs=4)

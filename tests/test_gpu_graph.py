@@ -47,10 +47,24 @@ def test_graph_matches_oracle(ctx, mix):
     orw, ocl, ovl = _oracle_graph(idx, dist, sg, rh, mix, 500)
     assert np.array_equal(r, orw) and np.array_equal(c, ocl)
     np.testing.assert_allclose(v, ovl, rtol=1e-5, atol=0)
-    # and against the oracle's own sigma the graph agrees to the search tolerance
+    # and against the graph built from the oracle's own (f32-searched) sigma:
+    # the FP64 search lands within the tolerance band of the same bandwidth,
+    # so weights agree to 1e-3 on the shared entries, and the filter threshold
+    # can flip only entries whose weight sits within that of max_w / n_epochs
     orw2, ocl2, ovl2 = _oracle_graph(idx, dist, so, ro, mix, 500)
-    if np.array_equal(r, orw2) and np.array_equal(c, ocl2):
-        np.testing.assert_allclose(v, ovl2, rtol=1e-3)
+    n = idx.shape[0]
+    ka, kb = r.astype(np.int64) * n + c, orw2.astype(np.int64) * n + ocl2
+    shared, ia, ib = np.intersect1d(ka, kb, assume_unique=True, return_indices=True)
+    flipped = ka.size + kb.size - 2 * shared.size
+    frac = flipped / max(ka.size, 1)
+    print(f"mix {mix}: {flipped} of {ka.size} entries differ from the oracle-sigma graph "
+          f"({frac:.2e}); max weight rel diff {np.max(np.abs(v[ia] - ovl2[ib]) / ovl2[ib]):.2e}")
+    assert frac < 1e-3
+    np.testing.assert_allclose(v[ia], ovl2[ib], rtol=1e-3)
+    thr = max(float(v.max()), float(ovl2.max())) / 500
+    only = np.concatenate([v[np.setdiff1d(np.arange(ka.size), ia)],
+                           ovl2[np.setdiff1d(np.arange(kb.size), ib)]])
+    assert np.all(np.abs(only - thr) <= 1e-3 * thr + 1e-7), "a flipped entry far from the threshold"
 
 
 def test_graph_csr_layout_matches_host_from_graph(ctx):
@@ -203,3 +217,23 @@ def test_graph_from_coo_rejects_bad_lists(ctx):
     h = M.FuzzyGraph.from_coo(ctx, n, r, c, v)
     with pytest.raises(M.ManifoldsError):
         h.sigma_rho()
+
+
+def test_precomputed_knn_index_out_of_range_rejected(ctx):
+    # the reference indexes its point arrays with every neighbour id and stops
+    # on one outside [0, n); the device graph refuses it (InvalidArgument)
+    # instead of dropping the entry and fitting a smaller graph
+    idx, dist = _knn(n=500)
+    for bad in (-1, 500, 10**6):
+        b = idx.copy()
+        b[123, 4] = bad
+        with pytest.raises(M.ManifoldsError) as e:
+            M.FuzzyGraph(ctx, b, dist, n_epochs=200)
+        assert e.value.status == M.MRS_INVALID_ARGUMENT
+        assert "outside" in str(e.value)
+    # a point listed as its own neighbour is skipped, as knn_to_coo skips it
+    b = idx.copy()
+    b[7, 0] = 7
+    g = M.FuzzyGraph(ctx, b, dist, n_epochs=200)
+    r, c, _ = g.coo()
+    assert not np.any(r == c)

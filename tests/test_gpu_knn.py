@@ -154,3 +154,45 @@ def test_pruned_sweep_two_dimensional(ctx):
     x = (centres[rng.integers(0, 40, size=67_000)] + rng.normal(size=(67_000, 2))).astype(np.float32)
     t = _check_pruned(ctx, x, 15, M.KNN_FP32)
     assert t["swept"] < 0.5, t
+
+
+@pytest.mark.parametrize("k,shortlist", [(15, 60), (30, 120)])
+def test_pruned_sweep_split_work_items(ctx, k, shortlist):
+    # Second-pass stage lists longer than the work-item chunk are cut into
+    # parts whose aux lists k_merge_aux folds back into the block's list (with
+    # the main list's S-th score as a floor).  By default only large problems
+    # split (1M rows: lists of up to 8 309 stages); a 64-stage chunk makes the
+    # 70k-row lists (~500 stages) split too.  Shortlists of 64 (k = 15) and
+    # 128 (k = 30) candidates.  Bit-identical to the full sweep and the oracle.
+    import synthetic
+
+    x, _ = synthetic.subspace(n=70_000, d=64, seed=3)
+    ctx.set_knn_work_chunk(64)
+    try:
+        t = _check_pruned(ctx, x, k, M.KNN_BF16_RERANK, shortlist=shortlist)
+    finally:
+        ctx.set_knn_work_chunk(0)
+    print(f"k {k}: {t['aux_items']} split work items, longest list {t['list_max']}")
+    assert t["aux_items"] > 0, t
+    _, _, t0 = M.knn_exhaustive(ctx, x, k, M.KNN_BF16_RERANK, shortlist, timed=True)
+    assert t0["aux_items"] == 0  # the default chunk does not split at this size
+
+
+def test_plain_fp32_form_knn_anisotropic_100k(ctx):
+    # The plain FP32 GPU form's data: 100 000 x 50, 20 anisotropic Gaussians
+    # with 5 % uniform background.  FP32 kNN (k = 15) bit-exact against the
+    # oracle's brute force on row slices and on 1 000 seeded rows.
+    import synthetic
+
+    x, _ = synthetic.anisotropic()
+    idx, dist, t = M.knn_exhaustive(ctx, x, 15, M.KNN_FP32, timed=True)
+    n = x.shape[0]
+    for q0, q1 in ((0, 256), (n // 2, n // 2 + 256), (n - 256, n)):
+        widx, wdist = orc.knn_exact(x, 15, q0, q1)
+        assert np.array_equal(idx[q0:q1], widx), (q0, q1)
+        assert np.array_equal(dist[q0:q1].view(np.uint32), wdist.view(np.uint32)), (q0, q1)
+    rows = np.sort(np.random.default_rng(2).choice(n, 1000, replace=False))
+    widx, wdist = orc.knn_exact_rows_f64(x, 15, rows)
+    assert np.array_equal(idx[rows], widx), "FP32 kNN vs FP64 brute force"
+    assert np.max(np.abs(dist[rows] - wdist) / wdist) < 1e-5
+    print(f"swept {t['swept']:.3f}, cells {t['cells']}")

@@ -457,3 +457,303 @@ def test_cpu_densmap_raises_radius_correlation():
 
     assert np.all(np.isfinite(dens))
     assert spearman(R, emb_radii(dens)) > spearman(R, emb_radii(plain)) + 0.05
+
+
+# ------------------------------------------------ fit_params, LUT, floors
+
+
+def _fit_loss_f64(min_dist, spread, a, b):
+    """L(a, b) = 1/(2 * 300) * sum_{x > 0} (1 / (1 + a x^(2b)) - y(x))^2 on the 300
+    points of fit_params (src/training/umap_optimisers.rs:145-164), in float64."""
+    x = np.arange(300) * (3.0 * spread / 299.0)
+    y = np.where(x < min_dist, 1.0, np.exp(-(x - min_dist) / spread))
+    x, y = x[1:], y[1:]
+    return float(((1.0 / (1.0 + a * x ** (2.0 * b)) - y) ** 2).sum() / 600.0)
+
+
+@pytest.mark.parametrize("md,sp,a,b", [(0.1, 1.0, 1.0, 1.0), (0.1, 1.0, 0.5, 1.5),
+                                       (0.5, 2.0, 3.0, 0.7), (0.0, 1.0, 1.6, 0.9)])
+def test_fit_params_gradient_is_gradient_of_its_loss(md, sp, a, b):
+    # fit_params steps (a, b) -= (grad_a, grad_b): those must be the gradient of
+    # the mean squared curve error (with the 1/2), checked by central
+    # differences in float64.  A dropped factor 2 or ln x, or a wrong sign,
+    # fails here.
+    ga, gb = orc.fit_params_grad(md, sp, a, b)
+    h = 1e-6
+    fa = (_fit_loss_f64(md, sp, a + h, b) - _fit_loss_f64(md, sp, a - h, b)) / (2 * h)
+    fb = (_fit_loss_f64(md, sp, a, b + h) - _fit_loss_f64(md, sp, a, b - h)) / (2 * h)
+    assert abs(ga - fa) <= 1e-5 + 1e-4 * abs(fa), (ga, fa)
+    assert abs(gb - fb) <= 1e-5 + 1e-4 * abs(fb), (gb, fb)
+
+
+def test_fit_params_is_projected_gradient_descent():
+    # n_iter steps of (a, b) <- clamp((a, b) - grad(a, b)) from (1, 1): one step
+    # from (1, 1), and the composition of steps, each through the gradient above
+    a1, b1 = orc.fit_params(0.1, 1.0, n_iter=1)
+    ga, gb = orc.fit_params_grad(0.1, 1.0, 1.0, 1.0)
+    assert (a1, b1) == (np.float32(1.0 - ga), np.float32(1.0 - gb))
+    a, b = 1.0, 1.0
+    for _ in range(5):
+        ga, gb = orc.fit_params_grad(0.8, 0.5, a, b)
+        a = float(np.clip(np.float32(a - ga), np.float32(0.001), np.float32(10.0)))
+        b = float(np.clip(np.float32(b - gb), np.float32(0.1), np.float32(2.0)))
+    assert orc.fit_params(0.8, 0.5, n_iter=5) == (np.float32(a), np.float32(b))
+    # spread 0.5 / min_dist 0.8 drives b into its upper clamp
+    assert orc.fit_params(0.8, 0.5)[1] == 2.0
+
+
+def test_fit_params_library_equals_oracle():
+    # the product's from_min_dist_spread computes the same a, b, bit for bit
+    for md, sp in [(0.1, 1.0), (0.5, 1.0), (0.0, 1.0), (0.25, 2.0), (0.8, 0.5), (1.5, 1.0)]:
+        p = mrs_amd.UmapOptimParams.from_min_dist_spread(md, sp)
+        a, b = orc.fit_params(md, sp)
+        assert np.float32(p.a).tobytes() == np.float32(a).tobytes(), (md, sp)
+        assert np.float32(p.b).tobytes() == np.float32(b).tobytes(), (md, sp)
+        assert (p.lr, p.gamma, p.n_epochs, p.neg_sample_rate) == (1.0, 1.0, 500, 5)
+        assert (np.float32(p.beta1), np.float32(p.beta2), np.float32(p.eps)) == (
+            np.float32(0.5), np.float32(0.9), np.float32(1e-7))
+    q = mrs_amd.UmapOptimParams.from_min_dist_spread(0.1, 1.0, lr=0.5, n_epochs=200,
+                                                     neg_sample_rate=3, beta1=0.9)
+    assert (q.lr, q.n_epochs, q.neg_sample_rate, np.float32(q.beta1)) == (0.5, 200, 3,
+                                                                          np.float32(0.9))
+    with pytest.raises(mrs_amd.ManifoldsError):
+        mrs_amd.UmapOptimParams.from_min_dist_spread(0.1, 0.0)
+    # new_default_2d: min_dist None -> 0.5, spread None -> 1.0
+    g = mrs_amd.UmapParamsGpu.new_default_2d()
+    a, b = orc.fit_params(0.5, 1.0)
+    assert (g.optim_params.a, g.optim_params.b) == (a, b) and g.n_dim == 2 and g.k == 15
+
+
+@pytest.mark.parametrize("b", [0.9, 0.5, 1.5, 1.8, 0.2])
+def test_lut_pow_within_interpolation_bound(b):
+    # FastPowLut (src/training/umap_optimisers.rs:325-397): x^b on 65 536 points
+    # of [0, 25], linear interpolation between them, exact powf from 25 up.
+    # On a cell [x0, x0 + h] the interpolation error of a smooth f is at most
+    # h^2/8 max|f''|; f'' = b(b-1) x^(b-2) is largest at the cell's end nearer
+    # zero for b < 2.  The first cell [0, h] is bounded by h^b max_t |t^b - t|.
+    # An index off by one costs ~h f'(x), far above these bounds.
+    rng = np.random.default_rng(int(b * 100))
+    h = np.float64(np.float32(25.0) / np.float32(65535))
+    x = np.concatenate([rng.uniform(0, 25, 20000), rng.uniform(0, 3 * h, 2000),
+                        rng.uniform(25, 40, 500), [0.0, h, 25.0, 24.9999]]).astype(np.float32)
+    got = orc.lut_pow(b, x).astype(np.float64)
+    xd = x.astype(np.float64)
+    want = xd ** b
+    x0 = np.floor(xd / h) * h
+    f2 = abs(b * (b - 1)) * np.maximum(np.where(x0 > 0, x0, h) ** (b - 2), (x0 + h) ** (b - 2))
+    t = np.linspace(0, 1, 10001)
+    first = h ** b * np.max(np.abs(t ** b - t))
+    bound = np.where(xd < h, first, h * h / 8 * f2) + 8 * np.finfo(np.float32).eps * want + 1e-30
+    above = xd >= 25
+    bound[above] = 2 * np.finfo(np.float32).eps * want[above]
+    bad = np.flatnonzero(np.abs(got - want) > bound)
+    assert bad.size == 0, (x[bad[:3]], got[bad[:3]], want[bad[:3]], bound[bad[:3]])
+
+
+def test_density_edge_coeff_values_and_floors():
+    # density_edge_coeff (src/training/umap_optimisers.rs:564-592) at a = b = 1,
+    # d^2 = 1: phi = 1/2, dphi_term = a b d^(2b)/d^2 phi = 1/2, common =
+    # 1 - b (1 - phi) = 1/2, so with unit phi sums and re = 0 both dr = 1/2 and
+    # coeff (w_i dr + w_o dr) / mu = 3 (0.5 + 1.0) / 0.5 = 9.
+    c = orc.density_edge_coeff(1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 0.0, 0.0, 1.0, 2.0, 3.0, 0.5)
+    assert abs(c - 9.0) < 1e-6
+    # re enters as exp(-re): re_i = ln 2 halves common / exp(re_i)
+    c = orc.density_edge_coeff(1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, math.log(2.0), 0.0, 1.0, 0.0,
+                               1.0, 1.0)
+    assert abs(c - 0.5 * (0.25 + 0.5)) < 1e-6
+    # the max(eps) floors: a zero phi sum and a zero edge weight are floored at
+    # f32 epsilon (2^-23) instead of dividing by zero
+    eps = 2.0 ** -23
+    c = orc.density_edge_coeff(1.0, 1.0, 1.0, 1.0, 2.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 1.0)
+    assert math.isfinite(c) and abs(c - 0.5 / eps) <= 1e-5 * (0.5 / eps)
+    c = orc.density_edge_coeff(1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 0.0, 0.0, 1.0, 0.0, 1.0, 0.0)
+    assert math.isfinite(c) and abs(c - 0.5 / eps) <= 1e-5 * (0.5 / eps)
+
+
+# ------------------------------------- CPU optimisers on two-node graphs
+#
+# Two nodes joined by one edge of weight 1: epochs_per_sample = 1, so the edge
+# fires at epochs 1, 2, 3, ... (never at 0).  Negatives are scheduled at a
+# period of 1/rate (0.2 for rate 5) from next_neg = 0.2: the first firing
+# draws floor((1 - 0.2) / 0.2) = 4 = rate - 1 negatives, every later one
+# floor(1 / 0.2) = 5 (both exact in f32).  With n = 2 a draw is the node
+# itself (skipped) or the other node, so the repulsive sum is a count times one
+# term.  The draws come from orc_hash_neg (pinned above) with the keys
+# (node, epoch, CSR slot 0, s).
+
+
+def _attr(a, b, yi, yo):
+    """2 * coeff * (y_o - y_i), coeff = 2ab d^(2b) / (d^2 (1 + a d^(2b))) (unclipped)."""
+    d2 = float(((yi - yo) ** 2).sum())
+    db = d2 ** b
+    return 2.0 * (2.0 * a * b * db / (d2 * (1.0 + a * db))) * (yo - yi)
+
+
+def _rep(a, b, gamma, yi, yk):
+    """clip(2 gamma b / ((d^2 + 0.001)(1 + a (d^2 + 0.001)^b)), 4) * (y_i - y_k)."""
+    s = float(((yi - yk) ** 2).sum()) + 0.001
+    return min(2.0 * gamma * b / (s * (1.0 + a * s ** b)), 4.0) * (yi - yk)
+
+
+def _draws_other(seed, node, epoch, count):
+    return sum(orc.hash_neg(seed, node, epoch, 0, s, 2) != node for s in range(count))
+
+
+def _two_node_params(n_epochs, a=1.0, b=1.0, eps=10.0):
+    # eps large: Adam's first steps stay proportional to the gradient
+    # (m / (sqrt(v) + eps_c) with eps_c >> sqrt(v)), so the gradient's size,
+    # not only its sign, shows in the result
+    return mrs_amd.UmapOptimParams(a=a, b=b, lr=1.0, gamma=1.0, n_epochs=n_epochs,
+                                   neg_sample_rate=5, min_dist=0.1, beta1=0.5, beta2=0.9,
+                                   eps=eps)
+
+
+@pytest.mark.parametrize("n_epochs", [2, 3, 4])
+def test_adam_parallel_two_nodes_hand_computed(n_epochs):
+    # optimise_embedding_adam_parallel (umap_optimisers.rs:1170-1492): node 0
+    # (the smaller endpoint) takes the edge's negatives, node 1 none; Adam's
+    # bias correction counts epochs (t = epoch + 1), not updates; lr decays as
+    # lr (1 - epoch / n_epochs)
+    p = _two_node_params(n_epochs)
+    y0 = np.array([[0.0, 0.0], [3.0, 1.0]])
+    adj = mrs_amd.adjacency_to_csr([[(1, 1.0)], [(0, 1.0)]])
+    seed = 7
+    y = y0.copy()
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    for e in range(1, n_epochs):
+        n_neg = 4 if e == 1 else 5
+        g0 = _attr(p.a, p.b, y[0], y[1]) + _draws_other(seed, 0, e, n_neg) * _rep(
+            p.a, p.b, p.gamma, y[0], y[1])
+        g1 = _attr(p.a, p.b, y[1], y[0])
+        g = np.stack([g0, g1])
+        m += (1 - p.beta1) * (g - m)
+        v += (1 - p.beta2) * (g * g - v)
+        t = e + 1
+        sq = math.sqrt(1 - p.beta2 ** t)
+        y = y + p.lr * (1 - e / n_epochs) * sq / (1 - p.beta1 ** t) * m / (np.sqrt(v) + sq * p.eps)
+    got = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, p, seed, use_lut=False)
+    np.testing.assert_allclose(got, y, rtol=2e-5, atol=2e-6)
+    # with the LUT (b != 1 takes it): the same within the LUT's error
+    q = _two_node_params(n_epochs, a=1.5, b=0.9)
+    yl = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, q, seed, use_lut=True)
+    ye = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, q, seed, use_lut=False)
+    np.testing.assert_allclose(yl, ye, rtol=0, atol=1e-4)
+    # and the draws matter: another seed draws another count, another result
+    assert any(_draws_other(seed, 0, e, 4 if e == 1 else 5) !=
+               _draws_other(seed + 1, 0, e, 4 if e == 1 else 5) for e in range(1, 4))
+
+
+def test_adam_parallel_two_nodes_densmap_hand_computed():
+    # the density term of the parallel optimiser (umap_optimisers.rs:1303-1317,
+    # 1369-1387) on two nodes: each node's only neighbour is the other, so both
+    # log radii are log(1e-8 + d^2), their variance is 0, std = sqrt(var_shift),
+    # cov = 0, the weights are r itself and coeff = lambda mu_tot / std.
+    n_epochs, seed = 3, 11
+    p = _two_node_params(n_epochs)
+    lam, frac, vs = 2.0, 1.0, 0.1  # frac 1: every epoch is a density epoch
+    r = np.array([1.0, 0.5], np.float32)
+    mu_tot = 2.0
+    y0 = np.array([[0.0, 0.0], [1.0, 0.5]])
+    adj = mrs_amd.adjacency_to_csr([[(1, 1.0)], [(0, 1.0)]])
+    y = y0.copy()
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    for e in range(1, n_epochs):
+        d2 = float(((y[0] - y[1]) ** 2).sum())
+        phi = 1.0 / (1.0 + p.a * d2 ** p.b)
+        re = math.log(1e-8 + d2)
+        coeff = lam * mu_tot / math.sqrt(vs)
+        dr = (1.0 - p.b * (1.0 - phi)) / math.exp(re) + p.a * p.b * d2 ** p.b / d2 * phi
+        cor = coeff * (r[0] + r[1]) * dr / 1.0
+        gs = []
+        for i, o in ((0, 1), (1, 0)):
+            gi = _attr(p.a, p.b, y[i], y[o]) + 2.0 * np.clip(2.0 * cor * (y[i] - y[o]), -4, 4)
+            if i == 0:
+                gi = gi + _draws_other(seed, 0, e, 4 if e == 1 else 5) * _rep(
+                    p.a, p.b, p.gamma, y[0], y[1])
+            gs.append(gi)
+        g = np.stack(gs)
+        m += (1 - p.beta1) * (g - m)
+        v += (1 - p.beta2) * (g * g - v)
+        t = e + 1
+        sq = math.sqrt(1 - p.beta2 ** t)
+        y = y + p.lr * (1 - e / n_epochs) * sq / (1 - p.beta1 ** t) * m / (np.sqrt(v) + sq * p.eps)
+    got = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, p, seed, use_lut=False,
+                                     dens_r=r, dens_params=(lam, frac, vs), mu_tot=mu_tot)
+    np.testing.assert_allclose(got, y, rtol=2e-5, atol=2e-6)
+    plain = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, p, seed, use_lut=False)
+    assert np.max(np.abs(got - plain)) > 1e-3  # the density term moved the points
+
+
+@pytest.mark.parametrize("a,yb", [(1.0, (3.0, 1.0)), (25.0, (0.2, 0.0))])
+def test_sgd_two_nodes_hand_computed(a, yb):
+    # optimise_embedding_sgd (umap_optimisers.rs:653-865): the adjacency list is
+    # walked as directed edges (0 -> 1 then 1 -> 0), each moving both endpoints
+    # in place by lr * clip(coeff * delta, 4) per coordinate, then drawing its
+    # source's negatives from the updated positions.  a = 25, d = 0.2 puts the
+    # attractive step above the clip (coeff * delta = 5).
+    n_epochs, seed = 3, 5
+    p = _two_node_params(n_epochs, a=a)
+    y0 = np.array([[0.0, 0.0], list(yb)])
+    adj = mrs_amd.adjacency_to_csr([[(1, 1.0)], [(0, 1.0)]])
+    y = y0.copy()
+    for e in range(1, n_epochs):
+        lr = p.lr * (1 - e / n_epochs)
+        n_neg = 4 if e == 1 else 5
+        for i, j in ((0, 1), (1, 0)):
+            d2 = float(((y[i] - y[j]) ** 2).sum())
+            db = d2 ** p.b
+            coeff = 2.0 * p.a * p.b * db / (d2 * (1.0 + p.a * db))
+            gd = np.clip(coeff * (y[j] - y[i]), -4.0, 4.0)
+            y[i] += gd * lr
+            y[j] -= gd * lr
+            for s in range(n_neg):
+                if orc.hash_neg(seed, i, e, 0, s, 2) == i:
+                    continue
+                y[i] += _rep(p.a, p.b, p.gamma, y[i], y[j]) * lr
+    got = orc.optimise_sgd(y0.astype(np.float32), *adj, p, seed)
+    np.testing.assert_allclose(got, y, rtol=2e-5, atol=2e-6)
+
+
+def test_adam_parallel_three_nodes_fractional_schedule():
+    # Edges (0, 1) weight 1 and (1, 2) weight 0.45.  Edge (1, 2): per =
+    # 1 / 0.45 = 2.222, negative period 0.4444, first negative due at 0.4444;
+    # it fires at epochs 3, 5 and 7 with floor((3 - 0.444) / 0.444) =
+    # floor(5.75) = 5, floor((5 - 2.667) / 0.444) = floor(5.25) = 5 and
+    # floor((7 - 4.889) / 0.444) = floor(4.75) = 4 negatives, drawn by node 1
+    # (its smaller endpoint) under CSR slot 1.  Edge (0, 1) fires every epoch
+    # from 1 (4, then 5 negatives, node 0, slot 0).  The fractional quotients
+    # make the floor visible: ceil or round would draw 6 at epoch 3 and 5 at
+    # epoch 7 (and with this seed the extra draws hit another node).
+    n_epochs, seed, n = 8, 3, 3
+    p = _two_node_params(n_epochs)
+    y0 = np.array([[0.0, 0.0], [2.0, 1.0], [4.0, -1.0]])
+    adj = mrs_amd.adjacency_to_csr([[(1, 1.0)], [(0, 1.0), (2, 0.45)], [(1, 0.45)]])
+    fires = {0: {e: (4 if e == 1 else 5) for e in range(1, n_epochs)}, 1: {3: 5, 5: 5, 7: 4}}
+    edges = {0: (0, 1, 0), 1: (1, 2, 1)}  # edge -> (smaller, larger, slot under smaller)
+    y = y0.copy()
+    m = np.zeros_like(y)
+    v = np.zeros_like(y)
+    for e in range(n_epochs):
+        g = np.zeros_like(y)
+        hu = np.zeros(n, bool)
+        for ed, (i, j, slot) in edges.items():
+            if e not in fires[ed]:
+                continue
+            hu[i] = hu[j] = True
+            g[i] += _attr(p.a, p.b, y[i], y[j])
+            g[j] += _attr(p.a, p.b, y[j], y[i])
+            for s in range(fires[ed][e]):
+                k = orc.hash_neg(seed, i, e, slot, s, n)
+                if k != i:
+                    g[i] += _rep(p.a, p.b, p.gamma, y[i], y[k])
+        upd = hu[:, None]
+        m = np.where(upd, m + (1 - p.beta1) * (g - m), m)
+        v = np.where(upd, v + (1 - p.beta2) * (g * g - v), v)
+        t = e + 1
+        sq = math.sqrt(1 - p.beta2 ** t)
+        step = p.lr * (1 - e / n_epochs) * sq / (1 - p.beta1 ** t) * m / (np.sqrt(v) + sq * p.eps)
+        y = np.where(upd, y + step, y)
+    got = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, p, seed, use_lut=False)
+    np.testing.assert_allclose(got, y, rtol=2e-5, atol=2e-6)
