@@ -38,60 +38,70 @@ std::shared_ptr<DevicePool> current_pool() { return t_pool; }
 
 // ------------------------------------------------------------ memory pool
 
+HipRT::Event HipRT::record(Stream s) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) {
+        if (hipEventRecord(e, s) == hipSuccess) return e;
+        (void)hipEventDestroy(e);
+    }
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s);  // no event: the stream's work is done instead
+    return nullptr;
+}
+
+void HipRT::wait(Stream s, Event e) {
+    if (e && hipStreamWaitEvent(s, e, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventSynchronize(e);
+    }
+}
+
+void HipRT::host_wait(Event e) {
+    if (e) (void)hipEventSynchronize(e);
+}
+
+void HipRT::destroy(Event e) {
+    if (e) (void)hipEventDestroy(e);
+}
+
+void* HipRT::alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void HipRT::release(void* p) { (void)hipFree(p); }
+
+void HipRT::sync_all() { (void)hipDeviceSynchronize(); }
+
 DevicePool::~DevicePool() {
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(device_);
-    for (auto& kv : free_) (void)hipFree(kv.second);  // hipFree waits for the device
+    cache_.trim();
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
 void* DevicePool::get(size_t bytes, size_t* got) {
-    bytes = (bytes + 255) / 256 * 256;
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        auto it = free_.lower_bound(bytes);
-        if (it != free_.end() && it->first <= 2 * bytes) {
-            void* p = it->second;
-            *got = it->first;
-            cached_ -= it->first;
-            free_.erase(it);
-            return p;
-        }
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        trim();
-        MRS_HIP(hipMalloc(&p, bytes));
-    }
-    *got = bytes;
+    hipStream_t s = nullptr;
+    const bool scoped = current_stream(&s);
+    void* p = cache_.get(bytes, got, scoped ? s : nullptr);
+    if (!p)
+        throw Error(MRS_DEVICE_ERROR, "hipMalloc of " + std::to_string(bytes) +
+                                          " bytes failed (device memory exhausted)");
     return p;
 }
 
 void DevicePool::put(void* p, size_t bytes) {
-    std::lock_guard<std::mutex> lk(mu_);
-    free_.emplace(bytes, p);
-    cached_ += bytes;
+    hipStream_t s = nullptr;
+    const bool scoped = current_stream(&s);
+    cache_.put(p, bytes, scoped ? s : nullptr);
 }
 
-void DevicePool::trim() {
-    std::multimap<size_t, void*> blocks;
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        blocks.swap(free_);
-        cached_ = 0;
-    }
-    if (blocks.empty()) return;
-    // a cached block may still be read by queued work of its stream
-    (void)hipDeviceSynchronize();
-    for (auto& kv : blocks) (void)hipFree(kv.second);
-}
-
-size_t DevicePool::cached_bytes() {
-    std::lock_guard<std::mutex> lk(mu_);
-    return cached_;
-}
+void DevicePool::trim() { cache_.trim(); }
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
@@ -417,12 +427,12 @@ int mrs_ctx_create(int32_t device, size_t hbm_budget, mrs_ctx** out) {
         auto* c = new mrs_ctx();
         c->device = device;
         c->hbm_budget = hbm_budget;
-        c->pool = std::make_shared<mrs::DevicePool>(device);
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;
             throw mrs::Error(MRS_DEVICE_ERROR, "hipStreamCreate failed");
         }
+        c->pool = std::make_shared<mrs::DevicePool>(device, c->stream);
         *out = c;
     });
 }

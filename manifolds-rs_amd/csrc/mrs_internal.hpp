@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/mrs_amd.h"
+#include "block_cache.hpp"
 
 namespace mrs {
 
@@ -61,29 +62,46 @@ int guarded(F&& body) {
     }
 }
 
-// Device memory kept for reuse by one context (one stream).  A block a DevBuf
-// releases goes back here instead of to hipFree, and a later DevBuf of the
-// same context takes it: every user of a context's blocks works on that
-// context's stream, so stream order alone keeps a block's old and new users
-// apart, and a fit of the same shape allocates nothing after the first.
-// Blocks are reused for requests of at least half their size.  When hipMalloc
-// fails, the cache is returned to the device and the allocation tried again.
+// The HIP runtime behind BlockCache: events without timing, hipStreamWaitEvent
+// for a stream-ordered fence, hipMalloc / hipFree.  Nothing here throws (blocks
+// are released from destructors): an event that cannot be recorded is replaced
+// by a host wait on its stream, and record returns nullptr.
+struct HipRT {
+    using Stream = hipStream_t;
+    using Event = hipEvent_t;
+    static Event record(Stream s);
+    static void wait(Stream s, Event e);
+    static void host_wait(Event e);
+    static void destroy(Event e);
+    static void* alloc(size_t bytes);
+    static void release(void* p);
+    static void sync_all();
+};
+
+// Device memory kept for reuse by one context (home stream: the context's).
+// A block a DevBuf releases goes back here instead of to hipFree, and a later
+// DevBuf of the same context takes it, so a fit of the same shape allocates
+// nothing after the first.  Blocks are reused for requests of at least half
+// their size.  Reuse across streams is fenced (block_cache.hpp): a block
+// released from, or handed to, another stream than the one its last user
+// worked on waits for that work through an event.  When hipMalloc fails, the
+// cache is returned to the device and the allocation tried again.
 class DevicePool {
   public:
-    explicit DevicePool(int device) : device_(device) {}
+    DevicePool(int device, hipStream_t home) : device_(device), cache_(home) {}
     ~DevicePool();  // frees every cached block
     DevicePool(const DevicePool&) = delete;
     DevicePool& operator=(const DevicePool&) = delete;
+    // for use on the calling thread's StreamScope stream (the host if none)
     void* get(size_t bytes, size_t* got);
+    // released by the calling thread's StreamScope stream (home if none)
     void put(void* p, size_t bytes);
     void trim();  // waits for the device, then frees the cached blocks
-    size_t cached_bytes();
+    size_t cached_bytes() { return cache_.cached_bytes(); }
 
   private:
-    std::mutex mu_;
-    std::multimap<size_t, void*> free_;
-    size_t cached_ = 0;
     int device_;
+    BlockCache<HipRT> cache_;
 };
 
 // The stream the calling thread works on inside the library (set for the

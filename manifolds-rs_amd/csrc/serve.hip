@@ -17,6 +17,12 @@
 //   * retires a fit after its last epoch (transpose to [n_dim][n]; the caller
 //     collects it with mrs_server_wait) and frees its memory in stream order.
 //
+// Memory: a prep worker builds a fit's optimiser state from its own context's
+// block cache, and the batcher uses and releases it on the batcher stream.
+// The cache fences such cross-stream reuse (block_cache.hpp): a block the
+// batcher returns carries an event recorded after the fit's last queued
+// kernel, and the prep stream that takes it next waits for that event.
+//
 // A fit's epochs depend only on its own state, seed and epoch counter (the
 // negative-sample hash is keyed by node, epoch and CSR slot), so batching
 // changes nothing for the node-exact strategy and only the FP32 atomic
@@ -46,11 +52,6 @@ namespace {
 constexpr int kWg = 256;
 
 }  // namespace
-
-// mrs_server_create refuses while this is false: one run of the served stream
-// (several prep streams plus the batcher) ended in a device memory fault that
-// the same build and inputs did not reproduce, and no cause has been shown.
-constexpr bool kServerEnabled = false;
 
 namespace {
 
@@ -229,11 +230,15 @@ void worker_loop(mrs_server* sv, int w) {
         int status = MRS_OK;
         std::string err;
         try {
-            StreamScope scope(ctx->stream);
+            // every buffer of the preparation comes from this worker's cache
+            StreamScope scope(ctx->stream, ctx->pool);
             FitTimings t;
             r->st = fit_prepare(ctx, r->data, r->n, r->d, nullptr, nullptr, 0, &r->params,
                                 nullptr, r->seed, 0, t);
             MRS_HIP(hipStreamSynchronize(ctx->stream));
+            // keep the cache to a share of the budget (blocks of many sizes pile up)
+            if (ctx->pool->cached_bytes() > sv->budget / (4 * sv->prep_ctx.size()))
+                ctx->pool->trim();
         } catch (const Error& e) {
             status = e.status;
             err = e.what();
@@ -243,7 +248,7 @@ void worker_loop(mrs_server* sv, int w) {
         }
         if (status != MRS_OK) {
             if (r->st) {
-                StreamScope scope(ctx->stream);
+                StreamScope scope(ctx->stream, ctx->pool);
                 r->st.reset();
             }
             finish(sv, r, status, err);
@@ -292,7 +297,9 @@ void batcher_loop(mrs_server* sv) {
     mrs_ctx* ctx = sv->batch_ctx;
     hipStream_t s = ctx->stream;
     (void)hipSetDevice(sv->device);
-    StreamScope scope(s);
+    // outputs come from the batcher's cache; states released here go back to
+    // their prep worker's cache fenced by an event on this stream
+    StreamScope scope(s, ctx->pool);
     std::vector<mrs_server_request*> active;
     struct Group {
         int dim, first, count;
@@ -428,10 +435,6 @@ int mrs_server_create(int32_t device, size_t hbm_budget, int32_t prep_workers, i
                       mrs_server** out) {
     return mrs::guarded([&] {
         if (!out) throw Error(MRS_INVALID_ARGUMENT, "out is NULL");
-        if (!mrs::kServerEnabled)
-            throw Error(MRS_INVALID_ARGUMENT,
-                        "the fit server is switched off: a device memory fault under concurrent "
-                        "serving has no identified cause yet");
         if (prep_workers < 1 || prep_workers > 16)
             throw Error(MRS_INVALID_ARGUMENT, "prep_workers must be 1..16");
         if (max_active < 1 || max_active > 4096)

@@ -17,12 +17,7 @@ import pytest
 import mrs_amd as M
 from metrics import trustworthiness
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the fit server is switched off (mrs_server_create refuses): one "
-                     "served-stream run ended in a device memory fault that the same build and "
-                     "inputs did not reproduce, and its cause has not been shown"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def mixture(n, d, n_clusters, seed):
