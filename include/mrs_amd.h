@@ -194,6 +194,18 @@ int mrs_last_fit_timings(double *out);
  * shortlist of max(shortlist, k) rounded up to 64 or 128 candidates). */
 int mrs_knn_exhaustive(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
                        int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
+/* The same search without any allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  x, idx and dist must be
+ * device memory; n <= 16384, d <= 128, max(k, shortlist) <= 64 (larger
+ * problems are MRS_INVALID_ARGUMENT).  Results equal mrs_knn_exhaustive's bit
+ * for bit.  The work is queued on the context's stream and the call returns
+ * without waiting for it (mrs_ctx_synchronize, or order later work on
+ * mrs_ctx_stream).  Its working memory is one 72 MiB device array per GPU,
+ * kept until the process ends and shared by every call on that GPU: calls
+ * that overlap on one GPU (other contexts' streams) race. */
+int mrs_knn_exhaustive_noalloc(mrs_ctx *ctx, const float *x, int64_t n, int32_t d, int32_t k,
+                               int32_t precision, int32_t shortlist, int32_t *idx, float *dist);
 /* Same, returning in ms_out[0..9]: the stage times in ms of pack (with the
  * cell layout when pruning), shortlist (every pass), merge+re-rank; database
  * splits used; shortlist size; the fraction of (query block, database stage)

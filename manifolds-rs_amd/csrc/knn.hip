@@ -37,6 +37,9 @@
 //   the shortlist, for both precisions.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -1145,7 +1148,7 @@ Shape shortlist_shape(bool bf16, int KP) {
 
 void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, int32_t k,
                         int32_t precision, int32_t shortlist, int32_t* out_idx, float* out_dist,
-                        KnnTimings* timings) {
+                        KnnTimings* timings, const KnnScratch* scratch) {
     if (n64 <= 1) throw Error(MRS_NO_DATA, "kNN needs at least two points (NoData)");
     if (k <= 0 || k > n64 - 1) throw Error(MRS_INVALID_ARGUMENT, "k must be in [1, n-1]");
     if (d <= 0 || d > 128) throw Error(MRS_INVALID_ARGUMENT, "exhaustive kNN supports 1 <= d <= 128");
@@ -1164,7 +1167,9 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     const uint32_t npad = ((n + 255) / 256) * 256;
     hipStream_t s = ctx->stream;
 
-    // stage timing events, destroyed on every exit (including a throw)
+    if (scratch && timings) throw Error(MRS_INVALID_ARGUMENT, "no timings from scratch memory");
+    // stage timing events, destroyed on every exit (including a throw); none
+    // with caller scratch (nothing is created or waited for then)
     struct Events {
         hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
         ~Events() {
@@ -1173,17 +1178,42 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         }
     } evs;
     hipEvent_t* ev = evs.e;
-    for (int i = 0; i < 4; i++) MRS_HIP(hipEventCreate(&ev[i]));
-    MRS_HIP(hipEventRecord(ev[0], s));
+    if (!scratch)
+        for (int i = 0; i < 4; i++) MRS_HIP(hipEventCreate(&ev[i]));
+    auto mark = [&](int i) {
+        if (ev[i]) MRS_HIP(hipEventRecord(ev[i], s));
+    };
+    mark(0);
+    // caller scratch, carved in 256-byte aligned pieces
+    size_t carved = 0;
+    auto carve = [&](size_t bytes) -> void* {
+        const size_t at = (carved + 255) / 256 * 256;
+        if (at + bytes > scratch->bytes)
+            throw Error(MRS_INVALID_ARGUMENT, "the kNN scratch is too small for this problem");
+        carved = at + bytes;
+        return scratch->base + at;
+    };
 
-    DevBuf<unsigned char> P((size_t)npad * KP * (bf16 ? 2 : 4));
-    DevBuf<float> negh(npad);
+    const size_t p_bytes = (size_t)npad * KP * (bf16 ? 2 : 4);
+    DevBuf<unsigned char> P_own;
+    DevBuf<float> negh_own;
+    unsigned char* Pp;
+    float* neghp;
+    if (scratch) {
+        Pp = static_cast<unsigned char*>(carve(p_bytes));
+        neghp = static_cast<float*>(carve(sizeof(float) * npad));
+    } else {
+        P_own.alloc(p_bytes);
+        negh_own.alloc(npad);
+        Pp = P_own.p;
+        neghp = negh_own.p;
+    }
     if (bf16)
         hipLaunchKernelGGL(k_pack<true>, dim3(div_up(npad, 4)), dim3(256), 0, s, x, n, d, npad, KP,
-                           (void*)P.p, negh.p);
+                           (void*)Pp, neghp);
     else
         hipLaunchKernelGGL(k_pack<false>, dim3(div_up(npad, 4)), dim3(256), 0, s, x, n, d, npad,
-                           KP, (void*)P.p, negh.p);
+                           KP, (void*)Pp, neghp);
     MRS_CHECK_LAUNCH();
 
     const Shape sh = shortlist_shape(bf16, KP);
@@ -1264,38 +1294,51 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     double swept = 1.0, list_mean = 0.0;
     uint32_t cells = 0, list_max = 0, aux_items = 0;
 
-    const bool prune = ctx->knn_prune && n >= kPruneMinRows && sh.W * kQPerWave == (int)kBlockRows;
+    const bool prune = !scratch && ctx->knn_prune && n >= kPruneMinRows &&
+                       sh.W * kQPerWave == (int)kBlockRows;
     if (!prune) {
         const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
-        splits = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
+        const uint32_t max_splits = scratch ? (uint32_t)std::max(1, scratch->max_splits) : 8u;
+        splits = (int)std::min<uint32_t>(max_splits,
+                                         std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
         uint32_t db_per = ((npad + splits - 1) / splits + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
         splits = (int)((npad + db_per - 1) / db_per);
-        DevBuf<uint64_t> lists((size_t)splits * npad * S);
-        DevBuf<uint64_t> queue((size_t)splits * npad * sh.QC);
+        const size_t n_list = (size_t)splits * npad * S, n_queue = (size_t)splits * npad * sh.QC;
+        DevBuf<uint64_t> lists_own, queue_own;
+        uint64_t *lists, *queue;
+        if (scratch) {
+            lists = static_cast<uint64_t*>(carve(sizeof(uint64_t) * n_list));
+            queue = static_cast<uint64_t*>(carve(sizeof(uint64_t) * n_queue));
+        } else {
+            lists_own.alloc(n_list);
+            queue_own.alloc(n_queue);
+            lists = lists_own.p;
+            queue = queue_own.p;
+        }
         ShortlistArgs sa{};
-        sa.P = P.p;
-        sa.negh = negh.p;
-        sa.lists = lists.p;
-        sa.queue = queue.p;
+        sa.P = Pp;
+        sa.negh = neghp;
+        sa.lists = lists;
+        sa.queue = queue;
         sa.n = n;
         sa.npad = npad;
         sa.KP = KP;
         sa.db_per_split = db_per;
-        MRS_HIP(hipEventRecord(ev[1], s));
+        mark(1);
         run_shortlist(sa, dim3(qblocks, splits), false);
-        MRS_HIP(hipEventRecord(ev[2], s));
+        mark(2);
         if (splits > 1) {
             if (E == 1)
                 hipLaunchKernelGGL(k_merge_splits<1>, dim3(div_up(npad, 4)), dim3(256), 0, s,
-                                   lists.p, npad, splits);
+                                   lists, npad, splits);
             else
                 hipLaunchKernelGGL(k_merge_splits<2>, dim3(div_up(npad, 4)), dim3(256), 0, s,
-                                   lists.p, npad, splits);
+                                   lists, npad, splits);
             MRS_CHECK_LAUNCH();
         }
-        rerank(lists.p);
-        MRS_HIP(hipEventRecord(ev[3], s));
-        MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
+        rerank(lists);
+        mark(3);
+        if (!scratch) MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
     } else {
         // ---- cell order (see the pruned-sweep kernels above)
         const int elem = bf16 ? 2 : 4;
@@ -1310,18 +1353,18 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         DevBuf<double> cell_d2(mpad);
         MRS_HIP(hipMemsetAsync(ccount.p, 0, ccount.bytes(), s));
         MRS_HIP(hipMemsetAsync(cell_d2.p, 0, cell_d2.bytes(), s));
-        const void* pp = P.p;
+        const void* pp = Pp;
         const dim3 ga(div_up(n, 128));
         if (bf16) {
             hipLaunchKernelGGL(k_seed_rows<true>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
                                seeds.p, snegh.p);
             hipLaunchKernelGGL(k_assign_cells<true>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
-                               snegh.p, mpad, cell.p, ccount.p, negh.p, dist2.p, cell_d2.p);
+                               snegh.p, mpad, cell.p, ccount.p, neghp, dist2.p, cell_d2.p);
         } else {
             hipLaunchKernelGGL(k_seed_rows<false>, dim3(mpad), dim3(64), 0, s, pp, KP, KS, n, m,
                                seeds.p, snegh.p);
             hipLaunchKernelGGL(k_assign_cells<false>, ga, dim3(256), 0, s, pp, KP, KS, n, seeds.p,
-                               snegh.p, mpad, cell.p, ccount.p, negh.p, dist2.p, cell_d2.p);
+                               snegh.p, mpad, cell.p, ccount.p, neghp, dist2.p, cell_d2.p);
         }
         MRS_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, s, cell.p, dist2.p,
@@ -1352,7 +1395,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         DevBuf<unsigned char> P2((size_t)npos * row_bytes);
         DevBuf<float> negh2(npos);
         hipLaunchKernelGGL(k_gather_rows, dim3(div_up((uint64_t)npos * (row_bytes / 16), 256)),
-                           dim3(256), 0, s, P.p, row_bytes, negh.p, rowmap.p, npos, P2.p, negh2.p);
+                           dim3(256), 0, s, Pp, row_bytes, neghp, rowmap.p, npos, P2.p, negh2.p);
         MRS_CHECK_LAUNCH();
         const uint32_t nblk = npos / kBlockRows, nst = npos / (uint32_t)sh.RT;
         DevBuf<float> bcen((size_t)nblk * KP), brad(nblk), bnorm(nblk);
@@ -1454,7 +1497,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.items = items1.p;
         sa.stage_ids = ids1.p;
         MRS_HIP(hipStreamSynchronize(s));  // hitems is read by the copy
-        MRS_HIP(hipEventRecord(ev[1], s));
+        mark(1);
         if (!hitems.empty()) run_shortlist(sa, dim3((unsigned)hitems.size(), 1), true);
         // pass 2: the stages the bounds cannot rule out
         DevBuf<uint32_t> counts(nblk), offs((size_t)nblk + 1);
@@ -1503,7 +1546,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
                                    dmerges.p, (uint32_t)hmerges.size());
             MRS_CHECK_LAUNCH();
         }
-        MRS_HIP(hipEventRecord(ev[2], s));
+        mark(2);
         // the re-rank in cell order, from X in cell order
         const int dp = (d + 3) / 4 * 4;
         DevBuf<float> xs((size_t)npos * dp);
@@ -1517,7 +1560,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
             hipLaunchKernelGGL(k_rerank_cells<2>, gr, dim3(256), 0, s, xs.p, dp, d, npos, lists.p,
                                rowmap.p, posmap.p, k, out_idx, out_dist);
         MRS_CHECK_LAUNCH();
-        MRS_HIP(hipEventRecord(ev[3], s));
+        mark(3);
         MRS_HIP(hipEventSynchronize(ev[3]));  // before the buffers go
         const double full = (double)(pstart[m] / kBlockRows) * (double)(pstart[m] / (uint32_t)sh.RT);
         swept = full > 0 ? ((double)pairs1 + (double)total) / full : 1.0;
@@ -1542,9 +1585,69 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     }
 }
 
+// ------------------------------------------- the search without allocation
+//
+// Scratch for mrs_knn_exhaustive_noalloc: the full sweep's packed operand,
+// norms, candidate lists and queues for up to kNoAllocRows rows of up to 128
+// dimensions, shortlists of 64 and four database splits (72 MiB).  One copy
+// lives on every GPU the library runs on, until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name in host code is not a device address).
+constexpr int64_t kNoAllocRows = 16384;
+constexpr int kNoAllocSplits = 4;
+constexpr size_t kNoAllocScratchBytes =
+    (size_t)kNoAllocRows * 128 * 4                                  // packed operand (FP32)
+    + (size_t)kNoAllocRows * 4                                      // -|y|^2 / 2
+    + 2 * (size_t)kNoAllocSplits * kNoAllocRows * 64 * 8            // lists and queues
+    + 4 * 256;                                                      // alignment of the pieces
+
+namespace {
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_addr;  // device -> address of its scratch
+
+// the current device's scratch address, looked up once per device
+unsigned char* device_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* ptr = nullptr;
+    MRS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(MRS_DEVICE_ERROR, "no address for the kNN scratch");
+    g_scratch_addr.emplace(device, static_cast<unsigned char*>(ptr));
+    return static_cast<unsigned char*>(ptr);
+}
+
+}  // namespace
+
 }  // namespace mrs
 
 extern "C" {
+
+int mrs_knn_exhaustive_noalloc(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32_t k,
+                               int32_t precision, int32_t shortlist, int32_t* idx, float* dist) {
+    return mrs::guarded([&] {
+        if (!ctx || !x || !idx || !dist) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
+        if (n > mrs::kNoAllocRows)
+            throw mrs::Error(MRS_INVALID_ARGUMENT,
+                             "n = " + std::to_string(n) + " exceeds the no-allocation search's " +
+                                 std::to_string(mrs::kNoAllocRows) + " rows");
+        if (std::max(shortlist, k) > 64)
+            throw mrs::Error(MRS_INVALID_ARGUMENT,
+                             "the no-allocation search keeps shortlists of at most 64");
+        MRS_HIP(hipSetDevice(ctx->device));
+        if (!mrs::is_device_ptr(x) || !mrs::is_device_ptr(idx) || !mrs::is_device_ptr(dist))
+            throw mrs::Error(MRS_INVALID_ARGUMENT,
+                             "the no-allocation search takes device buffers only (staging host "
+                             "memory would allocate)");
+        mrs::KnnScratch scr{mrs::device_scratch(ctx->device), mrs::kNoAllocScratchBytes,
+                            mrs::kNoAllocSplits};
+        mrs::StreamScope scope(ctx->stream);
+        mrs::knn_exhaustive_dev(ctx, x, n, d, k, precision, shortlist, idx, dist, nullptr, &scr);
+    });
+}
 
 int mrs_knn_exhaustive(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32_t k,
                        int32_t precision, int32_t shortlist, int32_t* idx, float* dist) {

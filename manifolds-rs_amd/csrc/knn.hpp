@@ -15,9 +15,19 @@ struct KnnTimings {
     uint32_t aux_items = 0;  // pass-2 work items that fill aux lists (k_merge_aux folds them in)
 };
 
+// Caller-provided device memory for the full sweep's buffers (packed operand,
+// norms, candidate lists and queues), at most max_splits database splits.
+// With it the search allocates nothing, creates no event and does not wait
+// for the device: it is queued on the context's stream and returns.
+struct KnnScratch {
+    unsigned char* base;
+    size_t bytes;
+    int max_splits;
+};
+
 // x: device [n][d] f32; out_idx/out_dist: device [n][k].
 void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n, int32_t d, int32_t k,
                         int32_t precision, int32_t shortlist, int32_t* out_idx, float* out_dist,
-                        KnnTimings* timings);
+                        KnnTimings* timings, const KnnScratch* scratch = nullptr);
 
 }  // namespace mrs

@@ -205,6 +205,7 @@ def lib() -> C.CDLL:
         "mrs_umap_state_kernel_times": (C.c_int, [vp, C.POINTER(C.c_double),
                                                   C.POINTER(C.c_int64)]),
         "mrs_knn_exhaustive": (C.c_int, [vp, vp, i64, i32, i32, i32, i32, vp, vp]),
+        "mrs_knn_exhaustive_noalloc": (C.c_int, [vp, vp, i64, i32, i32, i32, i32, vp, vp]),
         "mrs_knn_exhaustive_timed": (C.c_int, [vp, vp, i64, i32, i32, i32, i32, vp, vp,
                                                C.POINTER(C.c_double)]),
         "mrs_fuzzy_graph": (C.c_int, [vp, vp, vp, i64, i32, C.POINTER(UmapGraphParams), i32,
@@ -378,6 +379,16 @@ def knn_exhaustive(ctx: Context, x, k: int, precision: int = KNN_FP32, shortlist
     _check(lib().mrs_knn_exhaustive(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
                                     _ptr(idx), _ptr(dist)))
     return idx, dist
+
+
+def knn_exhaustive_noalloc(ctx: Context, x, idx, dist, k: int, precision: int = KNN_FP32,
+                           shortlist: int = 0) -> None:
+    """The exhaustive kNN into caller-provided device tensors, allocating
+    nothing (n <= 16384, d <= 128, shortlist <= 64); queued on the context's
+    stream, not waited for."""
+    n, d = x.shape
+    _check(lib().mrs_knn_exhaustive_noalloc(ctx.handle, _ptr(x), n, d, k, precision, shortlist,
+                                            _ptr(idx), _ptr(dist)))
 
 
 class FuzzyGraph:
