@@ -493,68 +493,66 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 
 // ------------------------------------------------------ edge-parallel epoch
 
-// One edge-parallel epoch, with the firing edges compacted per wavefront.
-// About one edge in five fires in an epoch, scattered over the edge list, so
-// with one thread per edge nearly every wavefront would have a firing lane and
-// run the whole negative-sampling body for all 64 lanes.  Here each wavefront owns
-// kEdgeSpan consecutive edges: it first advances their schedule cursors and
-// queues the firing ones in its own LDS slice, in edge order (so runs of equal
-// i stay together for the segmented sum), then works through the queue 64
-// edges at a time.  The queue is built from ballots: no counter is shared
-// between wavefronts.
-// Span: 256 edges per wavefront (256 and 512 measured alike on the 1M fit
-// graph; 1024 and 2048 slower).  Below kWideSpanEdges edges that leaves too few
-// wavefronts to fill the chip, and spans of 64 are used.
+// One edge-parallel epoch: each wavefront owns kEdgeSpan consecutive edges
+// and works through the firing ones compacted in LDS (epoch_edge_span).
+// Spans of 128 edges (128 and 64 alike and 256 slower on the 1M fit graph:
+// 317, 320 and 327 us); below kWideSpanEdges edges spans of 64 leave enough
+// wavefronts to fill the chip.
 constexpr uint32_t kWideSpanEdges = 1u << 20;
+constexpr int kWideSpan = 128;
 
 template <int DIM, bool DENS, int kEdgeSpan>
 __global__ __launch_bounds__(kWg) void k_epoch_edges(
-    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
-    float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
+    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
+    const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
-    static_assert(kEdgeSpan % 64 == 0, "whole wavefront rounds");
     constexpr int kWaves = kWg / 64;
     __shared__ uint32_t queue[kWaves][kEdgeSpan];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
+    const int wave = threadIdx.x >> 6;
     const uint64_t base = ((uint64_t)blockIdx.x * kWaves + wave) * kEdgeSpan;
-    uint32_t count = 0;  // wavefront-uniform
-    for (int r = 0; r < kEdgeSpan / 64; r++) {
-        const uint64_t e = base + (uint64_t)r * 64 + lane;
-        bool fire = false;
-        if (e < n_edges) {
-            const float nx = next[e];
-            if (nx <= epoch_f) {
-                fire = true;
-                next[e] = nx + per[e];
-            }
-        }
-        const uint64_t m = __ballot(fire);
-        if (fire) queue[wave][count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
-        count += (uint32_t)__popcll(m);
-    }
-    __syncthreads();  // the queue is read by other lanes than the ones that wrote it
-    for (uint32_t q0 = 0; q0 < count; q0 += 64) {
-        const uint32_t slot = q0 + lane;
-        const bool active = slot < count;
-        const uint32_t e = active ? queue[wave][slot] : 0u;
-        edge_work<DIM, DENS>(active, e, edges, embd, grad, flag, epoch, seed, fm, a, b, two_a_b,
-                             two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
-    }
+    epoch_edge_span<DIM, DENS, kEdgeSpan>(base, queue[wave], edges, per, next, embd, grad,
+                                          n_edges, epoch, epoch_f, seed, fm, a, b, two_a_b,
+                                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
 }
 
 template <int DIM>
 __global__ __launch_bounds__(kWg) void k_adam_clear(float* __restrict__ grad,
-                                                    uint32_t* __restrict__ flag,
                                                     float* __restrict__ m, float* __restrict__ v,
                                                     float* __restrict__ embd, uint32_t n,
                                                     float fused_lr, float epsc, float omb1,
                                                     float omb2, float* __restrict__ norm_partial) {
     const uint32_t node = blockIdx.x * kWg + threadIdx.x;
     if (node >= n) return;
-    adam_clear_node<DIM>(node, grad, flag, m, v, embd, fused_lr, epsc, omb1, omb2, norm_partial);
+    adam_clear_node<DIM>(node, grad, m, v, embd, fused_lr, epsc, omb1, omb2, norm_partial);
+}
+
+// grad = -0.0 everywhere: no node has an update yet (the edge strategy's
+// cleared state; a kernel, not a memset, so it may sit in a captured graph)
+__global__ __launch_bounds__(kWg) void k_fill_no_update(float* __restrict__ grad, uint64_t count) {
+    const uint64_t q = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (q < count) grad[q] = -0.0f;
+}
+
+// has_update[i] from the -0.0 markers (what a caller reads after an edge epoch)
+__global__ __launch_bounds__(kWg) void k_flags_from_marks(const float* __restrict__ grad,
+                                                          uint32_t n, int dim,
+                                                          uint32_t* __restrict__ flag) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    uint32_t any = 0;
+    for (int d = 0; d < dim; d++) any |= __float_as_uint(grad[(size_t)i * dim + d]) != kNoUpdateBits;
+    flag[i] = any;
+}
+
+// the -0.0 markers from has_update (a caller wrote gradients and flags)
+__global__ __launch_bounds__(kWg) void k_marks_from_flags(float* __restrict__ grad,
+                                                          const uint32_t* __restrict__ flag,
+                                                          uint32_t n, int dim) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n || flag[i]) return;
+    for (int d = 0; d < dim; d++) grad[(size_t)i * dim + d] = -0.0f;
 }
 
 template <class F>
@@ -676,12 +674,31 @@ void launch_dens_update_dev(mrs_umap_state* st, bool reference_order) {
     MRS_CHECK_LAUNCH();
 }
 
+// The edge strategy's cleared state: every gradient at the -0.0 marker.
+void clear_update_marks(mrs_umap_state* st) {
+    const uint64_t count = (uint64_t)st->n * st->dim;
+    hipLaunchKernelGGL(k_fill_no_update, dim3(div_up(count, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->grad.p, count);
+    MRS_CHECK_LAUNCH();
+    st->grad_clean = true;
+}
+
+void derive_update_flags(mrs_umap_state* st) {
+    const uint32_t n = (uint32_t)st->n;
+    hipLaunchKernelGGL(k_flags_from_marks, dim3(div_up(n, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->grad.p, n, st->dim, st->has_update.p);
+    MRS_CHECK_LAUNCH();
+}
+
+void marks_from_flags(mrs_umap_state* st) {
+    const uint32_t n = (uint32_t)st->n;
+    hipLaunchKernelGGL(k_marks_from_flags, dim3(div_up(n, kWg)), dim3(kWg), 0, st->ctx->stream,
+                       st->grad.p, st->has_update.p, n, st->dim);
+    MRS_CHECK_LAUNCH();
+}
+
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
-    if (!st->grad_clean) {
-        MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), st->ctx->stream));
-        MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), st->ctx->stream));
-        st->grad_clean = true;
-    }
+    if (!st->grad_clean) clear_update_marks(st);
     const mrs_umap_optim_params& p = st->params;
     const float two = 2.0f;
     FastMod fm = make_fastmod((uint32_t)st->n);
@@ -689,36 +706,25 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     hipStream_t s = st->ctx->stream;
     // one wavefront per span of edges
     const bool wide = ne >= kWideSpanEdges;
-    const unsigned blocks = div_up(div_up(ne, wide ? 256 : 64), kWg / 64);
+    const unsigned blocks = div_up(div_up(ne, wide ? kWideSpan : 64), kWg / 64);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edges.p, st->per.p,
-                           st->next.p, st->embd.p, st->grad.p,
-                           st->has_update.p, ne, (uint32_t)epoch, (float)epoch, seed, fm, p.a,
-                           p.b, two * p.a * p.b, two * p.gamma * p.b, p.neg_sample_rate,
-                           st->mu.p, reinterpret_cast<const float4*>(st->dens_pack.p),
-                           st->dens_scalars.p);
+                           st->next.p, st->embd.p, st->grad.p, ne, (uint32_t)epoch, (float)epoch,
+                           seed, fm, p.a, p.b, two * p.a * p.b, two * p.gamma * p.b,
+                           p.neg_sample_rate, st->mu.p,
+                           reinterpret_cast<const float4*>(st->dens_pack.p), st->dens_scalars.p);
+    };
+    auto by_span = [&](auto d) {
+        constexpr int D = decltype(d)::value;
+        if (wide) dens ? go(k_epoch_edges<D, true, kWideSpan>) : go(k_epoch_edges<D, false, kWideSpan>);
+        else dens ? go(k_epoch_edges<D, true, 64>) : go(k_epoch_edges<D, false, 64>);
     };
     switch (st->dim) {
-        case 1:
-            if (wide) dens ? go(k_epoch_edges<1, true, 256>) : go(k_epoch_edges<1, false, 256>);
-            else dens ? go(k_epoch_edges<1, true, 64>) : go(k_epoch_edges<1, false, 64>);
-            break;
-        case 2:
-            if (wide) dens ? go(k_epoch_edges<2, true, 256>) : go(k_epoch_edges<2, false, 256>);
-            else dens ? go(k_epoch_edges<2, true, 64>) : go(k_epoch_edges<2, false, 64>);
-            break;
-        case 3:
-            if (wide) dens ? go(k_epoch_edges<3, true, 256>) : go(k_epoch_edges<3, false, 256>);
-            else dens ? go(k_epoch_edges<3, true, 64>) : go(k_epoch_edges<3, false, 64>);
-            break;
-        case 4:
-            if (wide) dens ? go(k_epoch_edges<4, true, 256>) : go(k_epoch_edges<4, false, 256>);
-            else dens ? go(k_epoch_edges<4, true, 64>) : go(k_epoch_edges<4, false, 64>);
-            break;
-        case 8:
-            if (wide) dens ? go(k_epoch_edges<8, true, 256>) : go(k_epoch_edges<8, false, 256>);
-            else dens ? go(k_epoch_edges<8, true, 64>) : go(k_epoch_edges<8, false, 64>);
-            break;
+        case 1: by_span(Dim<1>{}); break;
+        case 2: by_span(Dim<2>{}); break;
+        case 3: by_span(Dim<3>{}); break;
+        case 4: by_span(Dim<4>{}); break;
+        case 8: by_span(Dim<8>{}); break;
         default:
             throw Error(MRS_INVALID_ARGUMENT,
                         "edge-parallel epoch supports n_dim in {1,2,3,4,8}; use node-exact");
@@ -732,8 +738,8 @@ void launch_adam_clear(mrs_umap_state* st, int32_t epoch, float* norm_partial) {
     const float omb1 = 1.0f - st->params.beta1, omb2 = 1.0f - st->params.beta2;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(div_up(n, kWg)), dim3(kWg), 0, st->ctx->stream,
-                           st->grad.p, st->has_update.p, st->m.p, st->v.p, st->embd.p, n,
-                           sc.fused_lr, sc.epsc, omb1, omb2, norm_partial);
+                           st->grad.p, st->m.p, st->v.p, st->embd.p, n, sc.fused_lr, sc.epsc,
+                           omb1, omb2, norm_partial);
     };
     switch (st->dim) {
         case 1: go(k_adam_clear<1>); break;
@@ -813,13 +819,9 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
         return;
     }
     // Capture the whole loop once and replay it: one launch per fit instead of
-    // 2-3 per epoch.  The accumulate-by-atomics buffers are cleared outside the
-    // capture (no memset nodes inside the graph).
-    if (strategy == MRS_EPOCH_EDGE_ATOMIC && !st->grad_clean) {
-        MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), s));
-        MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), s));
-        st->grad_clean = true;
-    }
+    // 2-3 per epoch.  The accumulate-by-atomics buffer is cleared outside the
+    // capture.
+    if (strategy == MRS_EPOCH_EDGE_ATOMIC && !st->grad_clean) clear_update_marks(st);
     const int64_t key[4] = {first, count, (int64_t)seed, strategy};
     const bool hit = st->graph_exec && std::equal(key, key + 4, st->graph_key);
     if (!hit && !std::equal(key, key + 4, st->last_key)) {
@@ -918,7 +920,7 @@ void state_alloc_nodes(mrs_umap_state* st, const float* embd) {
     MRS_HIP(hipMemsetAsync(st->v.p, 0, st->v.bytes(), s));
     MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), s));
     MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), s));
-    st->grad_clean = true;
+    st->grad_clean = false;  // zeros, not the edge strategy's -0.0 markers
     st->dens_scalars.alloc(4);
     MRS_HIP(hipMemsetAsync(st->dens_scalars.p, 0, st->dens_scalars.bytes(), s));
     if (!st->dens_enabled) {

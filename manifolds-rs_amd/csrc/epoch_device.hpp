@@ -51,35 +51,99 @@ __device__ inline bool seg_reduce(uint32_t key, float (&v)[D]) {
     return head;
 }
 
+// "No update this epoch" is marked in the accumulated gradient itself: the
+// accumulate-by-atomics buffer starts each epoch at -0.0 in every component.
+// Every contribution an edge adds is a sum that started at +0.0, and a sum of
+// such values is never -0.0 (in round-to-nearest, x + y = -0 only for
+// x = y = -0), so one atomic add of any contribution, even an exact zero,
+// turns the component into something other than -0.0.  A node is flagged when
+// any component differs from -0.0.  (Only a contribution that is a negative
+// denormal in every component and flushed by the atomic unit could leave all
+// markers standing; contributions are differences of coordinates times
+// bounded factors, far from the denormal range.)  This replaces a per-node
+// flag array, whose store cost one more scattered request per firing edge.
+constexpr uint32_t kNoUpdateBits = 0x80000000u;  // -0.0f
+
+template <int DIM>
+__device__ inline bool has_update_mark(const float* g) {
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) any |= __float_as_uint(g[d]) != kNoUpdateBits;
+    return any;
+}
+
+// Repulsive term of one negative sample k of node y (umap_grad_accum's
+// negative loop, umap_optimiser_gpu.rs:806-849): skipped for k = the node.
+template <int DIM>
+__device__ inline void repulse(const float (&y)[DIM], const float (&yk)[DIM], bool self, float a,
+                               float b, float two_gamma_b, float (&g)[DIM]) {
+    if (self) return;
+    const float one = 1.0f;
+    float dsk = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        float diff = y[d] - yk[d];
+        dsk += diff * diff;
+    }
+    float safe = dsk + kGradRepEps;
+    float gc = clampf(two_gamma_b / (safe * (one + a * pos_powf(safe, b))), kGradClipVal);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) g[d] += gc * (y[d] - yk[d]);
+}
+
+// Negatives of one endpoint (node, CSR slot loc), added to g in s order.
+// Their rows are gathered kNegBatch at a time, all in flight together.
+constexpr int kNegBatch = 8;
+template <int DIM>
+__device__ inline void negatives(uint32_t node, uint32_t loc, const float (&y)[DIM],
+                                 const float* __restrict__ embd, uint32_t epoch, uint32_t seed,
+                                 FastMod fm, float a, float b, float two_gamma_b, int neg_rate,
+                                 float (&g)[DIM]) {
+    for (int s0 = 0; s0 < neg_rate; s0 += kNegBatch) {
+        uint32_t k[kNegBatch];
+        float yk[kNegBatch][DIM];
+#pragma unroll
+        for (int q = 0; q < kNegBatch; q++)
+            if (s0 + q < neg_rate) {
+                k[q] = hash_neg(seed, node, epoch, loc, (uint32_t)(s0 + q), fm);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) yk[q][d] = embd[(size_t)k[q] * DIM + d];
+            }
+#pragma unroll
+        for (int q = 0; q < kNegBatch; q++)
+            if (s0 + q < neg_rate) repulse<DIM>(y, yk[q], k[q] == node, a, b, two_gamma_b, g);
+    }
+}
+
 // The work of one firing undirected edge (i < j) of one Adam epoch: gathers
-// y_i, y_j and 2*neg_rate negatives, scatters the larger endpoint's gradient
-// with FP32 atomics and the smaller endpoint's after a segmented sum over the
-// wavefront (edges arrive sorted by i).  Every lane of the wavefront calls it
-// (the reduction is wavefront-wide); lanes with active == false only join the
-// reduction.  DIM is the compile-time embedding dimension.
+// y_i, y_j, computes the attractive (and densMAP) term once for both
+// endpoints, then j's negatives, whose sum goes to grad[j] by FP32 atomics
+// at once (while y_j's line is still in L2), then i's negatives; i's total
+// goes out after a segmented sum over the wavefront (edges arrive sorted by
+// i).  Every lane of the wavefront calls it (the reduction is
+// wavefront-wide); lanes with active == false only join the reduction.  DIM
+// is the compile-time embedding dimension.
 template <int DIM, bool DENS>
 __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* __restrict__ edges,
-    const float* __restrict__ embd, float* __restrict__ grad, uint32_t* __restrict__ flag,
-    uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
-    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
+    const float* __restrict__ embd, float* __restrict__ grad, uint32_t epoch, uint32_t seed,
+    FastMod fm, float a, float b, float two_a_b, float two_gamma_b, int neg_rate,
+    const float* __restrict__ mu, const float4* __restrict__ dens_pack,
+    const float* __restrict__ dens_scalars) {
     const float two = 2.0f, one = 1.0f;
     uint32_t key = 0xFFFFFFFFu;
-    float gi[DIM], gj[DIM];
+    float gi[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) gi[d] = gj[d] = 0.0f;
-    uint32_t j = 0;
+    for (int d = 0; d < DIM; d++) gi[d] = 0.0f;
     if (active) {
         const uint4 rec = edges[e];  // {i, j, CSR slot under i, under j}
-        const uint32_t i = rec.x;
-        j = rec.y;
+        const uint32_t i = rec.x, j = rec.y;
         key = i;
-        const uint32_t li = rec.z, lj = rec.w;
-        float yi[DIM], yj[DIM];
+        float yi[DIM], yj[DIM], gj[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             yi[d] = embd[(size_t)i * DIM + d];
             yj[d] = embd[(size_t)j * DIM + d];
+            gj[d] = 0.0f;
         }
         float dist_sq = 0.0f;
 #pragma unroll
@@ -115,87 +179,78 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* 
                 }
             }
         }
-        // negatives of both endpoints: gather all rows first, then consume
-        for (int s = 0; s < neg_rate; s++) {
-            uint32_t ki = hash_neg(seed, i, epoch, li, (uint32_t)s, fm);
-            uint32_t kj = hash_neg(seed, j, epoch, lj, (uint32_t)s, fm);
-            float yki[DIM], ykj[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                yki[d] = embd[(size_t)ki * DIM + d];
-                ykj[d] = embd[(size_t)kj * DIM + d];
-            }
-            if (ki != i) {
-                float dsk = 0.0f;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    float diff = yi[d] - yki[d];
-                    dsk += diff * diff;
-                }
-                float safe = dsk + kGradRepEps;
-                float gc = clampf(two_gamma_b / (safe * (one + a * pos_powf(safe, b))), kGradClipVal);
-#pragma unroll
-                for (int d = 0; d < DIM; d++) gi[d] += gc * (yi[d] - yki[d]);
-            }
-            if (kj != j) {
-                float dsk = 0.0f;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    float diff = yj[d] - ykj[d];
-                    dsk += diff * diff;
-                }
-                float safe = dsk + kGradRepEps;
-                float gc = clampf(two_gamma_b / (safe * (one + a * pos_powf(safe, b))), kGradClipVal);
-#pragma unroll
-                for (int d = 0; d < DIM; d++) gj[d] += gc * (yj[d] - ykj[d]);
-            }
-        }
+        negatives<DIM>(j, rec.w, yj, embd, epoch, seed, fm, a, b, two_gamma_b, neg_rate, gj);
 #pragma unroll
         for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)j * DIM + d], gj[d]);
-        flag[j] = 1u;
+        negatives<DIM>(i, rec.z, yi, embd, epoch, seed, fm, a, b, two_gamma_b, neg_rate, gi);
     }
     // smaller endpoint: combine runs of equal i within the wavefront first
     const bool head = seg_reduce<DIM>(key, gi);
     if (head && key != 0xFFFFFFFFu) {
 #pragma unroll
         for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)key * DIM + d], gi[d]);
-        flag[key] = 1u;
     }
 }
 
-// One undirected edge of one Adam epoch, for thread-per-edge kernels: e is
-// this thread's edge (e >= n_edges: an idle lane that still joins the
-// wavefront's reduction).  Reads next/per coalesced and advances a firing
-// edge's schedule cursor in place, then does the edge's work.
-template <int DIM, bool DENS>
-__device__ __forceinline__ void edge_epoch(uint32_t e, const uint4* __restrict__ edges,
-    const float* __restrict__ per, float* __restrict__ next, const float* __restrict__ embd,
-    float* __restrict__ grad, uint32_t* __restrict__ flag, uint32_t n_edges, uint32_t epoch,
+// One wavefront's span of SPAN consecutive edges [base, base + SPAN) of one
+// Adam epoch.  About one edge in five fires in an epoch, scattered over the
+// edge list, so with one lane per edge nearly every wavefront would have a
+// firing lane and run the whole negative-sampling body for all 64 lanes.
+// Here the wavefront first advances the span's schedule cursors (coalesced)
+// and queues the firing edges in its LDS slice `queue`, in edge order (runs of
+// equal i stay together for the segmented sum), then works through the queue
+// 64 edges at a time.  The queue is built from ballots: no counter is shared
+// between wavefronts.  queue is this wavefront's LDS slice (SPAN entries).
+// Every wavefront of the workgroup must call this (it holds a workgroup
+// barrier between building and reading the queues).
+template <int DIM, bool DENS, int SPAN>
+__device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restrict__ queue,
+    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
+    const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
-    bool active = false;
-    if (e < n_edges) {
-        float nx = next[e];
-        if (nx <= epoch_f) {
-            active = true;
-            next[e] = nx + per[e];
+    static_assert(SPAN % 64 == 0, "whole wavefront rounds");
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t count = 0;  // wavefront-uniform
+    for (int r = 0; r < SPAN / 64; r++) {
+        const uint64_t e = base + (uint64_t)r * 64 + lane;
+        bool fire = false;
+        if (e < n_edges) {
+            const float nx = next[e];
+            if (nx <= epoch_f) {
+                fire = true;
+                next[e] = nx + per[e];
+            }
         }
+        const uint64_t m = __ballot(fire);
+        if (fire) queue[count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
+        count += (uint32_t)__popcll(m);
     }
-    edge_work<DIM, DENS>(active, e, edges, embd, grad, flag, epoch, seed, fm, a, b, two_a_b,
-                         two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+    __syncthreads();  // the queue is read by other lanes than the ones that wrote it
+    for (uint32_t q0 = 0; q0 < count; q0 += 64) {
+        const uint32_t slot = q0 + lane;
+        const bool active = slot < count;
+        const uint32_t e = active ? queue[slot] : 0u;
+        edge_work<DIM, DENS>(active, e, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
+                             two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+    }
 }
 
-// Adam step of one node if flagged (umap_adam_update, umap_optimiser_gpu.rs:887-923),
-// then clear its grad and flag for the next accumulate-by-atomics epoch.
+// Adam step of one node if it got an update this epoch (umap_adam_update,
+// umap_optimiser_gpu.rs:887-923), then its gradient back to the -0.0 marker
+// for the next accumulate-by-atomics epoch.
 template <int DIM>
 __device__ __forceinline__ void adam_clear_node(uint32_t node, float* __restrict__ grad,
-                                                uint32_t* __restrict__ flag,
                                                 float* __restrict__ m, float* __restrict__ v,
                                                 float* __restrict__ embd, float fused_lr,
                                                 float epsc, float omb1, float omb2,
                                                 float* __restrict__ norm_partial) {
-    if (flag[node] == 0u) {
+    float g[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) g[d] = grad[(size_t)node * DIM + d];
+    if (!has_update_mark<DIM>(g)) {
         if (norm_partial) norm_partial[node] = 0.0f;
         return;
     }
@@ -203,18 +258,16 @@ __device__ __forceinline__ void adam_clear_node(uint32_t node, float* __restrict
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const size_t q = (size_t)node * DIM + d;
-        float g = grad[q];
-        nrm += g * g;
+        nrm += g[d] * g[d];
         float mo = m[q];
-        float mn = mo + omb1 * (g - mo);
+        float mn = mo + omb1 * (g[d] - mo);
         m[q] = mn;
         float vo = v[q];
-        float vn = vo + omb2 * (g * g - vo);
+        float vn = vo + omb2 * (g[d] * g[d] - vo);
         v[q] = vn;
         embd[q] += fused_lr * mn / (sqrtf(vn) + epsc);
-        grad[q] = 0.0f;
+        grad[q] = -0.0f;
     }
-    flag[node] = 0u;
     if (norm_partial) norm_partial[node] = nrm;
 }
 

@@ -339,6 +339,7 @@ int mrs_launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed,
                                    hipMemcpyDefault, st->ctx->stream));
         }
         mrs::launch_epoch_edges(st, epoch, seed, dens);
+        mrs::derive_update_flags(st);  // has_update as the reference's kernel leaves it
         MRS_HIP(hipStreamSynchronize(st->ctx->stream));
     });
 }
@@ -346,8 +347,11 @@ int mrs_launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed,
 int mrs_launch_adam_clear(mrs_umap_state* st, int32_t epoch) {
     return mrs::guarded([&] {
         if (!st) throw Error(MRS_INVALID_ARGUMENT, "state is NULL");
+        // flagged = has_update; gradients a caller wrote carry no markers yet
+        if (!st->grad_clean) mrs::marks_from_flags(st);
         mrs::launch_adam_clear(st, epoch, nullptr);
         st->grad_clean = true;
+        MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), st->ctx->stream));
         MRS_HIP(hipStreamSynchronize(st->ctx->stream));
     });
 }
@@ -395,7 +399,7 @@ int mrs_umap_state_reset(mrs_umap_state* st, const float* embd) {
         MRS_HIP(hipMemsetAsync(st->grad.p, 0, st->grad.bytes(), s));
         MRS_HIP(hipMemsetAsync(st->has_update.p, 0, st->has_update.bytes(), s));
         MRS_HIP(hipMemcpyAsync(st->next.p, st->per.p, st->per.bytes(), hipMemcpyDeviceToDevice, s));
-        st->grad_clean = true;
+        st->grad_clean = false;
         MRS_HIP(hipStreamSynchronize(s));
     });
 }

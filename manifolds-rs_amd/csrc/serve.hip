@@ -62,7 +62,6 @@ struct FitDesc {
     float* next;
     float* embd;
     float* grad;
-    uint32_t* flag;
     float* m;
     float* v;
     uint32_t n_edges, n;
@@ -89,14 +88,21 @@ __device__ inline int fit_of_block(const FitDesc* __restrict__ fits, int n_fits,
     return lo;
 }
 
+// One epoch of every fit of the launch: a block covers kWg consecutive edges of
+// one fit, as four wavefronts of 64-edge spans with the firing edges compacted
+// (epoch_edge_span, the single-fit kernel's code).
 template <int DIM>
 __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restrict__ fits,
                                                          int n_fits) {
+    constexpr int kWaves = kWg / 64;
+    __shared__ uint32_t queue[kWaves][64];
     const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x, true)];
-    const uint32_t e = (blockIdx.x - f.blk_edge0) * kWg + threadIdx.x;
-    edge_epoch<DIM, false>(e, f.edges, f.per, f.next, f.embd, f.grad, f.flag,
-                           f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b, f.two_a_b,
-                           f.two_gamma_b, f.neg_rate, nullptr, nullptr, nullptr);
+    const int wave = threadIdx.x >> 6;
+    const uint64_t base = ((uint64_t)(blockIdx.x - f.blk_edge0) * kWaves + wave) * 64;
+    epoch_edge_span<DIM, false, 64>(base, queue[wave], f.edges, f.per, f.next, f.embd, f.grad,
+                                    f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b,
+                                    f.two_a_b, f.two_gamma_b, f.neg_rate, nullptr, nullptr,
+                                    nullptr);
 }
 
 template <int DIM>
@@ -105,8 +111,8 @@ __global__ __launch_bounds__(kWg) void k_seg_adam_clear(const FitDesc* __restric
     const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x, false)];
     const uint32_t node = (blockIdx.x - f.blk_node0) * kWg + threadIdx.x;
     if (node >= f.n) return;
-    adam_clear_node<DIM>(node, f.grad, f.flag, f.m, f.v, f.embd, f.fused_lr, f.epsc, f.omb1,
-                         f.omb2, nullptr);
+    adam_clear_node<DIM>(node, f.grad, f.m, f.v, f.embd, f.fused_lr, f.epsc, f.omb1, f.omb2,
+                         nullptr);
 }
 
 double now_ms() {
@@ -272,7 +278,6 @@ void fill_desc(FitDesc& f, const mrs_umap_state* st, int32_t epoch, uint32_t see
     f.next = st->next.p;
     f.embd = st->embd.p;
     f.grad = st->grad.p;
-    f.flag = st->has_update.p;
     f.m = st->m.p;
     f.v = st->v.p;
     f.n_edges = (uint32_t)st->n_edges;
@@ -333,9 +338,11 @@ void batcher_loop(mrs_server* sv) {
             for (int dim : {1, 2, 3, 4, 8}) {
                 Group g{dim, nd, 0, 0, 0};
                 for (mrs_server_request* r : active) {
-                    const mrs_umap_state* st = r->st.get();
+                    mrs_umap_state* st = r->st.get();
                     if (r->params.epoch_strategy != MRS_EPOCH_EDGE_ATOMIC || st->dim != dim)
                         continue;
+                    // a new fit's gradients to the -0.0 markers, on this stream
+                    if (!st->grad_clean) clear_update_marks(st);
                     FitDesc& f = hd[nd++];
                     fill_desc(f, st, r->epoch, (uint32_t)r->seed);
                     f.blk_edge0 = g.blocks_e;

@@ -25,8 +25,9 @@ struct mrs_umap_state {
     mrs::DevBuf<uint4> edges;        // [E]
     mrs::DevBuf<float> per, next, mu;  // [E]
 
-    // grad/has_update are all-zero (ready for the accumulate-by-atomics kernel)
-    bool grad_clean = true;
+    // grad holds the edge strategy's cleared state: -0.0 everywhere, "no update
+    // yet" (epoch_device.hpp); has_update is kept by the node-exact strategy
+    bool grad_clean = false;
 
     // densMAP
     bool dens_enabled = false;
@@ -103,6 +104,10 @@ void launch_schedule(mrs_umap_state* st, int32_t epoch);
 // summed per node in the reference's sequential order (else 16 lanes per node)
 void launch_dens_update_dev(mrs_umap_state* st, bool reference_order);
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
+// grad to -0.0 (no update yet), has_update from / to the markers
+void clear_update_marks(mrs_umap_state* st);
+void derive_update_flags(mrs_umap_state* st);
+void marks_from_flags(mrs_umap_state* st);
 void launch_adam_clear(mrs_umap_state* st, int32_t epoch, float* norm_partial);
 void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed, int strategy);
 // Wait for and accumulate the events recorded while profiling.

@@ -487,7 +487,7 @@ def _random_sparse_graph(n, per_node, seed, w_lo=0.05):
 
 
 def test_edge_kernel_wide_span_teacher_forced(ctx):
-    # Graphs of 2^20 or more edges take the 256-edge-span instantiation of the
+    # Graphs of 2^20 or more edges take the 128-edge-span instantiation of the
     # edge kernel (every headline fit does); below that the 64-edge span runs.
     # Eight epochs on a 1.1M-edge graph, each replayed on the oracle from the
     # GPU's state: schedule exact, gradients and Adam within the per-node bar.
