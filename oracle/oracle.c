@@ -1150,3 +1150,413 @@ int orc_optimise_sgd(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_o
     free(per); free(next); free(per_neg); free(next_neg); free(src);
     return 0;
 }
+
+/* ------------------------------------------------------------ spectral init */
+
+/* Counter-based stand-ins for the reference's StdRng draws: its streams
+ * (rand 0.9.5) cannot be reproduced here, their distributions can.  stream
+ * separates the uses (start vector, noise, random fill). */
+static uint64_t smix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double u01(uint64_t seed, uint64_t stream, uint64_t i) {
+    return (double)(smix(smix(seed * 0x100000001B3ull + stream) + i) >> 11) *
+           (1.0 / 9007199254740992.0);
+}
+static double std_normal(uint64_t seed, uint64_t stream, uint64_t i) { /* Box-Muller */
+    double u = u01(seed, stream, 2 * i), v = u01(seed, stream, 2 * i + 1);
+    if (u < 1e-300) u = 1e-300;
+    return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+}
+
+/* Symmetric eigenproblem of a small dense matrix (the tridiagonal T of
+ * Lanczos, src/utils/math.rs:366-388, which the reference hands to faer's
+ * self_adjoint_eigen): cyclic Jacobi rotations.  A [m][m] is destroyed;
+ * eigenvalues ascending, eigenvectors as the columns of V [m][m]. */
+static void jacobi_eigen(double *A, int m, double *evals, double *V) {
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) V[i * m + j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) off += A[p * m + q] * A[p * m + q];
+        if (off < 1e-30) break;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                double apq = A[p * m + q];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; k++) { /* columns p, q */
+                    double akp = A[k * m + p], akq = A[k * m + q];
+                    A[k * m + p] = c * akp - s * akq;
+                    A[k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; k++) { /* rows p, q */
+                    double apk = A[p * m + k], aqk = A[q * m + k];
+                    A[p * m + k] = c * apk - s * aqk;
+                    A[q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; k++) {
+                    double vkp = V[k * m + p], vkq = V[k * m + q];
+                    V[k * m + p] = c * vkp - s * vkq;
+                    V[k * m + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    /* sort ascending (selection sort keeps equal values in index order) */
+    int *ord = (int *)malloc(sizeof(int) * (size_t)m);
+    for (int i = 0; i < m; i++) ord[i] = i;
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++)
+            if (A[ord[j] * m + ord[j]] < A[ord[i] * m + ord[i]]) {
+                int t = ord[i];
+                ord[i] = ord[j];
+                ord[j] = t;
+            }
+    double *W = (double *)malloc(sizeof(double) * (size_t)m * m);
+    memcpy(W, V, sizeof(double) * (size_t)m * m);
+    for (int j = 0; j < m; j++) {
+        evals[j] = A[ord[j] * m + ord[j]];
+        for (int i = 0; i < m; i++) V[i * m + j] = W[i * m + ord[j]];
+    }
+    free(W);
+    free(ord);
+}
+
+static double dotp(const double *a, const double *b, int64_t n) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static) if (n > 65536)
+    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+/* lanczos_eigendecomp + select_eigenpairs (smallest), src/utils/math.rs:401-627,
+ * on the CSR matrix M (f64): min(4 n_eigs + 30, n) iterations with full
+ * modified Gram-Schmidt, the tridiagonal problem, Ritz vectors of the n_eigs
+ * smallest Ritz values normalised and cast to f32 into evecs [n][n_eigs].
+ * Returns how many pairs were produced (fewer on an early breakdown). */
+static int lanczos_smallest(int64_t n, const int64_t *ip, const int32_t *ix, const double *dv,
+                            int n_eigs, uint64_t seed, float *evecs) {
+    int n_iter = n_eigs * 4 + 30;
+    if ((int64_t)n_iter > n) n_iter = (int)n;
+    double *v = (double *)malloc(sizeof(double) * (size_t)n);
+    double *v_old = (double *)calloc((size_t)n, sizeof(double));
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    double *basis = (double *)malloc(sizeof(double) * (size_t)n * n_iter);
+    double *alpha = (double *)calloc((size_t)n_iter, sizeof(double));
+    double *beta = (double *)calloc((size_t)n_iter, sizeof(double));
+    for (int64_t i = 0; i < n; i++) v[i] = u01(seed, 1, (uint64_t)i) - 0.5;
+    double nv = sqrt(dotp(v, v, n));
+    for (int64_t i = 0; i < n; i++) v[i] /= nv;
+    int n_done = 0;
+    for (int j = 0; j < n_iter; j++) {
+        memcpy(basis + (size_t)j * n, v, sizeof(double) * (size_t)n);
+        n_done = j + 1;
+#pragma omp parallel for schedule(static) if (n > 65536)
+        for (int64_t i = 0; i < n; i++) {
+            double acc = 0.0;
+            for (int64_t q = ip[i]; q < ip[i + 1]; q++) acc += dv[q] * v[ix[q]];
+            w[i] = acc;
+        }
+        alpha[j] = dotp(w, v, n);
+        const double a = alpha[j], bprev = j > 0 ? beta[j - 1] : 0.0;
+#pragma omp parallel for schedule(static) if (n > 65536)
+        for (int64_t i = 0; i < n; i++) {
+            w[i] -= a * v[i];
+            if (j > 0) w[i] -= bprev * v_old[i];
+        }
+        for (int k = 0; k <= j; k++) { /* modified Gram-Schmidt against every basis vector */
+            const double *bk = basis + (size_t)k * n;
+            const double coeff = dotp(w, bk, n);
+#pragma omp parallel for schedule(static) if (n > 65536)
+            for (int64_t i = 0; i < n; i++) w[i] -= coeff * bk[i];
+        }
+        beta[j] = sqrt(dotp(w, w, n));
+        if (beta[j] < 1e-12) break;
+        memcpy(v_old, v, sizeof(double) * (size_t)n);
+        for (int64_t i = 0; i < n; i++) v[i] = w[i] / beta[j];
+    }
+    const int m = n_done;
+    double *T = (double *)calloc((size_t)m * m, sizeof(double));
+    double *ev = (double *)malloc(sizeof(double) * (size_t)m);
+    double *V = (double *)malloc(sizeof(double) * (size_t)m * m);
+    for (int i = 0; i < m; i++) {
+        T[i * m + i] = alpha[i];
+        if (i + 1 < m) T[i * m + i + 1] = T[(i + 1) * m + i] = beta[i];
+    }
+    jacobi_eigen(T, m, ev, V);
+    const int n_sel = n_eigs < m ? n_eigs : m;
+    double *e = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int s = 0; s < n_sel; s++) {
+        for (int64_t i = 0; i < n; i++) e[i] = 0.0;
+        for (int j = 0; j < m; j++) { /* j outer: every element accumulates in ascending j */
+            const double cj = V[j * m + s];
+            const double *bj = basis + (size_t)j * n;
+            for (int64_t i = 0; i < n; i++) e[i] += bj[i] * cj;
+        }
+        double ne = 0.0;
+        for (int64_t i = 0; i < n; i++) ne += e[i] * e[i];
+        ne = sqrt(ne);
+        for (int64_t i = 0; i < n; i++) evecs[i * n_eigs + s] = (float)(ne > 0.0 ? e[i] / ne : e[i]);
+    }
+    free(v); free(v_old); free(w); free(basis); free(alpha); free(beta);
+    free(T); free(ev); free(V); free(e);
+    return n_sel;
+}
+
+/* graph_to_negative_normalised_adjacency, src/data/init.rs:210-256: degrees
+ * summed in COO order, M = -D^-1/2 A D^-1/2 in CSR (columns ascending), the
+ * diagonal left out */
+static void neg_norm_adj(int64_t n, const int32_t *r, const int32_t *c, const float *v,
+                         int64_t nnz, int64_t **ip_out, int32_t **ix_out, double **dv_out) {
+    double *deg = (double *)calloc((size_t)n, sizeof(double));
+    for (int64_t e = 0; e < nnz; e++) deg[r[e]] += (double)v[e];
+    double *dis = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i < n; i++) dis[i] = deg[i] > 1e-8 ? 1.0 / sqrt(deg[i]) : 0.0;
+    /* CSR sorted by (row, col) */
+    int64_t *ip = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < nnz; e++) ip[r[e] + 1]++;
+    for (int64_t i = 0; i < n; i++) ip[i + 1] += ip[i];
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    memcpy(cur, ip, sizeof(int64_t) * (size_t)n);
+    int32_t *cc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    double *vv = (double *)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    for (int64_t e = 0; e < nnz; e++) {
+        int64_t q = cur[r[e]]++;
+        cc[q] = c[e];
+        vv[q] = (double)v[e];
+    }
+    for (int64_t i = 0; i < n; i++) /* insertion sort of each row by column */
+        for (int64_t a = ip[i] + 1; a < ip[i + 1]; a++)
+            for (int64_t b = a; b > ip[i] && cc[b - 1] > cc[b]; b--) {
+                int32_t tc = cc[b]; cc[b] = cc[b - 1]; cc[b - 1] = tc;
+                double tv = vv[b]; vv[b] = vv[b - 1]; vv[b - 1] = tv;
+            }
+    int64_t *op = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    int32_t *oi = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    double *od = (double *)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    int64_t w = 0;
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t q = ip[i]; q < ip[i + 1]; q++)
+            if (cc[q] != i) {
+                oi[w] = cc[q];
+                od[w] = -dis[i] * vv[q] * dis[cc[q]];
+                w++;
+            }
+        op[i + 1] = w;
+    }
+    free(deg); free(dis); free(ip); free(cur); free(cc); free(vv);
+    *ip_out = op;
+    *ix_out = oi;
+    *dv_out = od;
+}
+
+/* single_component_spectral_raw, src/data/init.rs:277-317: eigenvectors
+ * 1..n_comp of M (random in [-1, 1) for a graph of <= n_comp + 1 vertices, or
+ * for a dimension Lanczos did not deliver) into raw [n][n_comp] */
+static void spectral_raw(int64_t n, const int32_t *r, const int32_t *c, const float *v,
+                         int64_t nnz, int n_comp, uint64_t seed, float *raw) {
+    if (n <= n_comp + 1) {
+        for (int64_t i = 0; i < n * n_comp; i++) raw[i] = (float)(2.0 * u01(seed, 2, (uint64_t)i) - 1.0);
+        return;
+    }
+    int64_t *ip;
+    int32_t *ix;
+    double *dv;
+    neg_norm_adj(n, r, c, v, nnz, &ip, &ix, &dv);
+    const int n_eigs = n_comp + 1 < n ? n_comp + 1 : (int)n;
+    float *ev = (float *)malloc(sizeof(float) * (size_t)n * n_eigs);
+    const int got = lanczos_smallest(n, ip, ix, dv, n_eigs, seed, ev);
+    for (int d = 0; d < n_comp; d++)
+        for (int64_t i = 0; i < n; i++)
+            raw[i * n_comp + d] = d + 1 < got ? ev[i * n_eigs + d + 1]
+                                              : (float)(2.0 * u01(seed + (uint64_t)d, 3, (uint64_t)i) - 1.0);
+    free(ev); free(ip); free(ix); free(dv);
+}
+
+static float max_abs_f(const float *x, int64_t m) {
+    float mx = 0.0f;
+    for (int64_t i = 0; i < m; i++) {
+        float a = fabsf(x[i]);
+        if (a > mx) mx = a;
+    }
+    return mx;
+}
+
+/* centre each column (f32 sums in row order), scale max |x| to range */
+static void centre_scale(float *emb, int64_t n, int n_comp, float range) {
+    const float nt = (float)n;
+    for (int d = 0; d < n_comp; d++) {
+        float s = 0.0f;
+        for (int64_t i = 0; i < n; i++) s += emb[i * n_comp + d];
+        const float mean = s / nt;
+        for (int64_t i = 0; i < n; i++) emb[i * n_comp + d] -= mean;
+    }
+    const float mx = max_abs_f(emb, n * n_comp);
+    if (mx > 1e-8f) {
+        const float sc = range / mx;
+        for (int64_t i = 0; i < n * n_comp; i++) emb[i] *= sc;
+    }
+}
+
+/* spectral_layout, src/data/init.rs:986-1004, with find_connected_components
+ * (:330-391, BFS in ascending start vertex over the COO's rows),
+ * single_component_spectral + finalise_spectral_embedding (:877-957) and
+ * multi_component_init / component_meta_layout / meta_data_range
+ * (:413-785).  graph: a symmetric COO of n vertices; data [n][d] optional
+ * (NULL: simplex meta layout).  out [n][n_comp].  The reference's StdRng
+ * draws (start vector, noise, random fills) are replaced by counter-based
+ * draws of the same distributions. */
+int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
+                        int64_t nnz, int32_t n_comp, uint64_t seed, float range, const float *data,
+                        int32_t d, float *out) {
+    if (n <= 0) return 1;
+    /* adjacency in COO order per row, then BFS */
+    int64_t *ip = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < nnz; e++) ip[rows[e] + 1]++;
+    for (int64_t i = 0; i < n; i++) ip[i + 1] += ip[i];
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    memcpy(cur, ip, sizeof(int64_t) * (size_t)n);
+    int32_t *adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    for (int64_t e = 0; e < nnz; e++) adj[cur[rows[e]]++] = cols[e];
+    int64_t *label = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *local = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *order = (int64_t *)malloc(sizeof(int64_t) * (size_t)n); /* members, component by component */
+    int64_t *cstart = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    for (int64_t i = 0; i < n; i++) label[i] = -1;
+    int64_t nc = 0, filled = 0;
+    for (int64_t s = 0; s < n; s++) {
+        if (label[s] >= 0) continue;
+        cstart[nc] = filled;
+        int64_t head = filled;
+        order[filled++] = s;
+        label[s] = nc;
+        while (head < filled) {
+            int64_t node = order[head];
+            local[node] = head - cstart[nc];
+            head++;
+            for (int64_t q = ip[node]; q < ip[node + 1]; q++)
+                if (label[adj[q]] < 0) {
+                    label[adj[q]] = nc;
+                    order[filled++] = adj[q];
+                }
+        }
+        nc++;
+    }
+    cstart[nc] = filled;
+    if (nc == 1) {
+        spectral_raw(n, rows, cols, vals, nnz, n_comp, seed, out);
+        centre_scale(out, n, n_comp, range);
+        for (int64_t i = 0; i < n * n_comp; i++)
+            out[i] += (float)std_normal(seed + 9999, 4, (uint64_t)i) * 1e-4f;
+    } else {
+        /* component_meta_layout */
+        float *meta = (float *)calloc((size_t)nc * n_comp, sizeof(float));
+        int spectral_meta = nc > 2 * n_comp && data && nc <= 2000;
+        if (nc > 2 * n_comp && data && nc > 2000) {
+            for (int64_t l = 0; l < nc * n_comp; l++) meta[l] = (float)(2.0 * u01(seed, 5, (uint64_t)l) - 1.0);
+        } else if (spectral_meta) {
+            float *cen = (float *)calloc((size_t)nc * d, sizeof(float));
+            for (int64_t l = 0; l < nc; l++) {
+                const float inv = 1.0f / (float)(cstart[l + 1] - cstart[l]);
+                for (int64_t q = cstart[l]; q < cstart[l + 1]; q++)
+                    for (int f = 0; f < d; f++) cen[l * d + f] += data[order[q] * d + f];
+                for (int f = 0; f < d; f++) cen[l * d + f] *= inv;
+            }
+            float *d2 = (float *)calloc((size_t)nc * nc, sizeof(float));
+            float maxd2 = 0.0f;
+            for (int64_t a = 0; a < nc; a++)
+                for (int64_t b = a + 1; b < nc; b++) {
+                    float s = 0.0f;
+                    for (int f = 0; f < d; f++) {
+                        float df = cen[a * d + f] - cen[b * d + f];
+                        s += df * df;
+                    }
+                    d2[a * nc + b] = d2[b * nc + a] = s;
+                    if (s > maxd2) maxd2 = s;
+                }
+            const float scale = maxd2 > 1e-12f ? 1.0f / maxd2 : 1.0f;
+            int64_t m = nc * (nc - 1);
+            int32_t *ar = (int32_t *)malloc(sizeof(int32_t) * (size_t)m);
+            int32_t *ac = (int32_t *)malloc(sizeof(int32_t) * (size_t)m);
+            float *av = (float *)malloc(sizeof(float) * (size_t)m);
+            int64_t q = 0;
+            for (int64_t a = 0; a < nc; a++)
+                for (int64_t b = 0; b < nc; b++) {
+                    if (a == b) continue;
+                    ar[q] = (int32_t)a;
+                    ac[q] = (int32_t)b;
+                    av[q] = (float)exp(-(double)(d2[a * nc + b] * scale));
+                    q++;
+                }
+            spectral_raw(nc, ar, ac, av, m, n_comp, seed, meta);
+            free(cen); free(d2); free(ar); free(ac); free(av);
+        }
+        float mx = max_abs_f(meta, nc * n_comp);
+        if (!(nc > 2 * n_comp && data) || mx <= 1e-8f) { /* simplex */
+            const int64_t k = (nc + 1) / 2;
+            memset(meta, 0, sizeof(float) * (size_t)nc * n_comp);
+            for (int64_t l = 0; l < nc; l++) {
+                const int64_t idx = l < k ? l : l - k;
+                if (idx < n_comp) meta[l * n_comp + idx] = l < k ? 1.0f : -1.0f;
+            }
+            mx = max_abs_f(meta, nc * n_comp);
+        }
+        if (mx > 1e-8f)
+            for (int64_t l = 0; l < nc * n_comp; l++) meta[l] /= mx;
+        /* each component around its centre */
+        uint64_t draw = 0;
+        for (int64_t l = 0; l < nc; l++) {
+            const int64_t cn = cstart[l + 1] - cstart[l];
+            float range_l = 0.0f; /* meta_data_range: half the largest distance to another centre */
+            for (int64_t o = 0; o < nc; o++) {
+                float s = 0.0f;
+                for (int dd = 0; dd < n_comp; dd++) {
+                    float df = meta[l * n_comp + dd] - meta[o * n_comp + dd];
+                    s += df * df;
+                }
+                if (sqrtf(s) > range_l) range_l = sqrtf(s);
+            }
+            range_l /= 2.0f;
+            if (cn < 2 * n_comp) {
+                for (int64_t q = cstart[l]; q < cstart[l + 1]; q++)
+                    for (int dd = 0; dd < n_comp; dd++)
+                        out[order[q] * n_comp + dd] =
+                            meta[l * n_comp + dd] + (float)(2.0 * u01(seed, 6, draw++) - 1.0) * range_l;
+                continue;
+            }
+            /* the component's subgraph in COO order, vertices renamed to local */
+            int64_t sm = 0;
+            for (int64_t e = 0; e < nnz; e++) sm += label[rows[e]] == l && label[cols[e]] == l;
+            int32_t *sr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(sm ? sm : 1));
+            int32_t *sc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(sm ? sm : 1));
+            float *sv = (float *)malloc(sizeof(float) * (size_t)(sm ? sm : 1));
+            int64_t q = 0;
+            for (int64_t e = 0; e < nnz; e++)
+                if (label[rows[e]] == l && label[cols[e]] == l) {
+                    sr[q] = (int32_t)local[rows[e]];
+                    sc[q] = (int32_t)local[cols[e]];
+                    sv[q] = vals[e];
+                    q++;
+                }
+            float *sub = (float *)malloc(sizeof(float) * (size_t)cn * n_comp);
+            spectral_raw(cn, sr, sc, sv, sm, n_comp, seed + (uint64_t)l, sub);
+            const float smx = max_abs_f(sub, cn * n_comp);
+            const float expansion = smx > 1e-8f ? range_l / smx : 1.0f;
+            for (int64_t lq = 0; lq < cn; lq++)
+                for (int dd = 0; dd < n_comp; dd++)
+                    out[order[cstart[l] + lq] * n_comp + dd] =
+                        meta[l * n_comp + dd] + sub[lq * n_comp + dd] * expansion;
+            free(sr); free(sc); free(sv); free(sub);
+        }
+        centre_scale(out, n, n_comp, range);
+        free(meta);
+    }
+    free(ip); free(cur); free(adj); free(label); free(local); free(order); free(cstart);
+    return 0;
+}

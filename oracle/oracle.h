@@ -184,6 +184,15 @@ int orc_optimise_sgd(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_o
                      const int32_t *adj_cols, const float *adj_vals, const orc_optim_params *p,
                      uint64_t seed);
 
+/* spectral_layout, src/data/init.rs:986-1004 (components, Lanczos with
+ * modified Gram-Schmidt, src/utils/math.rs:401-627, meta layouts for
+ * disconnected graphs, centring, scaling, noise).  A symmetric COO of n
+ * vertices; data [n][d] or NULL; out [n][n_comp].  StdRng draws replaced by
+ * counter-based draws of the same distributions.  0 ok, 1 NoData. */
+int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
+                        int64_t nnz, int32_t n_comp, uint64_t seed, float range, const float *data,
+                        int32_t d, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -757,3 +757,94 @@ def test_adam_parallel_three_nodes_fractional_schedule():
         y = np.where(upd, y + step, y)
     got = orc.optimise_adam_parallel(y0.astype(np.float32), *adj, p, seed, use_lut=False)
     np.testing.assert_allclose(got, y, rtol=2e-5, atol=2e-6)
+
+
+# ------------------------------------------------------ spectral layout
+
+
+def _graph(x, k=15):
+    from helpers import umap_graph_from_data
+
+    _, _, (r, c, v), _ = umap_graph_from_data(x, k)
+    return r, c, v
+
+
+@pytest.mark.parametrize("n_dim", [2, 3])
+def test_spectral_layout_single_component_spans_eigenvectors(n_dim):
+    # one component of 40 vertices: 4 (n_dim + 1) + 30 >= 40 Lanczos steps span
+    # the space, so the layout columns are exact eigenvectors 1..n_dim, centred,
+    # scaled to max |x| = 10, plus N(0, 1e-4) noise
+    from spectral_checks import check_components, rectangle
+
+    x = rectangle(40, 4.0, 8, seed=3)
+    r, c, v = _graph(x)
+    emb = orc.spectral_layout(40, r, c, v, n_comp=n_dim, seed=5).astype(np.float64)
+    check_components(emb, r, c, v, [slice(0, 40)], n_dim)
+    assert abs(np.max(np.abs(emb)) - 10.0) < 1e-3
+    assert np.all(np.abs(emb.mean(0)) < 1e-3)
+
+
+def test_spectral_layout_simplex_meta_and_expansion():
+    # three components (<= 2 n_dim): centres on the simplex rows (+e1, +e2, -e1)
+    # up to the final centring and scaling; each component reaches half its
+    # centre's largest distance to another centre
+    from spectral_checks import check_components, rectangle
+
+    parts = [rectangle(40, 4.0, 8, seed=s) + 100.0 * s for s in (1, 2, 3)]
+    x = np.concatenate(parts)
+    r, c, v = _graph(x)
+    emb = orc.spectral_layout(120, r, c, v, seed=5).astype(np.float64)
+    centre, reach = check_components(emb, r, c, v, [slice(40 * k, 40 * k + 40) for k in range(3)], 2)
+    simplex = np.array([[1.0, 0.0], [0.0, 1.0], [-1.0, 0.0]])
+    # centre = s * (simplex - mean) for one scale s
+    s = np.linalg.lstsq((simplex - simplex.mean(0)).ravel()[:, None],
+                        (centre - centre.mean(0)).ravel(), rcond=None)[0][0]
+    np.testing.assert_allclose(centre - centre.mean(0), s * (simplex - simplex.mean(0)), atol=1e-3)
+    half = 0.5 * np.sqrt(((centre[:, None] - centre[None]) ** 2).sum(-1)).max(1)
+    np.testing.assert_allclose(reach, half, rtol=1e-3)
+
+
+def test_spectral_layout_data_centroid_meta():
+    # six components (> 2 n_dim) with data: centres from the centroid affinity's
+    # spectral layout, each component expanded to half its reach
+    from spectral_checks import check_components, check_meta_spectral, rectangle
+
+    rng = np.random.default_rng(29)
+    offsets = rng.normal(0.0, 60.0, size=(6, 8))
+    parts = [rectangle(40, 4.0, 8, seed=s) + offsets[s] for s in range(6)]
+    x = np.concatenate(parts).astype(np.float32)
+    r, c, v = _graph(x)
+    emb = orc.spectral_layout(240, r, c, v, seed=5, data=x).astype(np.float64)
+    centre, reach = check_components(emb, r, c, v, [slice(40 * k, 40 * k + 40) for k in range(6)], 2)
+    check_meta_spectral(centre, parts, 2)
+    half = 0.5 * np.sqrt(((centre[:, None] - centre[None]) ** 2).sum(-1)).max(1)
+    np.testing.assert_allclose(reach, half, rtol=1e-3)
+    # without data the same graph falls back to the simplex: only 2 n_dim = 4
+    # distinct centres, the other two at the origin of the meta layout
+    emb0 = orc.spectral_layout(240, r, c, v, seed=5).astype(np.float64)
+    c0, _ = check_components(emb0, r, c, v, [slice(40 * k, 40 * k + 40) for k in range(6)], 2)
+    simplex = np.array([[1, 0], [0, 1], [0, 0], [-1, 0], [0, -1], [0, 0]], np.float64)
+    s = np.linalg.lstsq((simplex - simplex.mean(0)).ravel()[:, None],
+                        (c0 - c0.mean(0)).ravel(), rcond=None)[0][0]
+    np.testing.assert_allclose(c0 - c0.mean(0), s * (simplex - simplex.mean(0)), atol=1e-3)
+
+
+def test_spectral_layout_small_component_is_scattered_near_its_centre():
+    # a component of 3 < 2 n_dim vertices is placed uniformly within its data
+    # range of its centre, not spectrally
+    from spectral_checks import rectangle
+
+    big = rectangle(40, 4.0, 8, seed=1)
+    r, c, v = _graph(big)
+    # three extra vertices joined to each other only
+    extra = [(40, 41), (41, 42), (40, 42)]
+    r2 = np.concatenate([r, [a for a, b in extra] + [b for a, b in extra]])
+    c2 = np.concatenate([c, [b for a, b in extra] + [a for a, b in extra]])
+    v2 = np.concatenate([v, np.full(6, 0.5, np.float32)])
+    o = np.lexsort((c2, r2))
+    emb = orc.spectral_layout(43, r2[o], c2[o], v2[o], seed=3).astype(np.float64)
+    assert np.all(np.isfinite(emb)) and abs(np.max(np.abs(emb)) - 10.0) < 1e-3
+    # the two centres (+e1, +e2 on the simplex) are sqrt(2) apart, so the small
+    # component stays within sqrt(2)/2 of its centre in meta units
+    small = emb[40:43]
+    assert np.ptp(small, axis=0).max() > 0  # scattered, not collapsed
