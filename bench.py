@@ -258,6 +258,12 @@ def bench_fit(args):
         step()
         log(f"[bench] warmup fit: {M.last_fit_timings()}")
     torch.cuda.synchronize()
+    # the timed fits copy the kNN they compute into these (one device-to-device
+    # copy of n*k entries per fit, ~0.1 ms at 1M): the recall check runs on the
+    # timed fit's own neighbour lists
+    fit_idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    fit_dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    M.fit_knn_copy_to(fit_idx, fit_dist)
     times, stages = [], []
     for _ in range(args.steps):
         torch.cuda.synchronize()
@@ -267,8 +273,18 @@ def bench_fit(args):
         times.append((time.perf_counter() - t_s) * 1e3)
         stages.append(M.last_fit_timings())
         log(f"[bench] fit {times[-1]:.1f} ms: {stages[-1]}")
+    M.fit_knn_copy_to(None, None)
     ms_per_step = float(np.mean(times))
     st = {kk: float(np.mean([s_[kk] for s_ in stages])) for kk in stages[0]}
+    # the same fit from host memory to host memory, as the reference's umap_gpu
+    # is called (data copied in, [2][n] result copied out)
+    h_times = []
+    for _ in range(max(1, min(args.steps, 3))):
+        t_s = time.perf_counter()
+        M.umap_gpu(ctx, x, p, seed=42)
+        h_times.append((time.perf_counter() - t_s) * 1e3)
+    host_ms = float(np.mean(h_times))
+    log(f"[bench] host-to-host fit {host_ms:.1f} ms (device-resident {ms_per_step:.1f} ms)")
 
     # kNN: the shortlist sweeps only the (query block, stage) pairs its bounds
     # cannot rule out; the MFMA work done is that fraction of the 2 n^2 d FLOPs
@@ -322,6 +338,10 @@ def bench_fit(args):
                     "64 candidates, spectral init, 500 Adam epochs, 2-D output",
                     n=n, d=d, k=k, n_epochs=n_epochs, n_edges=E, edge_updates=A_total),
         stages_ms=st,
+        host_to_host=dict(ms_per_fit=host_ms, fits=len(h_times),
+                          note="data [n][64] f32 from host memory, [2][n] result to host "
+                               "memory, as umap_gpu's callers pass them; value above has the "
+                               "data resident in HBM"),
         knn_fp32_vs_bf16=knn_cmp,
         knn_pruned_vs_full_sweep=knn_prune,
         edge_updates_per_s=A_total / (st["epochs_ms"] * 1e-3),
@@ -342,8 +362,10 @@ def bench_fit(args):
                         brute_force_flops=flops),
     )
     if not args.no_cpu:
-        knn_host = (idx.cpu().numpy(), dist.cpu().numpy())
+        # the last timed fit's own kNN lists
+        knn_host = (fit_idx.cpu().numpy(), fit_dist.cpu().numpy())
         res["cpu_baseline"] = cpu_baseline_fit(x, g, knn_host, p, args)
+        res["recall"] = res["cpu_baseline"]["knn_recall_vs_fp64"]
     return res
 
 
@@ -351,7 +373,10 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
     """Reference CPU path on bounded samples of the same fit: exhaustive kNN
     (FP32 brute force, OpenMP) for Q query rows against all n, scaled by n/Q;
     host graph (smooth_knn_dist, knn_to_coo, symmetrise, filter) on the full
-    kNN; parallel Adam (LUT pow, OpenMP) for S of 500 epochs, scaled."""
+    kNN; spectral init (components, Lanczos, layout) on the full graph;
+    parallel Adam (LUT pow, OpenMP) for S of 500 epochs, scaled.  Also the
+    checker: recall@k and distance error of the timed fit's kNN against an
+    FP64 brute force on R seeded rows."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
 
@@ -382,6 +407,9 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
     orc.filter_weak_edges(rr, cc, vv, p.optim_params.n_epochs)
     t_graph = time.perf_counter() - t0
     r, c, v = g.coo()
+    t0 = time.perf_counter()
+    orc.spectral_layout(n, r, c, v, n_comp=2, seed=42, data=x)
+    t_spec = time.perf_counter() - t0
     sys.path.insert(0, str(ROOT / "tests"))
     from helpers import coo_to_adjacency
 
@@ -392,12 +420,15 @@ def cpu_baseline_fit(x, g, knn_host, p, args):
     t0 = time.perf_counter()
     orc.optimise_adam_parallel(init, *adj, q2, 42, use_lut=True, n_threads=cores)
     t_adam = (time.perf_counter() - t0) * p.optim_params.n_epochs / args.cpu_epochs
-    return dict(value=t_knn + t_graph + t_adam, unit="s", cores=cores, kind="port",
+    return dict(value=t_knn + t_graph + t_spec + t_adam, unit="s", cores=cores, kind="port",
+                build="gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (oracle/Makefile)",
+                extrapolated=True,
                 knn_recall_vs_fp64=recall,
                 sample=f"kNN: {Q} of {n} query rows x all {n} (scaled x{n / Q:g}); graph: "
-                       f"full; parallel Adam: {args.cpu_epochs} of {p.optim_params.n_epochs} "
-                       f"epochs on the same graph (scaled); spectral init not included",
-                knn_s=t_knn, graph_s=t_graph, adam_s=t_adam)
+                       f"full; spectral init: full; parallel Adam: {args.cpu_epochs} of "
+                       f"{p.optim_params.n_epochs} epochs on the same graph (scaled x"
+                       f"{p.optim_params.n_epochs / args.cpu_epochs:g})",
+                knn_s=t_knn, graph_s=t_graph, spectral_s=t_spec, adam_s=t_adam)
 
 
 def fit_steps(M, ctx, xd, p, args, dens=None):
@@ -657,7 +688,35 @@ def bench_densmap(args):
     xd = torch.from_numpy(x).cuda()
     ms, st, out = fit_steps(M, ctx, xd, p, args, dens=dp)
     idx, dist = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 0)
-    rho_full, _ = spearman_of(x, out.T.cpu().numpy(), idx, dist)
+    rho_full, g_full = spearman_of(x, out.T.cpu().numpy(), idx, dist)
+    # the dominant kernel: the edge epoch (350 plain and 150 density launches),
+    # timed with HIP events on a state of the fit's graph and density state.
+    # Algorithmic bytes per launch: 16 E + A (36 + 16 r), and for a density
+    # launch another A (2 * 16 + 4) (the two endpoints' packed density factors
+    # and the edge weight)
+    r_o, ms_o, tot_o = g_full.dens_state(idx, dist)
+    ds = M.DensState(params=dp, r=r_o.ctypes.data, mu_sum=ms_o.ctypes.data, mu_tot=tot_o)
+    e0 = torch.zeros((n, 2), device="cuda")
+    sd = M.UmapGpuState(ctx, e0, params=p.optim_params, dens=ds, graph=g_full)
+    E = sd.n_edges
+    act = M.schedule_active_counts(sd.read(M.BUF_PER), n_epochs)
+    rneg = p.optim_params.neg_sample_rate
+    dens_ep = np.array([(e + 1) / n_epochs > 1.0 - dp.frac for e in range(n_epochs)])
+    alg_bytes = float(np.sum(16.0 * E + act * (36.0 + 16.0 * rneg) + dens_ep * act * 36.0))
+    sd.reset(out.T.contiguous())
+    sd.set_profiling(True)
+    sd.run_epochs(0, n_epochs, 42, M.EPOCH_EDGE_ATOMIC)
+    sd.set_profiling(False)
+    kt = sd.kernel_times()
+    edge_ms, edge_n = kt["epoch_edges"]
+    dens_gbs = alg_bytes / (edge_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", kernel="k_epoch_edges (plain and density launches)",
+                achieved=dens_gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=dens_gbs / HBM_PEAK_GBS,
+                traffic=None, avg_launch_us=1e3 * edge_ms / edge_n,
+                algorithmic_bytes_per_launch=alg_bytes / n_epochs,
+                density_update_ms_per_call=kt["dens_update"][0] / max(kt["dens_update"][1], 1))
+    log(f"[bench] densMAP epoch kernel {roof['avg_launch_us']:.1f} us/launch, "
+        f"{dens_gbs:.0f} GB/s algorithmic")
     plain = M.umap_gpu(ctx, xd, p, seed=42).T.cpu().numpy()
     rho_plain, _ = spearman_of(x, plain, idx, dist)
     log(f"[bench] Spearman(R_o, R_e) at n={n}: densMAP {rho_full:.4f}, plain UMAP {rho_plain:.4f}")
@@ -691,6 +750,7 @@ def bench_densmap(args):
                     "BF16 kNN + exact re-rank, spectral init, 500 Adam epochs",
                     n=n, d=50, k=k, n_epochs=n_epochs),
         stages_ms=st,
+        roofline=roof,
         local_radius_spearman=dict(densmap_gpu=rho_full, plain_umap_gpu=rho_plain,
                                    sample_n=ns, sample_gpu=rho_gpu_s,
                                    sample_cpu_densmap=rho_cpu_s),
@@ -820,7 +880,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fit")
     ap.add_argument("--cpu-queries", type=int, default=512)
-    ap.add_argument("--recall-rows", type=int, default=1000)
+    ap.add_argument("--recall-rows", type=int, default=10000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-epochs", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -178,6 +178,11 @@ int mrs_densmap_gpu(mrs_ctx *ctx, const float *data, int64_t n, int32_t d,
                     const int32_t *knn_idx, const float *knn_dist, int32_t knn_k,
                     const mrs_umap_params_gpu *params, const mrs_dens_params *dens,
                     uint64_t seed, int32_t verbose, float *out);
+/* Measurement hook (no counterpart in the reference): the calling thread's
+ * later fits that search their own kNN copy the lists into idx / dist, device
+ * buffers of at least n*k entries (capacity; smaller fits only), with one
+ * device-to-device copy on the context's stream.  NULL, NULL switches it off. */
+int mrs_fit_knn_copy_to(int32_t *idx, float *dist, int64_t capacity);
 /* Stage times of the calling thread's last fit (no counterpart in the reference;
  * its timings are verbose prints): out[0..7] = kNN ms, graph ms, densMAP state
  * ms, init ms, epochs ms, total ms, undirected edges, kNN shortlist-kernel ms. */

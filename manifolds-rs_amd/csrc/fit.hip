@@ -65,6 +65,15 @@ std::string lower(const char* s) {
 
 thread_local FitTimings g_last_fit;
 
+// Where the calling thread's fits copy the kNN lists they computed (device
+// buffers of the caller, capacity in entries; null: nowhere).
+struct KnnCopyOut {
+    int32_t* idx = nullptr;
+    float* dist = nullptr;
+    int64_t capacity = 0;
+};
+thread_local KnnCopyOut g_knn_out;
+
 void random_layout_dev(mrs_ctx* ctx, float* out, int64_t n, int32_t dim, uint64_t seed,
                        float range) {
     const uint64_t total = (uint64_t)n * dim;
@@ -129,6 +138,13 @@ std::unique_ptr<mrs_umap_state> fit_prepare(mrs_ctx* ctx, const float* data, int
         kidx = knn_idx_own.p;
         kdist = knn_dist_own.p;
         t.knn_shortlist_ms = kt.shortlist_ms;
+        const int64_t entries = n * (int64_t)k;
+        if (g_knn_out.idx && entries <= g_knn_out.capacity) {
+            MRS_HIP(hipMemcpyAsync(g_knn_out.idx, kidx, sizeof(int32_t) * entries,
+                                   hipMemcpyDeviceToDevice, s));
+            MRS_HIP(hipMemcpyAsync(g_knn_out.dist, kdist, sizeof(float) * entries,
+                                   hipMemcpyDeviceToDevice, s));
+        }
     }
     MRS_HIP(hipStreamSynchronize(s));
     t.knn_ms = now_ms() - t0;
@@ -297,6 +313,18 @@ int mrs_densmap_gpu(mrs_ctx* ctx, const float* data, int64_t n, int32_t d,
         mrs::StreamScope scope(ctx->stream, ctx->pool);
         mrs::umap_fit(ctx, data, n, d, knn_idx, knn_dist, knn_k, params, dens, seed, verbose,
                       out);
+    });
+}
+
+int mrs_fit_knn_copy_to(int32_t* idx, float* dist, int64_t capacity) {
+    return mrs::guarded([&] {
+        if ((idx == nullptr) != (dist == nullptr) || capacity < 0)
+            throw mrs::Error(MRS_INVALID_ARGUMENT, "idx and dist: both or neither");
+        if (idx && (!mrs::is_device_ptr(idx) || !mrs::is_device_ptr(dist)))
+            throw mrs::Error(MRS_INVALID_ARGUMENT, "the kNN copy takes device buffers");
+        mrs::g_knn_out.idx = idx;
+        mrs::g_knn_out.dist = dist;
+        mrs::g_knn_out.capacity = idx ? capacity : 0;
     });
 }
 

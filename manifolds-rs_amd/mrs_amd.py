@@ -226,6 +226,7 @@ def lib() -> C.CDLL:
         "mrs_densmap_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, i32, C.POINTER(UmapParamsGpu),
                                       C.POINTER(DensParams), u64, i32, vp]),
         "mrs_last_fit_timings": (C.c_int, [vp]),
+        "mrs_fit_knn_copy_to": (C.c_int, [vp, vp, i64]),
         "mrs_fit_bytes_estimate": (C.c_size_t, [i64, i32, i32, i32]),
         "mrs_tridiag_eigen": (C.c_int, [vp, vp, i32, vp, vp]),
         "mrs_server_create": (C.c_int, [i32, C.c_size_t, i32, i32, C.POINTER(vp)]),
@@ -735,6 +736,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def fit_knn_copy_to(idx=None, dist=None) -> None:
+    """Later fits on this thread copy the kNN lists they compute into these
+    device tensors ([n, k] each, n*k entries at most); None switches it off."""
+    cap = 0 if idx is None else int(idx.numel())
+    _check(lib().mrs_fit_knn_copy_to(_ptr(idx), _ptr(dist), cap))
 
 
 def last_fit_timings() -> dict:
