@@ -74,14 +74,14 @@ def pmc_traffic(workload: str, kernel_prefix: str):
 def request_floor(ctx, n: int, firing_per_launch: float, neg_rate: int, dim: int,
                   avg_launch_us: float):
     """The epoch kernel's bound in random requests.  Per firing edge it makes
-    y_j and 2*neg_rate negative gathers, the larger endpoint's `dim` FP32 atomics and
-    flag store, and about one request for the smaller endpoint's run-summed
-    atomics (y_i is an L2 hit: edges are sorted by i).  Each of these touches a
+    y_j and 2*neg_rate negative gathers, the larger endpoint's `dim` FP32 atomics
+    (a node with no update keeps its -0.0 markers, so there is no flag store), and
+    about one request for the smaller endpoint's run-summed atomics (y_i is an L2 hit: edges are sorted by i).  Each of these touches a
     random line of an n-row table that does not fit one XCD's L2, so the
     kernel can go no faster than that many requests at the measured rate of
     uniform random 8-byte gathers from an n-row table."""
     rate = ctx.probe_random_gather(n, 1 << 24, 10)
-    per_edge = 1 + 2 * neg_rate + dim + 1 + 1
+    per_edge = 1 + 2 * neg_rate + dim + 1
     requests = firing_per_launch * per_edge
     floor_us = requests / rate * 1e6
     return dict(probe="uniform random float2 gathers, n-row table, 2^24 per launch, "
@@ -710,9 +710,17 @@ def bench_densmap(args):
     kt = sd.kernel_times()
     edge_ms, edge_n = kt["epoch_edges"]
     dens_gbs = alg_bytes / (edge_ms * 1e-3) / 1e9
+    # counter traffic: the plain and density instantiations' per-launch HBM
+    # bytes from the profile summary, weighted by their launches in the fit
+    span = 128 if E >= (1 << 20) else 64
+    t_plain, src = pmc_traffic("densmap", f"k_epoch_edges<2, false, {span}>")
+    t_dens, _ = pmc_traffic("densmap", f"k_epoch_edges<2, true, {span}>")
+    n_dens = int(dens_ep.sum())
+    traffic = (None if t_plain is None or t_dens is None
+               else ((n_epochs - n_dens) * t_plain + n_dens * t_dens) / n_epochs)
     roof = dict(bound="hbm", kernel="k_epoch_edges (plain and density launches)",
                 achieved=dens_gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=dens_gbs / HBM_PEAK_GBS,
-                traffic=None, avg_launch_us=1e3 * edge_ms / edge_n,
+                traffic=traffic, traffic_source=src, avg_launch_us=1e3 * edge_ms / edge_n,
                 algorithmic_bytes_per_launch=alg_bytes / n_epochs,
                 density_update_ms_per_call=kt["dens_update"][0] / max(kt["dens_update"][1], 1))
     log(f"[bench] densMAP epoch kernel {roof['avg_launch_us']:.1f} us/launch, "

@@ -1359,6 +1359,27 @@ static void neg_norm_adj(int64_t n, const int32_t *r, const int32_t *c, const fl
 /* single_component_spectral_raw, src/data/init.rs:277-317: eigenvectors
  * 1..n_comp of M (random in [-1, 1) for a graph of <= n_comp + 1 vertices, or
  * for a dimension Lanczos did not deliver) into raw [n][n_comp] */
+/* The sign of an eigenvector is free (the reference's follows its StdRng start
+ * vector).  With a target `align` [n][n_comp] (test use: another
+ * implementation's layout of the same vertices), each column of raw is
+ * flipped to agree with it: centred dot product >= 0. */
+static void align_signs(float *raw, const float *align, int64_t n, int n_comp) {
+    if (!align) return;
+    for (int d = 0; d < n_comp; d++) {
+        double mr = 0.0, ma = 0.0, dot = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            mr += raw[i * n_comp + d];
+            ma += align[i * n_comp + d];
+        }
+        mr /= (double)n;
+        ma /= (double)n;
+        for (int64_t i = 0; i < n; i++)
+            dot += (raw[i * n_comp + d] - mr) * (align[i * n_comp + d] - ma);
+        if (dot < 0.0)
+            for (int64_t i = 0; i < n; i++) raw[i * n_comp + d] = -raw[i * n_comp + d];
+    }
+}
+
 static void spectral_raw(int64_t n, const int32_t *r, const int32_t *c, const float *v,
                          int64_t nnz, int n_comp, uint64_t seed, float *raw) {
     if (n <= n_comp + 1) {
@@ -1415,6 +1436,14 @@ static void centre_scale(float *emb, int64_t n, int n_comp, float range) {
 int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
                         int64_t nnz, int32_t n_comp, uint64_t seed, float range, const float *data,
                         int32_t d, float *out) {
+    return orc_spectral_layout_aligned(n, rows, cols, vals, nnz, n_comp, seed, range, data, d,
+                                       NULL, out);
+}
+
+int orc_spectral_layout_aligned(int64_t n, const int32_t *rows, const int32_t *cols,
+                                const float *vals, int64_t nnz, int32_t n_comp, uint64_t seed,
+                                float range, const float *data, int32_t d, const float *align,
+                                float *out) {
     if (n <= 0) return 1;
     /* adjacency in COO order per row, then BFS */
     int64_t *ip = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
@@ -1451,6 +1480,7 @@ int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, con
     cstart[nc] = filled;
     if (nc == 1) {
         spectral_raw(n, rows, cols, vals, nnz, n_comp, seed, out);
+        if (n > n_comp + 1) align_signs(out, align, n, n_comp);
         centre_scale(out, n, n_comp, range);
         for (int64_t i = 0; i < n * n_comp; i++)
             out[i] += (float)std_normal(seed + 9999, 4, (uint64_t)i) * 1e-4f;
@@ -1495,6 +1525,16 @@ int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, con
                     q++;
                 }
             spectral_raw(nc, ar, ac, av, m, n_comp, seed, meta);
+            if (align && nc > n_comp + 1) { /* target: the aligned layout's component centres */
+                float *tc = (float *)calloc((size_t)nc * n_comp, sizeof(float));
+                for (int64_t l = 0; l < nc; l++) {
+                    for (int64_t q2 = cstart[l]; q2 < cstart[l + 1]; q2++)
+                        for (int dd = 0; dd < n_comp; dd++) tc[l * n_comp + dd] += align[order[q2] * n_comp + dd];
+                    for (int dd = 0; dd < n_comp; dd++) tc[l * n_comp + dd] /= (float)(cstart[l + 1] - cstart[l]);
+                }
+                align_signs(meta, tc, nc, n_comp);
+                free(tc);
+            }
             free(cen); free(d2); free(ar); free(ac); free(av);
         }
         float mx = max_abs_f(meta, nc * n_comp);
@@ -1546,6 +1586,14 @@ int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, con
                 }
             float *sub = (float *)malloc(sizeof(float) * (size_t)cn * n_comp);
             spectral_raw(cn, sr, sc, sv, sm, n_comp, seed + (uint64_t)l, sub);
+            if (align && cn > n_comp + 1) {
+                float *ta = (float *)malloc(sizeof(float) * (size_t)cn * n_comp);
+                for (int64_t lq = 0; lq < cn; lq++)
+                    for (int dd = 0; dd < n_comp; dd++)
+                        ta[lq * n_comp + dd] = align[order[cstart[l] + lq] * n_comp + dd];
+                align_signs(sub, ta, cn, n_comp);
+                free(ta);
+            }
             const float smx = max_abs_f(sub, cn * n_comp);
             const float expansion = smx > 1e-8f ? range_l / smx : 1.0f;
             for (int64_t lq = 0; lq < cn; lq++)

@@ -192,6 +192,13 @@ int orc_optimise_sgd(float *embd, int64_t n, int32_t n_dim, const int64_t *adj_o
 int orc_spectral_layout(int64_t n, const int32_t *rows, const int32_t *cols, const float *vals,
                         int64_t nnz, int32_t n_comp, uint64_t seed, float range, const float *data,
                         int32_t d, float *out);
+/* The same with each eigenvector's free sign (components, meta layout) chosen
+ * to agree with align [n][n_comp], another layout of the same graph (NULL:
+ * none).  Test use: compares a layout up to those signs. */
+int orc_spectral_layout_aligned(int64_t n, const int32_t *rows, const int32_t *cols,
+                                const float *vals, int64_t nnz, int32_t n_comp, uint64_t seed,
+                                float range, const float *data, int32_t d, const float *align,
+                                float *out);
 
 #ifdef __cplusplus
 }

@@ -320,17 +320,23 @@ def optimise_sgd(embd, adj_off, adj_cols, adj_vals, params, seed):
     return out
 
 
-def spectral_layout(n, rows, cols, vals, n_comp=2, seed=42, range_=10.0, data=None):
-    """spectral_layout of a symmetric COO graph: [n, n_comp] float32."""
+def spectral_layout(n, rows, cols, vals, n_comp=2, seed=42, range_=10.0, data=None, align=None):
+    """spectral_layout of a symmetric COO graph: [n, n_comp] float32.  align
+    ([n, n_comp], optional): another layout of the graph whose eigenvector
+    signs the oracle adopts (each component's and the meta layout's)."""
     rows, cols, vals = i32(rows), i32(cols), f32(vals)
     out = np.zeros((n, n_comp), np.float32)
     d = 0
     if data is not None:
         data = f32(data)
         d = data.shape[1]
-    st = lib().orc_spectral_layout(C.c_int64(n), P(rows), P(cols), P(vals), C.c_int64(rows.size),
-                                   C.c_int32(n_comp), C.c_uint64(seed), C.c_float(range_),
-                                   P(data), C.c_int32(d), P(out))
+    if align is not None:
+        align = f32(align)
+        assert align.shape == (n, n_comp)
+    st = lib().orc_spectral_layout_aligned(C.c_int64(n), P(rows), P(cols), P(vals),
+                                           C.c_int64(rows.size), C.c_int32(n_comp),
+                                           C.c_uint64(seed), C.c_float(range_), P(data),
+                                           C.c_int32(d), P(align), P(out))
     if st:
         raise RuntimeError(f"oracle status {st}")
     return out

@@ -462,11 +462,15 @@ def test_fit_same_with_and_without_knn_pruning(ctx):
 
 def test_spectral_layout_equals_oracle_up_to_eigenvector_signs(ctx):
     # On components small enough for Lanczos to span them, the device layout
-    # and the oracle's restatement of spectral_layout agree up to the sign of
-    # each eigenvector (their Lanczos start vectors differ, as the reference's
-    # StdRng stream differs from both): one component (centred, scaled,
-    # N(0, 1e-4) noise), and six components placed by the data-centroid meta
-    # layout (each component's own signs, and the meta layout's).
+    # and the oracle's restatement of spectral_layout agree once the oracle
+    # takes the device's eigenvector signs (their Lanczos start vectors differ,
+    # as the reference's StdRng stream differs from both, and each start vector
+    # fixes the signs): one component (centred, scaled, N(0, 1e-4) noise), and
+    # six components placed by the data-centroid meta layout.  A component's
+    # centre moves with its own eigenvectors' signs (it holds expansion times
+    # their mean), so the multi-component layouts compare only with every
+    # sign aligned.  Bar: 2e-3 on coordinates scaled to max |x| = 10 (Lanczos
+    # residuals and the independent noise draws).
     import synthetic  # noqa: F401  (keeps the import order of the other tests)
     from spectral_checks import rectangle
 
@@ -474,11 +478,9 @@ def test_spectral_layout_equals_oracle_up_to_eigenvector_signs(ctx):
     idx, dist = M.knn_exhaustive(ctx, x, 15)
     g = M.FuzzyGraph(ctx, idx, dist)
     r, c, v = g.coo()
-    dev = g.spectral_layout(2, seed=5, data=x).astype(np.float64)
-    ref = orc.spectral_layout(g.n, r, c, v, n_comp=2, seed=5, data=x).astype(np.float64)
-    for dd in range(2):
-        s = np.sign(np.dot(dev[:, dd], ref[:, dd]))
-        assert np.max(np.abs(dev[:, dd] - s * ref[:, dd])) < 2e-3, dd
+    dev = g.spectral_layout(2, seed=5, data=x)
+    ref = orc.spectral_layout(g.n, r, c, v, n_comp=2, seed=5, data=x, align=dev)
+    assert np.max(np.abs(dev.astype(np.float64) - ref)) < 2e-3
 
     rng = np.random.default_rng(29)
     offsets = rng.normal(0.0, 60.0, size=(6, 8))
@@ -487,15 +489,9 @@ def test_spectral_layout_equals_oracle_up_to_eigenvector_signs(ctx):
     idx, dist = M.knn_exhaustive(ctx, x, 15)
     g = M.FuzzyGraph(ctx, idx, dist)
     r, c, v = g.coo()
-    dev = g.spectral_layout(2, seed=5, data=x).astype(np.float64)
-    ref = orc.spectral_layout(g.n, r, c, v, n_comp=2, seed=5, data=x).astype(np.float64)
-    cd = np.stack([dev[40 * k:40 * k + 40].mean(0) for k in range(6)])
-    cr = np.stack([ref[40 * k:40 * k + 40].mean(0) for k in range(6)])
-    for dd in range(2):
-        s = np.sign(np.dot(cd[:, dd], cr[:, dd]))
-        assert np.max(np.abs(cd[:, dd] - s * cr[:, dd])) < 2e-3, ("centres", dd)
-        for k in range(6):
-            a = dev[40 * k:40 * k + 40, dd] - cd[k, dd]
-            b = ref[40 * k:40 * k + 40, dd] - cr[k, dd]
-            sk = np.sign(np.dot(a, b))
-            assert np.max(np.abs(a - sk * b)) < 2e-3, ("component", k, dd)
+    labels = np.repeat(np.arange(6), 40)
+    assert np.all(labels[r] == labels[c])  # six components, 6 > 2 * n_dim: data-centroid meta
+    dev = g.spectral_layout(2, seed=5, data=x)
+    ref = orc.spectral_layout(g.n, r, c, v, n_comp=2, seed=5, data=x, align=dev)
+    err = np.abs(dev.astype(np.float64) - ref)
+    assert np.max(err) < 2e-3, np.max(err, axis=0)

@@ -784,6 +784,28 @@ def test_spectral_layout_single_component_spans_eigenvectors(n_dim):
     assert np.all(np.abs(emb.mean(0)) < 1e-3)
 
 
+def test_spectral_layout_alignment_only_chooses_signs():
+    # align= picks each eigenvector's sign and nothing else: aligned to the
+    # negated layout, the single-component layout is the negated layout, up to
+    # the N(0, 1e-4) noise added after the sign is fixed (twice the noise)
+    from spectral_checks import rectangle
+    from scipy.spatial.distance import cdist
+
+    x = rectangle(40, 4.0, 8, seed=3)
+    d2 = cdist(x, x, "sqeuclidean")
+    np.fill_diagonal(d2, np.inf)
+    w = np.exp(-d2 / np.median(d2[np.isfinite(d2)]))
+    w[w < 1e-3] = 0.0
+    w = np.maximum(w, w.T)
+    r, c = np.nonzero(w)
+    v = w[r, c].astype(np.float32)
+    ref = orc.spectral_layout(40, r, c, v, seed=5).astype(np.float64)
+    for flip in ((1, 1), (-1, 1), (1, -1), (-1, -1)):
+        target = (ref * np.array(flip)).astype(np.float32)
+        got = orc.spectral_layout(40, r, c, v, seed=5, align=target).astype(np.float64)
+        assert np.max(np.abs(got - target)) < 1e-3, flip
+
+
 def test_spectral_layout_simplex_meta_and_expansion():
     # three components (<= 2 n_dim): centres on the simplex rows (+e1, +e2, -e1)
     # up to the final centring and scaling; each component reaches half its
