@@ -72,33 +72,62 @@ struct FitDesc {
     int32_t neg_rate;
     float a, b, two_a_b, two_gamma_b;
     float fused_lr, epsc, omb1, omb2;
-    uint32_t blk_edge0, blk_node0;  // first block of this fit in the edge / node launch
+    uint32_t blk_node0;  // first block of this fit in the node launch
 };
 
-// index of the fit owning block `blk` (fits sorted by first block); uniform per block
-__device__ inline int fit_of_block(const FitDesc* __restrict__ fits, int n_fits, uint32_t blk,
-                                   bool edges) {
+// Workgroups of a launch go to the chip's 8 XCDs round-robin: block b runs on
+// XCD b mod 8.  The edge launch gives each XCD its own bin of edge blocks, so
+// that a fit's edges run on as few XCDs as possible and its embedding, the
+// target of every negative-sample gather, stays in those XCDs' L2 (each XCD
+// has its own).  Bins are filled in fit order to equal size; a fit larger
+// than the space left in a bin continues in the next.  A piece is one fit's
+// run of blocks inside one bin.
+constexpr uint32_t kXcds = 8;
+struct Piece {
+    uint32_t fit;    // index into the launch's descriptors
+    uint32_t blk0;   // the piece's first edge block within its fit
+    uint32_t slot0;  // its first slot in the bin
+    uint32_t count;  // blocks
+};
+struct Bins {
+    uint32_t first[kXcds + 1];  // pieces of bin x: [first[x], first[x + 1]), by slot
+};
+
+// index of the fit owning block `blk` of the node launch (fits sorted by first
+// block); uniform per block
+__device__ inline int fit_of_block(const FitDesc* __restrict__ fits, int n_fits, uint32_t blk) {
     int lo = 0, hi = n_fits - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const uint32_t b0 = edges ? fits[mid].blk_edge0 : fits[mid].blk_node0;
-        if (b0 <= blk) lo = mid;
+        if (fits[mid].blk_node0 <= blk) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
-// One epoch of every fit of the launch: a block covers kWg consecutive edges of
-// one fit, as four wavefronts of 64-edge spans with the firing edges compacted
-// (epoch_edge_span, the single-fit kernel's code).
+// One epoch of every fit of the launch: block b takes slot b / 8 of bin b % 8,
+// i.e. kWg consecutive edges of one fit, as four wavefronts of 64-edge spans
+// with the firing edges compacted (epoch_edge_span, the single-fit kernel's
+// code).  Slots past the end of their bin exit at once (the whole block).
 template <int DIM>
 __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restrict__ fits,
-                                                         int n_fits) {
+                                                         const Piece* __restrict__ pieces,
+                                                         Bins bins) {
     constexpr int kWaves = kWg / 64;
     __shared__ uint32_t queue[kWaves][64];
-    const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x, true)];
+    const uint32_t x = blockIdx.x % kXcds, t = blockIdx.x / kXcds;
+    uint32_t lo = bins.first[x], hi = bins.first[x + 1];
+    if (lo == hi) return;
+    while (hi - lo > 1) {  // the last piece of the bin starting at or before slot t
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pieces[mid].slot0 <= t) lo = mid;
+        else hi = mid;
+    }
+    const Piece pc = pieces[lo];
+    if (t < pc.slot0 || t - pc.slot0 >= pc.count) return;
+    const FitDesc& f = fits[pc.fit];
     const int wave = threadIdx.x >> 6;
-    const uint64_t base = ((uint64_t)(blockIdx.x - f.blk_edge0) * kWaves + wave) * 64;
+    const uint64_t base = ((uint64_t)(pc.blk0 + (t - pc.slot0)) * kWaves + wave) * 64;
     epoch_edge_span<DIM, false, 64>(base, queue[wave], f.edges, f.per, f.next, f.embd, f.grad,
                                     f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b,
                                     f.two_a_b, f.two_gamma_b, f.neg_rate, nullptr, nullptr,
@@ -108,7 +137,7 @@ __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restri
 template <int DIM>
 __global__ __launch_bounds__(kWg) void k_seg_adam_clear(const FitDesc* __restrict__ fits,
                                                         int n_fits) {
-    const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x, false)];
+    const FitDesc& f = fits[fit_of_block(fits, n_fits, blockIdx.x)];
     const uint32_t node = (blockIdx.x - f.blk_node0) * kWg + threadIdx.x;
     if (node >= f.n) return;
     adam_clear_node<DIM>(node, f.grad, f.m, f.v, f.embd, f.fused_lr, f.epsc, f.omb1, f.omb2,
@@ -183,10 +212,13 @@ struct mrs_server {
     // per-tick descriptor upload: a ring of pinned host tables and of device
     // tables (max_active entries per slot), one event per tick in flight; a
     // tick never writes the table the previous tick's kernels read
+    // A slot holds the tick's descriptors followed by its pieces (at most
+    // max_active + 8 per dimension group), in units of FitDesc.
     static constexpr int kRing = 8;
     mrs::FitDesc* host_desc[kRing] = {nullptr};
     hipEvent_t tick_ev[kRing] = {nullptr};
     mrs::DevBuf<mrs::FitDesc> dev_desc;
+    size_t slot_units = 0;
 };
 
 namespace mrs {
@@ -309,6 +341,8 @@ void batcher_loop(mrs_server* sv) {
     struct Group {
         int dim, first, count;
         uint32_t blocks_e, blocks_n;
+        uint32_t piece0, slots;  // the group's first piece; slots per XCD bin
+        Bins bins;
     };
     int64_t tick = 0;
     for (;;) {
@@ -336,7 +370,7 @@ void batcher_loop(mrs_server* sv) {
             std::vector<Group> groups;
             int nd = 0;
             for (int dim : {1, 2, 3, 4, 8}) {
-                Group g{dim, nd, 0, 0, 0};
+                Group g{dim, nd, 0, 0, 0, 0, 0, {}};
                 for (mrs_server_request* r : active) {
                     mrs_umap_state* st = r->st.get();
                     if (r->params.epoch_strategy != MRS_EPOCH_EDGE_ATOMIC || st->dim != dim)
@@ -345,7 +379,6 @@ void batcher_loop(mrs_server* sv) {
                     if (!st->grad_clean) clear_update_marks(st);
                     FitDesc& f = hd[nd++];
                     fill_desc(f, st, r->epoch, (uint32_t)r->seed);
-                    f.blk_edge0 = g.blocks_e;
                     f.blk_node0 = g.blocks_n;
                     g.blocks_e += div_up(st->n_edges, kWg);
                     g.blocks_n += div_up(st->n, kWg);
@@ -354,13 +387,42 @@ void batcher_loop(mrs_server* sv) {
                 if (g.count) groups.push_back(g);
             }
             if (nd) {
-                FitDesc* dtab = sv->dev_desc.p + (size_t)slot * sv->max_active;
-                MRS_HIP(hipMemcpyAsync(dtab, hd, sizeof(FitDesc) * nd,
-                                       hipMemcpyHostToDevice, s));
+                // the pieces follow the descriptors in the same table
+                Piece* hp = reinterpret_cast<Piece*>(hd + nd);
+                uint32_t np = 0;
+                for (Group& g : groups) {
+                    g.piece0 = np;
+                    g.slots = div_up(g.blocks_e, kXcds);
+                    uint32_t x = 0, filled = 0, cnt = 0;
+                    g.bins.first[0] = 0;
+                    for (int q = 0; q < g.count; q++) {
+                        uint32_t left = div_up(hd[g.first + q].n_edges, kWg), blk0 = 0;
+                        while (left) {
+                            const uint32_t take = std::min(left, g.slots - filled);
+                            hp[np + cnt++] = Piece{(uint32_t)q, blk0, filled, take};
+                            blk0 += take;
+                            left -= take;
+                            filled += take;
+                            if (filled == g.slots) {
+                                g.bins.first[++x] = cnt;
+                                filled = 0;
+                            }
+                        }
+                    }
+                    for (uint32_t q = x + 1; q <= kXcds; q++) g.bins.first[q] = cnt;
+                    np += cnt;
+                }
+                const size_t bytes = sizeof(FitDesc) * nd + sizeof(Piece) * np;
+                if (bytes > sizeof(FitDesc) * sv->slot_units)
+                    throw Error(MRS_DEVICE_ERROR, "fit server: tick table overflow");
+                FitDesc* dtab = sv->dev_desc.p + (size_t)slot * sv->slot_units;
+                MRS_HIP(hipMemcpyAsync(dtab, hd, bytes, hipMemcpyHostToDevice, s));
+                const Piece* dpieces = reinterpret_cast<const Piece*>(dtab + nd);
                 for (const Group& g : groups) {
                     const FitDesc* dd = dtab + g.first;
                     auto go = [&](auto ke, auto ka) {
-                        hipLaunchKernelGGL(ke, dim3(g.blocks_e), dim3(kWg), 0, s, dd, g.count);
+                        hipLaunchKernelGGL(ke, dim3(g.slots * kXcds), dim3(kWg), 0, s, dd,
+                                           dpieces + g.piece0, g.bins);
                         hipLaunchKernelGGL(ka, dim3(g.blocks_n), dim3(kWg), 0, s, dd, g.count);
                     };
                     switch (g.dim) {
@@ -465,15 +527,20 @@ int mrs_server_create(int32_t device, size_t hbm_budget, int32_t prep_workers, i
                 hbm_budget = free_b / 2;
             }
             sv->budget = hbm_budget;
+            // descriptors, then pieces: at most one per fit plus one per bin
+            // boundary in each of the five dimension groups
+            sv->slot_units = (size_t)max_active +
+                             mrs::div_up(sizeof(mrs::Piece) * ((size_t)max_active + 5 * mrs::kXcds),
+                                         sizeof(mrs::FitDesc));
             for (int q = 0; q < mrs_server::kRing; q++) {
-                MRS_HIP(hipHostMalloc((void**)&sv->host_desc[q], sizeof(mrs::FitDesc) * max_active,
-                                      hipHostMallocDefault));
+                MRS_HIP(hipHostMalloc((void**)&sv->host_desc[q],
+                                      sizeof(mrs::FitDesc) * sv->slot_units, hipHostMallocDefault));
                 MRS_HIP(hipEventCreateWithFlags(&sv->tick_ev[q], hipEventDisableTiming));
                 MRS_HIP(hipEventRecord(sv->tick_ev[q], sv->batch_ctx->stream));
             }
             {
                 mrs::StreamScope scope(sv->batch_ctx->stream);
-                sv->dev_desc.alloc((size_t)max_active * mrs_server::kRing);
+                sv->dev_desc.alloc(sv->slot_units * mrs_server::kRing);
             }
             for (int w = 0; w < prep_workers; w++) sv->workers.emplace_back(mrs::worker_loop, sv, w);
             sv->batcher = std::thread(mrs::batcher_loop, sv);
