@@ -926,6 +926,12 @@ def main():
         res = bench_plain(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")
+    if "autotune" not in res:
+        import mrs_amd as M
+
+        # launch shapes the run chose by measurement (family, shape class,
+        # choice, milliseconds per candidate); decided in the warm-up
+        res["autotune"] = M.autotune_records()
     print(json.dumps(res))
 
 

@@ -414,6 +414,45 @@ int mrs_server_stats(mrs_server *sv, int64_t *out);
 /* Finishes every queued request, then stops the threads and frees everything. */
 int mrs_server_destroy(mrs_server *sv);
 
+/* ------------------------------------------------------------- autotuning
+ * Launch shapes chosen by measurement.  No counterpart in the reference,
+ * whose workgroup sizes are compile-time constants
+ * (src/training/umap_optimiser_gpu.rs:72-82).  Two families have several
+ * valid shapes with the same results:
+ *   MRS_TUNE_EPOCH_SPAN  edges per wavefront of the edge-parallel epoch
+ *                        (64, 128, 256); only the order of the FP32 atomic
+ *                        additions differs between them;
+ *   MRS_TUNE_KNN_SPLITS  database splits of the unpruned kNN sweep (1, 2, 4,
+ *                        8); the lists are bit-identical.
+ * With MRS_TUNE_ON_FIRST_USE (the default) the first call that meets an
+ * undecided shape class (dimension, density term, and edge and node counts
+ * as powers of two; precision, padded width, shortlist and rows as a power
+ * of two)
+ * times every candidate on its own data with HIP events and records the
+ * fastest: 12 epochs in the middle of a run's stretch with one density flag
+ * (at least 12 epochs long) alternate the spans; the first search runs the
+ * shortlist once per split count.  Later
+ * calls of the class use the record.  MRS_TUNE_OFF keeps the built-in
+ * choices (spans of 128 from 2^20 edges, else 64; splits filling the chip
+ * with 1024 workgroups).  Process-wide; thread-safe. */
+#define MRS_TUNE_OFF 0
+#define MRS_TUNE_ON_FIRST_USE 1
+#define MRS_TUNE_EPOCH_SPAN 1
+#define MRS_TUNE_KNN_SPLITS 2
+int mrs_autotune_set_mode(int32_t mode);
+/* Use `variant` for every call of the family (overrides records and mode);
+ * -1 releases it.  For tests and experiments. */
+int mrs_autotune_force(int32_t family, int32_t variant);
+/* Forget every recorded decision. */
+int mrs_autotune_clear(void);
+/* The records as text, one per line: "family shape variant cand:ms ...".
+ * Writes at most cap bytes (NUL-terminated) and returns the size needed. */
+int64_t mrs_autotune_dump(char *buf, int64_t cap);
+/* Save / load the records (text; the first line names the device they were
+ * measured on, and a table from another device is refused). */
+int mrs_autotune_save(const char *path);
+int mrs_autotune_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

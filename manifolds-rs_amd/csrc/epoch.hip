@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "autotune.hpp"
 #include "epoch_device.hpp"
 #include "umap_state.hpp"
 
@@ -495,8 +496,9 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 
 // One edge-parallel epoch: each wavefront owns kEdgeSpan consecutive edges
 // and works through the firing ones compacted in LDS (epoch_edge_span).
-// Spans of 128 edges (128 and 64 alike and 256 slower on the 1M fit graph:
-// 317, 320 and 327 us); below kWideSpanEdges edges spans of 64 leave enough
+// Built-in choice (the autotuner's fallback): spans of 128 edges from
+// kWideSpanEdges edges (128 and 64 alike and 256 slower on the 1M fit graph:
+// 317, 320 and 327 us), below that spans of 64, which leave enough
 // wavefronts to fill the chip.
 constexpr uint32_t kWideSpanEdges = 1u << 20;
 constexpr int kWideSpan = 128;
@@ -697,16 +699,38 @@ void marks_from_flags(mrs_umap_state* st) {
     MRS_CHECK_LAUNCH();
 }
 
-void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens) {
+namespace {
+
+// shape class of an edge epoch: dimension, density term, and edges and nodes
+// as powers of two (the node count sets how much of the embedding, the target
+// of the random gathers, each XCD's L2 holds)
+uint64_t epoch_span_shape(const mrs_umap_state* st, bool dens) {
+    return (uint64_t)st->dim | (uint64_t)dens << 4 | (uint64_t)ceil_log2(st->n_edges) << 8 |
+           (uint64_t)ceil_log2(st->n) << 16;
+}
+
+}  // namespace
+
+int epoch_span_choice(const mrs_umap_state* st, bool dens) {
+    const int forced = tune_forced(kTuneEpochSpan);
+    if (forced > 0) return forced;
+    const int rec = tune_lookup(kTuneEpochSpan, epoch_span_shape(st, dens));
+    if (rec > 0) return rec;
+    return (uint32_t)st->n_edges >= kWideSpanEdges ? kWideSpan : 64;
+}
+
+void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens, int span) {
     if (!st->grad_clean) clear_update_marks(st);
     const mrs_umap_optim_params& p = st->params;
     const float two = 2.0f;
     FastMod fm = make_fastmod((uint32_t)st->n);
     const uint32_t ne = (uint32_t)st->n_edges;
     hipStream_t s = st->ctx->stream;
+    if (span == 0) span = epoch_span_choice(st, dens);
+    if (span != 64 && span != 128 && span != 256)
+        throw Error(MRS_INVALID_ARGUMENT, "edge epoch span must be 64, 128 or 256");
     // one wavefront per span of edges
-    const bool wide = ne >= kWideSpanEdges;
-    const unsigned blocks = div_up(div_up(ne, wide ? kWideSpan : 64), kWg / 64);
+    const unsigned blocks = div_up(div_up(ne, (unsigned)span), kWg / 64);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWg), 0, s, st->edges.p, st->per.p,
                            st->next.p, st->embd.p, st->grad.p, ne, (uint32_t)epoch, (float)epoch,
@@ -716,7 +740,8 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     };
     auto by_span = [&](auto d) {
         constexpr int D = decltype(d)::value;
-        if (wide) dens ? go(k_epoch_edges<D, true, kWideSpan>) : go(k_epoch_edges<D, false, kWideSpan>);
+        if (span == 256) dens ? go(k_epoch_edges<D, true, 256>) : go(k_epoch_edges<D, false, 256>);
+        else if (span == 128) dens ? go(k_epoch_edges<D, true, 128>) : go(k_epoch_edges<D, false, 128>);
         else dens ? go(k_epoch_edges<D, true, 64>) : go(k_epoch_edges<D, false, 64>);
     };
     switch (st->dim) {
@@ -796,6 +821,76 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
         int d = st->dim;
         if (!(d == 1 || d == 2 || d == 3 || d == 4 || d == 8)) strategy = MRS_EPOCH_NODE_EXACT;
     }
+    // Span tuning (autotune.hpp).  For each stretch of the run with one
+    // density flag whose shape class is undecided, a window of kTuneEpochs
+    // epochs alternates the candidate spans with an event pair around each
+    // edge launch, and the median per span decides.  The window sits in the
+    // middle of the stretch: in the first epochs only the edges whose period
+    // has already come round fire, so they are lighter than the rest.
+    // Directly launched runs only (not under capture).
+    constexpr int kSpans[3] = {64, 128, 256};
+    constexpr int kTuneEpochs = 12;
+    struct Window {
+        int32_t start;
+        bool dens;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    };
+    std::vector<Window> windows;
+    auto drop_windows = [&]() {
+        for (auto& w : windows)
+            for (auto& ev : w.ev) {
+                if (ev.first) (void)hipEventDestroy(ev.first);
+                if (ev.second) (void)hipEventDestroy(ev.second);
+            }
+        windows.clear();
+    };
+    auto plan_tuning = [&]() {
+        if (strategy != MRS_EPOCH_EDGE_ATOMIC || !tune_enabled() || tune_forced(kTuneEpochSpan) > 0)
+            return;
+        auto dens_at = [&](int32_t e) {
+            return st->dens_enabled && dens_is_active(st->dens_params, e, N);
+        };
+        for (int32_t a = first; a < first + count;) {
+            const bool d = dens_at(a);
+            int32_t b = a + 1;
+            while (b < first + count && dens_at(b) == d) b++;
+            const int32_t len = b - a;
+            bool planned = false;
+            for (auto& w : windows) planned |= w.dens == d;
+            if (len >= kTuneEpochs && !planned &&
+                tune_lookup(kTuneEpochSpan, epoch_span_shape(st, d)) <= 0) {
+                Window w{a + (len - kTuneEpochs) / 2, d, {}};
+                w.ev.resize(kTuneEpochs, {nullptr, nullptr});
+                windows.push_back(std::move(w));
+                for (auto& ev : windows.back().ev) {
+                    MRS_HIP(hipEventCreate(&ev.first));
+                    MRS_HIP(hipEventCreate(&ev.second));
+                }
+            }
+            a = b;
+        }
+    };
+    auto finish_window = [&](Window& w) {
+        std::vector<std::vector<float>> t(3);
+        for (int q = 0; q < kTuneEpochs; q++) {
+            float ms = 0.0f;
+            MRS_HIP(hipEventSynchronize(w.ev[q].second));
+            MRS_HIP(hipEventElapsedTime(&ms, w.ev[q].first, w.ev[q].second));
+            t[q % 3].push_back(ms);
+        }
+        std::vector<float> med;
+        for (auto& v : t) {
+            std::sort(v.begin(), v.end());
+            med.push_back(v[v.size() / 2]);
+        }
+        tune_record(kTuneEpochSpan, epoch_span_shape(st, w.dens), {kSpans[0], kSpans[1], kSpans[2]},
+                    med);
+    };
+    auto window_of = [&](int32_t e) -> Window* {
+        for (auto& w : windows)
+            if (e >= w.start && e < w.start + kTuneEpochs) return &w;
+        return nullptr;
+    };
     auto launches = [&]() {
         for (int32_t e = first; e < first + count; e++) {
             bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
@@ -807,6 +902,14 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
                 { ProfScope ps(st, 2); launch_grad_accum_node(st, e, seed, dens); }
                 { ProfScope ps(st, 3); launch_adam_elem(st, e); }
                 { ProfScope ps(st, 4); launch_schedule(st, e); }
+            } else if (Window* w = window_of(e)) {
+                const int q = e - w->start;
+                if (!st->grad_clean) clear_update_marks(st);
+                MRS_HIP(hipEventRecord(w->ev[q].first, st->ctx->stream));
+                { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens, kSpans[q % 3]); }
+                MRS_HIP(hipEventRecord(w->ev[q].second, st->ctx->stream));
+                { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
+                if (q + 1 == kTuneEpochs) finish_window(*w);
             } else {
                 { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens); }
                 { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
@@ -815,7 +918,14 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
     };
     hipStream_t s = st->ctx->stream;
     if (st->profiling || !st->use_graphs || count < 16) {
-        launches();
+        try {
+            plan_tuning();
+            launches();
+        } catch (...) {
+            drop_windows();
+            throw;
+        }
+        drop_windows();
         return;
     }
     // Capture the whole loop once and replay it: one launch per fit instead of
@@ -826,7 +936,14 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
     const bool hit = st->graph_exec && std::equal(key, key + 4, st->graph_key);
     if (!hit && !std::equal(key, key + 4, st->last_key)) {
         std::copy(key, key + 4, st->last_key);
-        launches();
+        try {
+            plan_tuning();
+            launches();
+        } catch (...) {
+            drop_windows();
+            throw;
+        }
+        drop_windows();
         return;
     }
     if (!hit) {

@@ -47,6 +47,7 @@
 #include <cfloat>
 #include <cmath>
 
+#include "autotune.hpp"
 #include "knn.hpp"
 
 namespace mrs {
@@ -1299,11 +1300,25 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     if (!prune) {
         const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
         const uint32_t max_splits = scratch ? (uint32_t)std::max(1, scratch->max_splits) : 8u;
-        splits = (int)std::min<uint32_t>(max_splits,
-                                         std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks));
-        uint32_t db_per = ((npad + splits - 1) / splits + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
-        splits = (int)((npad + db_per - 1) / db_per);
-        const size_t n_list = (size_t)splits * npad * S, n_queue = (size_t)splits * npad * sh.QC;
+        // split count: forced, recorded for this shape class, tuned now, or the
+        // built-in choice (enough query-block x split workgroups, 1024, to fill
+        // the chip); the lists are the same for every split count
+        auto layout = [&](uint32_t want, uint32_t* dbp) {
+            const uint32_t sp = std::min(max_splits, std::max(1u, want));
+            *dbp = ((npad + sp - 1) / sp + kSplitAlign - 1) / kSplitAlign * kSplitAlign;
+            return (int)((npad + *dbp - 1) / *dbp);
+        };
+        const uint64_t shape = (uint64_t)bf16 | (uint64_t)E << 1 | (uint64_t)ceil_log2(npad) << 3 |
+                               (uint64_t)KP << 9;
+        int want = tune_forced(kTuneKnnSplits);
+        if (want <= 0) want = tune_lookup(kTuneKnnSplits, shape);
+        const bool tune = want <= 0 && !scratch && tune_enabled();
+        if (want <= 0) want = (int)std::max<uint32_t>(1, (1024 + qblocks - 1) / qblocks);
+        uint32_t db_per = 0, db_max = 0;
+        splits = layout((uint32_t)want, &db_per);
+        const int alloc_splits = tune ? layout(max_splits, &db_max) : splits;
+        const size_t n_list = (size_t)alloc_splits * npad * S,
+                     n_queue = (size_t)alloc_splits * npad * sh.QC;
         DevBuf<uint64_t> lists_own, queue_own;
         uint64_t *lists, *queue;
         if (scratch) {
@@ -1323,18 +1338,58 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         sa.n = n;
         sa.npad = npad;
         sa.KP = KP;
-        sa.db_per_split = db_per;
-        mark(1);
-        run_shortlist(sa, dim3(qblocks, splits), false);
-        mark(2);
-        if (splits > 1) {
-            if (E == 1)
-                hipLaunchKernelGGL(k_merge_splits<1>, dim3(div_up(npad, 4)), dim3(256), 0, s,
-                                   lists, npad, splits);
-            else
-                hipLaunchKernelGGL(k_merge_splits<2>, dim3(div_up(npad, 4)), dim3(256), 0, s,
-                                   lists, npad, splits);
-            MRS_CHECK_LAUNCH();
+        auto sweep = [&](int sp, uint32_t dbp) {
+            sa.db_per_split = dbp;
+            run_shortlist(sa, dim3(qblocks, sp), false);
+            if (sp > 1) {
+                if (E == 1)
+                    hipLaunchKernelGGL(k_merge_splits<1>, dim3(div_up(npad, 4)), dim3(256), 0, s,
+                                       lists, npad, sp);
+                else
+                    hipLaunchKernelGGL(k_merge_splits<2>, dim3(div_up(npad, 4)), dim3(256), 0, s,
+                                       lists, npad, sp);
+                MRS_CHECK_LAUNCH();
+            }
+        };
+        if (tune) {
+            // every distinct split count once, shortlist and merge timed with
+            // events; the lists of the last run are the result (all runs give
+            // the same lists)
+            std::vector<int> cands;
+            std::vector<uint32_t> dbps;
+            for (uint32_t c : {1u, 2u, 4u, 8u}) {
+                uint32_t dbp = 0;
+                const int sp = layout(c, &dbp);
+                if (std::find(cands.begin(), cands.end(), sp) == cands.end()) {
+                    cands.push_back(sp);
+                    dbps.push_back(dbp);
+                }
+            }
+            std::vector<hipEvent_t> te(cands.size() + 1, nullptr);
+            try {
+                for (auto& e : te) MRS_HIP(hipEventCreate(&e));
+                mark(1);
+                MRS_HIP(hipEventRecord(te[0], s));
+                for (size_t c = 0; c < cands.size(); c++) {
+                    sweep(cands[c], dbps[c]);
+                    MRS_HIP(hipEventRecord(te[c + 1], s));
+                }
+                mark(2);
+                MRS_HIP(hipEventSynchronize(te.back()));
+                std::vector<float> ms(cands.size());
+                for (size_t c = 0; c < cands.size(); c++)
+                    MRS_HIP(hipEventElapsedTime(&ms[c], te[c], te[c + 1]));
+                splits = tune_record(kTuneKnnSplits, shape, cands, ms);
+            } catch (...) {
+                for (auto e : te)
+                    if (e) (void)hipEventDestroy(e);
+                throw;
+            }
+            for (auto e : te) (void)hipEventDestroy(e);
+        } else {
+            mark(1);
+            sweep(splits, db_per);
+            mark(2);
         }
         rerank(lists);
         mark(3);

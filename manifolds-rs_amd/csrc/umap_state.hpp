@@ -103,7 +103,12 @@ void launch_schedule(mrs_umap_state* st, int32_t epoch);
 // radii + FP64 stats + weights, coeff on device; reference_order: the radii
 // summed per node in the reference's sequential order (else 16 lanes per node)
 void launch_dens_update_dev(mrs_umap_state* st, bool reference_order);
-void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens);
+// span: edges per wavefront (64, 128 or 256); 0 = the recorded or built-in
+// choice for this state (autotune.hpp)
+void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens,
+                        int span = 0);
+// The span launch_epoch_edges uses for this state when given 0.
+int epoch_span_choice(const mrs_umap_state* st, bool dens);
 // grad to -0.0 (no update yet), has_update from / to the markers
 void clear_update_marks(mrs_umap_state* st);
 void derive_update_flags(mrs_umap_state* st);

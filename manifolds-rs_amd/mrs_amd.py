@@ -236,6 +236,12 @@ def lib() -> C.CDLL:
         "mrs_server_wait": (C.c_int, [vp, i64, vp, vp]),
         "mrs_server_stats": (C.c_int, [vp, vp]),
         "mrs_server_destroy": (C.c_int, [vp]),
+        "mrs_autotune_set_mode": (C.c_int, [i32]),
+        "mrs_autotune_force": (C.c_int, [i32, i32]),
+        "mrs_autotune_clear": (C.c_int, []),
+        "mrs_autotune_dump": (i64, [C.c_char_p, i64]),
+        "mrs_autotune_save": (C.c_int, [C.c_char_p]),
+        "mrs_autotune_load": (C.c_int, [C.c_char_p]),
         "mrs_optimise_embedding_adam_gpu": (C.c_int, [vp, vp, i64, i32, vp, vp, vp,
                                                       C.POINTER(UmapOptimParams),
                                                       C.POINTER(DensState), u64, i32, i32]),
@@ -743,6 +749,48 @@ def fit_knn_copy_to(idx=None, dist=None) -> None:
     device tensors ([n, k] each, n*k entries at most); None switches it off."""
     cap = 0 if idx is None else int(idx.numel())
     _check(lib().mrs_fit_knn_copy_to(_ptr(idx), _ptr(dist), cap))
+
+
+TUNE_OFF, TUNE_ON_FIRST_USE = 0, 1
+TUNE_EPOCH_SPAN, TUNE_KNN_SPLITS = 1, 2
+
+
+def autotune_set_mode(mode: int) -> None:
+    """TUNE_ON_FIRST_USE (default): undecided launch shapes are timed on first
+    use; TUNE_OFF: built-in choices."""
+    _check(lib().mrs_autotune_set_mode(mode))
+
+
+def autotune_force(family: int, variant: int) -> None:
+    """Force a family's variant (TUNE_EPOCH_SPAN: 64/128/256 edges per
+    wavefront; TUNE_KNN_SPLITS: 1/2/4/8); -1 releases it."""
+    _check(lib().mrs_autotune_force(family, variant))
+
+
+def autotune_clear() -> None:
+    _check(lib().mrs_autotune_clear())
+
+
+def autotune_records() -> list[dict]:
+    """The recorded decisions: family, shape class, chosen variant and the
+    measured milliseconds of every candidate."""
+    need = lib().mrs_autotune_dump(None, 0)
+    buf = C.create_string_buffer(int(need))
+    lib().mrs_autotune_dump(buf, need)
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        out.append(dict(family=int(f[0]), shape=int(f[1]), variant=int(f[2]),
+                        ms={int(a): float(b) for a, b in (t.split(":") for t in f[3:])}))
+    return out
+
+
+def autotune_save(path: str) -> None:
+    _check(lib().mrs_autotune_save(str(path).encode()))
+
+
+def autotune_load(path: str) -> None:
+    _check(lib().mrs_autotune_load(str(path).encode()))
 
 
 def last_fit_timings() -> dict:

@@ -96,3 +96,23 @@ def test_tridiag_eigen_matches_numpy(m, seed):
     np.testing.assert_allclose(ev, want, atol=1e-12 * max(1.0, np.abs(want).max()))
     np.testing.assert_allclose(V.T @ V, np.eye(m), atol=1e-12)
     np.testing.assert_allclose(T @ V, V * ev[None, :], atol=1e-11)
+
+
+def test_autotune_arguments_and_empty_table(tmp_path):
+    # host-side checks of the autotuner's ABI (no device work)
+    M.autotune_clear()
+    assert M.autotune_records() == []
+    for fam, var in ((M.TUNE_EPOCH_SPAN, 96), (M.TUNE_KNN_SPLITS, 3), (7, 64)):
+        with pytest.raises(M.ManifoldsError) as e:
+            M.autotune_force(fam, var)
+        assert e.value.status == M.MRS_INVALID_ARGUMENT
+    with pytest.raises(M.ManifoldsError):
+        M.autotune_set_mode(2)
+    M.autotune_force(M.TUNE_EPOCH_SPAN, 256)
+    M.autotune_force(M.TUNE_EPOCH_SPAN, -1)
+    bad = tmp_path / "t.txt"
+    bad.write_text("not a table\n")
+    with pytest.raises(M.ManifoldsError):
+        M.autotune_load(bad)
+    with pytest.raises(M.ManifoldsError):
+        M.autotune_load(tmp_path / "missing.txt")
