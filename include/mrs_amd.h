@@ -166,6 +166,25 @@ int mrs_umap_optim_params_from_min_dist_spread(float min_dist, float spread, flo
                                                int32_t neg_sample_rate, float beta1, float beta2,
                                                float eps, mrs_umap_optim_params *out);
 
+/* pca_layout (src/data/init.rs:1065-1140): out [n][n_comp] (host or device)
+ * = the first n_comp principal-component scores of data [n][d] (host or
+ * device), each column centred and scaled to standard deviation `range` (NaN:
+ * PCA_RANGE = 1).  The components are the exact leading eigenvectors of the
+ * centred covariance (FP64), for randomised = 0 or 1 alike (the reference's
+ * randomised SVD approximates them; seed is accepted for that signature).
+ * Each component's sign makes its largest-magnitude loading positive.
+ * n_comp 1..min(16, d, n); d <= 2048.  The fit takes this path for
+ * initialisation "pca" and, as the reference does, for unknown strings. */
+int mrs_pca_layout(mrs_ctx *ctx, const float *data, int64_t n, int32_t d, int32_t n_comp,
+                   int32_t randomised, float range, uint64_t seed, float *out);
+/* The n_comp leading eigenvectors of the symmetric matrix a [d][d] (FP64) as
+ * rows of vecs [n_comp][d] (largest eigenvalue first, each with its
+ * largest-magnitude entry positive) and evals [n_comp]: the host eigensolver
+ * of the PCA layout (Householder tridiagonalisation, implicit QL).  No
+ * counterpart in the reference (faer's SVD); host only, needs no device. */
+int mrs_sym_eigen_leading(const double *a, int32_t d, int32_t n_comp, double *vecs,
+                          double *evals);
+
 /* umap_gpu (src/lib.rs:3465-3487 -> umap_gpu_inner :3510-3708): data [n*d] f32
  * row-major; optional precomputed kNN (knn_idx/knn_dist [n*knn_k], self
  * excluded, squared distances; pass NULL to search); out [n_dim*n] is the

@@ -17,6 +17,7 @@
 
 #include "graph.hpp"
 #include "knn.hpp"
+#include "pca.hpp"
 #include "spectral.hpp"
 #include "fit.hpp"
 #include "umap_state.hpp"
@@ -183,13 +184,18 @@ std::unique_ptr<mrs_umap_state> fit_prepare(mrs_ctx* ctx, const float* data, int
     DevBuf<float> embd0((size_t)n * dim);
     const std::string init = lower(p->initialisation);
     const bool has_range = !std::isnan(p->init_range);
+    // parse_initilisation (src/data/init.rs:75-85): "spectral", "random",
+    // "pca"; any other string is PCA (src/lib.rs:3526-3540)
     if (init == "random") {
         random_layout_dev(ctx, embd0.p, n, dim, seed, has_range ? p->init_range : 10.0f);
-    } else {
-        if (init != "spectral" && verbose > 0)
-            std::printf("Initialisation \"%s\": using the spectral layout\n", init.c_str());
+    } else if (init == "spectral") {
         spectral_layout_dev(ctx, g.get(), dim, seed, has_range ? p->init_range : 10.0f,
                             embd0.p, xv.p, xv.p ? d : 0);
+    } else {
+        if (init != "pca" && verbose > 0)
+            std::printf("Unknown initialisation provided: \"%s\". Defaulting to PCA.\n",
+                        init.c_str());
+        pca_layout_dev(ctx, xv.p, n, d, dim, has_range ? p->init_range : 1.0f, embd0.p);
     }
     MRS_HIP(hipStreamSynchronize(s));
     t.init_ms = now_ms() - t0;

@@ -236,6 +236,7 @@ def lib() -> C.CDLL:
         "mrs_server_wait": (C.c_int, [vp, i64, vp, vp]),
         "mrs_server_stats": (C.c_int, [vp, vp]),
         "mrs_server_destroy": (C.c_int, [vp]),
+        "mrs_pca_layout": (C.c_int, [vp, vp, i64, i32, i32, i32, C.c_float, u64, vp]),
         "mrs_autotune_set_mode": (C.c_int, [i32]),
         "mrs_autotune_force": (C.c_int, [i32, i32]),
         "mrs_autotune_clear": (C.c_int, []),
@@ -749,6 +750,19 @@ def fit_knn_copy_to(idx=None, dist=None) -> None:
     device tensors ([n, k] each, n*k entries at most); None switches it off."""
     cap = 0 if idx is None else int(idx.numel())
     _check(lib().mrs_fit_knn_copy_to(_ptr(idx), _ptr(dist), cap))
+
+
+def pca_layout(ctx: "Context", x, n_comp: int = 2, randomised: bool = True, range_=None,
+               seed: int = 42):
+    """pca_layout (src/data/init.rs:1065-1140): [n, n_comp] float32 principal
+    component scores, each column centred and scaled to std `range_` (None: 1)."""
+    xa = np.ascontiguousarray(x, dtype=np.float32)
+    n, d = xa.shape
+    out = np.zeros((n, n_comp), np.float32)
+    rng = float("nan") if range_ is None else float(range_)
+    _check(lib().mrs_pca_layout(ctx.handle, xa.ctypes.data, n, d, n_comp, int(randomised), rng,
+                                seed, out.ctypes.data))
+    return out
 
 
 TUNE_OFF, TUNE_ON_FIRST_USE = 0, 1

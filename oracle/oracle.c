@@ -1608,3 +1608,55 @@ int orc_spectral_layout_aligned(int64_t n, const int32_t *rows, const int32_t *c
     free(ip); free(cur); free(adj); free(label); free(local); free(order); free(cstart);
     return 0;
 }
+
+/* ------------------------------------------------------------------ PCA */
+
+/* pca_layout, src/data/init.rs:1065-1140: centre the data (column means,
+ * :1077-1085), the first n_comp principal-component scores U S (:1088-1114;
+ * here X_c V with V the leading eigenvectors of the centred covariance, FP64,
+ * by cyclic Jacobi), each score column centred and scaled to standard deviation
+ * `range` (population variance; a column of std <= 1e-8 keeps scale 1,
+ * :1117-1137).  Eigenvector signs: the entry of largest magnitude (the first
+ * of equals) positive.  out [n][n_comp]. */
+int orc_pca_layout(const float *x, int64_t n, int32_t d, int32_t n_comp, float range, float *out) {
+    if (n <= 0) return 1;
+    if (n_comp <= 0 || n_comp > d || n_comp > n) return 5;
+    double *mean = (double *)calloc((size_t)d, sizeof(double));
+    for (int64_t r = 0; r < n; r++)
+        for (int j = 0; j < d; j++) mean[j] += (double)x[r * d + j];
+    for (int j = 0; j < d; j++) mean[j] /= (double)n;
+    double *cov = (double *)calloc((size_t)d * d, sizeof(double));
+    double *row = (double *)malloc(sizeof(double) * (size_t)d);
+    for (int64_t r = 0; r < n; r++) {
+        for (int j = 0; j < d; j++) row[j] = (double)x[r * d + j] - mean[j];
+        for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++) cov[i * d + j] += row[i] * row[j];
+    }
+    /* eigenvalues ascending, eigenvectors as columns (cyclic Jacobi above) */
+    double *vec = (double *)malloc(sizeof(double) * (size_t)d * d);
+    double *ev = (double *)malloc(sizeof(double) * (size_t)d);
+    jacobi_eigen(cov, d, ev, vec);
+    double *score = (double *)calloc((size_t)n, sizeof(double));
+    for (int c = 0; c < n_comp; c++) {
+        const int col = d - 1 - c; /* largest first */
+        int big = 0;
+        for (int i = 1; i < d; i++)
+            if (fabs(vec[i * d + col]) > fabs(vec[big * d + col])) big = i;
+        const double sg = vec[big * d + col] < 0.0 ? -1.0 : 1.0;
+        double s = 0.0;
+        for (int64_t r = 0; r < n; r++) {
+            double t = 0.0;
+            for (int j = 0; j < d; j++) t += ((double)x[r * d + j] - mean[j]) * sg * vec[j * d + col];
+            score[r] = t;
+            s += t;
+        }
+        const double mu = s / (double)n;
+        double var = 0.0;
+        for (int64_t r = 0; r < n; r++) var += (score[r] - mu) * (score[r] - mu);
+        const double sd = sqrt(var / (double)n);
+        const double sc = sd > 1e-8 ? (double)range / sd : 1.0;
+        for (int64_t r = 0; r < n; r++) out[r * n_comp + c] = (float)((score[r] - mu) * sc);
+    }
+    free(mean); free(cov); free(row); free(vec); free(ev); free(score);
+    return 0;
+}

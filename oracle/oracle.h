@@ -200,6 +200,12 @@ int orc_spectral_layout_aligned(int64_t n, const int32_t *rows, const int32_t *c
                                 float range, const float *data, int32_t d, const float *align,
                                 float *out);
 
+/* pca_layout, src/data/init.rs:1065-1140: x [n][d], out [n][n_comp]; the
+ * leading eigenvectors of the centred covariance by cyclic Jacobi (FP64),
+ * each with its largest-magnitude entry positive; every score column centred
+ * and scaled to standard deviation `range`.  0 ok, 1 NoData, 5 bad n_comp. */
+int orc_pca_layout(const float *x, int64_t n, int32_t d, int32_t n_comp, float range, float *out);
+
 #ifdef __cplusplus
 }
 #endif

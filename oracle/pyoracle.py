@@ -340,3 +340,15 @@ def spectral_layout(n, rows, cols, vals, n_comp=2, seed=42, range_=10.0, data=No
     if st:
         raise RuntimeError(f"oracle status {st}")
     return out
+
+
+def pca_layout(x, n_comp=2, range_=1.0):
+    """pca_layout of x [n, d]: [n, n_comp] float32 (exact leading components)."""
+    x = f32(x)
+    n, d = x.shape
+    out = np.zeros((n, n_comp), np.float32)
+    st = lib().orc_pca_layout(P(x), C.c_int64(n), C.c_int32(d), C.c_int32(n_comp),
+                              C.c_float(range_), P(out))
+    if st:
+        raise RuntimeError(f"oracle status {st}")
+    return out

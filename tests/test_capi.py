@@ -116,3 +116,28 @@ def test_autotune_arguments_and_empty_table(tmp_path):
         M.autotune_load(bad)
     with pytest.raises(M.ManifoldsError):
         M.autotune_load(tmp_path / "missing.txt")
+
+
+@pytest.mark.parametrize("d,n_comp", [(1, 1), (2, 2), (5, 3), (64, 4), (200, 16)])
+def test_host_symmetric_eigensolver_matches_numpy(d, n_comp):
+    # mrs_sym_eigen_leading (Householder tridiagonalisation + implicit QL, the
+    # PCA layout's host step) against numpy's eigh: eigenvalues, and
+    # eigenvectors under the library's sign rule (largest-magnitude entry
+    # positive)
+    rng = np.random.default_rng(d)
+    q = np.linalg.qr(rng.normal(size=(d, d)))[0]
+    lam = np.sort(rng.uniform(0.1, 10.0, size=d))[::-1] * np.linspace(1.0, 2.0, d)
+    a = (q * lam) @ q.T
+    a = (a + a.T) / 2
+    vecs = np.zeros((n_comp, d))
+    evals = np.zeros(n_comp)
+    P = C.POINTER(C.c_double)
+    st = M.lib().mrs_sym_eigen_leading(np.ascontiguousarray(a).ctypes.data_as(P), d, n_comp,
+                                       vecs.ctypes.data_as(P), evals.ctypes.data_as(P))
+    assert st == 0
+    w, v = np.linalg.eigh(a)
+    for c in range(n_comp):
+        r = d - 1 - c
+        assert abs(evals[c] - w[r]) < 1e-10 * w[-1]
+        want = v[:, r] * (1.0 if v[np.argmax(np.abs(v[:, r])), r] >= 0 else -1.0)
+        assert np.max(np.abs(vecs[c] - want)) < 1e-8

@@ -870,3 +870,73 @@ def test_spectral_layout_small_component_is_scattered_near_its_centre():
     # component stays within sqrt(2)/2 of its centre in meta units
     small = emb[40:43]
     assert np.ptp(small, axis=0).max() > 0  # scattered, not collapsed
+
+
+# ----------------------------------------------------------------- PCA layout
+
+_PCA_5x3 = np.array([[1.0, 2.0, 1.0], [2.0, 3.0, 2.0], [3.0, 4.0, 1.5], [4.0, 5.0, 2.5],
+                     [5.0, 6.0, 2.0]], np.float32)
+
+
+def test_pca_layout_reference_known_answers():
+    # src/data/init.rs:1495-1566: on the 5 x 3 matrix the layout has mean 0
+    # (1e-6) and the first component's standard deviation is PCA_RANGE = 1
+    # (1e-4), or the custom range; repeated calls are identical; a 3 x 4
+    # matrix gives 3 x 2
+    e = orc.pca_layout(_PCA_5x3, 2).astype(np.float64)
+    assert e.shape == (5, 2)
+    assert np.all(np.abs(e.mean(0)) < 1e-6)
+    assert abs(e[:, 0].std() - 1.0) < 1e-4
+    for rng in (1.0, 0.5, 3.0):
+        er = orc.pca_layout(_PCA_5x3, 2, range_=rng).astype(np.float64)
+        assert abs(er[:, 0].std() - rng) < 1e-4 * rng
+    assert np.array_equal(orc.pca_layout(_PCA_5x3, 2), e.astype(np.float32))
+    x34 = np.array([[1, 2, 3, 4], [5, 6, 7, 8], [9, 10, 11, 12]], np.float32)
+    assert orc.pca_layout(x34, 2).shape == (3, 2)
+
+
+def _numpy_pca(x, n_comp, range_=1.0):
+    # the reference's construction with numpy's SVD in float64: scores U S of
+    # the centred data, each column centred and scaled to std range_ (a
+    # column of std <= 1e-8 keeps scale 1); the sign of each component makes
+    # its largest-magnitude loading positive
+    xc = x.astype(np.float64) - x.astype(np.float64).mean(0)
+    u, s, vt = np.linalg.svd(xc, full_matrices=False)
+    out = np.empty((x.shape[0], n_comp))
+    for c in range(n_comp):
+        sg = 1.0 if vt[c][np.argmax(np.abs(vt[c]))] >= 0 else -1.0
+        sc = u[:, c] * s[c] * sg
+        sd = sc.std()
+        out[:, c] = (sc - sc.mean()) * (range_ / sd if sd > 1e-8 else 1.0)
+    return out
+
+
+@pytest.mark.parametrize("n,d,n_comp", [(400, 12, 2), (300, 7, 3), (50, 3, 3), (2000, 40, 5)])
+def test_pca_layout_equals_numpy_svd(n, d, n_comp):
+    rng = np.random.default_rng(n + d)
+    scales = np.linspace(3.0, 0.2, d)  # distinct variances: a unique basis
+    x = (rng.normal(size=(n, d)) * scales @ np.linalg.qr(rng.normal(size=(d, d)))[0] + 5.0)
+    x = x.astype(np.float32)
+    got = orc.pca_layout(x, n_comp).astype(np.float64)
+    want = _numpy_pca(x, n_comp)
+    assert np.max(np.abs(got - want)) < 1e-5
+
+
+def test_pca_layout_rank_one_data_recovers_the_line():
+    # closed form: points a_r u + m on a line have one component, a_r scaled to
+    # unit std (up to the sign fixed by u's largest entry)
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=300)
+    u = np.array([0.2, -0.9, 0.3, 0.25])
+    u /= np.linalg.norm(u)
+    x = (a[:, None] * u + np.array([1.0, 2.0, -3.0, 0.5])).astype(np.float32)
+    got = orc.pca_layout(x, 1).astype(np.float64)[:, 0]
+    want = (a - a.mean()) / a.std() * -1.0  # u's largest-magnitude entry is negative
+    assert np.max(np.abs(got - want)) < 1e-4
+
+
+def test_pca_layout_rejects_bad_shapes():
+    x = np.ones((5, 3), np.float32)
+    for nc in (0, 4):
+        with pytest.raises(RuntimeError):
+            orc.pca_layout(x, nc)
