@@ -275,10 +275,14 @@ __device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __
 // the threshold can only belong to a larger row and is not queued.  Pruned
 // sweep (PR = true): stages come in cell order, so an equal score is queued
 // too and the merge, which keeps the S smallest (score, row) keys, decides the
-// tie by row; keys carry the caller's row (rowmap).
+// tie by row; keys carry the caller's row (rowmap).  The query's own row
+// never reaches here (the kernel sets its score to -inf first), and in the
+// pruned sweep the threshold never drops below -FLT_MAX, so padding rows
+// (-inf) fail the test like every row below the threshold: one compare per
+// score decides.
 template <int E, int QC, bool PR>
-__device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h, int col,
-                                int lane, float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
+__device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, int lane,
+                                float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
                                 uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap,
                                 float floor) {
     float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
@@ -288,14 +292,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h,
     const float m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
     auto reach = [&](float v) { return PR ? v >= thr : v > thr; };
     if (__ballot(reach(m)) == 0ull) return;
-    // The query's own row scores highest and may pass the test above; it is
-    // never queued.  Excluding it here, on the rare path, keeps the common
-    // path free of per-element index compares.  Padding rows score -inf.
-    const uint32_t self = q - rowbase - (uint32_t)h * 4u;  // q's offset among this lane's rows
-    auto take = [&](int r) {
-        return reach(acc[r]) && (!PR || acc[r] > -INFINITY) &&
-               (uint32_t)((r >> 2) * 8 + (r & 3)) != self;
-    };
+    auto take = [&](int r) { return reach(acc[r]); };
     uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) c += take(r) ? 1u : 0u;
@@ -326,34 +323,38 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, uint32_t q, int h,
     }
 }
 
-// Global -> registers: this thread's 16-byte chunks of an RT-row stage (rows
-// are contiguous in P) and, for tid < RT, the row's -0.5|y|^2.
-template <int CPT, int RT, int W>
-__device__ inline void stage_load(u32x4 (&r)[CPT], float& rn, const unsigned char* __restrict__ P,
-                                  const float* __restrict__ negh, uint32_t row0, int row_elems_bytes,
-                                  int chunks, int tid) {
-    const u32x4* base = reinterpret_cast<const u32x4*>(P + (size_t)row0 * row_elems_bytes);
+// Score of the lane's query row to -inf in the tile that holds the wave's
+// query rows: element r is row (r/4)*8 + h*4 + r%4 of the tile, the query's
+// row is col.
+__device__ inline void mask_self(f32x16& acc, int h, int col) {
 #pragma unroll
-    for (int j = 0; j < CPT; j++) {
-        const int c = tid + j * W * 64;
-        if (c < chunks) r[j] = base[c];
-    }
-    if (tid < RT) rn = negh[row0 + tid];
+    for (int r = 0; r < 16; r++)
+        if ((r >> 2) * 8 + h * 4 + (r & 3) == col) acc[r] = -INFINITY;
 }
 
-// Registers -> LDS stage buffer (rows padded to row_bytes against bank conflicts).
-template <int CPT, int RT, int W>
-__device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned char* tile,
-                                   float* tile_negh, int row_bytes, int cpr, int chunks, int tid) {
+// One RT-row stage copied from global memory straight into its LDS buffer
+// (global_load_lds: each wave instruction writes 64 consecutive 16-byte LDS
+// slots, lane i at slot i), with no registers in between.  The buffer keeps
+// rows padded to row_bytes against bank conflicts, so slot p of the stage is
+// chunk p % spr of row p / spr (spr = row_bytes / 16 slots per row); the
+// padding slot of a row receives a second copy of the row's chunk 0 and is
+// never read.  goff[j] is this lane's source offset within the stage for its
+// copy j (slot wave*64 + j*W*64 + lane), -1 past the stage.  The stage's RT
+// values of -0.5|y|^2 go to tile_negh the same way, 4 bytes per lane.
+#define MRS_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+template <int NCOPY, int RT, int W>
+__device__ inline void stage_copy(const unsigned char* __restrict__ P, const float* __restrict__ negh,
+                                  uint32_t row0, int row_elems_bytes, const int* goff,
+                                  unsigned char* tile, float* tile_negh, int wave, int lane) {
+    const unsigned char* src = P + (size_t)row0 * row_elems_bytes;
 #pragma unroll
-    for (int j = 0; j < CPT; j++) {
-        const int c = tid + j * W * 64;
-        if (c < chunks) {
-            const int row = c / cpr, o = c - row * cpr;
-            *reinterpret_cast<u32x4*>(tile + row * row_bytes + o * 16) = r[j];
-        }
+    for (int j = 0; j < NCOPY; j++) {
+        const int base = wave * 64 + j * W * 64;  // wave-uniform
+        if (goff[j] >= 0) __builtin_amdgcn_global_load_lds(src + goff[j], MRS_LDS(tile + base * 16), 16, 0, 0);
     }
-    if (tid < RT) tile_negh[tid] = rn;
+    for (int base = wave * 64; base < RT; base += W * 64)  // wave-uniform
+        if (base + lane < RT)
+            __builtin_amdgcn_global_load_lds(negh + row0 + base + lane, MRS_LDS(tile_negh + base), 4, 0, 0);
 }
 
 // The distance GEMM + top-S filter.  One workgroup = W waves x 32 queries; the
@@ -369,7 +370,9 @@ __device__ inline void stage_store(const u32x4 (&r)[CPT], float rn, unsigned cha
 template <bool BF16, int E, int RT, int QC, int W, int STEPS, bool PR>
 __global__ __launch_bounds__(W * 64)
 __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) void k_shortlist(ShortlistArgs a) {
-    constexpr int CPT = (kStageChunks * kWavesMax) / W;
+    // copies per thread and stage: the stage's 16-byte chunks (at most
+    // kStageChunks * kWavesMax * 64) plus one padding slot per row
+    constexpr int NCOPY = (kStageChunks * kWavesMax * 64 + RT + W * 64 - 1) / (W * 64);
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int S = 64 * E;
     constexpr int elem = BF16 ? 2 : 4;
@@ -422,15 +425,23 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     }
     // a pass that continues: the thresholds left by the earlier pass (an aux
     // item starts from the floor, its lists being empty)
-    const float floor = (PR && a.floor) ? a.floor[q] : -INFINITY;
-    float thr = -INFINITY;
-    if (PR && a.cont) thr = aux ? floor : key_score(lists[(size_t)col * S + S - 1]);
+    // pruned sweep: thresholds never below -FLT_MAX (see epilogue)
+    const float floor = PR ? fmaxf(a.floor ? a.floor[q] : -INFINITY, -FLT_MAX) : -INFINITY;
+    float thr = floor;
+    if (PR && a.cont && !aux) thr = fmaxf(key_score(lists[(size_t)col * S + S - 1]), floor);
     uint32_t qn = 0;
 
     const unsigned char* P = reinterpret_cast<const unsigned char*>(a.P);
     const int row_elems_bytes = KP * elem;
     const int cpr = row_elems_bytes / 16;  // 16-byte chunks per row
-    const int chunks = RT * cpr;
+    const int spr = cpr + 1;               // LDS slots per padded row
+    int goff[NCOPY];
+#pragma unroll
+    for (int j = 0; j < NCOPY; j++) {
+        const int slot = wave * 64 + j * W * 64 + lane;
+        const int row = slot / spr, c = slot - row * spr;
+        goff[j] = slot < RT * spr ? row * row_elems_bytes + (c < cpr ? c : 0) * 16 : -1;
+    }
     // the stages this workgroup sweeps: the split's consecutive stages, or
     // (pruned sweep) its list
     uint32_t nst = (db1 - db0) / RT;
@@ -440,30 +451,21 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
         sid = a.stage_ids + item.y;
     }
     auto srow = [&](uint32_t t) -> uint32_t { return PR ? sid[t] * (uint32_t)RT : db0 + t * RT; };
-    u32x4 R[CPT];
-    float Rn = 0.0f;
-    if (nst > 0) {
-        stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(0), row_elems_bytes, chunks, tid);
-        stage_store<CPT, RT, W>(R, Rn, tile0, negh0, row_bytes, cpr, chunks, tid);
-    }
-    // everything loaded so far (query fragments, stage 0) is complete here, so
-    // inside the loop only the one-stage-ahead prefetch is ever in flight and
-    // the MFMAs do not wait on it
+    if (nst > 0) stage_copy<NCOPY, RT, W>(P, a.negh, srow(0), row_elems_bytes, goff, tile0, negh0, wave, lane);
+    // everything loaded so far (query fragments, stage 0) is complete here
     __builtin_amdgcn_s_waitcnt(0);
-    if (nst > 1) stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(1), row_elems_bytes, chunks, tid);
     __syncthreads();
 
-    for (uint32_t st = 0; st < nst; st++) {
-        const bool odd = st & 1u;
-        const unsigned char* cur = odd ? tile1 : tile0;
-        const float* curn = odd ? negh1 : negh0;
-        // stage st+1 -> the other buffer (last read in stage st-1, before the
-        // barrier that closed it), then fetch stage st+2 into registers
+    // One stage: stage st+1 goes straight into the other buffer (last read in
+    // stage st-1, before the barrier that closed it) and lands while stage st
+    // computes.  The buffers come in as distinct restrict pointers so that the
+    // compiler knows the LDS reads of this stage cannot see the copy in flight
+    // and does not wait for it before reading.
+    auto step = [&](const unsigned char* __restrict__ cur, const float* __restrict__ curn,
+                    unsigned char* __restrict__ nxt, float* __restrict__ nxtn, uint32_t st) {
         if (st + 1 < nst)
-            stage_store<CPT, RT, W>(R, Rn, odd ? tile0 : tile1, odd ? negh0 : negh1, row_bytes, cpr,
-                                    chunks, tid);
-        if (st + 2 < nst)
-            stage_load<CPT, RT, W>(R, Rn, P, a.negh, srow(st + 2), row_elems_bytes, chunks, tid);
+            stage_copy<NCOPY, RT, W>(P, a.negh, srow(st + 1), row_elems_bytes, goff, nxt, nxtn,
+                                     wave, lane);
         const uint32_t r0 = srow(st);
 #pragma unroll
         for (int sp = 0; sp < RT / 32; sp += 2) {
@@ -529,13 +531,23 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
             }
-            epilogue<E, QC, PR>(acc0, r0 + sp * 32, q, h, col, lane, thr, qn, lists, myq, a.rowmap,
+            // the query's own row: only the tile at the wave's query rows holds
+            // it (a wave-uniform test), at offset col of the lane's query
+            if (r0 + sp * 32 == qbase) mask_self(acc0, h, col);
+            if (kPair && r0 + (sp + 1) * 32 == qbase) mask_self(acc1, h, col);
+            epilogue<E, QC, PR>(acc0, r0 + sp * 32, h, col, lane, thr, qn, lists, myq, a.rowmap,
                                 floor);
             if (kPair)
-                epilogue<E, QC, PR>(acc1, r0 + (sp + 1) * 32, q, h, col, lane, thr, qn, lists, myq,
+                epilogue<E, QC, PR>(acc1, r0 + (sp + 1) * 32, h, col, lane, thr, qn, lists, myq,
                                     a.rowmap, floor);
         }
+        // stage st+1's copies have landed in LDS before any wave passes the barrier
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+    };
+    for (uint32_t st = 0; st < nst; st++) {
+        if (st & 1u) step(tile1, negh1, tile0, negh0, st);
+        else step(tile0, negh0, tile1, negh1, st);
     }
     // final flush of every non-empty queue
     uint64_t need = __ballot(qn > 0) & 0xFFFFFFFFull;
