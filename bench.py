@@ -550,6 +550,48 @@ def bench_fp32(args):
     )
 
 
+def bench_knn(args):
+    """The exhaustive kNN of the precision configuration alone: 1M x 64 (50
+    Gaussians on an 8-D subspace), k = 15, BF16 MFMA shortlist of 64 with the
+    exact FP32 re-rank, data in HBM; --knn-full-sweep sweeps every database
+    stage (the brute force) instead of the pruned sweep.  Reports the search
+    time and the shortlist kernel's MFMA rate (the FLOPs of the stages swept)."""
+    import torch
+
+    import mrs_amd as M
+    import synthetic
+
+    n, d, k = args.n, 64, 15
+    x, _ = synthetic.subspace(n=n, d=d, n_clusters=50, seed=42)
+    xd = torch.from_numpy(x).cuda()
+    ctx = M.Context(0)
+    ctx.set_knn_pruning(not args.knn_full_sweep)
+    for _ in range(max(args.warmup, 1)):  # the first call of a shape also autotunes
+        M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 4 * k)
+    torch.cuda.synchronize()
+    times, kts = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, kt = M.knn_exhaustive(ctx, xd, k, M.KNN_BF16_RERANK, 4 * k, timed=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        kts.append(kt)
+    sl = float(np.mean([kt["shortlist_ms"] for kt in kts]))
+    flops = 2.0 * n * n * d * kts[-1]["swept"]
+    tflops = flops / (sl * 1e-3) / 1e12
+    return dict(metric="exhaustive kNN seconds @1M x 64 k=15 (BF16 shortlist + FP32 re-rank)",
+                value=float(np.mean(times)), unit="s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=1e3 * float(np.mean(times)),
+                higher_is_better=False, scaling="weak", vs_baseline=None, dtype="bf16-mfma/f32",
+                data="synthetic",
+                config=dict(workload=f"kNN {n} x {d}, k=15, "
+                                     f"{'full' if args.knn_full_sweep else 'pruned'} sweep"),
+                roofline=dict(bound="mfma", kernel="k_shortlist<bf16>", achieved=tflops,
+                              peak=2500.0, unit="TFLOP/s", frac=tflops / 2500.0, traffic=None,
+                              avg_launch_ms=sl, swept=kts[-1]["swept"]),
+                timings=kts[-1])
+
+
 def bench_plain(args):
     """The reference's CPU-runnable case beside the GPU fit: n=10,000, d=50 (10
     isotropic Gaussians, centres ~N(0, 25 I)), exhaustive kNN k=15, spectral
@@ -904,6 +946,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="serve: GPU_MAX_HW_QUEUES for the run (0 = leave as is; at most 32)")
     ap.add_argument("--max-active", type=int, default=256)
+    ap.add_argument("--knn-full-sweep", action="store_true",
+                    help="knn: sweep every database stage (no pruning)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a device buffer of this many MiB (a plain copy kernel of "
                          "known size, to scale hardware byte counters)")
@@ -927,6 +971,8 @@ def main():
         res = bench_densmap(args)
     elif args.workload == "plain":
         res = bench_plain(args)
+    elif args.workload == "knn":
+        res = bench_knn(args)
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if "autotune" not in res:
