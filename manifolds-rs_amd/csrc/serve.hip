@@ -106,15 +106,20 @@ __device__ inline int fit_of_block(const FitDesc* __restrict__ fits, int n_fits,
 }
 
 // One epoch of every fit of the launch: block b takes slot b / 8 of bin b % 8,
-// i.e. kWg consecutive edges of one fit, as four wavefronts of 64-edge spans
-// with the firing edges compacted (epoch_edge_span, the single-fit kernel's
-// code).  Slots past the end of their bin exit at once (the whole block).
+// i.e. kSegBlockEdges consecutive edges of one fit, as four wavefronts of
+// kSegSpan-edge spans with the firing edges compacted (epoch_edge_span, the
+// single-fit kernel's code).  About one edge in five fires per epoch, so a
+// 64-edge span left a wavefront ~13 firing edges for 64 lanes and the kernel
+// VALU-bound (75% VALU busy on the served stream); 256-edge spans fill the
+// lanes.  Slots past the end of their bin exit at once (the whole block).
+constexpr int kSegSpan = 256;
+constexpr uint32_t kSegBlockEdges = (kWg / 64) * kSegSpan;
 template <int DIM>
 __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restrict__ fits,
                                                          const Piece* __restrict__ pieces,
                                                          Bins bins) {
     constexpr int kWaves = kWg / 64;
-    __shared__ uint32_t queue[kWaves][64];
+    __shared__ uint32_t queue[kWaves][kSegSpan];
     const uint32_t x = blockIdx.x % kXcds, t = blockIdx.x / kXcds;
     uint32_t lo = bins.first[x], hi = bins.first[x + 1];
     if (lo == hi) return;
@@ -127,8 +132,8 @@ __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restri
     if (t < pc.slot0 || t - pc.slot0 >= pc.count) return;
     const FitDesc& f = fits[pc.fit];
     const int wave = threadIdx.x >> 6;
-    const uint64_t base = ((uint64_t)(pc.blk0 + (t - pc.slot0)) * kWaves + wave) * 64;
-    epoch_edge_span<DIM, false, 64>(base, queue[wave], f.edges, f.per, f.next, f.embd, f.grad,
+    const uint64_t base = ((uint64_t)(pc.blk0 + (t - pc.slot0)) * kWaves + wave) * kSegSpan;
+    epoch_edge_span<DIM, false, kSegSpan>(base, queue[wave], f.edges, f.per, f.next, f.embd, f.grad,
                                     f.n_edges, f.epoch, f.epoch_f, f.seed, f.fm, f.a, f.b,
                                     f.two_a_b, f.two_gamma_b, f.neg_rate, nullptr, nullptr,
                                     nullptr);
@@ -380,7 +385,7 @@ void batcher_loop(mrs_server* sv) {
                     FitDesc& f = hd[nd++];
                     fill_desc(f, st, r->epoch, (uint32_t)r->seed);
                     f.blk_node0 = g.blocks_n;
-                    g.blocks_e += div_up(st->n_edges, kWg);
+                    g.blocks_e += div_up(st->n_edges, kSegBlockEdges);
                     g.blocks_n += div_up(st->n, kWg);
                 }
                 g.count = nd - g.first;
@@ -396,7 +401,7 @@ void batcher_loop(mrs_server* sv) {
                     uint32_t x = 0, filled = 0, cnt = 0;
                     g.bins.first[0] = 0;
                     for (int q = 0; q < g.count; q++) {
-                        uint32_t left = div_up(hd[g.first + q].n_edges, kWg), blk0 = 0;
+                        uint32_t left = div_up(hd[g.first + q].n_edges, kSegBlockEdges), blk0 = 0;
                         while (left) {
                             const uint32_t take = std::min(left, g.slots - filled);
                             hp[np + cnt++] = Piece{(uint32_t)q, blk0, filled, take};
