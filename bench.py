@@ -72,23 +72,49 @@ def pmc_traffic(workload: str, kernel_prefix: str):
 
 
 def request_floor(ctx, n: int, firing_per_launch: float, neg_rate: int, dim: int,
-                  avg_launch_us: float):
-    """The epoch kernel's bound in random requests.  Per firing edge it makes
-    y_j and 2*neg_rate negative gathers, the larger endpoint's `dim` FP32 atomics
-    (a node with no update keeps its -0.0 markers, so there is no flag store), and
-    about one request for the smaller endpoint's run-summed atomics (y_i is an L2 hit: edges are sorted by i).  Each of these touches a
-    random line of an n-row table that does not fit one XCD's L2, so the
-    kernel can go no faster than that many requests at the measured rate of
-    uniform random 8-byte gathers from an n-row table."""
-    rate = ctx.probe_random_gather(n, 1 << 24, 10)
-    per_edge = 1 + 2 * neg_rate + dim + 1
-    requests = firing_per_launch * per_edge
-    floor_us = requests / rate * 1e6
-    return dict(probe="uniform random float2 gathers, n-row table, 2^24 per launch, "
-                      "8 in flight per thread (mrs_probe_random_gather)",
-                rate_g_per_s=rate / 1e9, requests_per_firing_edge=per_edge,
-                requests_per_launch=requests, floor_us=floor_us,
+                  avg_launch_us: float, workload: str, kernel_prefix: str):
+    """The epoch kernel's bound in random requests, of two kinds measured apart
+    on the same n-row table: gathers (y_j and 2*neg_rate negatives per firing
+    edge; y_i is an L2 hit, the edges being sorted by i) at the rate of
+    uniform random 8-byte gathers, and FP32 atomic adds (the larger endpoint's
+    `dim` per firing edge plus one per run of the smaller endpoint; counted
+    per launch by rocprofv3, TCP_TCC_ATOMIC_WITHOUT_RET_REQ, in the profile
+    summary) at the rate of uniform random atomic adds.  The two kinds overlap
+    in the kernel, so the floor is the larger of the two times."""
+    rate_g = ctx.probe_random_gather(n, 1 << 24, 10)
+    rate_a = ctx.probe_random_atomic(n, 1 << 24, 10)
+    gathers = firing_per_launch * (1 + 2 * neg_rate)
+    atom, src = pmc_field(workload, kernel_prefix, "atomic_req_per_launch")
+    atom_src = src
+    if atom is None:  # no counter pass: the larger endpoint's atomics only
+        atom, atom_src = firing_per_launch * dim, "model: dim per firing edge (no counter pass)"
+    t_g, t_a = gathers / rate_g * 1e6, atom / rate_a * 1e6
+    floor_us = max(t_g, t_a)
+    return dict(probe="uniform random float2 gathers and FP32 atomic adds on an n-row table, "
+                      "2^24 per launch, 8 per thread (mrs_probe_random_gather / _atomic)",
+                gather_rate_g_per_s=rate_g / 1e9, atomic_rate_g_per_s=rate_a / 1e9,
+                gathers_per_firing_edge=1 + 2 * neg_rate, gathers_per_launch=gathers,
+                atomics_per_launch=atom, atomics_source=atom_src,
+                gather_floor_us=t_g, atomic_floor_us=t_a, floor_us=floor_us,
+                bound="atomics" if t_a >= t_g else "gathers",
                 achieved_us=avg_launch_us, frac=floor_us / avg_launch_us)
+
+
+def pmc_field(workload: str, kernel_prefix: str, field: str):
+    """A per-launch counter figure of a kernel from the profile summaries'
+    JSON blocks (see pmc_traffic); None if absent."""
+    import re
+
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for block in re.findall(r"```json\n(.*?)```", f.read_text(), re.S):
+            try:
+                entries = json.loads(block)
+            except ValueError:
+                continue
+            for key, e in entries.items():
+                if key.startswith(f"{workload}:{kernel_prefix}") and field in e:
+                    return e[field], f"profiles/{f.name}"
+    return None, None
 
 
 def synthetic_knn_graph(n: int, k: int, n_epochs: int, seed: int = 42):
@@ -199,7 +225,8 @@ def bench_epochs(args):
                       traffic=traffic, traffic_source=traffic_src,
                       avg_launch_us=edge_avg_ms * 1e3,
                       algorithmic_bytes_per_launch=bytes_per_launch),
-        request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2, edge_avg_ms * 1e3),
+        request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2, edge_avg_ms * 1e3,
+                                    "epochs", "k_epoch_edges<2"),
         wall_s=wall,
     )
     if not args.no_cpu:
@@ -355,7 +382,7 @@ def bench_fit(args):
                       avg_launch_us=1e3 * edge_ms / edge_launches,
                       algorithmic_bytes_per_launch=bytes_edge),
         request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2,
-                                    1e3 * edge_ms / edge_launches),
+                                    1e3 * edge_ms / edge_launches, "fit", "k_epoch_edges<2"),
         knn_kernel=dict(bound="mfma", kernel="k_shortlist<bf16> (pruned sweep, both passes)",
                         achieved=achieved, peak=2500.0, unit="TFLOP/s", frac=achieved / 2500.0,
                         traffic=None, traffic_per_dispatch_mean=sl_traffic,

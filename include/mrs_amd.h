@@ -131,6 +131,11 @@ int mrs_ctx_set_knn_work_chunk(mrs_ctx *ctx, int32_t stages);
  * in the reference. */
 int mrs_probe_random_gather(mrs_ctx *ctx, int64_t rows, int64_t gathers, int32_t reps,
                             double *per_s);
+/* The same for FP32 atomic adds without return (the epoch kernel's gradient
+ * scatter) to the x of uniform random rows of the table: ops per second.
+ * No counterpart in the reference. */
+int mrs_probe_random_atomic(mrs_ctx *ctx, int64_t rows, int64_t ops, int32_t reps,
+                            double *per_s);
 
 /* parse_umap_optimiser_gpu, src/training/umap_optimiser_gpu.rs:44-52.
  * 0 adam_gpu, 1 adam_parallel, 2 adam, 3 sgd; -1 if unknown. */

@@ -220,6 +220,24 @@ __global__ __launch_bounds__(256) void k_probe_gather(const float2* __restrict__
     out[t] = acc;
 }
 
+// The same pattern as FP32 atomic adds without return (the epoch kernel's
+// gradient scatter): kProbeU per thread to the x of uniform pseudo-random rows.
+__global__ __launch_bounds__(256) void k_probe_atomic(float2* __restrict__ table, uint32_t rows,
+                                                      uint64_t threads, uint32_t salt) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= threads) return;
+#pragma unroll
+    for (int u = 0; u < kProbeU; u++) {
+        uint32_t h = (uint32_t)(t * kProbeU + u) * 0x9E3779B1u ^ salt * 0x85EBCA77u;
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        h *= 0x846CA68Bu;
+        h ^= h >> 16;
+        unsafeAtomicAdd(&table[(uint32_t)(((uint64_t)h * rows) >> 32)].x, 1e-9f);
+    }
+}
+
 // Stable LSD radix sort of u32 keys (8-bit digits) carrying u32 values.  A
 // workgroup of kSortWg threads owns a tile of kSortTile consecutive keys.
 constexpr int kSortWg = 256;
@@ -467,8 +485,12 @@ int mrs_ctx_release_cache(mrs_ctx* ctx) {
     });
 }
 
-int mrs_probe_random_gather(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t reps,
-                            double* per_s) {
+namespace {
+
+// ops per second of `reps` launches of the gather (atomic = false) or the
+// atomic probe, after two warm-up launches
+int probe_random(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t reps, double* per_s,
+                 bool atomic) {
     return mrs::guarded([&] {
         if (!ctx || !per_s) throw mrs::Error(MRS_INVALID_ARGUMENT, "NULL argument");
         if (rows < 1 || rows > (int64_t)UINT32_MAX)
@@ -482,18 +504,22 @@ int mrs_probe_random_gather(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t
         mrs::DevBuf<float2> table((size_t)rows), out((size_t)threads);
         MRS_HIP(hipMemsetAsync(table.p, 0, table.bytes(), s));
         const dim3 grid((unsigned)((threads + 255) / 256));
-        for (int w = 0; w < 2; w++)
-            hipLaunchKernelGGL(mrs::k_probe_gather, grid, dim3(256), 0, s, table.p, (uint32_t)rows,
-                               threads, (uint32_t)w, out.p);
+        auto launch = [&](uint32_t salt) {
+            if (atomic)
+                hipLaunchKernelGGL(mrs::k_probe_atomic, grid, dim3(256), 0, s, table.p,
+                                   (uint32_t)rows, threads, salt);
+            else
+                hipLaunchKernelGGL(mrs::k_probe_gather, grid, dim3(256), 0, s, table.p,
+                                   (uint32_t)rows, threads, salt, out.p);
+        };
+        for (int w = 0; w < 2; w++) launch((uint32_t)w);
         MRS_CHECK_LAUNCH();
         hipEvent_t e0 = nullptr, e1 = nullptr;
         MRS_HIP(hipEventCreate(&e0));
         MRS_HIP(hipEventCreate(&e1));
         float ms = 0.0f;
         hipError_t err = hipEventRecord(e0, s);
-        for (int r = 0; r < reps && err == hipSuccess; r++)
-            hipLaunchKernelGGL(mrs::k_probe_gather, grid, dim3(256), 0, s, table.p,
-                               (uint32_t)rows, threads, (uint32_t)(r + 2), out.p);
+        for (int r = 0; r < reps && err == hipSuccess; r++) launch((uint32_t)(r + 2));
         if (err == hipSuccess) err = hipGetLastError();
         if (err == hipSuccess) err = hipEventRecord(e1, s);
         if (err == hipSuccess) err = hipEventSynchronize(e1);
@@ -503,6 +529,18 @@ int mrs_probe_random_gather(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t
         MRS_HIP(err);
         *per_s = ms > 0.0f ? (double)threads * mrs::kProbeU * reps / (ms * 1e-3) : 0.0;
     });
+}
+
+}  // namespace
+
+int mrs_probe_random_gather(mrs_ctx* ctx, int64_t rows, int64_t gathers, int32_t reps,
+                            double* per_s) {
+    return probe_random(ctx, rows, gathers, reps, per_s, false);
+}
+
+int mrs_probe_random_atomic(mrs_ctx* ctx, int64_t rows, int64_t ops, int32_t reps,
+                            double* per_s) {
+    return probe_random(ctx, rows, ops, reps, per_s, true);
 }
 
 int mrs_ctx_set_knn_pruning(mrs_ctx* ctx, int32_t on) {

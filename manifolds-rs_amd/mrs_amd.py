@@ -179,6 +179,7 @@ def lib() -> C.CDLL:
         "mrs_umap_params_gpu_default_2d": (None, [C.c_float, C.c_float,
                                                   C.POINTER(UmapParamsGpu)]),
         "mrs_probe_random_gather": (C.c_int, [vp, i64, i64, i32, C.POINTER(C.c_double)]),
+        "mrs_probe_random_atomic": (C.c_int, [vp, i64, i64, i32, C.POINTER(C.c_double)]),
         "mrs_ctx_cached_bytes": (C.c_size_t, [vp]),
         "mrs_ctx_release_cache": (C.c_int, [vp]),
         "mrs_parse_umap_optimiser_gpu": (i32, [C.c_char_p]),
@@ -319,6 +320,14 @@ class Context:
         table (the epoch kernel's request pattern), measured on this context."""
         out = C.c_double(0.0)
         _check(lib().mrs_probe_random_gather(self.handle, int(rows), int(gathers), int(reps),
+                                             C.byref(out)))
+        return out.value
+
+    def probe_random_atomic(self, rows: int, ops: int = 1 << 24, reps: int = 10) -> float:
+        """FP32 atomic adds (no return) per second to uniform random rows of a
+        `rows`-row float2 table (the epoch kernel's gradient scatter)."""
+        out = C.c_double(0.0)
+        _check(lib().mrs_probe_random_atomic(self.handle, int(rows), int(ops), int(reps),
                                              C.byref(out)))
         return out.value
 

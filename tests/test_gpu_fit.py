@@ -441,6 +441,12 @@ def test_random_gather_probe(ctx):
         with pytest.raises(M.ManifoldsError) as e:
             ctx.probe_random_gather(rows, gathers, reps)
         assert e.value.status == M.MRS_INVALID_ARGUMENT
+    # the atomic scatter probe likewise
+    arate = ctx.probe_random_atomic(1 << 16, 1 << 20, 2)
+    assert 1e8 < arate < 1e13
+    with pytest.raises(M.ManifoldsError) as e:
+        ctx.probe_random_atomic(0, 1 << 20, 1)
+    assert e.value.status == M.MRS_INVALID_ARGUMENT
 
 
 def test_fit_same_with_and_without_knn_pruning(ctx):
