@@ -238,16 +238,21 @@ struct ShortlistArgs {
 };
 constexpr uint32_t kMainList = 0xFFFFFFFFu;
 
-// Wave-wide merge of query b's queue (c entries) into its sorted global list;
-// returns the list's new threshold key (its last entry).
+// Wave-wide merge of query b's queue into its sorted global list; returns the
+// list's new threshold key (its last entry).  The queue is two half-queues of
+// QC/2 entries, one per accumulator half (lanes b and b + 32), holding c0 and
+// c1 entries: lane l reads entry l % 32 of half l / 32.
 template <int E, int QC>
-__device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __restrict__ lists,
-                                    const uint64_t* __restrict__ myq) {
+__device__ inline uint64_t merge_query(int b, uint32_t c0, uint32_t c1, int lane,
+                                       uint64_t* __restrict__ lists,
+                                       const uint64_t* __restrict__ myq) {
+    static_assert(QC == 64, "two half-queues of 32: one entry per lane");
     constexpr int S = 64 * E;
     // the queue entries were stored by this wave's lanes: complete those
     // stores before any lane reads another lane's entry
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    uint64_t one[1] = {(uint32_t)lane < c ? myq[b * QC + lane] : kInfKey};
+    const uint32_t c = lane < 32 ? c0 : c1;
+    uint64_t one[1] = {(uint32_t)(lane & 31) < c ? myq[b * QC + lane] : kInfKey};
     // the list load is issued before the queue sort, so its latency overlaps
     // the sort instead of following it (each lane reads back only the list
     // entries it wrote itself)
@@ -266,11 +271,12 @@ __device__ inline uint64_t merge_query(int b, uint32_t c, int lane, uint64_t* __
 
 // Epilogue of one 32-row MFMA tile: the lane's 16 scores (one query, rows
 // rowbase + (r/4)*8 + h*4 + r%4) against the query's threshold thr (the score
-// of its list's last entry, never below floor).  A query's queue count qn and threshold live in
-// registers, identical in its two lanes (h = 0, 1); the halves learn each
-// other's insert count with one lane swap, so queueing costs no LDS round
-// trip.  Most tiles leave on the max/ballot test.  A queue that could overflow
-// on the next tile is merged.
+// of its list's last entry, never below floor).  Each of a query's two lanes
+// (h = 0, 1) appends to its own half-queue with its own count qn (a register),
+// so an insertion needs no count of the tile's candidates and no exchange
+// between the lanes; the threshold is identical in both.  Most tiles leave on
+// the max/ballot test.  A query with a half-queue that could overflow on the
+// next tile (more than QC/2 - 16 entries) is merged.
 //   Full sweep (PR = false): the stages come in row order, so a score equal to
 // the threshold can only belong to a larger row and is not queued.  Pruned
 // sweep (PR = true): stages come in cell order, so an equal score is queued
@@ -293,13 +299,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, in
     auto reach = [&](float v) { return PR ? v >= thr : v > thr; };
     if (__ballot(reach(m)) == 0ull) return;
     auto take = [&](int r) { return reach(acc[r]); };
-    uint32_t c = 0;
-#pragma unroll
-    for (int r = 0; r < 16; r++) c += take(r) ? 1u : 0u;
-    const uint32_t co = __shfl_xor(c, 32, 64);
-    uint32_t pos = qn + (h ? co : 0u);
-    qn += c + co;
-    uint64_t* dst = myq + col * QC;
+    uint64_t* dst = myq + col * QC + h * (QC / 2);
     // a group no element of which reaches the threshold is skipped as a whole
 #pragma unroll
     for (int g = 0; g < 4; g++)
@@ -308,14 +308,17 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, in
             for (int j = 0; j < 4; j++)
                 if (take(g * 4 + j)) {
                     const uint32_t row = rowbase + g * 8 + h * 4 + j;
-                    dst[pos++] = score_key(acc[g * 4 + j], PR ? rowmap[row] : row);
+                    dst[qn++] = score_key(acc[g * 4 + j], PR ? rowmap[row] : row);
                 }
         }
-    uint64_t need = __ballot(qn > (uint32_t)(QC - 32)) & 0xFFFFFFFFull;  // lanes 0..31: one per query
+    const uint64_t full = __ballot(qn > (uint32_t)(QC / 2 - 16));
+    uint64_t need = (full | (full >> 32)) & 0xFFFFFFFFull;  // one bit per query
     while (need) {
         const int b = __builtin_ctzll(need);
         need &= need - 1;
-        const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
+        const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b),
+                                              __builtin_amdgcn_readlane(qn, b + 32), lane, lists,
+                                              myq);
         if (col == b) {
             qn = 0;
             thr = fmaxf(key_score(t), floor);  // an aux list never goes below its floor
@@ -550,11 +553,13 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
         else step(tile0, negh0, tile1, negh1, st);
     }
     // final flush of every non-empty queue
-    uint64_t need = __ballot(qn > 0) & 0xFFFFFFFFull;
+    const uint64_t left = __ballot(qn > 0);
+    uint64_t need = (left | (left >> 32)) & 0xFFFFFFFFull;
     while (need) {
         const int b = __builtin_ctzll(need);
         need &= need - 1;
-        merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), lane, lists, myq);
+        merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), __builtin_amdgcn_readlane(qn, b + 32),
+                           lane, lists, myq);
     }
 }
 
