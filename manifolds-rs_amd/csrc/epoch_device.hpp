@@ -51,6 +51,39 @@ __device__ inline bool seg_reduce(uint32_t key, float (&v)[D]) {
     return head;
 }
 
+// grad[node] += v by FP32 atomics for every lane with on == true, all of a
+// node's coordinates in the same wave instruction.  FP32 atomics execute
+// beyond the L2 (about 21 G requests/s on this GPU whatever the scope or the
+// table size), and the atomics of one instruction that fall in one cache line
+// leave as one request; a node's coordinates share a line, so issuing them
+// together costs one request per node instead of DIM.  Lanes work in groups
+// of P >= DIM: in round r, lane L adds coordinate L % P of the node held by
+// lane r * (64 / P) + L / P.  Every lane of the wavefront must call this.
+template <int DIM>
+__device__ inline void atomic_add_rows(float* __restrict__ grad, uint32_t node, bool on,
+                                       const float (&v)[DIM]) {
+    constexpr int P = DIM == 1 ? 1 : DIM == 2 ? 2 : DIM <= 4 ? 4 : 8;
+    static_assert(DIM <= P && P <= 8, "at most 8 coordinates per node");
+    constexpr int PER = 64 / P;
+    const int lane = threadIdx.x & 63;
+    const int c = lane % P;
+    const uint64_t busy = __ballot(on);
+#pragma unroll
+    for (int r = 0; r < P; r++) {
+        if (((busy >> (r * PER)) & (PER == 64 ? ~0ull : ((1ull << PER) - 1ull))) == 0ull) continue;
+        const int src = r * PER + lane / P;
+        const uint32_t nd = __shfl(node, src, 64);
+        const bool o = (busy >> src) & 1ull;
+        float val = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float t = __shfl(v[d], src, 64);
+            if (d == c) val = t;
+        }
+        if (o && c < DIM) unsafeAtomicAdd(&grad[(size_t)nd * DIM + c], val);
+    }
+}
+
 // "No update this epoch" is marked in the accumulated gradient itself: the
 // accumulate-by-atomics buffer starts each epoch at -0.0 in every component.
 // Every contribution an edge adds is a sum that started at +0.0, and a sum of
@@ -115,94 +148,106 @@ __device__ inline void negatives(uint32_t node, uint32_t loc, const float (&y)[D
     }
 }
 
+// The attractive (and densMAP) term of edge e between y_i and y_j, added to
+// gi and subtracted from gj (umap_grad_accum's edge loop,
+// umap_optimiser_gpu.rs:740-800; the negation is exact).
+template <int DIM, bool DENS>
+__device__ __forceinline__ void attract(uint32_t i, uint32_t j, uint32_t e, const float (&yi)[DIM],
+                                        const float (&yj)[DIM], float a, float b, float two_a_b,
+                                        const float* __restrict__ mu,
+                                        const float4* __restrict__ dens_pack,
+                                        const float* __restrict__ dens_scalars, float (&gi)[DIM],
+                                        float (&gj)[DIM]) {
+    const float two = 2.0f, one = 1.0f;
+    float dist_sq = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        float diff = yi[d] - yj[d];
+        dist_sq += diff * diff;
+    }
+    if (dist_sq < kGradDistSqThreshold) return;
+    float dsb = pos_powf(dist_sq, b);
+    float denom = one + a * dsb;
+    float gc = two_a_b * dsb / (dist_sq * denom);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        float t = two * gc * (yj[d] - yi[d]);
+        gi[d] += t;
+        gj[d] -= t;
+    }
+    if (DENS) {
+        float phi = one / denom;
+        float dphi_term = a * b * dsb / dist_sq * phi;
+        float common = one - b * (one - phi);
+        // per node (w, 1 / phi_sum, 1 / exp(re)): one 16-byte gather per endpoint
+        const float4 pi = dens_pack[i], pj = dens_pack[j];
+        float dr_i = (phi * pi.y) * (common * pi.z + dphi_term);
+        float dr_j = (phi * pj.y) * (common * pj.z + dphi_term);
+        float cor = dens_scalars[3] * (pi.x * dr_i + pj.x * dr_j) / mu[e];
+        float two_cor = two * cor;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            float t = two * clampf(two_cor * (yi[d] - yj[d]), kGradClipVal);
+            gi[d] += t;
+            gj[d] -= t;
+        }
+    }
+}
+
 // The work of one firing undirected edge (i < j) of one Adam epoch: gathers
 // y_i, y_j, computes the attractive (and densMAP) term once for both
 // endpoints, then j's negatives, whose sum goes to grad[j] by FP32 atomics
 // at once (while y_j's line is still in L2), then i's negatives; i's total
 // goes out after a segmented sum over the wavefront (edges arrive sorted by
-// i).  Every lane of the wavefront calls it (the reduction is
-// wavefront-wide); lanes with active == false only join the reduction.  DIM
-// is the compile-time embedding dimension.
+// i).  Both sums leave through atomic_add_rows, one request per node.  Every
+// lane of the wavefront calls it (the reduction and the atomics are
+// wavefront-wide); lanes with active == false only join them.  DIM is the
+// compile-time embedding dimension.
 template <int DIM, bool DENS>
 __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* __restrict__ edges,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t epoch, uint32_t seed,
     FastMod fm, float a, float b, float two_a_b, float two_gamma_b, int neg_rate,
     const float* __restrict__ mu, const float4* __restrict__ dens_pack,
     const float* __restrict__ dens_scalars) {
-    const float two = 2.0f, one = 1.0f;
-    uint32_t key = 0xFFFFFFFFu;
-    float gi[DIM];
+    uint32_t key = 0xFFFFFFFFu, j = 0, slot_i = 0;
+    float gi[DIM], gj[DIM], yi[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) gi[d] = 0.0f;
+    for (int d = 0; d < DIM; d++) gi[d] = gj[d] = 0.0f;
     if (active) {
         const uint4 rec = edges[e];  // {i, j, CSR slot under i, under j}
-        const uint32_t i = rec.x, j = rec.y;
+        const uint32_t i = rec.x;
+        j = rec.y;
         key = i;
-        float yi[DIM], yj[DIM], gj[DIM];
+        slot_i = rec.z;
+        float yj[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             yi[d] = embd[(size_t)i * DIM + d];
             yj[d] = embd[(size_t)j * DIM + d];
-            gj[d] = 0.0f;
         }
-        float dist_sq = 0.0f;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            float diff = yi[d] - yj[d];
-            dist_sq += diff * diff;
-        }
-        if (dist_sq >= kGradDistSqThreshold) {
-            float dsb = pos_powf(dist_sq, b);
-            float denom = one + a * dsb;
-            float gc = two_a_b * dsb / (dist_sq * denom);
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                float t = two * gc * (yj[d] - yi[d]);
-                gi[d] += t;
-                gj[d] -= t;
-            }
-            if (DENS) {
-                float phi = one / denom;
-                float dphi_term = a * b * dsb / dist_sq * phi;
-                float common = one - b * (one - phi);
-                // per node (w, 1 / phi_sum, 1 / exp(re)): one 16-byte gather per endpoint
-                const float4 pi = dens_pack[i], pj = dens_pack[j];
-                float dr_i = (phi * pi.y) * (common * pi.z + dphi_term);
-                float dr_j = (phi * pj.y) * (common * pj.z + dphi_term);
-                float cor = dens_scalars[3] * (pi.x * dr_i + pj.x * dr_j) / mu[e];
-                float two_cor = two * cor;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    float t = two * clampf(two_cor * (yi[d] - yj[d]), kGradClipVal);
-                    gi[d] += t;
-                    gj[d] -= t;
-                }
-            }
-        }
+        attract<DIM, DENS>(i, j, e, yi, yj, a, b, two_a_b, mu, dens_pack, dens_scalars, gi, gj);
         negatives<DIM>(j, rec.w, yj, embd, epoch, seed, fm, a, b, two_gamma_b, neg_rate, gj);
-#pragma unroll
-        for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)j * DIM + d], gj[d]);
-        negatives<DIM>(i, rec.z, yi, embd, epoch, seed, fm, a, b, two_gamma_b, neg_rate, gi);
     }
+    atomic_add_rows<DIM>(grad, j, active, gj);
+    if (active)
+        negatives<DIM>(key, slot_i, yi, embd, epoch, seed, fm, a, b, two_gamma_b, neg_rate, gi);
     // smaller endpoint: combine runs of equal i within the wavefront first
     const bool head = seg_reduce<DIM>(key, gi);
-    if (head && key != 0xFFFFFFFFu) {
-#pragma unroll
-        for (int d = 0; d < DIM; d++) unsafeAtomicAdd(&grad[(size_t)key * DIM + d], gi[d]);
-    }
+    atomic_add_rows<DIM>(grad, key, head && key != 0xFFFFFFFFu, gi);
 }
 
 // One wavefront's span of SPAN consecutive edges [base, base + SPAN) of one
 // Adam epoch.  About one edge in five fires in an epoch, scattered over the
 // edge list, so with one lane per edge nearly every wavefront would have a
 // firing lane and run the whole negative-sampling body for all 64 lanes.
-// Here the wavefront first advances the span's schedule cursors (coalesced)
-// and queues the firing edges in its LDS slice `queue`, in edge order (runs of
-// equal i stay together for the segmented sum), then works through the queue
-// 64 edges at a time.  The queue is built from ballots: no counter is shared
-// between wavefronts.  queue is this wavefront's LDS slice (SPAN entries).
-// Every wavefront of the workgroup must call this (it holds a workgroup
-// barrier between building and reading the queues).
+// Here the wavefront first advances the span's schedule cursors (coalesced;
+// every cursor and period of the span is loaded in one round trip) and queues
+// the firing edges in its LDS slice `queue`, in edge order (runs of equal i
+// stay together for the segmented sum), then works through the queue 64 edges
+// at a time.  The queue is built from ballots: no counter is shared between
+// wavefronts.  queue is this wavefront's LDS slice (SPAN entries).  Every
+// wavefront of the workgroup must call this (it holds a workgroup barrier
+// between building and reading the queues).
 template <int DIM, bool DENS, int SPAN>
 __device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restrict__ queue,
     const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
@@ -211,19 +256,26 @@ __device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restr
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
     static_assert(SPAN % 64 == 0, "whole wavefront rounds");
+    constexpr int R = SPAN / 64;
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t count = 0;  // wavefront-uniform
-    for (int r = 0; r < SPAN / 64; r++) {
+    float nx[R], pr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const uint64_t e = base + (uint64_t)r * 64 + lane;
-        bool fire = false;
+        nx[r] = INFINITY;  // past the end: never fires
+        pr[r] = 0.0f;
         if (e < n_edges) {
-            const float nx = next[e];
-            if (nx <= epoch_f) {
-                fire = true;
-                next[e] = nx + per[e];
-            }
+            nx[r] = next[e];
+            pr[r] = per[e];
         }
+    }
+    uint32_t count = 0;  // wavefront-uniform
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint64_t e = base + (uint64_t)r * 64 + lane;
+        const bool fire = nx[r] <= epoch_f;
+        if (fire) next[e] = nx[r] + pr[r];
         const uint64_t m = __ballot(fire);
         if (fire) queue[count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
         count += (uint32_t)__popcll(m);
