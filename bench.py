@@ -50,6 +50,18 @@ METRIC = "UMAP fit sec @1M×64 k=15 (recall@15=1.0 vs FP64); edge-updates/s; ser
 HBM_PEAK_GBS = 8000.0
 
 
+def edge_kernel_bytes(E: int, A, r: int, dens_A=0.0):
+    """Algorithmic bytes of the edge-parallel epoch kernel over launches with A
+    active edges in total (dens_A of them in density epochs): per launch one
+    bit per edge of the firing bitmask's plane (E / 8 B), and per active edge
+    its 16-byte record, the 2 x 8 B endpoint gathers, 2r x 8 B negative-sample
+    gathers and 2 x 8 B gradient scatters, plus in a density epoch the two
+    endpoints' 16-byte packed density factors and the 4-byte edge weight.
+    The bitmask planes themselves are written ahead, 128 epochs at a time, by
+    k_fire_bits (timed apart: `fire_bits`)."""
+    return E / 8.0 + A * (48.0 + 16.0 * r) + dens_A * 36.0
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -189,12 +201,10 @@ def bench_epochs(args):
     kt = st.kernel_times()
     ms_per_step = float(np.mean(times))
 
-    # algorithmic bytes of the edge kernel: 16 B/edge streamed (i, j, next, per)
-    # + per active edge 4 B schedule write, 2x8 B endpoint gathers, 2r x8 B
-    # negative gathers, 2x8 B gradient scatter
+    # algorithmic bytes of the edge kernel (edge_kernel_bytes)
     r = p.neg_sample_rate
     edge_ms, edge_launches = kt["epoch_edges"]
-    bytes_edge_total = 16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)
+    bytes_edge_total = edge_kernel_bytes(E, A_total, r) + (n_epochs - 1) * E / 8.0
     edge_avg_ms = edge_ms / edge_launches
     bytes_per_launch = bytes_edge_total / n_epochs
     achieved = bytes_per_launch / (edge_avg_ms * 1e-3) / 1e9
@@ -354,8 +364,9 @@ def bench_fit(args):
     ktimes = stt.kernel_times()
     edge_ms, edge_launches = ktimes["epoch_edges"]
     r = p.optim_params.neg_sample_rate
-    bytes_edge = (16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)) / n_epochs
+    bytes_edge = (edge_kernel_bytes(E, A_total, r) + (n_epochs - 1) * E / 8.0) / n_epochs
     epoch_gbs = bytes_edge / (edge_ms / edge_launches * 1e-3) / 1e9
+    fb_ms, fb_n = ktimes.get("fire_bits", (0.0, 0))
     sl_traffic, sl_src = pmc_traffic("fit", "k_shortlist<true")
     ep_traffic, ep_src = pmc_traffic("fit", "k_epoch_edges<2")
 
@@ -380,7 +391,8 @@ def bench_fit(args):
                       peak=HBM_PEAK_GBS, unit="GB/s", frac=epoch_gbs / HBM_PEAK_GBS,
                       traffic=ep_traffic, traffic_source=ep_src,
                       avg_launch_us=1e3 * edge_ms / edge_launches,
-                      algorithmic_bytes_per_launch=bytes_edge),
+                      algorithmic_bytes_per_launch=bytes_edge,
+                      fire_bits_ms_per_fit=fb_ms, fire_bits_launches=fb_n),
         request_floor=request_floor(ctx, n, A_total / n_epochs, r, 2,
                                     1e3 * edge_ms / edge_launches, "fit", "k_epoch_edges<2"),
         knn_kernel=dict(bound="mfma", kernel="k_shortlist<bf16> (pruned sweep, both passes)",
@@ -663,7 +675,7 @@ def bench_plain(args):
     edge_ms, edge_launches = stt.kernel_times()["epoch_edges"]
     stt.close()
     r = p.optim_params.neg_sample_rate
-    bytes_edge = (16.0 * E * n_epochs + A_total * (36.0 + 16.0 * r)) / n_epochs
+    bytes_edge = (edge_kernel_bytes(E, A_total, r) + (n_epochs - 1) * E / 8.0) / n_epochs
     epoch_gbs = bytes_edge / (edge_ms / edge_launches * 1e-3) / 1e9
     # the reference's CPU run, whole: kNN, graph, SGD (same layout to start from)
     cores = os.cpu_count() or 1
@@ -763,9 +775,8 @@ def bench_densmap(args):
     rho_full, g_full = spearman_of(x, out.T.cpu().numpy(), idx, dist)
     # the dominant kernel: the edge epoch (350 plain and 150 density launches),
     # timed with HIP events on a state of the fit's graph and density state.
-    # Algorithmic bytes per launch: 16 E + A (36 + 16 r), and for a density
-    # launch another A (2 * 16 + 4) (the two endpoints' packed density factors
-    # and the edge weight)
+    # Algorithmic bytes per launch: edge_kernel_bytes (a density launch adds
+    # the two endpoints' packed density factors and the edge weight)
     r_o, ms_o, tot_o = g_full.dens_state(idx, dist)
     ds = M.DensState(params=dp, r=r_o.ctypes.data, mu_sum=ms_o.ctypes.data, mu_tot=tot_o)
     e0 = torch.zeros((n, 2), device="cuda")
@@ -774,7 +785,7 @@ def bench_densmap(args):
     act = M.schedule_active_counts(sd.read(M.BUF_PER), n_epochs)
     rneg = p.optim_params.neg_sample_rate
     dens_ep = np.array([(e + 1) / n_epochs > 1.0 - dp.frac for e in range(n_epochs)])
-    alg_bytes = float(np.sum(16.0 * E + act * (36.0 + 16.0 * rneg) + dens_ep * act * 36.0))
+    alg_bytes = float(np.sum(edge_kernel_bytes(E, act, rneg, dens_ep * act)))
     sd.reset(out.T.contiguous())
     sd.set_profiling(True)
     sd.run_epochs(0, n_epochs, 42, M.EPOCH_EDGE_ATOMIC)

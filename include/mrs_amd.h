@@ -362,7 +362,8 @@ int mrs_run_epochs(mrs_umap_state *st, int32_t first, int32_t count, uint64_t se
  * recording, enable=0 stops; mrs_umap_state_kernel_times waits for the
  * recorded work and returns accumulated milliseconds and launch counts per
  * kind: 0 edge epoch, 1 adam+clear, 2 node gradient, 3 adam, 4 schedule,
- * 5 densMAP update (arrays of 8). */
+ * 5 densMAP update, 6 firing bitmask of a window of edge-parallel epochs
+ * (arrays of 8). */
 int mrs_umap_state_set_profiling(mrs_umap_state *st, int32_t enable);
 int mrs_umap_state_kernel_times(mrs_umap_state *st, double *ms_out, int64_t *count_out);
 

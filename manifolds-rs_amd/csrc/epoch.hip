@@ -503,20 +503,74 @@ __global__ __launch_bounds__(kWg) void k_dens_weights(
 constexpr uint32_t kWideSpanEdges = 1u << 20;
 constexpr int kWideSpan = 128;
 
-template <int DIM, bool DENS, int kEdgeSpan>
+// BITS: the firing edges come from the epoch's plane of the firing bitmask
+// (k_fire_bits) instead of the schedule cursors, which that kernel advanced.
+template <int DIM, bool DENS, int kEdgeSpan, bool BITS>
 __global__ __launch_bounds__(kWg) void k_epoch_edges(
     const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars,
+    const uint64_t* __restrict__ plane, uint64_t words) {
     constexpr int kWaves = kWg / 64;
     __shared__ uint32_t queue[kWaves][kEdgeSpan];
     const int wave = threadIdx.x >> 6;
     const uint64_t base = ((uint64_t)blockIdx.x * kWaves + wave) * kEdgeSpan;
-    epoch_edge_span<DIM, DENS, kEdgeSpan>(base, queue[wave], edges, per, next, embd, grad,
-                                          n_edges, epoch, epoch_f, seed, fm, a, b, two_a_b,
-                                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+    uint32_t count;
+    if constexpr (BITS)
+        count = queue_from_bits<kEdgeSpan>(base, queue[wave], plane, words);
+    else
+        count = queue_from_schedule<kEdgeSpan>(base, queue[wave], per, next, n_edges, epoch_f);
+    work_queue<DIM, DENS>(queue[wave], count, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
+                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+}
+
+// Firing bitmask of a window of `count` epochs from `first`: each edge's
+// schedule replayed exactly as the per-epoch scan advances it (the edge fires
+// at epoch t when next <= t, then next += per, in f32), bit e mod 64 of word
+// e / 64 of plane t - first set when edge e fires at epoch t (words past the
+// last edge stay zero), and the cursors written back as they stand after the
+// window.  A wavefront owns kFireWords consecutive words (64 edges each) and
+// keeps their cursors and periods in registers through the window; per epoch
+// it writes its words as one 128-byte row.  The epoch kernels then read one
+// bit per edge instead of a cursor and a period (and a cursor written back).
+constexpr int kFireWords = 16;
+__global__ __launch_bounds__(kWg) void k_fire_bits(const float* __restrict__ per,
+                                                   float* __restrict__ next, uint32_t n_edges,
+                                                   uint64_t words, int32_t first, int32_t count,
+                                                   uint64_t* __restrict__ bits) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t w0 = (((uint64_t)blockIdx.x * kWg + threadIdx.x) >> 6) * kFireWords;
+    if (w0 >= words) return;  // wavefront-uniform
+    float nx[kFireWords], pr[kFireWords];
+#pragma unroll
+    for (int r = 0; r < kFireWords; r++) {
+        const uint64_t e = (w0 + r) * 64 + lane;
+        nx[r] = INFINITY;  // past the last edge: never fires
+        pr[r] = 0.0f;
+        if (e < n_edges) {
+            nx[r] = next[e];
+            pr[r] = per[e];
+        }
+    }
+    for (int32_t t = 0; t < count; t++) {
+        const float tf = (float)(first + t);
+        uint64_t mine = 0;
+#pragma unroll
+        for (int r = 0; r < kFireWords; r++) {
+            const bool fire = nx[r] <= tf;
+            if (fire) nx[r] = nx[r] + pr[r];
+            const uint64_t m = __ballot(fire);
+            if (lane == r) mine = m;
+        }
+        if (lane < kFireWords && w0 + lane < words) bits[(size_t)t * words + w0 + lane] = mine;
+    }
+#pragma unroll
+    for (int r = 0; r < kFireWords; r++) {
+        const uint64_t e = (w0 + r) * 64 + lane;
+        if (e < n_edges) next[e] = nx[r];
+    }
 }
 
 template <int DIM>
@@ -719,12 +773,14 @@ int epoch_span_choice(const mrs_umap_state* st, bool dens) {
     return (uint32_t)st->n_edges >= kWideSpanEdges ? kWideSpan : 64;
 }
 
-void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens, int span) {
+void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens, int span,
+                        const uint64_t* plane) {
     if (!st->grad_clean) clear_update_marks(st);
     const mrs_umap_optim_params& p = st->params;
     const float two = 2.0f;
     FastMod fm = make_fastmod((uint32_t)st->n);
     const uint32_t ne = (uint32_t)st->n_edges;
+    const uint64_t words = div_up((uint64_t)ne, (uint64_t)64);
     hipStream_t s = st->ctx->stream;
     if (span == 0) span = epoch_span_choice(st, dens);
     if (span != 64 && span != 128 && span != 256)
@@ -736,13 +792,19 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
                            st->next.p, st->embd.p, st->grad.p, ne, (uint32_t)epoch, (float)epoch,
                            seed, fm, p.a, p.b, two * p.a * p.b, two * p.gamma * p.b,
                            p.neg_sample_rate, st->mu.p,
-                           reinterpret_cast<const float4*>(st->dens_pack.p), st->dens_scalars.p);
+                           reinterpret_cast<const float4*>(st->dens_pack.p), st->dens_scalars.p,
+                           plane, words);
     };
     auto by_span = [&](auto d) {
         constexpr int D = decltype(d)::value;
-        if (span == 256) dens ? go(k_epoch_edges<D, true, 256>) : go(k_epoch_edges<D, false, 256>);
-        else if (span == 128) dens ? go(k_epoch_edges<D, true, 128>) : go(k_epoch_edges<D, false, 128>);
-        else dens ? go(k_epoch_edges<D, true, 64>) : go(k_epoch_edges<D, false, 64>);
+        auto by_source = [&](auto sp) {
+            constexpr int S = decltype(sp)::value;
+            if (plane) dens ? go(k_epoch_edges<D, true, S, true>) : go(k_epoch_edges<D, false, S, true>);
+            else dens ? go(k_epoch_edges<D, true, S, false>) : go(k_epoch_edges<D, false, S, false>);
+        };
+        if (span == 256) by_source(Dim<256>{});
+        else if (span == 128) by_source(Dim<128>{});
+        else by_source(Dim<64>{});
     };
     switch (st->dim) {
         case 1: by_span(Dim<1>{}); break;
@@ -754,6 +816,19 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
             throw Error(MRS_INVALID_ARGUMENT,
                         "edge-parallel epoch supports n_dim in {1,2,3,4,8}; use node-exact");
     }
+    MRS_CHECK_LAUNCH();
+}
+
+void launch_fire_bits(mrs_umap_state* st, int32_t first, int32_t count) {
+    const uint32_t ne = (uint32_t)st->n_edges;
+    const uint64_t words = div_up((uint64_t)ne, (uint64_t)64);
+    if (!words || count <= 0) return;
+    if (count > kFireWindow || st->fire_bits.n < (size_t)count * words)
+        throw Error(MRS_DEVICE_ERROR, "firing bitmask smaller than the window");
+    const uint64_t waves = div_up(words, (uint64_t)kFireWords);
+    hipLaunchKernelGGL(k_fire_bits, dim3((unsigned)div_up(waves, (uint64_t)(kWg / 64))), dim3(kWg),
+                       0, st->ctx->stream, st->per.p, st->next.p, ne, words, first, count,
+                       st->fire_bits.p);
     MRS_CHECK_LAUNCH();
 }
 
@@ -821,6 +896,30 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
         int d = st->dim;
         if (!(d == 1 || d == 2 || d == 3 || d == 4 || d == 8)) strategy = MRS_EPOCH_NODE_EXACT;
     }
+    // Edge-parallel runs of kFireMinEpochs or more take their firing edges
+    // from bitmask windows (launch_fire_bits).  The buffer is allocated here,
+    // before any capture; a captured loop holds its address, so a new buffer
+    // drops the captured loop.
+    const uint64_t fire_words = div_up((uint64_t)st->n_edges, (uint64_t)64);
+    const bool use_bits =
+        strategy == MRS_EPOCH_EDGE_ATOMIC && count >= kFireMinEpochs && fire_words > 0;
+    if (use_bits) {
+        const size_t need = (size_t)std::min(count, kFireWindow) * fire_words;
+        if (st->fire_bits.n < need) {
+            if (st->graph_exec) {
+                MRS_HIP(hipStreamSynchronize(st->ctx->stream));
+                (void)hipGraphExecDestroy(st->graph_exec);
+                st->graph_exec = nullptr;
+                std::fill(st->graph_key, st->graph_key + 4, -1);
+            }
+            StreamScope scope(st->ctx->stream, st->ctx->pool);
+            st->fire_bits.alloc(need);
+        }
+    }
+    auto plane_of = [&](int32_t e) -> const uint64_t* {
+        return use_bits ? st->fire_bits.p + (size_t)((e - first) % kFireWindow) * fire_words
+                        : nullptr;
+    };
     // Span tuning (autotune.hpp).  For each stretch of the run with one
     // density flag whose shape class is undecided, a window of kTuneEpochs
     // epochs alternates the candidate spans with an event pair around each
@@ -893,6 +992,10 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
     };
     auto launches = [&]() {
         for (int32_t e = first; e < first + count; e++) {
+            if (use_bits && (e - first) % kFireWindow == 0) {
+                ProfScope ps(st, 6);
+                launch_fire_bits(st, e, std::min(kFireWindow, first + count - e));
+            }
             bool dens = st->dens_enabled && dens_is_active(st->dens_params, e, N);
             if (dens) {
                 ProfScope ps(st, 5);
@@ -906,12 +1009,15 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
                 const int q = e - w->start;
                 if (!st->grad_clean) clear_update_marks(st);
                 MRS_HIP(hipEventRecord(w->ev[q].first, st->ctx->stream));
-                { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens, kSpans[q % 3]); }
+                {
+                    ProfScope ps(st, 0);
+                    launch_epoch_edges(st, e, seed, dens, kSpans[q % 3], plane_of(e));
+                }
                 MRS_HIP(hipEventRecord(w->ev[q].second, st->ctx->stream));
                 { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
                 if (q + 1 == kTuneEpochs) finish_window(*w);
             } else {
-                { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens); }
+                { ProfScope ps(st, 0); launch_epoch_edges(st, e, seed, dens, 0, plane_of(e)); }
                 { ProfScope ps(st, 1); launch_adam_clear(st, e, nullptr); }
             }
         }

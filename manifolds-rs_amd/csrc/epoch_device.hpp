@@ -236,25 +236,21 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* 
     atomic_add_rows<DIM>(grad, key, head && key != 0xFFFFFFFFu, gi);
 }
 
-// One wavefront's span of SPAN consecutive edges [base, base + SPAN) of one
-// Adam epoch.  About one edge in five fires in an epoch, scattered over the
-// edge list, so with one lane per edge nearly every wavefront would have a
-// firing lane and run the whole negative-sampling body for all 64 lanes.
-// Here the wavefront first advances the span's schedule cursors (coalesced;
-// every cursor and period of the span is loaded in one round trip) and queues
-// the firing edges in its LDS slice `queue`, in edge order (runs of equal i
-// stay together for the segmented sum), then works through the queue 64 edges
-// at a time.  The queue is built from ballots: no counter is shared between
-// wavefronts.  queue is this wavefront's LDS slice (SPAN entries).  Every
-// wavefront of the workgroup must call this (it holds a workgroup barrier
-// between building and reading the queues).
-template <int DIM, bool DENS, int SPAN>
-__device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restrict__ queue,
-    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
-    const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
-    float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
-    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
-    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
+// The firing edges of one wavefront's span of SPAN consecutive edges
+// [base, base + SPAN) of one Adam epoch, queued in edge order in the
+// wavefront's LDS slice `queue` (SPAN entries); returns their count
+// (wavefront-uniform).  About one edge in five fires in an epoch, scattered
+// over the edge list, so with one lane per edge nearly every wavefront would
+// have a firing lane and run the whole negative-sampling body for all 64
+// lanes; a queue runs it for full batches.  Runs of equal i stay together for
+// the segmented sum.  Built from ballots: no counter is shared between
+// wavefronts.  This one advances the span's schedule cursors (coalesced;
+// every cursor and period of the span is loaded in one round trip).
+template <int SPAN>
+__device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, uint32_t* __restrict__ queue,
+                                                        const float* __restrict__ per,
+                                                        float* __restrict__ next,
+                                                        uint32_t n_edges, float epoch_f) {
     static_assert(SPAN % 64 == 0, "whole wavefront rounds");
     constexpr int R = SPAN / 64;
     const int lane = threadIdx.x & 63;
@@ -270,7 +266,7 @@ __device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restr
             pr[r] = per[e];
         }
     }
-    uint32_t count = 0;  // wavefront-uniform
+    uint32_t count = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint64_t e = base + (uint64_t)r * 64 + lane;
@@ -280,6 +276,45 @@ __device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restr
         if (fire) queue[count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
         count += (uint32_t)__popcll(m);
     }
+    return count;
+}
+
+// The same queue from the epoch's plane of a firing bitmask (bit e mod 64 of
+// word e / 64 set when edge e fires; `words` words, zero past the last edge),
+// written ahead for a window of epochs by an exact replay of the schedule: one
+// 8-byte word per 64 edges instead of their cursors and periods.
+template <int SPAN>
+__device__ __forceinline__ uint32_t queue_from_bits(uint64_t base, uint32_t* __restrict__ queue,
+                                                    const uint64_t* __restrict__ plane,
+                                                    uint64_t words) {
+    static_assert(SPAN % 64 == 0, "whole wavefront rounds");
+    constexpr int R = SPAN / 64;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t w0 = base / 64;
+    uint64_t m[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) m[r] = w0 + r < words ? plane[w0 + r] : 0ull;
+    uint32_t count = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if ((m[r] >> lane) & 1ull)
+            queue[count + (uint32_t)__popcll(m[r] & below)] = (uint32_t)(base + (uint64_t)r * 64 + lane);
+        count += (uint32_t)__popcll(m[r]);
+    }
+    return count;
+}
+
+// Works through a wavefront's queue of `count` firing edges 64 at a time
+// (edge_work).  Every wavefront of the workgroup must call this: it holds a
+// workgroup barrier between building and reading the queues.
+template <int DIM, bool DENS>
+__device__ __forceinline__ void work_queue(const uint32_t* __restrict__ queue, uint32_t count,
+    const uint4* __restrict__ edges, const float* __restrict__ embd, float* __restrict__ grad,
+    uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
+    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
+    const int lane = threadIdx.x & 63;
     __syncthreads();  // the queue is read by other lanes than the ones that wrote it
     for (uint32_t q0 = 0; q0 < count; q0 += 64) {
         const uint32_t slot = q0 + lane;
@@ -288,6 +323,20 @@ __device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restr
         edge_work<DIM, DENS>(active, e, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
                              two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
     }
+}
+
+// One wavefront's span of one Adam epoch: queue from the schedule cursors,
+// then work through it.  Every wavefront of the workgroup must call this.
+template <int DIM, bool DENS, int SPAN>
+__device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restrict__ queue,
+    const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
+    const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
+    float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
+    float two_gamma_b, int neg_rate, const float* __restrict__ mu,
+    const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
+    const uint32_t count = queue_from_schedule<SPAN>(base, queue, per, next, n_edges, epoch_f);
+    work_queue<DIM, DENS>(queue, count, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
+                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
 }
 
 // Adam step of one node if it got an update this epoch (umap_adam_update,

@@ -24,6 +24,10 @@ struct mrs_umap_state {
     // one 16-byte record per edge, {i, j, CSR slot of the edge under i, under j}
     mrs::DevBuf<uint4> edges;        // [E]
     mrs::DevBuf<float> per, next, mu;  // [E]
+    // firing bitmask of a window of epochs (k_fire_bits): up to kFireWindow
+    // planes of ceil(E / 64) words; allocated by the first run of the
+    // edge-parallel loop long enough to use it
+    mrs::DevBuf<uint64_t> fire_bits;
 
     // grad holds the edge strategy's cleared state: -0.0 everywhere, "no update
     // yet" (epoch_device.hpp); has_update is kept by the node-exact strategy
@@ -46,7 +50,8 @@ struct mrs_umap_state {
     mrs::DevBuf<float> norm_partial;  // [n]
 
     // Optional per-kernel timing with HIP events on the context stream.
-    // kind: 0 edge epoch, 1 adam+clear, 2 node grad, 3 adam elem, 4 schedule, 5 densMAP update
+    // kind: 0 edge epoch, 1 adam+clear, 2 node grad, 3 adam elem, 4 schedule, 5 densMAP update,
+    // 6 firing bitmask window
     bool profiling = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     double prof_ms[8] = {0};
@@ -66,6 +71,10 @@ struct mrs_umap_state {
 namespace mrs {
 
 constexpr int kDensBlocks = 256;
+// Epochs per firing-bitmask window, and the shortest edge-parallel run that
+// builds one (shorter runs read the schedule cursors in the epoch kernel).
+constexpr int kFireWindow = 128;
+constexpr int kFireMinEpochs = 8;
 
 // Host CSR of an adjacency list (UmapCsrGraph::from_graph,
 // src/training/umap_optimiser_gpu.rs:132-209), plus the per-edge endpoint
@@ -104,9 +113,14 @@ void launch_schedule(mrs_umap_state* st, int32_t epoch);
 // summed per node in the reference's sequential order (else 16 lanes per node)
 void launch_dens_update_dev(mrs_umap_state* st, bool reference_order);
 // span: edges per wavefront (64, 128 or 256); 0 = the recorded or built-in
-// choice for this state (autotune.hpp)
+// choice for this state (autotune.hpp).  plane: the epoch's plane of the
+// firing bitmask (launch_fire_bits), or null to read and advance the schedule
+// cursors in the kernel.
 void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool dens,
-                        int span = 0);
+                        int span = 0, const uint64_t* plane = nullptr);
+// Firing bitmask of epochs [first, first + count), count <= kFireWindow, into
+// st->fire_bits; advances the schedule cursors past the window.
+void launch_fire_bits(mrs_umap_state* st, int32_t first, int32_t count);
 // The span launch_epoch_edges uses for this state when given 0.
 int epoch_span_choice(const mrs_umap_state* st, bool dens);
 // grad to -0.0 (no update yet), has_update from / to the markers

@@ -591,7 +591,7 @@ class UmapGpuState:
         _check(lib().mrs_umap_state_set_profiling(self.handle, 1 if on else 0))
 
     KERNEL_KINDS = ("epoch_edges", "adam_clear", "grad_accum_node", "adam_elem", "schedule",
-                    "dens_update")
+                    "dens_update", "fire_bits")
 
     def kernel_times(self) -> dict:
         """{kind: (total_ms, launches)} of the launches recorded while profiling."""

@@ -503,6 +503,57 @@ def test_edge_kernel_wide_span_teacher_forced(ctx):
     assert fired > n_edges // 4
 
 
+def test_edge_loop_firing_bitmask_windows(ctx):
+    # Edge-parallel runs of 8 or more epochs take each epoch's firing edges
+    # from bitmask windows of up to 128 epochs, built ahead by an exact replay
+    # of the schedule.  With lr = 0 the embedding stays put, so every epoch's
+    # gradient is a function of its firing set alone, and the oracle replays
+    # the same epochs from the same embedding.  Calls end just after window
+    # boundaries (8 | 130 = 128 + 2 | 129 = 128 + 1 | 33); after each call:
+    # schedule cursors exact; m (decay 0.5: the last few epochs' gradients
+    # dominate) within the gradient bar accumulated over the epochs; v likewise
+    # through g^2.  An edge that fires in the wrong epoch moves m of its
+    # endpoints by a whole term.
+    adj, n_edges = _random_sparse_graph(5000, 7, seed=67)
+    n = adj[0].size - 1
+    embd = np.random.default_rng(67).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    p = M.UmapOptimParams.default_2d()
+    p.n_epochs = 300
+    p.lr = 0.0
+    st, _ = make_state(ctx, embd, adj, p)
+    c = oracle_csr(adj)
+    gtol = 1e-4
+    m = np.zeros(n * 2, np.float32)
+    v = np.zeros(n * 2, np.float32)
+    nx = c["per"].copy()
+    dm = np.zeros(n * 2)  # accumulated bars of m and v
+    dv = np.zeros(n * 2)
+    e = 0
+    for count in (8, 130, 129, 33):
+        st.run_epochs(e, count, 42, M.EPOCH_EDGE_ATOMIC)
+        for t in range(e, e + count):
+            g, hu, absum = orc.grad_accum(embd, c, nx, t, 42, p.neg_sample_rate, p, with_abs=True,
+                                          exact=True)
+            ew, m2, v2 = orc.adam_update(g, hu, m, v, embd, p, t, p.n_epochs)
+            assert np.array_equal(ew, embd)
+            bar = gtol * absum.ravel()
+            upd = np.repeat(hu.astype(bool), 2)
+            dm = np.where(upd, p.beta1 * dm + (1 - p.beta1) * bar, dm)
+            dv = np.where(upd, p.beta2 * dv + (1 - p.beta2) * (2 * np.abs(g.ravel()) * bar + bar ** 2),
+                          dv)
+            m, v = m2.ravel(), v2.ravel()
+            nx = orc.schedule_update(c["per"], nx, t)
+        e += count
+        assert np.array_equal(st.read(M.BUF_NEXT), nx), f"schedule after epoch {e - 1}"
+        got_m, got_v = st.read(M.BUF_M), st.read(M.BUF_V)
+        mbad = np.flatnonzero(np.abs(got_m - m) > dm + 1e-6 * (1 + np.abs(m)))
+        vbad = np.flatnonzero(np.abs(got_v - v) > dv + 1e-6 * (1 + np.abs(v)))
+        assert mbad.size == 0, f"m after epoch {e - 1}: {mbad.size} coords over the bar"
+        assert vbad.size == 0, f"v after epoch {e - 1}: {vbad.size} coords over the bar"
+        assert np.array_equal(st.read(M.BUF_EMBD), embd.ravel())
+    print(f"firing bitmask windows: {n_edges} edges, 300 epochs, schedule exact, m and v in bars")
+
+
 def test_edge_kernel_hub_degree_above_65535(ctx):
     # Node 0 is joined to every other node (degree 69 999 > 65 535), the others
     # also form a ring.  The negatives of an edge are keyed by its CSR slot
