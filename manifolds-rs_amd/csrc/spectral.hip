@@ -224,23 +224,52 @@ __global__ __launch_bounds__(kWg) void k_perm_rows(const uint32_t* __restrict__ 
 // y = M x for M = -D^-1/2 W D^-1/2 (zero diagonal), from u = D^-1/2 x:
 // y_q = -dinv_p[q] * sum_p w_p u[col_p].  The row streams 8 bytes per entry
 // (FP32 weight, column) where precomputed FP64 products of M would take 12.
-// 16 lanes per row, strided partial sums and a fixed xor tree: the result does
-// not depend on scheduling.
+// kSpmvLanes lanes per row; lane s takes entries s, s + kSpmvLanes, ... in
+// order, kSpmvUnroll of them per round with every column and weight load of
+// the round issued together and then every gather (one dependent round trip
+// per round instead of per entry: a kNN row of ~23 entries takes one round);
+// the lanes' sums meet in a fixed xor tree, so the result does not depend on
+// scheduling.  Workgroups go to the 8 XCDs round-robin, so block b works on
+// row block (b mod 8) * (gridDim.x / 8) + b / 8: each XCD takes one
+// contiguous eighth of the rows (in component order).  The grid is a
+// multiple of 8 blocks (kSpmvXcds); blocks past the rows return.
+constexpr uint32_t kSpmvXcds = 8;
+constexpr int kSpmvLanes = 8;
+constexpr int kSpmvUnroll = 4;
 __global__ __launch_bounds__(kWg) void k_spmv_rows(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ col,
                                                    const float* __restrict__ w,
                                                    const double* __restrict__ u,
                                                    const double* __restrict__ dinv_p, uint32_t n,
                                                    double* __restrict__ y) {
-    const uint64_t t = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    const uint64_t row = t / kRowLanes;
-    const int sub = (int)(t % kRowLanes);
+    const uint32_t per_xcd = gridDim.x / kSpmvXcds;
+    const uint32_t blk = (blockIdx.x % kSpmvXcds) * per_xcd + blockIdx.x / kSpmvXcds;
+    const uint64_t t = (uint64_t)blk * kWg + threadIdx.x;
+    const uint64_t row = t / kSpmvLanes;
+    const uint32_t sub = (uint32_t)(t % kSpmvLanes);
     double acc = 0.0;
-    if (row < n)
-        for (uint32_t p = off[row] + sub; p < off[row + 1]; p += kRowLanes)
-            acc += (double)w[p] * u[col[p]];
+    if (row < n) {
+        const uint32_t end = off[row + 1];
+        for (uint32_t p0 = off[row] + sub; p0 < end; p0 += kSpmvLanes * kSpmvUnroll) {
+            uint32_t c[kSpmvUnroll];
+            float wv[kSpmvUnroll];
 #pragma unroll
-    for (int o = kRowLanes / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+            for (int i = 0; i < kSpmvUnroll; i++) {
+                const uint32_t p = p0 + (uint32_t)i * kSpmvLanes;
+                c[i] = p < end ? col[p] : 0u;
+                wv[i] = p < end ? w[p] : 0.0f;
+            }
+            double uv[kSpmvUnroll];
+#pragma unroll
+            for (int i = 0; i < kSpmvUnroll; i++)
+                uv[i] = p0 + (uint32_t)i * kSpmvLanes < end ? u[c[i]] : 0.0;
+#pragma unroll
+            for (int i = 0; i < kSpmvUnroll; i++)
+                if (p0 + (uint32_t)i * kSpmvLanes < end) acc += (double)wv[i] * uv[i];
+        }
+    }
+#pragma unroll
+    for (int o = kSpmvLanes / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kSpmvLanes);
     if (row < n && sub == 0) y[row] = -dinv_p[row] * acc;
 }
 
@@ -956,13 +985,15 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemcpyAsync(V.p, v.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_mul_dinv, gn, dim3(kWg), 0, s, v.p, dinv_p.p, n, v.p);
         MRS_CHECK_LAUNCH();
+        const unsigned spmv_blocks =
+            div_up(div_up((uint64_t)n * kSpmvLanes, kWg), kSpmvXcds) * kSpmvXcds;
         for (int j = 0; j < max_iter; j++) {
             double* alpha_j = alpha_d.p + (size_t)j * nl;
             double* beta_j = beta_d.p + (size_t)j * nl;
             hipLaunchKernelGGL(k_lanczos_gate, gq, dim3(kWg), 0, s, live.p, dn_done.p, dn_iter.p,
                                j > 0 ? beta_j - nl : nullptr, nl, j);
-            hipLaunchKernelGGL(k_spmv_rows, dim3(div_up((uint64_t)n * kRowLanes, kWg)), dim3(kWg), 0,
-                               s, poff.p, pcol.p, pw.p, v.p, dinv_p.p, n, w.p);
+            hipLaunchKernelGGL(k_spmv_rows, dim3(spmv_blocks), dim3(kWg), 0, s, poff.p, pcol.p,
+                               pw.p, v.p, dinv_p.p, n, w.p);
             // Orthogonalise w = M v_j against V_0..V_j by classical Gram-Schmidt
             // applied twice.  The first pass's projections on v_j and v_{j-1}
             // are alpha_j and beta_{j-1}, so it also performs the three-term
