@@ -54,7 +54,8 @@ def edge_kernel_bytes(E: int, A, r: int, dens_A=0.0):
     """Algorithmic bytes of the edge-parallel epoch kernel over launches with A
     active edges in total (dens_A of them in density epochs): per launch one
     bit per edge of the firing bitmask's plane (E / 8 B), and per active edge
-    its 16-byte record, the 2 x 8 B endpoint gathers, 2r x 8 B negative-sample
+    16 B of record (the 8-byte compact record {j, slot under j} and its run's
+    {i, slot offset}), the 2 x 8 B endpoint gathers, 2r x 8 B negative-sample
     gathers and 2 x 8 B gradient scatters, plus in a density epoch the two
     endpoints' 16-byte packed density factors and the 4-byte edge weight.
     The bitmask planes themselves are written ahead, 128 epochs at a time, by

@@ -512,18 +512,64 @@ __global__ __launch_bounds__(kWg) void k_epoch_edges(
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars,
-    const uint64_t* __restrict__ plane, uint64_t words) {
+    const uint64_t* __restrict__ plane, uint64_t words, CompactRecords cr) {
     constexpr int kWaves = kWg / 64;
-    __shared__ uint32_t queue[kWaves][kEdgeSpan];
-    const int wave = threadIdx.x >> 6;
+    __shared__ SpanQueue<kEdgeSpan> queue[kWaves];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const uint64_t base = ((uint64_t)blockIdx.x * kWaves + wave) * kEdgeSpan;
     uint32_t count;
     if constexpr (BITS)
-        count = queue_from_bits<kEdgeSpan>(base, queue[wave], plane, words);
+        count = queue_from_bits<kEdgeSpan>(base, queue[wave], cr, plane, words);
     else
-        count = queue_from_schedule<kEdgeSpan>(base, queue[wave], per, next, n_edges, epoch_f);
-    work_queue<DIM, DENS>(queue[wave], count, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
-                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+        count = queue_from_schedule<kEdgeSpan>(base, queue[wave], edges, per, next, n_edges,
+                                               epoch_f);
+    work_queue<DIM, DENS, kEdgeSpan>(queue[wave], count, embd, grad, epoch, seed, fm, a, b,
+                                     two_a_b, two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+}
+
+// The compact records of the bitmask epoch kernel (CompactRecords), from the
+// 16-byte records: per edge {j, slot under j}; per 64-edge word the edges
+// that start a run of equal i (ballot) and their count.  One wavefront per
+// word.
+__global__ __launch_bounds__(kWg) void k_run_starts(const uint4* __restrict__ edges,
+                                                    uint32_t n_edges, uint64_t words,
+                                                    uint2* __restrict__ pair,
+                                                    uint64_t* __restrict__ run_start,
+                                                    uint32_t* __restrict__ run_count) {
+    const uint64_t e = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint64_t w = e / 64;
+    const int lane = threadIdx.x & 63;
+    bool start = false;
+    if (e < n_edges) {
+        const uint4 r = edges[e];
+        pair[e] = make_uint2(r.y, r.w);
+        start = e == 0 || edges[e - 1].x != r.x;
+    }
+    const uint64_t m = __ballot(start);
+    if (lane == 0 && w < words) {
+        run_start[w] = m;
+        run_count[w] = (uint32_t)__popcll(m);
+    }
+}
+
+// run_info[run] = {i, slot under i of the run's first edge - its index} from
+// each run's first edge; word_run[w] = (runs started before edge 64w) - 1
+// (mod 2^32) from the exclusive scan of the words' run counts.
+__global__ __launch_bounds__(kWg) void k_run_info(const uint4* __restrict__ edges,
+                                                  uint32_t n_edges, uint64_t words,
+                                                  const uint64_t* __restrict__ run_start,
+                                                  const uint32_t* __restrict__ runs_before,
+                                                  uint32_t* __restrict__ word_run,
+                                                  uint2* __restrict__ run_info) {
+    const uint64_t e = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint64_t w = e / 64;
+    const int lane = threadIdx.x & 63;
+    if (w >= words) return;
+    if (lane == 0) word_run[w] = runs_before[w] - 1u;
+    if (e >= n_edges || !((run_start[w] >> lane) & 1ull)) return;
+    const uint32_t run = runs_before[w] + (uint32_t)__popcll(run_start[w] & ((1ull << lane) - 1ull));
+    const uint4 r = edges[e];
+    run_info[run] = make_uint2(r.x, r.z - (uint32_t)e);
 }
 
 // Firing bitmask of a window of `count` epochs from `first`: each edge's
@@ -782,6 +828,8 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
     const uint32_t ne = (uint32_t)st->n_edges;
     const uint64_t words = div_up((uint64_t)ne, (uint64_t)64);
     hipStream_t s = st->ctx->stream;
+    if (plane && !st->rec_pair.p) throw Error(MRS_DEVICE_ERROR, "compact edge records not built");
+    const CompactRecords cr{st->rec_pair.p, st->run_start.p, st->word_run.p, st->run_info.p};
     if (span == 0) span = epoch_span_choice(st, dens);
     if (span != 64 && span != 128 && span != 256)
         throw Error(MRS_INVALID_ARGUMENT, "edge epoch span must be 64, 128 or 256");
@@ -793,7 +841,7 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
                            seed, fm, p.a, p.b, two * p.a * p.b, two * p.gamma * p.b,
                            p.neg_sample_rate, st->mu.p,
                            reinterpret_cast<const float4*>(st->dens_pack.p), st->dens_scalars.p,
-                           plane, words);
+                           plane, words, cr);
     };
     auto by_span = [&](auto d) {
         constexpr int D = decltype(d)::value;
@@ -817,6 +865,27 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
                         "edge-parallel epoch supports n_dim in {1,2,3,4,8}; use node-exact");
     }
     MRS_CHECK_LAUNCH();
+}
+
+void build_compact_records(mrs_umap_state* st) {
+    const uint32_t ne = (uint32_t)st->n_edges;
+    const uint64_t words = div_up((uint64_t)ne, (uint64_t)64);
+    if (!words) return;
+    hipStream_t s = st->ctx->stream;
+    st->rec_pair.alloc(ne);
+    st->run_start.alloc(words);
+    st->word_run.alloc(words);
+    st->run_info.alloc((size_t)std::min<int64_t>(st->n, st->n_edges));
+    DevBuf<uint32_t> runs_before(words);
+    const unsigned blocks = div_up(words * 64, kWg);
+    hipLaunchKernelGGL(k_run_starts, dim3(blocks), dim3(kWg), 0, s, st->edges.p, ne, words,
+                       st->rec_pair.p, st->run_start.p, runs_before.p);
+    MRS_CHECK_LAUNCH();
+    exclusive_scan_u32(runs_before.p, runs_before.p, words, nullptr, s);
+    hipLaunchKernelGGL(k_run_info, dim3(blocks), dim3(kWg), 0, s, st->edges.p, ne, words,
+                       st->run_start.p, runs_before.p, st->word_run.p, st->run_info.p);
+    MRS_CHECK_LAUNCH();
+    // runs_before returns to the pool at scope exit; the stream orders its reuse
 }
 
 void launch_fire_bits(mrs_umap_state* st, int32_t first, int32_t count) {
@@ -914,6 +983,10 @@ void run_epochs(mrs_umap_state* st, int32_t first, int32_t count, uint32_t seed,
             }
             StreamScope scope(st->ctx->stream, st->ctx->pool);
             st->fire_bits.alloc(need);
+        }
+        if (!st->rec_pair.p) {
+            StreamScope scope(st->ctx->stream, st->ctx->pool);
+            build_compact_records(st);
         }
     }
     auto plane_of = [&](int32_t e) -> const uint64_t* {

@@ -194,7 +194,8 @@ __device__ __forceinline__ void attract(uint32_t i, uint32_t j, uint32_t e, cons
     }
 }
 
-// The work of one firing undirected edge (i < j) of one Adam epoch: gathers
+// The work of one firing undirected edge e (i < j; rec = its record {i, j,
+// CSR slot under i, under j}) of one Adam epoch: gathers
 // y_i, y_j, computes the attractive (and densMAP) term once for both
 // endpoints, then j's negatives, whose sum goes to grad[j] by FP32 atomics
 // at once (while y_j's line is still in L2), then i's negatives; i's total
@@ -204,7 +205,7 @@ __device__ __forceinline__ void attract(uint32_t i, uint32_t j, uint32_t e, cons
 // wavefront-wide); lanes with active == false only join them.  DIM is the
 // compile-time embedding dimension.
 template <int DIM, bool DENS>
-__device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* __restrict__ edges,
+__device__ __forceinline__ void edge_work(bool active, uint32_t e, uint4 rec,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t epoch, uint32_t seed,
     FastMod fm, float a, float b, float two_a_b, float two_gamma_b, int neg_rate,
     const float* __restrict__ mu, const float4* __restrict__ dens_pack,
@@ -214,7 +215,6 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* 
 #pragma unroll
     for (int d = 0; d < DIM; d++) gi[d] = gj[d] = 0.0f;
     if (active) {
-        const uint4 rec = edges[e];  // {i, j, CSR slot under i, under j}
         const uint32_t i = rec.x;
         j = rec.y;
         key = i;
@@ -236,25 +236,52 @@ __device__ __forceinline__ void edge_work(bool active, uint32_t e, const uint4* 
     atomic_add_rows<DIM>(grad, key, head && key != 0xFFFFFFFFu, gi);
 }
 
+// A wavefront's LDS queue of firing edges: their ids and 16-byte records.
+template <int SPAN>
+struct SpanQueue {
+    uint4 rec[SPAN];
+    uint32_t e[SPAN];
+};
+
+// One 64-edge round of a queue: the firing lanes (bits of m) append their
+// edge and record at count in lane order.  The other lanes write too, into the
+// slots just past the firing ones (which the next round's firing lanes
+// overwrite, or which lie past the final count and are never read; the last
+// round ends at most at SPAN): every lane stores, so the record loads feeding
+// the stores are issued unconditionally and all rounds' loads are in flight
+// together.
+template <int SPAN>
+__device__ __forceinline__ void queue_put(SpanQueue<SPAN>& q, uint32_t count, uint64_t m, int lane,
+                                          uint32_t e, uint4 rec) {
+    const uint64_t below = (1ull << lane) - 1ull;
+    const bool fire = (m >> lane) & 1ull;
+    const uint32_t slot = count + (fire ? (uint32_t)__popcll(m & below)
+                                        : (uint32_t)__popcll(m) + (uint32_t)__popcll(~m & below));
+    q.e[slot] = e;
+    q.rec[slot] = rec;
+}
+
 // The firing edges of one wavefront's span of SPAN consecutive edges
 // [base, base + SPAN) of one Adam epoch, queued in edge order in the
-// wavefront's LDS slice `queue` (SPAN entries); returns their count
+// wavefront's LDS queue (ids and records); returns their count
 // (wavefront-uniform).  About one edge in five fires in an epoch, scattered
 // over the edge list, so with one lane per edge nearly every wavefront would
 // have a firing lane and run the whole negative-sampling body for all 64
 // lanes; a queue runs it for full batches.  Runs of equal i stay together for
 // the segmented sum.  Built from ballots: no counter is shared between
-// wavefronts.  This one advances the span's schedule cursors (coalesced;
-// every cursor and period of the span is loaded in one round trip).
+// wavefronts.  Each firing lane loads its edge's record before queueing it,
+// so the gathers that follow the queue depend on no further load.  This one
+// advances the span's schedule cursors (coalesced; every cursor and period of
+// the span is loaded in one round trip).
 template <int SPAN>
-__device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, uint32_t* __restrict__ queue,
+__device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, SpanQueue<SPAN>& q,
+                                                        const uint4* __restrict__ edges,
                                                         const float* __restrict__ per,
                                                         float* __restrict__ next,
                                                         uint32_t n_edges, float epoch_f) {
     static_assert(SPAN % 64 == 0, "whole wavefront rounds");
     constexpr int R = SPAN / 64;
     const int lane = threadIdx.x & 63;
-    const uint64_t below = (1ull << lane) - 1ull;
     float nx[R], pr[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -266,6 +293,13 @@ __device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, uint32_t*
             pr[r] = per[e];
         }
     }
+    // every lane loads (a lane whose edge does not fire reads the span's first
+    // record, which one request serves for all of them): no branch, so the
+    // rounds' loads are all in flight together
+    uint4 rec[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        rec[r] = edges[nx[r] <= epoch_f ? base + (uint64_t)r * 64 + lane : base];
     uint32_t count = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -273,7 +307,7 @@ __device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, uint32_t*
         const bool fire = nx[r] <= epoch_f;
         if (fire) next[e] = nx[r] + pr[r];
         const uint64_t m = __ballot(fire);
-        if (fire) queue[count + (uint32_t)__popcll(m & below)] = (uint32_t)e;
+        queue_put(q, count, m, lane, (uint32_t)e, rec[r]);
         count += (uint32_t)__popcll(m);
     }
     return count;
@@ -282,24 +316,54 @@ __device__ __forceinline__ uint32_t queue_from_schedule(uint64_t base, uint32_t*
 // The same queue from the epoch's plane of a firing bitmask (bit e mod 64 of
 // word e / 64 set when edge e fires; `words` words, zero past the last edge),
 // written ahead for a window of epochs by an exact replay of the schedule: one
-// 8-byte word per 64 edges instead of their cursors and periods.
+// 8-byte word per 64 edges instead of their cursors and periods.  The records
+// come in compact form (CompactRecords), 8 bytes per firing edge instead of
+// 16.  base is wavefront-uniform, so the per-word data comes through the
+// scalar cache.
+struct CompactRecords {
+    const uint2* pair;         // [E] {j, CSR slot of the edge under j}
+    const uint64_t* run_start; // [words] bit b: edge 64w + b starts a run of equal i
+    const uint32_t* word_run;  // [words] (runs started before edge 64w) - 1, mod 2^32
+    const uint2* run_info;     // [runs] {i, CSR slot under i of the run's first edge - its index}
+};
 template <int SPAN>
-__device__ __forceinline__ uint32_t queue_from_bits(uint64_t base, uint32_t* __restrict__ queue,
+__device__ __forceinline__ uint32_t queue_from_bits(uint64_t base, SpanQueue<SPAN>& q,
+                                                    const CompactRecords cr,
                                                     const uint64_t* __restrict__ plane,
                                                     uint64_t words) {
     static_assert(SPAN % 64 == 0, "whole wavefront rounds");
     constexpr int R = SPAN / 64;
     const int lane = threadIdx.x & 63;
-    const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t w0 = base / 64;
-    uint64_t m[R];
+    const uint64_t upto = (2ull << lane) - 1ull;  // bits 0..lane (all 64 for lane 63)
+    uint64_t m[R], rs[R];
+    uint32_t wr[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) m[r] = w0 + r < words ? plane[w0 + r] : 0ull;
+    for (int r = 0; r < R; r++) {
+        const bool in = w0 + r < words;
+        m[r] = in ? plane[w0 + r] : 0ull;
+        rs[r] = in ? cr.run_start[w0 + r] : 0ull;
+        wr[r] = in ? cr.word_run[w0 + r] : 0u;
+    }
+    // Every lane loads (a lane whose edge does not fire reads the record of
+    // the span's first edge, which always exists and which one request serves
+    // for all such lanes): no branch, so the rounds' loads are all in flight
+    // together.  Edges are sorted by i and a run's slots under i rise by one
+    // per edge, so an edge's run is word_run + its marks up to its bit, and
+    // its slot under i is its index plus the run's offset.
+    uint2 pj[R], ri[R];
+    const uint32_t run0 = wr[0] + (uint32_t)(rs[0] & 1ull);  // the run of edge base
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const bool fire = (m[r] >> lane) & 1ull;
+        pj[r] = cr.pair[fire ? base + (uint64_t)r * 64 + lane : base];
+        ri[r] = cr.run_info[fire ? wr[r] + (uint32_t)__popcll(rs[r] & upto) : run0];
+    }
     uint32_t count = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        if ((m[r] >> lane) & 1ull)
-            queue[count + (uint32_t)__popcll(m[r] & below)] = (uint32_t)(base + (uint64_t)r * 64 + lane);
+        const uint32_t e = (uint32_t)(base + (uint64_t)r * 64 + lane);
+        queue_put(q, count, m[r], lane, e, make_uint4(ri[r].x, pj[r].x, e + ri[r].y, pj[r].y));
         count += (uint32_t)__popcll(m[r]);
     }
     return count;
@@ -308,9 +372,9 @@ __device__ __forceinline__ uint32_t queue_from_bits(uint64_t base, uint32_t* __r
 // Works through a wavefront's queue of `count` firing edges 64 at a time
 // (edge_work).  Every wavefront of the workgroup must call this: it holds a
 // workgroup barrier between building and reading the queues.
-template <int DIM, bool DENS>
-__device__ __forceinline__ void work_queue(const uint32_t* __restrict__ queue, uint32_t count,
-    const uint4* __restrict__ edges, const float* __restrict__ embd, float* __restrict__ grad,
+template <int DIM, bool DENS, int SPAN>
+__device__ __forceinline__ void work_queue(const SpanQueue<SPAN>& q, uint32_t count,
+    const float* __restrict__ embd, float* __restrict__ grad,
     uint32_t epoch, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
@@ -319,8 +383,9 @@ __device__ __forceinline__ void work_queue(const uint32_t* __restrict__ queue, u
     for (uint32_t q0 = 0; q0 < count; q0 += 64) {
         const uint32_t slot = q0 + lane;
         const bool active = slot < count;
-        const uint32_t e = active ? queue[slot] : 0u;
-        edge_work<DIM, DENS>(active, e, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
+        const uint32_t e = active ? q.e[slot] : 0u;
+        const uint4 rec = active ? q.rec[slot] : make_uint4(0u, 0u, 0u, 0u);
+        edge_work<DIM, DENS>(active, e, rec, embd, grad, epoch, seed, fm, a, b, two_a_b,
                              two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
     }
 }
@@ -328,15 +393,15 @@ __device__ __forceinline__ void work_queue(const uint32_t* __restrict__ queue, u
 // One wavefront's span of one Adam epoch: queue from the schedule cursors,
 // then work through it.  Every wavefront of the workgroup must call this.
 template <int DIM, bool DENS, int SPAN>
-__device__ __forceinline__ void epoch_edge_span(uint64_t base, uint32_t* __restrict__ queue,
+__device__ __forceinline__ void epoch_edge_span(uint64_t base, SpanQueue<SPAN>& q,
     const uint4* __restrict__ edges, const float* __restrict__ per, float* __restrict__ next,
     const float* __restrict__ embd, float* __restrict__ grad, uint32_t n_edges, uint32_t epoch,
     float epoch_f, uint32_t seed, FastMod fm, float a, float b, float two_a_b,
     float two_gamma_b, int neg_rate, const float* __restrict__ mu,
     const float4* __restrict__ dens_pack, const float* __restrict__ dens_scalars) {
-    const uint32_t count = queue_from_schedule<SPAN>(base, queue, per, next, n_edges, epoch_f);
-    work_queue<DIM, DENS>(queue, count, edges, embd, grad, epoch, seed, fm, a, b, two_a_b,
-                          two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
+    const uint32_t count = queue_from_schedule<SPAN>(base, q, edges, per, next, n_edges, epoch_f);
+    work_queue<DIM, DENS, SPAN>(q, count, embd, grad, epoch, seed, fm, a, b, two_a_b,
+                                two_gamma_b, neg_rate, mu, dens_pack, dens_scalars);
 }
 
 // Adam step of one node if it got an update this epoch (umap_adam_update,

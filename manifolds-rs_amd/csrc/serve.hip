@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kWg) void k_seg_epoch_edges(const FitDesc* __restri
                                                          const Piece* __restrict__ pieces,
                                                          Bins bins) {
     constexpr int kWaves = kWg / 64;
-    __shared__ uint32_t queue[kWaves][kSegSpan];
+    __shared__ SpanQueue<kSegSpan> queue[kWaves];
     const uint32_t x = blockIdx.x % kXcds, t = blockIdx.x / kXcds;
     uint32_t lo = bins.first[x], hi = bins.first[x + 1];
     if (lo == hi) return;

@@ -28,6 +28,11 @@ struct mrs_umap_state {
     // planes of ceil(E / 64) words; allocated by the first run of the
     // edge-parallel loop long enough to use it
     mrs::DevBuf<uint64_t> fire_bits;
+    // compact edge records of the bitmask epoch kernel (CompactRecords,
+    // epoch_device.hpp), built from `edges` with the first bitmask
+    mrs::DevBuf<uint2> rec_pair, run_info;
+    mrs::DevBuf<uint64_t> run_start;
+    mrs::DevBuf<uint32_t> word_run;
 
     // grad holds the edge strategy's cleared state: -0.0 everywhere, "no update
     // yet" (epoch_device.hpp); has_update is kept by the node-exact strategy
@@ -121,6 +126,9 @@ void launch_epoch_edges(mrs_umap_state* st, int32_t epoch, uint32_t seed, bool d
 // Firing bitmask of epochs [first, first + count), count <= kFireWindow, into
 // st->fire_bits; advances the schedule cursors past the window.
 void launch_fire_bits(mrs_umap_state* st, int32_t first, int32_t count);
+// The compact edge records (rec_pair, run_start, word_run, run_info) from
+// st->edges; inside a pooled StreamScope.
+void build_compact_records(mrs_umap_state* st);
 // The span launch_epoch_edges uses for this state when given 0.
 int epoch_span_choice(const mrs_umap_state* st, bool dens);
 // grad to -0.0 (no update yet), has_update from / to the markers
