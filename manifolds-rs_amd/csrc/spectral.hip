@@ -232,7 +232,19 @@ __global__ __launch_bounds__(kWg) void k_perm_rows(const uint32_t* __restrict__ 
 // scheduling.  Workgroups go to the 8 XCDs round-robin, so block b works on
 // row block (b mod 8) * (gridDim.x / 8) + b / 8: each XCD takes one
 // contiguous eighth of the rows (in component order).  The grid is a
-// multiple of 8 blocks (kSpmvXcds); blocks past the rows return.
+// multiple of 8 blocks (kSpmvXcds); blocks past the rows return.  Block 0
+// also opens Lanczos step j (lanczos_gate): nothing in the product reads the
+// liveness flags.
+// Start of Lanczos step j for component q: a component past its iteration
+// budget, or whose previous residual vanished (breakdown, math.rs:490-495),
+// stops; a live one records that step j belongs to its basis.
+__device__ inline void lanczos_gate(uint8_t* __restrict__ live, int* __restrict__ n_done,
+                                    const int* __restrict__ n_iter,
+                                    const double* __restrict__ beta_prev, uint32_t q, int j) {
+    if (live[q] && (j >= n_iter[q] || (beta_prev && beta_prev[q] < 1e-12))) live[q] = 0;
+    if (live[q]) n_done[q] = j + 1;
+}
+
 constexpr uint32_t kSpmvXcds = 8;
 constexpr int kSpmvLanes = 8;
 constexpr int kSpmvUnroll = 4;
@@ -241,7 +253,13 @@ __global__ __launch_bounds__(kWg) void k_spmv_rows(const uint32_t* __restrict__ 
                                                    const float* __restrict__ w,
                                                    const double* __restrict__ u,
                                                    const double* __restrict__ dinv_p, uint32_t n,
-                                                   double* __restrict__ y) {
+                                                   double* __restrict__ y, uint8_t* __restrict__ live,
+                                                   int* __restrict__ n_done,
+                                                   const int* __restrict__ n_iter,
+                                                   const double* __restrict__ beta_prev,
+                                                   uint32_t nl, int j) {
+    if (blockIdx.x == 0)
+        for (uint32_t q = threadIdx.x; q < nl; q += kWg) lanczos_gate(live, n_done, n_iter, beta_prev, q, j);
     const uint32_t per_xcd = gridDim.x / kSpmvXcds;
     const uint32_t blk = (blockIdx.x % kSpmvXcds) * per_xcd + blockIdx.x / kSpmvXcds;
     const uint64_t t = (uint64_t)blk * kWg + threadIdx.x;
@@ -369,6 +387,104 @@ __global__ __launch_bounds__(kWg) void k_cgs_update(const Chunk* __restrict__ ch
     if (threadIdx.x == 0) norm_partial[blockIdx.x] = sh[0];
 }
 
+// The coefficients h[k] (k < m) of one component from its chunks' partial
+// dot products (chunks [c0, c1)), in the order of k_multidot_finish: lane l
+// adds chunks c0 + l, c0 + l + 64, ... in order, then a fixed xor tree.  Wave
+// w of the block takes k = w, w + kWavesPerWg, ...; the results go to sh_h.
+__device__ inline void component_coefs(const double* __restrict__ partial, uint32_t c0,
+                                       uint32_t c1, int m, double* sh_h) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = wave; k < m; k += kWavesPerWg) {
+        double t = 0.0;
+        for (uint32_t q = c0 + lane; q < c1; q += 64) t += partial[(size_t)q * m + k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
+        if (lane == 0) sh_h[k] = t;
+    }
+    __syncthreads();
+}
+
+// k_multidot_finish and k_cgs_update in one launch, for components of at most
+// kFusedChunks chunks: each chunk's block sums its component's partials itself
+// (the same fixed order, so the same bits) and the component's first chunk
+// writes alpha.  Saves two launches per Lanczos step, most of the time of a
+// small component's step.
+constexpr uint32_t kFusedChunks = 64;
+constexpr int kMaxLanczos = 128;  // Lanczos steps: 4 * (n_comp + 1) + 30 <= 98
+__global__ __launch_bounds__(kWg) void k_cgs_update_fused(const Chunk* __restrict__ chunks,
+                                                          const uint32_t* __restrict__ comp_chunk0,
+                                                          double* __restrict__ w,
+                                                          const double* __restrict__ V, uint32_t n,
+                                                          const double* __restrict__ mpart, int m,
+                                                          const uint8_t* __restrict__ live,
+                                                          double* __restrict__ alpha, int pass,
+                                                          double* __restrict__ norm_partial) {
+    __shared__ double sh[kWg];
+    __shared__ double sh_h[kMaxLanczos];
+    const Chunk c = chunks[blockIdx.x];
+    const uint32_t c0 = comp_chunk0[c.comp];
+    component_coefs(mpart, c0, comp_chunk0[c.comp + 1], m, sh_h);
+    if (threadIdx.x == 0 && blockIdx.x == c0)
+        alpha[c.comp] = pass ? alpha[c.comp] + sh_h[m - 1] : sh_h[m - 1];
+    const bool on = live[c.comp] != 0;
+    double acc = 0.0;
+    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
+        double x = w[q];
+        if (on) {
+            double t = 0.0;
+            for (int k = 0; k < m; k++) t += V[(size_t)k * n + q] * sh_h[k];
+            x -= t;
+            w[q] = x;
+        }
+        acc += x * x;
+    }
+    if (!norm_partial) return;  // uniform per block
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kWg / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) norm_partial[blockIdx.x] = sh[0];
+}
+
+// k_seg_finish (sqrt) and k_advance in one launch, one block per chunk, for
+// components of at most kFusedChunks chunks: the block's first wave sums the
+// component's norm partials (the order of k_seg_finish), the component's
+// first chunk writes beta, and a live component's chunk rows advance.
+__global__ __launch_bounds__(kWg) void k_advance_fused(const Chunk* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ comp_chunk0,
+                                                       const double* __restrict__ partial,
+                                                       double* __restrict__ beta,
+                                                       double* __restrict__ u,
+                                                       double* __restrict__ vnext,
+                                                       const double* __restrict__ w,
+                                                       const double* __restrict__ dinv_p,
+                                                       const uint8_t* __restrict__ live) {
+    __shared__ double sh_b;
+    const Chunk c = chunks[blockIdx.x];
+    const uint32_t c0 = comp_chunk0[c.comp];
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        double t = 0.0;
+        for (uint32_t q = c0 + lane; q < comp_chunk0[c.comp + 1]; q += 64) t += partial[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
+        if (lane == 0) {
+            sh_b = sqrt(t);
+            if (blockIdx.x == c0) beta[c.comp] = sh_b;
+        }
+    }
+    __syncthreads();
+    if (!live[c.comp]) return;  // uniform per block
+    const double b = sh_b;
+    for (uint32_t i = c.begin + threadIdx.x; i < c.end; i += kWg) {
+        const double x = w[i] / b;
+        u[i] = x * dinv_p[i];
+        if (vnext) vnext[i] = x;
+    }
+}
+
 __device__ inline double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -435,19 +551,6 @@ __global__ __launch_bounds__(kWg) void k_mul_dinv(const double* v,
     if (i < n) u[i] = v[i] * dinv_p[i];
 }
 
-// Start of Lanczos step j: a component past its iteration budget, or whose
-// previous residual vanished (breakdown, math.rs:490-495), stops; a live one
-// records that step j belongs to its basis.
-__global__ __launch_bounds__(kWg) void k_lanczos_gate(uint8_t* __restrict__ live,
-                                                      int* __restrict__ n_done,
-                                                      const int* __restrict__ n_iter,
-                                                      const double* __restrict__ beta_prev,
-                                                      uint32_t nl, int j) {
-    const uint32_t q = blockIdx.x * kWg + threadIdx.x;
-    if (q >= nl) return;
-    if (live[q] && (j >= n_iter[q] || (beta_prev && beta_prev[q] < 1e-12))) live[q] = 0;
-    if (live[q]) n_done[q] = j + 1;
-}
 
 __global__ __launch_bounds__(kWg) void k_scale_seg(double* __restrict__ v,
                                                    const double* __restrict__ s,
@@ -973,7 +1076,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_HIP(hipMemsetAsync(live.p, 1, nl, s));
         MRS_HIP(hipMemsetAsync(dn_done.p, 0, dn_done.bytes(), s));
         MRS_HIP(hipMemcpyAsync(dn_iter.p, n_iter_c.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
-        const dim3 gn(div_up(n, kWg)), gq(div_up(nl, kWg)), gqw(div_up(nl, kWavesPerWg));
+        const dim3 gn(div_up(n, kWg)), gqw(div_up(nl, kWavesPerWg));
         auto seg = [&](const double* a, const double* b, double* outp, int do_sqrt) {
             hipLaunchKernelGGL(k_seg_dot, dim3(nch), dim3(kWg), 0, s, dchunks.p, a, b, partial.p);
             hipLaunchKernelGGL(k_seg_finish, gqw, dim3(kWg), 0, s, partial.p, dchunk0.p, nl,
@@ -987,13 +1090,18 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_CHECK_LAUNCH();
         const unsigned spmv_blocks =
             div_up(div_up((uint64_t)n * kSpmvLanes, kWg), kSpmvXcds) * kSpmvXcds;
+        // components of at most kFusedChunks chunks take the fused kernels
+        // (six launches per step instead of nine)
+        uint32_t max_chunks = 0;
+        for (uint32_t q = 0; q < nl; q++) max_chunks = std::max(max_chunks, chunk0[q + 1] - chunk0[q]);
+        const bool fused = max_chunks <= kFusedChunks;
+        if (max_iter > kMaxLanczos) throw Error(MRS_INVALID_ARGUMENT, "too many Lanczos steps");
         for (int j = 0; j < max_iter; j++) {
             double* alpha_j = alpha_d.p + (size_t)j * nl;
             double* beta_j = beta_d.p + (size_t)j * nl;
-            hipLaunchKernelGGL(k_lanczos_gate, gq, dim3(kWg), 0, s, live.p, dn_done.p, dn_iter.p,
-                               j > 0 ? beta_j - nl : nullptr, nl, j);
             hipLaunchKernelGGL(k_spmv_rows, dim3(spmv_blocks), dim3(kWg), 0, s, poff.p, pcol.p,
-                               pw.p, v.p, dinv_p.p, n, w.p);
+                               pw.p, v.p, dinv_p.p, n, w.p, live.p, dn_done.p, dn_iter.p,
+                               j > 0 ? beta_j - nl : nullptr, nl, j);
             // Orthogonalise w = M v_j against V_0..V_j by classical Gram-Schmidt
             // applied twice.  The first pass's projections on v_j and v_{j-1}
             // are alpha_j and beta_{j-1}, so it also performs the three-term
@@ -1003,17 +1111,28 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             for (int pass = 0; pass < 2; pass++) {
                 hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
                                    dchunks.p, V.p, n, w.p, m, mpart.p);
-                hipLaunchKernelGGL(k_multidot_finish, dim3(div_up((uint64_t)nl * m, kWavesPerWg)),
-                                   dim3(kWg), 0, s, mpart.p, dchunk0.p, nl, m, hcoef.p, alpha_j,
-                                   pass);
-                hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, w.p, V.p,
-                                   n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
+                if (fused) {
+                    hipLaunchKernelGGL(k_cgs_update_fused, dim3(nch), dim3(kWg), 0, s, dchunks.p,
+                                       dchunk0.p, w.p, V.p, n, mpart.p, m, live.p, alpha_j, pass,
+                                       pass ? partial.p : nullptr);
+                } else {
+                    hipLaunchKernelGGL(k_multidot_finish,
+                                       dim3(div_up((uint64_t)nl * m, kWavesPerWg)), dim3(kWg), 0,
+                                       s, mpart.p, dchunk0.p, nl, m, hcoef.p, alpha_j, pass);
+                    hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(kWg), 0, s, dchunks.p, w.p,
+                                       V.p, n, hcoef.p, m, live.p, pass ? partial.p : nullptr);
+                }
             }
-            hipLaunchKernelGGL(k_seg_finish, gqw, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
-                               beta_j);
-            hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p,
-                               j + 1 < max_iter ? V.p + (size_t)(j + 1) * n : nullptr, w.p, beta_j,
-                               dinv_p.p, dlc.p, live.p, n);
+            double* vnext = j + 1 < max_iter ? V.p + (size_t)(j + 1) * n : nullptr;
+            if (fused) {
+                hipLaunchKernelGGL(k_advance_fused, dim3(nch), dim3(kWg), 0, s, dchunks.p,
+                                   dchunk0.p, partial.p, beta_j, v.p, vnext, w.p, dinv_p.p, live.p);
+            } else {
+                hipLaunchKernelGGL(k_seg_finish, gqw, dim3(kWg), 0, s, partial.p, dchunk0.p, nl, 1,
+                                   beta_j);
+                hipLaunchKernelGGL(k_advance, gn, dim3(kWg), 0, s, v.p, vnext, w.p, beta_j,
+                                   dinv_p.p, dlc.p, live.p, n);
+            }
             MRS_CHECK_LAUNCH();
         }
         MRS_HIP(hipMemcpyAsync(n_done.data(), dn_done.p, sizeof(int) * nl, hipMemcpyDeviceToHost, s));
