@@ -170,21 +170,21 @@ __global__ __launch_bounds__(kScanWg) void k_scan_tiles(const uint32_t* __restri
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
+// Adds each tile's offset; with total, the last tile also writes the grand
+// total (its offset plus its own sum).
 __global__ __launch_bounds__(kScanWg) void k_scan_add(uint32_t* __restrict__ out, uint64_t n,
-                                                      const uint32_t* __restrict__ tile_offs) {
+                                                      const uint32_t* __restrict__ tile_offs,
+                                                      const uint32_t* __restrict__ tile_sums,
+                                                      uint32_t* __restrict__ total) {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     const uint32_t add = tile_offs[blockIdx.x];
+    if (total && threadIdx.x == 0 && blockIdx.x + 1 == gridDim.x) *total = add + tile_sums[blockIdx.x];
     for (int q = threadIdx.x; q < kScanTile; q += kScanWg)
         if (base + q < n) out[base + q] += add;
 }
 
 // total = last exclusive prefix + the saved last input element (last_in holds
 // exactly one value: the input may have been overwritten by an in-place scan)
-__global__ void k_copy_total(const uint32_t* __restrict__ scanned,
-                             const uint32_t* __restrict__ last_in, uint64_t n,
-                             uint32_t* __restrict__ total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = scanned[n - 1] + last_in[0];
-}
 
 constexpr int kProbeU = 8;  // gathers in flight per thread in the probe below
 
@@ -357,24 +357,24 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t*
         return;
     }
     const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 1) {
+        // one tile: its sum is the total
+        DevBuf<uint32_t> sum1(total_dev ? 0 : 1);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanWg), 0, s, in, out, n,
+                           total_dev ? total_dev : sum1.p);
+        MRS_CHECK_LAUNCH();
+        if (sum1.freed_at_once()) MRS_HIP(hipStreamSynchronize(s));
+        return;
+    }
     DevBuf<uint32_t> sums(tiles), offs(tiles);
-    // the last input element is needed for the total when scanning in place
-    DevBuf<uint32_t> last(1);
-    MRS_HIP(hipMemcpyAsync(last.p, in + n - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanWg), 0, s, in, out, n,
                        sums.p);
     MRS_CHECK_LAUNCH();
-    if (tiles > 1) {
-        uint32_t* dummy_total = nullptr;
-        exclusive_scan_u32(sums.p, offs.p, tiles, dummy_total, s);
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanWg), 0, s, out, n,
-                           offs.p);
-        MRS_CHECK_LAUNCH();
-    }
-    if (total_dev) {
-        hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, out, last.p, n, total_dev);
-        MRS_CHECK_LAUNCH();
-    }
+    exclusive_scan_u32(sums.p, offs.p, tiles, nullptr, s);
+    // the last tile writes the total: its offset plus its sum (sums is unchanged)
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanWg), 0, s, out, n, offs.p,
+                       sums.p, total_dev);
+    MRS_CHECK_LAUNCH();
     // temporaries are freed on return: unless that free is stream-ordered it
     // is immediate, so the queued work must be finished first
     if (sums.freed_at_once()) MRS_HIP(hipStreamSynchronize(s));

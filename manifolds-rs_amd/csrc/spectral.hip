@@ -407,8 +407,13 @@ __device__ inline void component_coefs(const double* __restrict__ partial, uint3
 // k_multidot_finish and k_cgs_update in one launch, for components of at most
 // kFusedChunks chunks: each chunk's block sums its component's partials itself
 // (the same fixed order, so the same bits) and the component's first chunk
-// writes alpha.  Saves two launches per Lanczos step, most of the time of a
-// small component's step.
+// writes alpha.  With next_partial (the first pass), the block then computes
+// the second pass's partial products of its chunk, V_k . w for k < m, from
+// the rows it has just updated (each thread's rows stay in registers and its
+// V entries come back from cache), in place of that pass's k_multidot: each
+// thread adds its rows in order, a fixed xor tree joins a wave's lanes and the
+// waves are added in order.  A small component's Lanczos step is then five
+// launches instead of nine.
 constexpr uint32_t kFusedChunks = 64;
 constexpr int kMaxLanczos = 128;  // Lanczos steps: 4 * (n_comp + 1) + 30 <= 98
 __global__ __launch_bounds__(kWg) void k_cgs_update_fused(const Chunk* __restrict__ chunks,
@@ -418,9 +423,12 @@ __global__ __launch_bounds__(kWg) void k_cgs_update_fused(const Chunk* __restric
                                                           const double* __restrict__ mpart, int m,
                                                           const uint8_t* __restrict__ live,
                                                           double* __restrict__ alpha, int pass,
-                                                          double* __restrict__ norm_partial) {
+                                                          double* __restrict__ norm_partial,
+                                                          double* __restrict__ next_partial) {
+    constexpr int kRows = kChunk / kWg;  // rows per thread
     __shared__ double sh[kWg];
     __shared__ double sh_h[kMaxLanczos];
+    __shared__ double sh_p[kWavesPerWg][kMaxLanczos];
     const Chunk c = chunks[blockIdx.x];
     const uint32_t c0 = comp_chunk0[c.comp];
     component_coefs(mpart, c0, comp_chunk0[c.comp + 1], m, sh_h);
@@ -428,15 +436,43 @@ __global__ __launch_bounds__(kWg) void k_cgs_update_fused(const Chunk* __restric
         alpha[c.comp] = pass ? alpha[c.comp] + sh_h[m - 1] : sh_h[m - 1];
     const bool on = live[c.comp] != 0;
     double acc = 0.0;
-    for (uint32_t q = c.begin + threadIdx.x; q < c.end; q += kWg) {
-        double x = w[q];
-        if (on) {
-            double t = 0.0;
-            for (int k = 0; k < m; k++) t += V[(size_t)k * n + q] * sh_h[k];
-            x -= t;
-            w[q] = x;
+    double xs[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t q = c.begin + threadIdx.x + (uint32_t)r * kWg;
+        xs[r] = 0.0;
+        if (q < c.end) {
+            double x = w[q];
+            if (on) {
+                double t = 0.0;
+                for (int k = 0; k < m; k++) t += V[(size_t)k * n + q] * sh_h[k];
+                x -= t;
+                w[q] = x;
+            }
+            xs[r] = x;
+            acc += x * x;
         }
-        acc += x * x;
+    }
+    if (next_partial) {  // uniform per block
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int k = 0; k < m; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRows; r++) {
+                const uint32_t q = c.begin + threadIdx.x + (uint32_t)r * kWg;
+                if (q < c.end) t += V[(size_t)k * n + q] * xs[r];
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0) sh_p[wave][k] = t;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += kWg) {
+            double t = sh_p[0][k];
+#pragma unroll
+            for (int wv = 1; wv < kWavesPerWg; wv++) t += sh_p[wv][k];
+            next_partial[(size_t)blockIdx.x * m + k] = t;
+        }
     }
     if (!norm_partial) return;  // uniform per block
     sh[threadIdx.x] = acc;
@@ -1068,7 +1104,8 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         MRS_CHECK_LAUNCH();
         DevBuf<double> v(n), w(n), V((size_t)max_iter * n), partial(nch), sc(nl),
             alpha_d((size_t)max_iter * nl), beta_d((size_t)max_iter * nl),
-            mpart((size_t)nch * max_iter), hcoef((size_t)nl * max_iter);
+            mpart((size_t)nch * max_iter), mpart2((size_t)nch * max_iter),
+            hcoef((size_t)nl * max_iter);
         // per-component control stays on the device: the loop below issues no
         // host synchronisation until the tridiagonal coefficients are read back
         DevBuf<uint8_t> live(nl);
@@ -1091,7 +1128,7 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
         const unsigned spmv_blocks =
             div_up(div_up((uint64_t)n * kSpmvLanes, kWg), kSpmvXcds) * kSpmvXcds;
         // components of at most kFusedChunks chunks take the fused kernels
-        // (six launches per step instead of nine)
+        // (five launches per step instead of nine)
         uint32_t max_chunks = 0;
         for (uint32_t q = 0; q < nl; q++) max_chunks = std::max(max_chunks, chunk0[q + 1] - chunk0[q]);
         const bool fused = max_chunks <= kFusedChunks;
@@ -1109,12 +1146,16 @@ void spectral_layout_dev(mrs_ctx* ctx, const mrs_graph* g, int32_t dim, uint64_t
             // projections, and the second pass emits the norm partials of w.
             const int m = j + 1;
             for (int pass = 0; pass < 2; pass++) {
-                hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
-                                   dchunks.p, V.p, n, w.p, m, mpart.p);
+                // fused: the second pass's partial products come from the first
+                // pass's update (into mpart2)
+                if (!fused || pass == 0)
+                    hipLaunchKernelGGL(k_multidot, dim3(nch, div_up(m, kKB)), dim3(kWg), 0, s,
+                                       dchunks.p, V.p, n, w.p, m, mpart.p);
                 if (fused) {
                     hipLaunchKernelGGL(k_cgs_update_fused, dim3(nch), dim3(kWg), 0, s, dchunks.p,
-                                       dchunk0.p, w.p, V.p, n, mpart.p, m, live.p, alpha_j, pass,
-                                       pass ? partial.p : nullptr);
+                                       dchunk0.p, w.p, V.p, n, pass ? mpart2.p : mpart.p, m,
+                                       live.p, alpha_j, pass, pass ? partial.p : nullptr,
+                                       pass ? nullptr : mpart2.p);
                 } else {
                     hipLaunchKernelGGL(k_multidot_finish,
                                        dim3(div_up((uint64_t)nl * m, kWavesPerWg)), dim3(kWg), 0,
