@@ -797,8 +797,8 @@ def bench_densmap(args):
     # counter traffic: the plain and density instantiations' per-launch HBM
     # bytes from the profile summary, weighted by their launches in the fit
     span = 128 if E >= (1 << 20) else 64
-    t_plain, src = pmc_traffic("densmap", f"k_epoch_edges<2, false, {span}>")
-    t_dens, _ = pmc_traffic("densmap", f"k_epoch_edges<2, true, {span}>")
+    t_plain, src = pmc_traffic("densmap", f"k_epoch_edges<2, false, {span},")
+    t_dens, _ = pmc_traffic("densmap", f"k_epoch_edges<2, true, {span},")
     n_dens = int(dens_ep.sum())
     traffic = (None if t_plain is None or t_dens is None
                else ((n_epochs - n_dens) * t_plain + n_dens * t_dens) / n_epochs)
