@@ -503,7 +503,8 @@ def test_edge_kernel_wide_span_teacher_forced(ctx):
     assert fired > n_edges // 4
 
 
-def test_edge_loop_firing_bitmask_windows(ctx):
+@pytest.mark.parametrize("dim", [2, 1, 3, 8])
+def test_edge_loop_firing_bitmask_windows(ctx, dim):
     # Edge-parallel runs of 8 or more epochs take each epoch's firing edges
     # from bitmask windows of up to 128 epochs, built ahead by an exact replay
     # of the schedule.  With lr = 0 the embedding stays put, so every epoch's
@@ -513,21 +514,22 @@ def test_edge_loop_firing_bitmask_windows(ctx):
     # schedule cursors exact; m (decay 0.5: the last few epochs' gradients
     # dominate) within the gradient bar accumulated over the epochs; v likewise
     # through g^2.  An edge that fires in the wrong epoch moves m of its
-    # endpoints by a whole term.
+    # endpoints by a whole term.  Each dimension has its own lane layout for
+    # the gradient atomics (a node's coordinates in one instruction).
     adj, n_edges = _random_sparse_graph(5000, 7, seed=67)
     n = adj[0].size - 1
-    embd = np.random.default_rng(67).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    embd = np.random.default_rng(67).uniform(-10, 10, size=(n, dim)).astype(np.float32)
     p = M.UmapOptimParams.default_2d()
     p.n_epochs = 300
     p.lr = 0.0
     st, _ = make_state(ctx, embd, adj, p)
     c = oracle_csr(adj)
     gtol = 1e-4
-    m = np.zeros(n * 2, np.float32)
-    v = np.zeros(n * 2, np.float32)
+    m = np.zeros(n * dim, np.float32)
+    v = np.zeros(n * dim, np.float32)
     nx = c["per"].copy()
-    dm = np.zeros(n * 2)  # accumulated bars of m and v
-    dv = np.zeros(n * 2)
+    dm = np.zeros(n * dim)  # accumulated bars of m and v
+    dv = np.zeros(n * dim)
     e = 0
     for count in (8, 130, 129, 33):
         st.run_epochs(e, count, 42, M.EPOCH_EDGE_ATOMIC)
@@ -537,7 +539,7 @@ def test_edge_loop_firing_bitmask_windows(ctx):
             ew, m2, v2 = orc.adam_update(g, hu, m, v, embd, p, t, p.n_epochs)
             assert np.array_equal(ew, embd)
             bar = gtol * absum.ravel()
-            upd = np.repeat(hu.astype(bool), 2)
+            upd = np.repeat(hu.astype(bool), dim)
             dm = np.where(upd, p.beta1 * dm + (1 - p.beta1) * bar, dm)
             dv = np.where(upd, p.beta2 * dv + (1 - p.beta2) * (2 * np.abs(g.ravel()) * bar + bar ** 2),
                           dv)
@@ -551,7 +553,8 @@ def test_edge_loop_firing_bitmask_windows(ctx):
         assert mbad.size == 0, f"m after epoch {e - 1}: {mbad.size} coords over the bar"
         assert vbad.size == 0, f"v after epoch {e - 1}: {vbad.size} coords over the bar"
         assert np.array_equal(st.read(M.BUF_EMBD), embd.ravel())
-    print(f"firing bitmask windows: {n_edges} edges, 300 epochs, schedule exact, m and v in bars")
+    print(f"firing bitmask windows, dim {dim}: {n_edges} edges, 300 epochs, schedule exact, "
+          f"m and v in bars")
 
 
 def test_edge_kernel_hub_degree_above_65535(ctx):
