@@ -113,9 +113,10 @@ void *mrs_ctx_stream(mrs_ctx *ctx);
 size_t mrs_ctx_cached_bytes(mrs_ctx *ctx);
 int mrs_ctx_release_cache(mrs_ctx *ctx);
 /* Exhaustive kNN on this context skips database stages that provably hold no
- * candidate (on = 1, the default) or sweeps every stage (on = 0).  Both give
- * the same results; the switch exists for comparisons.  No counterpart in the
- * reference. */
+ * candidate (on = 1, the default), or sweeps every stage: in cell order, each
+ * query block's own cell first (on = 0), or in row order (on = -1; below
+ * 65 536 rows every sweep runs in row order).  All give the same results;
+ * the switch exists for comparisons.  No counterpart in the reference. */
 int mrs_ctx_set_knn_pruning(mrs_ctx *ctx, int32_t on);
 /* The pruned sweep cuts a query block's second-pass stage list longer than
  * `stages` into work items whose partial lists a merge kernel folds back in;

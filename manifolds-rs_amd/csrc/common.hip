@@ -546,7 +546,7 @@ int mrs_probe_random_atomic(mrs_ctx* ctx, int64_t rows, int64_t ops, int32_t rep
 int mrs_ctx_set_knn_pruning(mrs_ctx* ctx, int32_t on) {
     return mrs::guarded([&] {
         if (!ctx) throw mrs::Error(MRS_INVALID_ARGUMENT, "ctx is NULL");
-        ctx->knn_prune = on ? 1 : 0;
+        ctx->knn_prune = on > 0 ? 1 : (on < 0 ? -1 : 0);
     });
 }
 

@@ -241,11 +241,15 @@ constexpr uint32_t kMainList = 0xFFFFFFFFu;
 // Wave-wide merge of query b's queue into its sorted global list; returns the
 // list's new threshold key (its last entry).  The queue is two half-queues of
 // QC/2 entries, one per accumulator half (lanes b and b + 32), holding c0 and
-// c1 entries: lane l reads entry l % 32 of half l / 32.
+// c1 entries: lane l reads entry l % 32 of half l / 32.  With a rowmap, queue
+// entries carry database positions, which become the caller's rows here (one
+// gather per merge rather than one dependent load per queued candidate), so
+// the list keys, and the (score, row) order of the merge, are the caller's.
 template <int E, int QC>
 __device__ inline uint64_t merge_query(int b, uint32_t c0, uint32_t c1, int lane,
                                        uint64_t* __restrict__ lists,
-                                       const uint64_t* __restrict__ myq) {
+                                       const uint64_t* __restrict__ myq,
+                                       const uint32_t* __restrict__ rowmap) {
     static_assert(QC == 64, "two half-queues of 32: one entry per lane");
     constexpr int S = 64 * E;
     // the queue entries were stored by this wave's lanes: complete those
@@ -253,6 +257,8 @@ __device__ inline uint64_t merge_query(int b, uint32_t c0, uint32_t c1, int lane
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const uint32_t c = lane < 32 ? c0 : c1;
     uint64_t one[1] = {(uint32_t)(lane & 31) < c ? myq[b * QC + lane] : kInfKey};
+    if (rowmap && one[0] != kInfKey)
+        one[0] = (one[0] & 0xFFFFFFFF00000000ull) | rowmap[(uint32_t)one[0]];
     // the list load is issued before the queue sort, so its latency overlaps
     // the sort instead of following it (each lane reads back only the list
     // entries it wrote itself)
@@ -277,27 +283,38 @@ __device__ inline uint64_t merge_query(int b, uint32_t c0, uint32_t c1, int lane
 // between the lanes; the threshold is identical in both.  Most tiles leave on
 // the max/ballot test.  A query with a half-queue that could overflow on the
 // next tile (more than QC/2 - 16 entries) is merged.
-//   Full sweep (PR = false): the stages come in row order, so a score equal to
-// the threshold can only belong to a larger row and is not queued.  Pruned
-// sweep (PR = true): stages come in cell order, so an equal score is queued
-// too and the merge, which keeps the S smallest (score, row) keys, decides the
-// tie by row; keys carry the caller's row (rowmap).  The query's own row
-// never reaches here (the kernel sets its score to -inf first), and in the
+//   Row-order sweep (PR = false): the stages come in row order, so a score
+// equal to the threshold can only belong to a larger row and is not queued.
+// Cell-order sweep (PR = true): an equal score is queued too and the merge,
+// which keeps the S smallest (score, row) keys, decides the tie by row; queued
+// keys carry the database position, which the merge maps to the caller's row
+// (rowmap).  In the tile that holds the wave's own query rows (self_tile, a
+// wave-uniform flag) the query's own score is set to -inf before any score is
+// queued; it is the largest score of the tile, so that tile always gets past
+// the max/ballot test, and the test costs nothing extra elsewhere.  In the
 // pruned sweep the threshold never drops below -FLT_MAX, so padding rows
 // (-inf) fail the test like every row below the threshold: one compare per
-// score decides.
-template <int E, int QC, bool PR>
-__device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, int lane,
-                                float& thr, uint32_t& qn, uint64_t* __restrict__ lists,
-                                uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap,
-                                float floor) {
-    float gm[4];  // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3)
+// score decides.  gm and m (group_max) come in from the caller, which computes
+// them while the next tile's MFMAs run.
+__device__ inline void mask_self(f32x16& acc, int h, int col);
+
+__device__ inline void group_max(const f32x16& acc, float (&gm)[4], float& m) {
+    // maxima of the four 4-row groups (rows i*8 + h*4 + 0..3), then of the tile
 #pragma unroll
     for (int g = 0; g < 4; g++)
         gm[g] = fmaxf(fmaxf(acc[g * 4], acc[g * 4 + 1]), fmaxf(acc[g * 4 + 2], acc[g * 4 + 3]));
-    const float m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
+    m = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
+}
+
+template <int E, int QC, bool PR>
+__device__ inline void epilogue(f32x16 acc, const float (&gm)[4], float m, bool self_tile,
+                                uint32_t rowbase, int h, int col, int lane, float& thr,
+                                uint32_t& qn, uint64_t* __restrict__ lists,
+                                uint64_t* __restrict__ myq, const uint32_t* __restrict__ rowmap,
+                                float floor) {
     auto reach = [&](float v) { return PR ? v >= thr : v > thr; };
     if (__ballot(reach(m)) == 0ull) return;
+    if (self_tile) mask_self(acc, h, col);
     auto take = [&](int r) { return reach(acc[r]); };
     uint64_t* dst = myq + col * QC + h * (QC / 2);
     // a group no element of which reaches the threshold is skipped as a whole
@@ -308,7 +325,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, in
             for (int j = 0; j < 4; j++)
                 if (take(g * 4 + j)) {
                     const uint32_t row = rowbase + g * 8 + h * 4 + j;
-                    dst[qn++] = score_key(acc[g * 4 + j], PR ? rowmap[row] : row);
+                    dst[qn++] = score_key(acc[g * 4 + j], row);
                 }
         }
     const uint64_t full = __ballot(qn > (uint32_t)(QC / 2 - 16));
@@ -318,7 +335,7 @@ __device__ inline void epilogue(f32x16 acc, uint32_t rowbase, int h, int col, in
         need &= need - 1;
         const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b),
                                               __builtin_amdgcn_readlane(qn, b + 32), lane, lists,
-                                              myq);
+                                              myq, PR ? rowmap : nullptr);
         if (col == b) {
             qn = 0;
             thr = fmaxf(key_score(t), floor);  // an aux list never goes below its floor
@@ -391,7 +408,13 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     const int lane = tid & 63, h = lane >> 5, col = lane & 31;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     uint4 item = make_uint4(blockIdx.x, 0u, 0u, kMainList);
-    if (PR) item = a.items[blockIdx.x];
+    if (PR) {
+        // the same item for every lane: readfirstlane tells the compiler, so
+        // the stage loop, its bounds and the list addresses are scalar
+        const uint4 it = a.items[blockIdx.x];
+        item = make_uint4(__builtin_amdgcn_readfirstlane(it.x), __builtin_amdgcn_readfirstlane(it.y),
+                          __builtin_amdgcn_readfirstlane(it.z), __builtin_amdgcn_readfirstlane(it.w));
+    }
     const uint32_t blk_id = item.x;
     const bool aux = PR && item.w != kMainList;
     const uint32_t qbase = (blk_id * W + wave) * kQPerWave;
@@ -453,8 +476,23 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
         nst = item.z;
         sid = a.stage_ids + item.y;
     }
-    auto srow = [&](uint32_t t) -> uint32_t { return PR ? sid[t] * (uint32_t)RT : db0 + t * RT; };
-    if (nst > 0) stage_copy<NCOPY, RT, W>(P, a.negh, srow(0), row_elems_bytes, goff, tile0, negh0, wave, lane);
+    // The list's stage ids come 64 at a time in a register (lane i: stage
+    // c*64 + i of chunk c; idn holds chunk c + 1, loaded a chunk ahead), and
+    // readlane hands each id to the scalar unit: no stage waits on a load of
+    // its id, and no wait on such a load also waits on the stage copy in flight.
+    auto load_ids = [&](uint32_t t0) -> uint32_t { return t0 + lane < nst ? sid[t0 + lane] : 0u; };
+    uint32_t idc = 0, idn = 0;
+    if (PR) {
+        idc = load_ids(0);
+        idn = load_ids(64);
+    }
+    // row of stage t, for t in the current chunk or the next one
+    auto srow = [&](uint32_t t, bool next_chunk) -> uint32_t {
+        if (!PR) return db0 + t * RT;
+        const uint32_t v = __builtin_amdgcn_readlane(next_chunk ? idn : idc, (int)(t & 63u));
+        return v * (uint32_t)RT;
+    };
+    if (nst > 0) stage_copy<NCOPY, RT, W>(P, a.negh, srow(0, false), row_elems_bytes, goff, tile0, negh0, wave, lane);
     // everything loaded so far (query fragments, stage 0) is complete here
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -467,88 +505,131 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     auto step = [&](const unsigned char* __restrict__ cur, const float* __restrict__ curn,
                     unsigned char* __restrict__ nxt, float* __restrict__ nxtn, uint32_t st) {
         if (st + 1 < nst)
-            stage_copy<NCOPY, RT, W>(P, a.negh, srow(st + 1), row_elems_bytes, goff, nxt, nxtn,
-                                     wave, lane);
-        const uint32_t r0 = srow(st);
+            stage_copy<NCOPY, RT, W>(P, a.negh, srow(st + 1, ((st + 1) & 63u) == 0u),
+                                     row_elems_bytes, goff, nxt, nxtn, wave, lane);
+        const uint32_t r0 = srow(st, false);
+        if constexpr (BF16 && STEPS > 0 && RT >= 64) {
+            // Tiles in a software pipeline: tile t + 1's bias loads and MFMA
+            // chain are issued before tile t's group maxima, so the maxima's
+            // VALU work runs while the MFMAs do, in this wave as in others.
+            constexpr int NT = RT / 32;
+            f32x16 acc[2];
+            auto init_tile = [&](f32x16& c, int t) {
 #pragma unroll
-        for (int sp = 0; sp < RT / 32; sp += 2) {
-            // Two independent accumulation chains: the stage's two 32-row MFMA
-            // tiles, or (32-row stages) the even and odd K steps of one tile.
-            constexpr bool kPair = RT >= 64;
-            f32x16 acc0, acc1;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float4 n0 = *reinterpret_cast<const float4*>(curn + sp * 32 + i * 8 + h * 4);
-                acc0[i * 4 + 0] = n0.x; acc0[i * 4 + 1] = n0.y; acc0[i * 4 + 2] = n0.z; acc0[i * 4 + 3] = n0.w;
-                if (kPair) {
-                    const float4 n1 = *reinterpret_cast<const float4*>(curn + (sp + 1) * 32 + i * 8 + h * 4);
-                    acc1[i * 4 + 0] = n1.x; acc1[i * 4 + 1] = n1.y; acc1[i * 4 + 2] = n1.z; acc1[i * 4 + 3] = n1.w;
-                } else {
-                    acc1[i * 4 + 0] = 0.0f; acc1[i * 4 + 1] = 0.0f; acc1[i * 4 + 2] = 0.0f; acc1[i * 4 + 3] = 0.0f;
+                for (int i = 0; i < 4; i++) {
+                    const float4 nv = *reinterpret_cast<const float4*>(curn + t * 32 + i * 8 + h * 4);
+                    c[i * 4 + 0] = nv.x;
+                    c[i * 4 + 1] = nv.y;
+                    c[i * 4 + 2] = nv.z;
+                    c[i * 4 + 3] = nv.w;
                 }
+            };
+            auto chain_tile = [&](f32x16& c, int t) {
+                const bf16x8* ap = reinterpret_cast<const bf16x8*>(cur + (t * 32 + col) * row_bytes + h * KP);
+#pragma unroll
+                for (int s = 0; s < STEPS; s++)
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[s], bq[s], c, 0, 0, 0);
+            };
+            init_tile(acc[0], 0);
+            chain_tile(acc[0], 0);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                if (t + 1 < NT) {
+                    init_tile(acc[(t + 1) & 1], t + 1);
+                    chain_tile(acc[(t + 1) & 1], t + 1);
+                }
+                float gm[4], m;
+                group_max(acc[t & 1], gm, m);
+                epilogue<E, QC, PR>(acc[t & 1], gm, m, r0 + t * 32 == qbase, r0 + t * 32, h, col,
+                                    lane, thr, qn, lists, myq, a.rowmap, floor);
             }
-            const unsigned char* rowA0 = cur + (sp * 32 + col) * row_bytes;
-            const unsigned char* rowA1 = kPair ? cur + ((sp + 1) * 32 + col) * row_bytes : rowA0;
-            if (BF16) {
-                const bf16x8* a0 = reinterpret_cast<const bf16x8*>(rowA0 + h * KP);
-                const bf16x8* a1 = reinterpret_cast<const bf16x8*>(rowA1 + h * KP);
+        } else {
 #pragma unroll
-                for (int s = 0; s < MAXSTEPS; s++)
-                    if (s < steps) {
-                        if (kPair) {
-                            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
-                            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[s], bq[s], acc1, 0, 0, 0);
-                        } else if (s & 1) {
-                            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc1, 0, 0, 0);
-                        } else {
-                            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
-                        }
+            for (int sp = 0; sp < RT / 32; sp += 2) {
+                // Two independent accumulation chains: the stage's two 32-row MFMA
+                // tiles, or (32-row stages) the even and odd K steps of one tile.
+                constexpr bool kPair = RT >= 64;
+                f32x16 acc0, acc1;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float4 n0 = *reinterpret_cast<const float4*>(curn + sp * 32 + i * 8 + h * 4);
+                    acc0[i * 4 + 0] = n0.x; acc0[i * 4 + 1] = n0.y; acc0[i * 4 + 2] = n0.z; acc0[i * 4 + 3] = n0.w;
+                    if (kPair) {
+                        const float4 n1 = *reinterpret_cast<const float4*>(curn + (sp + 1) * 32 + i * 8 + h * 4);
+                        acc1[i * 4 + 0] = n1.x; acc1[i * 4 + 1] = n1.y; acc1[i * 4 + 2] = n1.z; acc1[i * 4 + 3] = n1.w;
+                    } else {
+                        acc1[i * 4 + 0] = 0.0f; acc1[i * 4 + 1] = 0.0f; acc1[i * 4 + 2] = 0.0f; acc1[i * 4 + 3] = 0.0f;
                     }
-            } else {
-                // K steps come in fours (KP is a multiple of 8): one 16-byte LDS
-                // read feeds four MFMAs
-                const float4* a0 = reinterpret_cast<const float4*>(rowA0) + h * (KP / 8);
-                const float4* a1 = reinterpret_cast<const float4*>(rowA1) + h * (KP / 8);
+                }
+                const unsigned char* rowA0 = cur + (sp * 32 + col) * row_bytes;
+                const unsigned char* rowA1 = kPair ? cur + ((sp + 1) * 32 + col) * row_bytes : rowA0;
+                if (BF16) {
+                    const bf16x8* a0 = reinterpret_cast<const bf16x8*>(rowA0 + h * KP);
+                    const bf16x8* a1 = reinterpret_cast<const bf16x8*>(rowA1 + h * KP);
 #pragma unroll
-                for (int s4 = 0; s4 < MAXSTEPS / 4; s4++)
-                    if (s4 * 4 < steps) {
-                        const float4 x0 = a0[s4];
-                        const float4 x1 = kPair ? a1[s4] : x0;
-                        const float v0[4] = {x0.x, x0.y, x0.z, x0.w};
-                        const float v1[4] = {x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int s = s4 * 4 + u;
+                    for (int s = 0; s < MAXSTEPS; s++)
+                        if (s < steps) {
                             if (kPair) {
-                                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
-                                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1[u], bqf[s], acc1, 0, 0, 0);
-                            } else if (u & 1) {
-                                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc1, 0, 0, 0);
+                                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
+                                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[s], bq[s], acc1, 0, 0, 0);
+                            } else if (s & 1) {
+                                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc1, 0, 0, 0);
                             } else {
-                                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
+                                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], bq[s], acc0, 0, 0, 0);
                             }
                         }
-                    }
-            }
-            if (!kPair) {
+                } else {
+                    // K steps come in fours (KP is a multiple of 8): one 16-byte LDS
+                    // read feeds four MFMAs
+                    const float4* a0 = reinterpret_cast<const float4*>(rowA0) + h * (KP / 8);
+                    const float4* a1 = reinterpret_cast<const float4*>(rowA1) + h * (KP / 8);
 #pragma unroll
-                for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
+                    for (int s4 = 0; s4 < MAXSTEPS / 4; s4++)
+                        if (s4 * 4 < steps) {
+                            const float4 x0 = a0[s4];
+                            const float4 x1 = kPair ? a1[s4] : x0;
+                            const float v0[4] = {x0.x, x0.y, x0.z, x0.w};
+                            const float v1[4] = {x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int s = s4 * 4 + u;
+                                if (kPair) {
+                                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
+                                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1[u], bqf[s], acc1, 0, 0, 0);
+                                } else if (u & 1) {
+                                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc1, 0, 0, 0);
+                                } else {
+                                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], bqf[s], acc0, 0, 0, 0);
+                                }
+                            }
+                        }
+                }
+                if (!kPair) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) acc0[r] += acc1[r];
+                }
+                // the query's own row: only the tile at the wave's query rows holds
+                // it (a wave-uniform test), at offset col of the lane's query
+                float gm[4], m;
+                group_max(acc0, gm, m);
+                epilogue<E, QC, PR>(acc0, gm, m, r0 + sp * 32 == qbase, r0 + sp * 32, h, col, lane,
+                                    thr, qn, lists, myq, a.rowmap, floor);
+                if (kPair) {
+                    group_max(acc1, gm, m);
+                    epilogue<E, QC, PR>(acc1, gm, m, r0 + (sp + 1) * 32 == qbase, r0 + (sp + 1) * 32,
+                                        h, col, lane, thr, qn, lists, myq, a.rowmap, floor);
+                }
             }
-            // the query's own row: only the tile at the wave's query rows holds
-            // it (a wave-uniform test), at offset col of the lane's query
-            if (r0 + sp * 32 == qbase) mask_self(acc0, h, col);
-            if (kPair && r0 + (sp + 1) * 32 == qbase) mask_self(acc1, h, col);
-            epilogue<E, QC, PR>(acc0, r0 + sp * 32, h, col, lane, thr, qn, lists, myq, a.rowmap,
-                                floor);
-            if (kPair)
-                epilogue<E, QC, PR>(acc1, r0 + (sp + 1) * 32, h, col, lane, thr, qn, lists, myq,
-                                    a.rowmap, floor);
         }
         // stage st+1's copies have landed in LDS before any wave passes the barrier
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     };
     for (uint32_t st = 0; st < nst; st++) {
+        if (PR && st > 0 && (st & 63u) == 0u) {
+            idc = idn;
+            idn = load_ids(st + 64);
+        }
         if (st & 1u) step(tile1, negh1, tile0, negh0, st);
         else step(tile0, negh0, tile1, negh1, st);
     }
@@ -559,7 +640,7 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
         const int b = __builtin_ctzll(need);
         need &= need - 1;
         merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), __builtin_amdgcn_readlane(qn, b + 32),
-                           lane, lists, myq);
+                           lane, lists, myq, PR ? a.rowmap : nullptr);
     }
 }
 
@@ -982,9 +1063,10 @@ __global__ __launch_bounds__(256) void k_stage_lists(
     const float* __restrict__ scen, const float* __restrict__ srad,
     const float* __restrict__ snorm, const uint32_t* __restrict__ scnt,
     const uint2* __restrict__ own, const float* __restrict__ negh2,
-    const uint64_t* __restrict__ lists, int S, int KP, uint32_t nblk, uint32_t nst,
+    const uint64_t* __restrict__ lists, int S, int KP, uint32_t nblk, uint32_t nst, int all,
     uint32_t* __restrict__ counts, const uint32_t* __restrict__ offs, uint32_t* __restrict__ ids) {
-    // WRITE reads counts (the count pass's totals) as a bound
+    // WRITE reads counts (the count pass's totals) as a bound; all: every
+    // stage outside the block's own cell, no bound (the unpruned sweep)
     __shared__ __align__(16) float cb[kListBlocks][128];
     __shared__ float dmax[kListBlocks];
     __shared__ uint32_t wcnt[kListBlocks][4];
@@ -1021,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_stage_lists(
     for (int j = 0; j < kListBlocks; j++) {
         const uint32_t b = b0 + j;
         vb[j] = b < nblk && bcnt[b] > 0;
-        D[j] = dmax[j];
+        D[j] = all ? INFINITY : dmax[j];
         rb[j] = vb[j] ? brad[b] : 0.0f;
         nb[j] = vb[j] ? sqrtf(bnorm[b]) : 0.0f;
         ob[j] = vb[j] ? own[b] : make_uint2(0u, 0u);
@@ -1312,9 +1394,15 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
     double swept = 1.0, list_mean = 0.0;
     uint32_t cells = 0, list_max = 0, aux_items = 0;
 
-    const bool prune = !scratch && ctx->knn_prune && n >= kPruneMinRows &&
-                       sh.W * kQPerWave == (int)kBlockRows;
-    if (!prune) {
+    // From kPruneMinRows rows the sweep runs in cell order: each query block
+    // first sweeps its own cell, then (pruned) the stages its bound cannot rule
+    // out, or (pruning off) every other stage.  The own cell fills the lists
+    // with near-final thresholds, so the rest of the sweep, even when it visits
+    // every stage, rarely inserts: most tiles leave on the max/ballot test.
+    const bool by_cell = !scratch && n >= kPruneMinRows && sh.W * kQPerWave == (int)kBlockRows &&
+                         ctx->knn_prune >= 0;
+    const bool prune = by_cell && ctx->knn_prune > 0;
+    if (!by_cell) {
         const uint32_t qblocks = npad / (uint32_t)(sh.W * kQPerWave);
         const uint32_t max_splits = scratch ? (uint32_t)std::max(1, scratch->max_splits) : 8u;
         // split count: forced, recorded for this shape class, tuned now, or the
@@ -1576,7 +1664,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         const dim3 gl(div_up(nblk, kListBlocks));
         hipLaunchKernelGGL(k_stage_lists<false>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
                            bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
-                           nst, counts.p, nullptr, nullptr);
+                           nst, prune ? 0 : 1, counts.p, nullptr, nullptr);
         MRS_CHECK_LAUNCH();
         exclusive_scan_u32(counts.p, offs.p, nblk, offs.p + nblk, s);
         fetch_counts(counts.p);
@@ -1584,7 +1672,7 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         DevBuf<uint32_t> ids(total ? total : 1);
         hipLaunchKernelGGL(k_stage_lists<true>, gl, dim3(256), 0, s, bcen.p, brad.p, bnorm.p,
                            bcnt.p, scen, srad, snorm, scnt, range.p, negh2.p, lists.p, S, KP, nblk,
-                           nst, counts.p, offs.p, ids.p);
+                           nst, prune ? 0 : 1, counts.p, offs.p, ids.p);
         MRS_CHECK_LAUNCH();
         // chunks: a tenth of a round of workgroups' share, at least 1024 stages
         // (or the context's chunk, which the tests lower to split small problems)

@@ -262,6 +262,8 @@ struct mrs_ctx {
     std::shared_ptr<mrs::DevicePool> pool;  // this context's reusable device memory
     size_t hbm_budget = 0;
     int verbose = 0;
-    int knn_prune = 1;  // exhaustive kNN skips stages that provably hold no candidate
+    // exhaustive kNN: 1 skips stages that provably hold no candidate, 0 sweeps
+    // every stage in cell order, -1 sweeps every stage in row order
+    int knn_prune = 1;
     uint32_t knn_chunk = 0;  // pass-2 work-item length in stages (0: the size-based default)
 };

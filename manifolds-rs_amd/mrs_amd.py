@@ -305,10 +305,12 @@ class Context:
         """Wait for the device and return the context's cached memory to it."""
         _check(lib().mrs_ctx_release_cache(self.handle))
 
-    def set_knn_pruning(self, on: bool) -> None:
+    def set_knn_pruning(self, on) -> None:
         """Exhaustive kNN skips database stages that provably hold no candidate
-        (default) or sweeps every stage; both give the same results."""
-        _check(lib().mrs_ctx_set_knn_pruning(self.handle, 1 if on else 0))
+        (True or 1, the default), or sweeps every stage: in cell order (False or
+        0) or in row order (-1).  All give the same results."""
+        mode = on if isinstance(on, int) and not isinstance(on, bool) else (1 if on else 0)
+        _check(lib().mrs_ctx_set_knn_pruning(self.handle, mode))
 
     def set_knn_work_chunk(self, stages: int) -> None:
         """Second-pass stage lists longer than `stages` are cut into work items

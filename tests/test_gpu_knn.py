@@ -76,14 +76,23 @@ def test_knn_device_tensors(ctx):
 
 
 def _both_sweeps(ctx, x, k, precision, shortlist):
-    """(idx, dist, timings) with pruning, and (idx, dist) of the full sweep."""
+    """(idx, dist, timings) with pruning, and (idx, dist) of the full sweep in
+    row order; the full sweep in cell order (every stage, own cell first) is
+    checked against the row-order one bit for bit on the way."""
     got = M.knn_exhaustive(ctx, x, k, precision, shortlist, timed=True)
-    ctx.set_knn_pruning(False)
     try:
+        ctx.set_knn_pruning(-1)
         full = M.knn_exhaustive(ctx, x, k, precision, shortlist, timed=True)
+        ctx.set_knn_pruning(0)
+        cfull = M.knn_exhaustive(ctx, x, k, precision, shortlist, timed=True)
     finally:
         ctx.set_knn_pruning(True)
     assert full[2]["swept"] == 1.0 and full[2]["cells"] == 0
+    # every stage that holds a row (cells are padded to whole blocks: stages of
+    # padding alone are skipped, so the fraction of padded pairs is below 1)
+    assert cfull[2]["cells"] > 0 and cfull[2]["swept"] >= got[2]["swept"]
+    assert np.array_equal(cfull[0], full[0])
+    assert np.array_equal(cfull[1].view(np.uint32), full[1].view(np.uint32))
     return got, full
 
 
