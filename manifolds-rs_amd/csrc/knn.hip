@@ -238,41 +238,78 @@ struct ShortlistArgs {
 };
 constexpr uint32_t kMainList = 0xFFFFFFFFu;
 
-// Wave-wide merge of query b's queue into its sorted global list; returns the
-// list's new threshold key (its last entry).  The queue is two half-queues of
-// QC/2 entries, one per accumulator half (lanes b and b + 32), holding c0 and
-// c1 entries: lane l reads entry l % 32 of half l / 32.  With a rowmap, queue
-// entries carry database positions, which become the caller's rows here (one
-// gather per merge rather than one dependent load per queued candidate), so
-// the list keys, and the (score, row) order of the merge, are the caller's.
+// Wave-wide merges of queries' queues into their sorted global lists.  A
+// query's queue is two half-queues of QC/2 entries, one per accumulator half
+// (lanes b and b + 32), holding c0 and c1 entries: lane l reads entry l % 32
+// of half l / 32; the wave sorts the queue and merges it into the list, whose
+// new last entry is the query's threshold key.  With a rowmap, queue entries
+// carry database positions, which become the caller's rows here (one gather
+// per merge rather than one dependent load per queued candidate), so the list
+// keys, and the (score, row) order of the merge, are the caller's.
+//   One call merges up to kMergeBatch queries bs[0..nb): every queue, list
+// and rowmap load of the batch is issued before the first sort, so a batch
+// waits out two memory latencies rather than two per query, and the sorts
+// then run back to back.  qn holds the lanes' half-queue counts (lanes b and
+// b + 32 hold query b's); tk[j] returns query bs[j]'s new threshold key.
+constexpr int kMergeBatch = 4;
 template <int E, int QC>
-__device__ inline uint64_t merge_query(int b, uint32_t c0, uint32_t c1, int lane,
-                                       uint64_t* __restrict__ lists,
-                                       const uint64_t* __restrict__ myq,
-                                       const uint32_t* __restrict__ rowmap) {
+__device__ inline void merge_queries(const int (&bs)[kMergeBatch], int nb, uint32_t qn, int lane,
+                                     uint64_t* __restrict__ lists,
+                                     const uint64_t* __restrict__ myq,
+                                     const uint32_t* __restrict__ rowmap,
+                                     uint64_t (&tk)[kMergeBatch]) {
     static_assert(QC == 64, "two half-queues of 32: one entry per lane");
     constexpr int S = 64 * E;
     // the queue entries were stored by this wave's lanes: complete those
     // stores before any lane reads another lane's entry
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    const uint32_t c = lane < 32 ? c0 : c1;
-    uint64_t one[1] = {(uint32_t)(lane & 31) < c ? myq[b * QC + lane] : kInfKey};
-    if (rowmap && one[0] != kInfKey)
-        one[0] = (one[0] & 0xFFFFFFFF00000000ull) | rowmap[(uint32_t)one[0]];
-    // the list load is issued before the queue sort, so its latency overlaps
-    // the sort instead of following it (each lane reads back only the list
-    // entries it wrote itself)
-    uint64_t v[E];
+    uint64_t one[kMergeBatch][1], v[kMergeBatch][E];
 #pragma unroll
-    for (int e = 0; e < E; e++) v[e] = lists[(size_t)b * S + e * 64 + lane];
-    __builtin_amdgcn_sched_barrier(0);  // keep the load ahead of the sort
-    wave_sort<1>(one);
-    wave_merge<E>(v, one[0]);
+    for (int j = 0; j < kMergeBatch; j++) {
+        one[j][0] = kInfKey;
+        if (j < nb) {  // wave-uniform
+            const int b = bs[j];
+            const uint32_t c = lane < 32 ? __builtin_amdgcn_readlane(qn, b)
+                                         : __builtin_amdgcn_readlane(qn, b + 32);
+            if ((uint32_t)(lane & 31) < c) one[j][0] = myq[b * QC + lane];
 #pragma unroll
-    for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[e];
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v[E - 1] >> 32), 63);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v[E - 1], 63);
-    return ((uint64_t)hi << 32) | lo;
+            for (int e = 0; e < E; e++) v[j][e] = lists[(size_t)b * S + e * 64 + lane];
+        }
+    }
+    if (rowmap) {
+#pragma unroll
+        for (int j = 0; j < kMergeBatch; j++)
+            if (j < nb && one[j][0] != kInfKey)
+                one[j][0] = (one[j][0] & 0xFFFFFFFF00000000ull) | rowmap[(uint32_t)one[j][0]];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every load of the batch ahead of the sorts
+#pragma unroll
+    for (int j = 0; j < kMergeBatch; j++) {
+        if (j >= nb) break;  // wave-uniform
+        const int b = bs[j];
+        wave_sort<1>(one[j]);
+        wave_merge<E>(v[j], one[j][0]);
+#pragma unroll
+        for (int e = 0; e < E; e++) lists[(size_t)b * S + e * 64 + lane] = v[j][e];
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v[j][E - 1] >> 32), 63);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v[j][E - 1], 63);
+        tk[j] = ((uint64_t)hi << 32) | lo;
+    }
+}
+
+// Pops up to kMergeBatch query indices (lowest first) from the bit set need.
+__device__ inline int pop_queries(uint64_t& need, int (&bs)[kMergeBatch]) {
+    int nb = 0;
+#pragma unroll
+    for (int j = 0; j < kMergeBatch; j++) {
+        bs[j] = 0;
+        if (need) {
+            bs[j] = __builtin_ctzll(need);
+            need &= need - 1;
+            nb = j + 1;
+        }
+    }
+    return nb;
 }
 
 // Epilogue of one 32-row MFMA tile: the lane's 16 scores (one query, rows
@@ -331,15 +368,16 @@ __device__ inline void epilogue(f32x16 acc, const float (&gm)[4], float m, bool 
     const uint64_t full = __ballot(qn > (uint32_t)(QC / 2 - 16));
     uint64_t need = (full | (full >> 32)) & 0xFFFFFFFFull;  // one bit per query
     while (need) {
-        const int b = __builtin_ctzll(need);
-        need &= need - 1;
-        const uint64_t t = merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b),
-                                              __builtin_amdgcn_readlane(qn, b + 32), lane, lists,
-                                              myq, PR ? rowmap : nullptr);
-        if (col == b) {
-            qn = 0;
-            thr = fmaxf(key_score(t), floor);  // an aux list never goes below its floor
-        }
+        int bs[kMergeBatch];
+        const int nb = pop_queries(need, bs);
+        uint64_t tk[kMergeBatch];
+        merge_queries<E, QC>(bs, nb, qn, lane, lists, myq, PR ? rowmap : nullptr, tk);
+#pragma unroll
+        for (int j = 0; j < kMergeBatch; j++)
+            if (j < nb && col == bs[j]) {
+                qn = 0;
+                thr = fmaxf(key_score(tk[j]), floor);  // an aux list never goes below its floor
+            }
     }
 }
 
@@ -637,10 +675,10 @@ __attribute__((amdgpu_waves_per_eu(BF16 && STEPS > 0 ? 4 : (BF16 ? 1 : 3)))) voi
     const uint64_t left = __ballot(qn > 0);
     uint64_t need = (left | (left >> 32)) & 0xFFFFFFFFull;
     while (need) {
-        const int b = __builtin_ctzll(need);
-        need &= need - 1;
-        merge_query<E, QC>(b, __builtin_amdgcn_readlane(qn, b), __builtin_amdgcn_readlane(qn, b + 32),
-                           lane, lists, myq, PR ? a.rowmap : nullptr);
+        int bs[kMergeBatch];
+        const int nb = pop_queries(need, bs);
+        uint64_t tk[kMergeBatch];
+        merge_queries<E, QC>(bs, nb, qn, lane, lists, myq, PR ? a.rowmap : nullptr, tk);
     }
 }
 
