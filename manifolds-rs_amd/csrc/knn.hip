@@ -1088,6 +1088,131 @@ __global__ __launch_bounds__(256) void k_block_ranges(const uint32_t* __restrict
     range[b] = make_uint2(pstart[lo] / RT, (pstart[lo + 1] - pstart[lo]) / RT);
 }
 
+// Pass-1 floors.  For each query, a score that at least S rows of its own
+// cell (itself excluded) reach: any such value is at most the S-th best score
+// of the cell, and so of the whole database, so every row of the query's
+// final list still passes it, and pass 1, starting from it, queues only the
+// rows at or above it (about a third of what it queued from -inf, where the
+// threshold climbs from nothing).  The scores are the shortlist kernel's bits:
+// the same bias, the same MFMA chain over the K steps in order, the query's
+// own row at -inf.  One sweep over the cell counts each query's rows in
+// kFloorBins bins of squared distance d2 = |x|^2 - 2 score on a log scale
+// (sixteen bins per octave, from r2 / 8 to 2 r2, r2 the block's squared
+// radius, rows beyond them in the end bins), in LDS; the floor is the score
+// at the far edge of the bin after the one where the count from the nearest
+// reaches S (a bin, at least 3% of d2, above any rounding), or -inf if no
+// such bin lies within the window or the floor would lose its meaning to
+// cancellation against |x|^2.  One workgroup per pass-1 work item (a
+// 128-query block; items come largest cell first, so no long block starts
+// last), wave w taking queries w*32 .. w*32 + 31 as in the shortlist kernel;
+// rows are read from global memory (the cell is a few hundred KB and stays
+// in L2), one tile ahead of their MFMA chain.
+constexpr int kFloorBins = 64;
+constexpr int kFloorPerOct = 16;  // bins per octave of d2
+constexpr int kFloorBelow = 48;   // bins below r2 (three octaves); one octave above
+template <int STEPS>
+__global__ __launch_bounds__(256) void k_cell_floor(const u16* __restrict__ P,
+                                                    const float* __restrict__ negh,
+                                                    const uint2* __restrict__ range,
+                                                    const float* __restrict__ brad,
+                                                    const uint4* __restrict__ items, int KP,
+                                                    int RT, int S, float* __restrict__ floor_out) {
+    __shared__ uint32_t hist[4][kFloorBins][kQPerWave];  // bank = query
+    const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t blk = __builtin_amdgcn_readfirstlane(items[blockIdx.x].x);
+    const uint2 rg = range[blk];
+    const uint32_t qbase = (blk * 4 + wave) * kQPerWave;
+    const uint32_t q = qbase + col;
+    bf16x8 bq[STEPS];
+    {
+        const bf16x8* src = reinterpret_cast<const bf16x8*>(P + (size_t)q * KP + h * (KP / 2));
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) bq[s] = src[s];
+    }
+    const uint32_t r0 = __builtin_amdgcn_readfirstlane(rg.x * (uint32_t)RT);
+    const uint32_t r1 = __builtin_amdgcn_readfirstlane((rg.x + rg.y) * (uint32_t)RT);
+    const float xn2 = -2.0f * negh[q];  // |x|^2 of the query's rounded coordinates
+    const float r2 = brad[blk] * brad[blk];
+    const bool scaled = r2 > 0.0f && r2 < INFINITY && xn2 < INFINITY;
+    const float inv_r2 = scaled ? 1.0f / r2 : 0.0f;
+    uint32_t* hq = &hist[wave][0][0];
+    for (int i = lane; i < kQPerWave * kFloorBins; i += 64) hq[i] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    struct Ops {
+        float4 nv[4];
+        bf16x8 a[STEPS];
+    };
+    auto load = [&](uint32_t row0, Ops& o) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) o.nv[i] = *reinterpret_cast<const float4*>(negh + row0 + i * 8 + h * 4);
+        const bf16x8* ap =
+            reinterpret_cast<const bf16x8*>(P + (size_t)(row0 + col) * KP + h * (KP / 2));
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) o.a[s] = ap[s];
+    };
+    // bin of a score from the bits of d2 / r2 (its exponent and top four
+    // mantissa bits: sixteen bins per octave), clamped to the window
+    auto bin_of = [&](float sc) {
+        const float d2 = fmaxf(xn2 - 2.0f * sc, 0.0f) * inv_r2;
+        const int bits = (int)(__float_as_uint(d2) >> 19);  // exponent * 16 + top four mantissa bits
+        return min(kFloorBins - 1, max(0, bits - (127 * kFloorPerOct - kFloorBelow)));
+    };
+    auto count = [&](uint32_t row0, const Ops& o) {
+        f32x16 c;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            c[i * 4 + 0] = o.nv[i].x;
+            c[i * 4 + 1] = o.nv[i].y;
+            c[i * 4 + 2] = o.nv[i].z;
+            c[i * 4 + 3] = o.nv[i].w;
+        }
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a[s], bq[s], c, 0, 0, 0);
+        if (row0 == qbase) mask_self(c, h, col);
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (scaled && c[r] != -INFINITY) atomicAdd(&hist[wave][bin_of(c[r])][col], 1u);
+    };
+    if (r0 < r1 && __ballot(scaled)) {
+        // two operand sets take turns: one loads while the other's MFMAs run
+        Ops o0, o1;
+        load(r0, o0);
+        for (uint32_t row0 = r0; row0 < r1; row0 += 64) {
+            if (row0 + 32 < r1) load(row0 + 32, o1);
+            count(row0, o0);
+            if (row0 + 32 < r1) {
+                if (row0 + 64 < r1) load(row0 + 64, o0);
+                count(row0 + 32, o1);
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (h == 0) {
+        float fl = -INFINITY;
+        if (scaled) {
+            uint32_t cum = 0;
+            int b = 0;
+            for (; b < kFloorBins; b++) {
+                cum += hist[wave][b][col];
+                if (cum >= (uint32_t)S) break;
+            }
+            // every row counted so far lies below the far edge of bin b, so
+            // (a bin further) above the score at the far edge of bin b + 1
+            if (b + 2 < kFloorBins) {
+                const int e = b + 2 - kFloorBelow;
+                const float edge = ldexpf(r2, e >> 4) * (1.0f + (float)(e & 15) / 16.0f);
+                if (edge >= 1e-4f * xn2) fl = 0.5f * (xn2 - edge);
+            }
+        }
+        floor_out[q] = fl;
+    }
+}
+
 // Pass-2 stage lists.  A workgroup takes kListBlocks query blocks; each
 // thread tests one stage against all of them per round.  A block's bound D is
 // the largest distance^2 threshold of its queries after pass 1 (|x|^2 - 2 thr,
@@ -1692,6 +1817,29 @@ void knn_exhaustive_dev(mrs_ctx* ctx, const float* x, int64_t n64, int32_t d, in
         fetch_counts(counts1.p);
         make_items(0xFFFFFFFFu);
         upload(items1, hitems);
+        // pass-1 floors (BF16 kernels whose tiles are one MFMA chain each)
+        DevBuf<float> floor1;
+        const int ksteps1 = bf16 ? KP / 16 : 0;
+        if (bf16 && sh.RT >= 64 && ksteps1 >= 1 && ksteps1 <= 8 && !hitems.empty()) {
+            floor1.alloc(npos);
+            auto fl = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3((unsigned)hitems.size()), dim3(256), 0, s,
+                                   (const u16*)P2.p, negh2.p, range.p, brad.p, items1.p, KP,
+                                   sh.RT, S, floor1.p);
+            };
+            switch (ksteps1) {
+                case 1: fl(k_cell_floor<1>); break;
+                case 2: fl(k_cell_floor<2>); break;
+                case 3: fl(k_cell_floor<3>); break;
+                case 4: fl(k_cell_floor<4>); break;
+                case 5: fl(k_cell_floor<5>); break;
+                case 6: fl(k_cell_floor<6>); break;
+                case 7: fl(k_cell_floor<7>); break;
+                default: fl(k_cell_floor<8>); break;
+            }
+            MRS_CHECK_LAUNCH();
+            sa.floor = floor1.p;
+        }
         sa.items = items1.p;
         sa.stage_ids = ids1.p;
         MRS_HIP(hipStreamSynchronize(s));  // hitems is read by the copy
