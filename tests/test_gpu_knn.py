@@ -145,6 +145,23 @@ def test_pruned_sweep_lattice_ties(ctx, precision):
     _check_pruned(ctx, x, 15, precision, shortlist=60)
 
 
+def test_pruned_sweep_far_from_origin(ctx):
+    # Clustered data under a common offset of 400 in every coordinate: |x|^2
+    # dwarfs the neighbour distances, so the pass-1 floors (scores at a squared
+    # distance d2, computed as |x|^2 - 2 score) would lose their meaning to
+    # cancellation and are not used for most queries.  BF16 cannot resolve the
+    # neighbours at this offset, so only the sweeps are compared, bit for bit:
+    # pruned, full in cell order and full in row order.
+    import synthetic
+
+    x, _ = synthetic.subspace(n=66_000, d=32, n_clusters=30, seed=21)
+    x = (x + 400.0).astype(np.float32)
+    (idx, dist, t), (fidx, fdist, _) = _both_sweeps(ctx, x, 15, M.KNN_BF16_RERANK, 60)
+    assert t["cells"] > 0
+    assert np.array_equal(idx, fidx)
+    assert np.array_equal(dist.view(np.uint32), fdist.view(np.uint32))
+
+
 def test_pruned_sweep_wide_rows(ctx):
     # d = 100: 64-row stages (separate stage and block bounds) and the 7-step
     # BF16 kernel
