@@ -83,11 +83,16 @@ __global__ __launch_bounds__(kWg) void k_cc_init(uint32_t* __restrict__ lab, uin
 
 // Hook the components of u and v together: the larger of their labels is
 // lowered to the smaller one (a root's label is itself after a jump pass).
+// Labels only decrease, so where a plain read already shows lab[hi] <= lo the
+// atomic would change nothing and is skipped: many entries join the same two
+// fragments, and their atomics on one root's label would otherwise queue up
+// behind each other (densMAP's 500k graph: 1.17 ms per sweep).
 __device__ inline void cc_hook_pair(uint32_t u, uint32_t v, uint32_t* __restrict__ lab,
                                     uint32_t* __restrict__ changed) {
     uint32_t a = lab[u], b = lab[v];
     if (a == b) return;
     uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    if (__hip_atomic_load(&lab[hi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= lo) return;
     uint32_t old = atomicMin(&lab[hi], lo);
     if (old > lo) *changed = 1u;
 }
