@@ -416,9 +416,10 @@ __device__ inline void stage_copy(const unsigned char* __restrict__ P, const flo
 }
 
 // The distance GEMM + top-S filter.  One workgroup = W waves x 32 queries; the
-// database split [db0, db1) streams through two LDS stage buffers of RT rows
-// (double buffering: one barrier per stage), each stage prefetched into
-// registers one stage ahead.  QC = per-query queue capacity.  STEPS > 0
+// database split [db0, db1) (PR: the work item's list of stages, in cell
+// order) streams through two LDS stage buffers of RT rows (double buffering:
+// one barrier per stage), each stage copied straight into LDS one stage ahead
+// (global_load_lds).  QC = per-query queue capacity.  STEPS > 0
 // compiles the K-step count in (BF16: KP / 16), so the query fragments take
 // only the registers they need and the MFMA chain is straight-line code.
 //   The loop is latency-bound (waves mostly wait on LDS, the stage barrier
