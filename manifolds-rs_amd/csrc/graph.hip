@@ -49,6 +49,11 @@ __device__ inline float lc_distance(const float* d, int len, float lc) {
 }
 
 // smooth_knn_dist (bandwidth search in FP64) + knn_to_coo weights (f32).
+// kSmoothLanes lanes per row: lane l takes the row's neighbours l, l + 16, ...
+// and a fixed xor tree joins their terms, so every lane of a row sees the
+// same sum and takes the same branch of the search, and a wave's rows wait
+// for the slowest of four rows rather than of sixty-four.
+constexpr int kSmoothLanes = 16;
 __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ dist,
                                                     const int32_t* __restrict__ idx, uint32_t n,
                                                     int k, float lc, double tol, double target,
@@ -58,8 +63,9 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
                                                     uint32_t* __restrict__ out_cnt,
                                                     uint32_t* __restrict__ in_cnt,
                                                     uint32_t* __restrict__ bad_idx) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = (blockIdx.x * kWg + threadIdx.x) / kSmoothLanes;
+    const int sub = threadIdx.x & (kSmoothLanes - 1);
+    if (i >= n) return;  // whole row groups: the xor trees stay within a group
     const float* d = dist + (size_t)i * k;
     const float r = lc_distance(d, k, lc);
     const double sentinel = (double)FLT_MAX;
@@ -69,11 +75,13 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
         // reference does (graph.rs:538-540)
         const double inv_mid = 1.0 / mid;
         double s = 0.0;
-        for (int j = 0; j < k; j++) {
+        for (int j = sub; j < k; j += kSmoothLanes) {
             float a = d[j] - r;
             a = a > 0.0f ? a : 0.0f;
             s += exp(-(double)a * inv_mid);
         }
+#pragma unroll
+        for (int o = kSmoothLanes / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, kSmoothLanes);
         if (fabs(s - target) < tol) break;
         if (s > target) {
             hi = mid;
@@ -85,12 +93,14 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
         }
     }
     const float sg = (float)mid;
-    sigma[i] = sg;
-    rho[i] = r;
+    if (sub == 0) {
+        sigma[i] = sg;
+        rho[i] = r;
+    }
     const bool smooth = sg > 0.0f;
     const float inv_sigma = smooth ? 1.0f / sg : 0.0f;
     uint32_t cnt = 0;
-    for (int j = 0; j < k; j++) {
+    for (int j = sub; j < k; j += kSmoothLanes) {
         const int32_t c = idx[(size_t)i * k + j];
         float a = d[j] - r;
         a = a > 0.0f ? a : 0.0f;
@@ -103,7 +113,9 @@ __global__ __launch_bounds__(kWg) void k_smooth_knn(const float* __restrict__ di
             atomicAdd(&in_cnt[c], 1u);
         }
     }
-    out_cnt[i] = cnt;
+#pragma unroll
+    for (int o = kSmoothLanes / 2; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, kSmoothLanes);
+    if (sub == 0) out_cnt[i] = cnt;
 }
 
 __global__ __launch_bounds__(kWg) void k_add_u32(const uint32_t* __restrict__ a,
@@ -764,7 +776,8 @@ mrs_graph* build_graph(mrs_ctx* ctx, const int32_t* knn_idx, const float* knn_di
         MRS_HIP(hipMemsetAsync(in_cnt.p, 0, in_cnt.bytes(), s));
         MRS_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), s));
         MRS_HIP(hipMemsetAsync(raw_cnt.p, 0, raw_cnt.bytes(), s));
-        hipLaunchKernelGGL(k_smooth_knn, dim3(div_up(n, kWg)), dim3(kWg), 0, s, knn_dist, knn_idx,
+        hipLaunchKernelGGL(k_smooth_knn, dim3(div_up((uint64_t)n * kSmoothLanes, kWg)), dim3(kWg), 0,
+                           s, knn_dist, knn_idx,
                            n, k, local_connectivity, (double)bandwidth, std::log((double)k), 64,
                            g->sigma.p, g->rho.p, w.p, out_cnt.p, in_cnt.p, bad_idx.p);
         MRS_CHECK_LAUNCH();
