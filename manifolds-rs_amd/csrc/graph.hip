@@ -410,13 +410,24 @@ __global__ __launch_bounds__(kWg) void k_periods(const float* __restrict__ w, ui
 // xor tree, so the result does not depend on scheduling.  An entry's distance
 // is looked up in row i's kNN list, else in row j's.
 constexpr int kDensLanes = 16;
+constexpr int kDensStage = 128;
 __global__ __launch_bounds__(kWg) void k_dens_rows(
     const uint32_t* __restrict__ u_off, const uint32_t* __restrict__ u_col,
     const float* __restrict__ u_w, const uint32_t* __restrict__ keep,
     const int32_t* __restrict__ knn_idx, const float* __restrict__ knn_dist, int k, uint32_t n,
     int squared, double* __restrict__ ro, float* __restrict__ mu_sum) {
+    // row i's kNN indices staged in LDS (k <= kDensStage), where the row
+    // group's lanes scan them for every entry (a broadcast read each)
+    __shared__ int32_t own[kWg / kDensLanes][kDensStage];
     const uint32_t i = (blockIdx.x * kWg + threadIdx.x) / kDensLanes;
     const uint32_t sub = threadIdx.x & (kDensLanes - 1);
+    const uint32_t grp = threadIdx.x / kDensLanes;
+    const bool staged = k <= kDensStage;
+    if (staged && i < n)
+        for (int t = (int)sub; t < k; t += kDensLanes) own[grp][t] = knn_idx[(size_t)i * k + t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // a row group lies within one wave
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double acc = 0.0, ms = 0.0;
     if (i < n) {
         for (uint32_t p = u_off[i] + sub; p < u_off[i + 1]; p += kDensLanes) {
@@ -425,7 +436,7 @@ __global__ __launch_bounds__(kWg) void k_dens_rows(
             float raw = 0.0f;
             bool found = false;
             for (int t = 0; t < k; t++)
-                if ((uint32_t)knn_idx[(size_t)i * k + t] == j) {
+                if ((uint32_t)(staged ? own[grp][t] : knn_idx[(size_t)i * k + t]) == j) {
                     raw = knn_dist[(size_t)i * k + t];
                     found = true;
                     break;
