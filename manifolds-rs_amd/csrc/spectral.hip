@@ -419,7 +419,9 @@ __device__ inline void component_coefs(const double* __restrict__ partial, uint3
 // thread adds its rows in order, a fixed xor tree joins a wave's lanes and the
 // waves are added in order.  A small component's Lanczos step is then five
 // launches instead of nine.
-constexpr uint32_t kFusedChunks = 64;
+// (up to 131 072 rows: each block re-reads its component's partials, 128 x
+// m doubles, which past that costs more than the launches it saves)
+constexpr uint32_t kFusedChunks = 128;
 constexpr int kMaxLanczos = 128;  // Lanczos steps: 4 * (n_comp + 1) + 30 <= 98
 __global__ __launch_bounds__(kWg) void k_cgs_update_fused(const Chunk* __restrict__ chunks,
                                                           const uint32_t* __restrict__ comp_chunk0,
