@@ -878,6 +878,10 @@ def bench_serve(args):
     mrs_umap_gpu."""
     import threading
 
+    if args.prep_workers > 8 or args.hw_queues > 16:
+        # one run at 12 workers and 32 hardware queues ended in a device memory
+        # fault with no cause found; settings past those that ran cleanly are refused
+        raise SystemExit("serve: at most 8 prep workers and 16 hardware queues")
     if args.hw_queues:
         # hardware queues per process: each prep worker and the batcher get one,
         # so their small kernels overlap instead of queueing behind each other
@@ -1011,7 +1015,7 @@ def main():
     ap.add_argument("--seq-sample", type=int, default=40, help="serve: sequential sample size")
     ap.add_argument("--prep-workers", type=int, default=8)
     ap.add_argument("--hw-queues", type=int, default=16,
-                    help="serve: GPU_MAX_HW_QUEUES for the run (0 = leave as is; at most 32)")
+                    help="serve: GPU_MAX_HW_QUEUES for the run (0 = leave as is; at most 16)")
     ap.add_argument("--max-active", type=int, default=256)
     ap.add_argument("--knn-full-sweep", action="store_true",
                     help="knn: sweep every database stage (no pruning)")

@@ -414,7 +414,9 @@ typedef struct mrs_server mrs_server;
 /* Upper estimate of one fit's peak device bytes (preparation and resident state). */
 size_t mrs_fit_bytes_estimate(int64_t n, int32_t d, int32_t k, int32_t n_dim);
 
-/* hbm_budget 0 = half the device's free memory at creation. */
+/* hbm_budget 0 = half the device's free memory at creation.  prep_workers
+ * 1..8 (larger counts are refused: a run with 12 workers ended in a device
+ * memory fault whose cause has not been found); max_active 1..4096. */
 int mrs_server_create(int32_t device, size_t hbm_budget, int32_t prep_workers, int32_t max_active,
                       mrs_server **out);
 

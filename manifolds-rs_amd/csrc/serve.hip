@@ -75,6 +75,9 @@ struct FitDesc {
     uint32_t blk_node0;  // first block of this fit in the node launch
 };
 
+// prep worker threads per server (the most that have run without fault)
+constexpr int kMaxPrepWorkers = 8;
+
 // Workgroups of a launch go to the chip's 8 XCDs round-robin: block b runs on
 // XCD b mod 8.  The edge launch gives each XCD its own bin of edge blocks, so
 // that a fit's edges run on as few XCDs as possible and its embedding, the
@@ -509,8 +512,12 @@ int mrs_server_create(int32_t device, size_t hbm_budget, int32_t prep_workers, i
                       mrs_server** out) {
     return mrs::guarded([&] {
         if (!out) throw Error(MRS_INVALID_ARGUMENT, "out is NULL");
-        if (prep_workers < 1 || prep_workers > 16)
-            throw Error(MRS_INVALID_ARGUMENT, "prep_workers must be 1..16");
+        // Up to 8 workers have run cleanly; one run with 12 workers (and 32
+        // hardware queues) ended in a device memory fault whose cause has not
+        // been found, so larger counts are refused until it is.
+        if (prep_workers < 1 || prep_workers > mrs::kMaxPrepWorkers)
+            throw Error(MRS_INVALID_ARGUMENT,
+                        "prep_workers must be 1.." + std::to_string(mrs::kMaxPrepWorkers));
         if (max_active < 1 || max_active > 4096)
             throw Error(MRS_INVALID_ARGUMENT, "max_active must be 1..4096");
         auto* sv = new mrs_server();

@@ -66,9 +66,10 @@ def test_fit_bytes_estimate_grows_with_every_size():
 
 
 def test_server_create_checks_arguments_and_fails_cleanly_without_a_device():
-    # bad worker / slot counts are refused before anything starts; on a host
-    # with no HIP device creation reports a device error and returns no handle
-    for workers, slots in ((0, 8), (17, 8), (2, 0), (2, 5000)):
+    # bad worker / slot counts are refused before anything starts (at most 8
+    # prep workers); on a host with no HIP device creation reports a device
+    # error and returns no handle
+    for workers, slots in ((0, 8), (9, 8), (12, 8), (17, 8), (2, 0), (2, 5000)):
         h = C.c_void_p()
         assert M.lib().mrs_server_create(0, 0, workers, slots, C.byref(h)) == M.MRS_INVALID_ARGUMENT
         assert not h.value
