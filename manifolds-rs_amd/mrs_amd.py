@@ -694,7 +694,7 @@ class Server:
     umap_gpu fit and returns a ticket; wait(ticket) returns its [n_dim, n]
     embedding (numpy for host data, a device tensor for device data) and the
     request's {queued, preparing, latency} milliseconds.  Submitted arrays are
-    kept alive here until their request is collected."""
+    kept alive here until their request is collected.  prep_workers 1..8."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, prep_workers: int = 4,
                  max_active: int = 256):
